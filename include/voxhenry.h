@@ -1,0 +1,102 @@
+/*
+ * voxhenry.h -- C ABI of the MI355X-native VoxHenry solve path.
+ *
+ * Conventions (all entry points):
+ *   - return 0 on success, a positive VH_ERR_* code on failure; the message of
+ *     the last failure on the calling thread is returned by vh_last_error();
+ *     no C++ exception crosses this boundary;
+ *   - complex data are interleaved (re, im) doubles or floats;
+ *   - indices are 0-based int64; voxel order is MATLAB column-major order of
+ *     the non-empty voxels (idxS of pre_define_structure_params.m:8-13);
+ *   - the unknown vector of one right-hand side has n = num_curr + num_node
+ *     entries ordered [Jx; Jy; Jz; J2d; J3d; phi] (idxS5, :16);
+ *   - pointers flagged "device" are HIP device pointers, all others are host;
+ *   - one context per thread; calls on one context are not thread-safe.
+ */
+#ifndef VOXHENRY_H
+#define VOXHENRY_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VH_C128 0 /* complex double */
+#define VH_C64 1  /* complex float */
+
+#define VH_ERR_ARG 1
+#define VH_ERR_HIP 2
+#define VH_ERR_STATE 3
+#define VH_ERR_NUMERIC 4
+#define VH_ERR_ALLOC 5
+#define VH_ERR_INTERNAL 6
+
+typedef struct vh_ctx_s vh_ctx;
+typedef struct vh_geom_s vh_geom;
+typedef struct vh_precon_s vh_precon;
+
+/* ---- context ------------------------------------------------------------ */
+/* Process-wide device context (the MEX gateway keeps one alive with mexLock). */
+int vh_ctx_create(int device, vh_ctx** out);
+int vh_ctx_destroy(vh_ctx* ctx);
+/* Launch all work of this context on `hip_stream` (a hipStream_t, NULL = default stream). */
+int vh_ctx_set_stream(vh_ctx* ctx, void* hip_stream);
+int vh_ctx_synchronize(vh_ctx* ctx);
+const char* vh_last_error(void);
+int vh_version(void);
+
+/* ---- geometry -------------------------------------------------------------
+ * Grid L x M x N of voxels of side dx with nvox non-empty voxels at the
+ * 0-based column-major linear indices idx (strictly increasing).
+ * pad_mode 0 pads to exactly (2L,2M,2N) like lse_generate_circulant_tensor.m:776
+ * (then 2L,2M,2N must be {2,3,5,7}-smooth); pad_mode 1 picks the smallest
+ * smooth length >= 2n-1 per axis (same result on the voxel grid). */
+int vh_geom_create(vh_ctx* ctx, int64_t L, int64_t M, int64_t N, double dx, int64_t nvox, const int64_t* idx,
+                   int pad_mode, vh_geom** out);
+int vh_geom_destroy(vh_geom* g);
+/* info[0..9] = nv, num_curr, num_node, Px, Py, Pz, nlines, dense, has_tensor, has_precon-ready */
+int vh_geom_info(const vh_geom* g, int64_t* info);
+
+/* Green's tensor, replaces lse_generate_circulant_tensor(dx,1,L,M,N,0)
+ * (lse_generate_circulant_tensor.m:1-833): generated on the GPU.  st2 receives
+ * st_sparse_precon2 (3 complex, :772). */
+int vh_geom_generate_green(vh_geom* g, double* st2);
+/* Drop-in alternative: the reference's padded spectrum fN (Px*Py*Pz*7 complex,
+ * column-major) with fN*scale equal to the ko=1 spectrum fN_all2. */
+int vh_geom_set_green_spectrum(vh_geom* g, const double* fN, int64_t P0, int64_t P1, int64_t P2, double scale_re,
+                               double scale_im, const double* st2);
+/* Real-space normalised tensor G_mn (L*M*N*7 complex, column-major, host). */
+int vh_geom_set_green_realspace(vh_geom* g, const double* G, const double* st2);
+/* Copy the real-space tensor generated on the GPU back to the host (L*M*N*7 complex). */
+int vh_geom_get_green_realspace(vh_geom* g, double* G);
+
+/* Node-incidence matrix Ar (num_node x 5*nvox) in CSC form (MATLAB sparse),
+ * port rows already removed (VoxHenry_executer.m:181-192). */
+int vh_geom_set_incidence(vh_geom* g, int64_t num_node, int64_t nnz, const int64_t* colptr, const int64_t* rowind,
+                          const double* val);
+/* z = z_real + z_imag per non-empty voxel (nvox complex), VoxHenry_executer.m:231-259. */
+int vh_geom_set_impedance(vh_geom* g, const double* z);
+
+/* ---- matvec ---------------------------------------------------------------
+ * y = A x, replaces lse_matvect_mult(J, fN_all, Ae, z_real, z_imag, dx, idx, ...)
+ * (lse_matvect_mult.m:1-153) for nrhs vectors with leading dimensions ldx, ldy
+ * (in complex elements).  jwmu = j*omega*mu scales the Green's product
+ * (VoxHenry_executer.m:266).  prec: VH_C128 / VH_C64.  mode 0: full operator;
+ * 1: FFT block + resistive diagonal (current rows only); 2: convolution only.
+ * on_device != 0: x and y are device pointers. */
+int vh_matvec(vh_geom* g, double jwmu_re, double jwmu_im, int prec, int mode, int64_t nrhs, const void* x,
+              int64_t ldx, void* y, int64_t ldy, int on_device);
+
+/* Timing helper: run the matvec `reps` times on device-resident data and
+ * report the average time of one call (ms) and of each fused pass. */
+int vh_matvec_timed(vh_geom* g, double jwmu_re, double jwmu_im, int prec, int mode, int64_t nrhs, const void* x,
+                    int64_t ldx, void* y, int64_t ldy, int reps, double* ms_total, double* ms_pass, int* npass);
+
+/* Test hook: in-place 3-D FFT (dir -1 forward, +1 inverse unnormalised) of ncomp
+ * device grids of the geometry's padded size. */
+int vh_fft3d(vh_geom* g, void* data, int ncomp, int prec, int dir);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VOXHENRY_H */

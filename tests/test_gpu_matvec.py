@@ -1,0 +1,157 @@
+"""GPU parity of the FFT engine, the Green's tensor producer and the matvec
+against the FP64 oracle (lse_matvect_mult.m / lse_generate_circulant_tensor.m)."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import topology, green, matvec as omv
+from tests import problems
+
+pytestmark = pytest.mark.gpu
+
+MU0 = 4 * np.pi * 1e-7
+TOL64 = 1e-12     # FP64 path vs oracle, relative 2-norm
+TOL32 = 2e-5      # FP32 path vs oracle, relative 2-norm
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize('dims,pad_mode', [((2, 4, 4), 0), ((30, 10, 10), 0), ((7, 5, 3), 1), ((64, 12, 6), 1)])
+@pytest.mark.parametrize('prec', [0, 1])
+def test_fft3d(ctx, dims, pad_mode, prec):
+    import torch
+    import voxhenry_amd as vh
+    say('fft3d', dims, pad_mode, prec)
+    L, M, N = dims
+    idx = np.arange(L * M * N)
+    G = vh.Geometry(ctx, dims, 1e-6, idx, pad_mode=pad_mode)
+    P = G.pad
+    rng = np.random.default_rng(1)
+    a = rng.standard_normal((2,) + P[::-1]) + 1j * rng.standard_normal((2,) + P[::-1])   # [c][z][y][x]
+    dt = torch.complex128 if prec == 0 else torch.complex64
+    t = torch.tensor(a, dtype=dt, device='cuda').contiguous()
+    G.fft3d(t, prec, -1)
+    torch.cuda.synchronize()
+    ref = np.fft.fftn(a, axes=(1, 2, 3))
+    err = rel(t.cpu().numpy(), ref)
+    assert err < (1e-14 if prec == 0 else 2e-6), err
+    G.fft3d(t, prec, +1)
+    torch.cuda.synchronize()
+    back = t.cpu().numpy() / np.prod(P)
+    assert rel(back, a) < (1e-14 if prec == 0 else 2e-6)
+
+
+@pytest.mark.parametrize('dims', [(2, 4, 4), (3, 3, 3), (6, 5, 4), (12, 7, 9)])
+def test_green_realspace(ctx, dims):
+    import voxhenry_amd as vh
+    say('green', dims)
+    L, M, N = dims
+    dx = 1e-6
+    G = vh.Geometry(ctx, dims, dx, np.arange(L * M * N), pad_mode=1)
+    st = G.generate_green()
+    Gd = G.get_green_realspace()
+    Go, sto = green.generate_G(L, M, N, dx)
+    for c in range(7):
+        e = np.abs(Gd[..., c] - Go[..., c]).max() / np.abs(Go[..., c]).max()
+        assert e < 1e-13, (c, e)
+    assert np.allclose(st, sto, rtol=1e-14, atol=0)
+
+
+def _problem(name):
+    if name == 'wire':
+        geo = problems.wire_vhr()
+    elif name == 'cube3':
+        geo = problems.bar(3, 3, 3)
+    elif name == 'bar432':
+        geo = problems.bar(4, 3, 2)
+    elif name == 'cfg1':
+        geo = problems.bar(30, 10, 10)
+    elif name == 'hollow':
+        geo = problems.bar(9, 6, 5)
+        geo['sigma'][3:6, 2:4, 1:4] = 0.0
+    return geo
+
+
+@pytest.mark.parametrize('name', ['wire', 'cube3', 'bar432', 'hollow', 'cfg1'])
+@pytest.mark.parametrize('tensor', ['oracle', 'gpu'])
+def test_matvec_fp64_vs_oracle(ctx, name, tensor):
+    import voxhenry_amd as vh
+    say('matvec64', name, tensor)
+    geo = _problem(name)
+    s = topology.build_system(geo)
+    L, M, N = s['dims']
+    f = 2.5e9
+    jwmu = 1j * 2 * np.pi * f * MU0
+    fN2, st2 = green.circulant_tensor(L, M, N, geo['dx'])
+    pad_mode = 0 if tensor == 'oracle' else 1
+    G = vh.Geometry(ctx, s['dims'], geo['dx'], s['idxS'], pad_mode=pad_mode)
+    if tensor == 'oracle':
+        G.set_green_spectrum(fN2, 1.0, st2)
+    else:
+        G.generate_green()
+    G.set_incidence(s['Ar'])
+    z = problems.voxel_z(geo, s, f)
+    G.set_impedance(z)
+    zr, zi = topology.impedance(geo, s['idxS'], f)
+    rng = np.random.default_rng(7)
+    n = s['num_curr'] + s['num_node']
+    X = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+    Y = G.matvec(X, jwmu, vh.VH_C128, 0)
+    for k in range(3):
+        ref = omv.lse_matvect_mult(X[k], jwmu * fN2, s['Ar'], zr, zi, geo['dx'], s['idxS5'])
+        assert rel(Y[k], ref) < TOL64, (k, rel(Y[k], ref))
+    # FP32 path
+    Y32 = G.matvec(X.astype(np.complex64), jwmu, vh.VH_C64, 0)
+    ref = omv.lse_matvect_mult(X[0], jwmu * fN2, s['Ar'], zr, zi, geo['dx'], s['idxS5'])
+    assert rel(Y32[0].astype(complex), ref) < TOL32
+
+
+def test_matvec_device_tensors_and_modes(ctx):
+    import torch
+    import voxhenry_amd as vh
+    geo = problems.bar(10, 6, 4)
+    s = topology.build_system(geo)
+    f = 1e8
+    jwmu = 1j * 2 * np.pi * f * MU0
+    G = problems.device_geometry(ctx, geo, s, pad_mode=1, f=f)
+    rng = np.random.default_rng(3)
+    n = G.n
+    X = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    Yh = G.matvec(X, jwmu, vh.VH_C128, 0)
+    Xd = torch.tensor(X, device='cuda')
+    Yd = G.matvec(Xd, jwmu, vh.VH_C128, 0)
+    torch.cuda.synchronize()
+    assert np.abs(Yd.cpu().numpy() - Yh).max() == 0.0
+    # mode 1 (FFT block + diagonal) equals mode 0 minus the incidence terms
+    Y1 = G.matvec(X[:, :s['num_curr']].copy(), jwmu, vh.VH_C128, 1)
+    inc = -(s['Ar'].T @ X[0, s['num_curr']:])
+    assert rel(Y1[0] + inc, Yh[0, :s['num_curr']]) < 1e-13
+
+
+def test_golden_solution_residual(ctx):
+    """numex1 golden x (reference FP64 solution at 2.5 GHz) through the GPU operator."""
+    import os
+    import voxhenry_amd as vh
+    say('golden residual')
+    geo = problems.numex1()
+    s = topology.build_system(geo)
+    d = np.load(os.path.join(problems.GOLDEN, 'numex1_x.npz'))
+    x = d['x']
+    nc = s['num_curr']
+    xr = np.concatenate([x[:nc], x[nc:][s['keep']]])
+    f = float(d['freq'])
+    G = problems.device_geometry(ctx, geo, s, pad_mode=1, f=f)
+    jwmu = 1j * 2 * np.pi * f * MU0
+    b = s['rhs'][:, 0]
+    r = b - G.matvec(xr, jwmu, vh.VH_C128, 0)
+    relres = np.linalg.norm(r) / np.linalg.norm(b)
+    say('golden relres', relres)
+    assert 1e-10 < relres < 3e-9

@@ -1,0 +1,177 @@
+"""Object layer over the C ABI: a device context and a device-resident geometry
+(grid, voxel map, Green's tensor spectrum, impedance, incidence matrix).
+
+Host vectors are numpy arrays (interleaved complex); device vectors are torch
+tensors on the HIP device (torch provides device memory and streams only).
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import lib, check, ptr, VH_C128, VH_C64
+
+MU0 = 4 * np.pi * 1e-7        # EMconstants.m:13
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class Context:
+    """One HIP device context (include/voxhenry.h: vh_ctx_*)."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().vh_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+
+    def use_torch_stream(self):
+        """Launch on torch's current HIP stream so torch events time our kernels."""
+        torch = _torch()
+        s = torch.cuda.current_stream(self.device)
+        check(lib().vh_ctx_set_stream(self.h, C.c_void_p(s.cuda_stream)))
+
+    def synchronize(self):
+        check(lib().vh_ctx_synchronize(self.h))
+
+    def close(self):
+        if self.h:
+            check(lib().vh_ctx_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Geometry:
+    """Device geometry (vh_geom_*).
+
+    idx: 0-based column-major linear indices of the non-empty voxels (idxS).
+    pad_mode 0 = (2L,2M,2N) like the reference; 1 = smallest smooth >= 2n-1.
+    """
+
+    def __init__(self, ctx, dims, dx, idx, pad_mode=1):
+        self.ctx = ctx
+        self.dims = tuple(int(d) for d in dims)
+        self.dx = float(dx)
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        h = C.c_void_p()
+        check(lib().vh_geom_create(ctx.h, *self.dims, self.dx, len(idx), ptr(idx), int(pad_mode), C.byref(h)))
+        self.h = h
+        self.st2 = None
+        self._refresh()
+
+    def _refresh(self):
+        info = np.zeros(10, dtype=np.int64)
+        check(lib().vh_geom_info(self.h, ptr(info)))
+        self.nv, self.num_curr, self.num_node = int(info[0]), int(info[1]), int(info[2])
+        self.pad = (int(info[3]), int(info[4]), int(info[5]))
+        self.nlines = int(info[6])
+
+    @property
+    def n(self):
+        return self.num_curr + self.num_node
+
+    # -- tensor -------------------------------------------------------------
+    def generate_green(self):
+        st = np.zeros(6)
+        check(lib().vh_geom_generate_green(self.h, ptr(st)))
+        self.st2 = st[0::2] + 1j * st[1::2]
+        return self.st2
+
+    def set_green_spectrum(self, fN, scale=1.0, st2=None):
+        fN = np.asfortranarray(fN, dtype=np.complex128)
+        P = fN.shape[:3]
+        st = None if st2 is None else np.ascontiguousarray(np.stack([np.real(st2), np.imag(st2)], 1).ravel())
+        flat = np.ascontiguousarray(fN.ravel(order='F'))
+        check(lib().vh_geom_set_green_spectrum(self.h, ptr(flat), P[0], P[1], P[2], float(np.real(scale)),
+                                               float(np.imag(scale)), ptr(st)))
+        if st2 is not None:
+            self.st2 = np.asarray(st2)
+
+    def set_green_realspace(self, G, st2=None):
+        flat = np.ascontiguousarray(np.asarray(G, dtype=np.complex128).ravel(order='F'))
+        st = None if st2 is None else np.ascontiguousarray(np.stack([np.real(st2), np.imag(st2)], 1).ravel())
+        check(lib().vh_geom_set_green_realspace(self.h, ptr(flat), ptr(st)))
+        if st2 is not None:
+            self.st2 = np.asarray(st2)
+
+    def get_green_realspace(self):
+        L, M, N = self.dims
+        out = np.zeros(L * M * N * 7, dtype=np.complex128)
+        check(lib().vh_geom_get_green_realspace(self.h, ptr(out)))
+        return out.reshape((L, M, N, 7), order='F')
+
+    # -- operator data --------------------------------------------------------
+    def set_incidence(self, Ar):
+        """Ar: scipy.sparse matrix num_node x num_curr (port rows removed)."""
+        A = Ar.tocsc()
+        A.sort_indices()
+        cp = np.ascontiguousarray(A.indptr, dtype=np.int64)
+        ri = np.ascontiguousarray(A.indices, dtype=np.int64)
+        v = np.ascontiguousarray(A.data, dtype=np.float64)
+        check(lib().vh_geom_set_incidence(self.h, A.shape[0], A.nnz, ptr(cp), ptr(ri), ptr(v)))
+        self._refresh()
+
+    def set_impedance(self, z):
+        """z = z_real + z_imag per non-empty voxel (complex, nv)."""
+        z = np.ascontiguousarray(np.asarray(z, dtype=np.complex128))
+        assert z.shape == (self.nv,)
+        check(lib().vh_geom_set_impedance(self.h, ptr(z)))
+
+    # -- matvec ---------------------------------------------------------------
+    def matvec(self, x, jwmu, prec=VH_C128, mode=0):
+        """y = A x for a host (numpy) or device (torch) vector/stack of vectors.
+
+        x: shape (n,) or (nrhs, n); complex128 for VH_C128, complex64 for VH_C64.
+        """
+        n = self.n if mode == 0 else self.num_curr
+        if isinstance(x, np.ndarray):
+            dt = np.complex128 if prec == VH_C128 else np.complex64
+            xx = np.ascontiguousarray(np.atleast_2d(x), dtype=dt)
+            nrhs, ld = xx.shape
+            y = np.zeros_like(xx)
+            check(lib().vh_matvec(self.h, float(np.real(jwmu)), float(np.imag(jwmu)), prec, mode, nrhs, ptr(xx), ld,
+                                  ptr(y), ld, 0))
+            return y.reshape(np.shape(x))
+        torch = _torch()
+        xx = x if x.dim() == 2 else x.unsqueeze(0)
+        assert xx.is_contiguous()
+        nrhs, ld = xx.shape
+        y = torch.zeros_like(xx)
+        self.ctx.use_torch_stream()
+        check(lib().vh_matvec(self.h, float(np.real(jwmu)), float(np.imag(jwmu)), prec, mode, nrhs, ptr(xx), ld,
+                              ptr(y), ld, 1))
+        return y.reshape(x.shape)
+
+    def matvec_timed(self, x, y, jwmu, prec, mode, reps):
+        """Average ms of one matvec call on device-resident torch tensors."""
+        xx = x if x.dim() == 2 else x.unsqueeze(0)
+        yy = y if y.dim() == 2 else y.unsqueeze(0)
+        nrhs, ld = xx.shape
+        ms = np.zeros(1)
+        self.ctx.use_torch_stream()
+        check(lib().vh_matvec_timed(self.h, float(np.real(jwmu)), float(np.imag(jwmu)), prec, mode, nrhs, ptr(xx), ld,
+                                    ptr(yy), yy.shape[1], reps, ptr(ms), None, None))
+        return float(ms[0])
+
+    def fft3d(self, data, prec, direction):
+        self.ctx.use_torch_stream()
+        ncomp = data.numel() // (self.pad[0] * self.pad[1] * self.pad[2])
+        check(lib().vh_fft3d(self.h, ptr(data), ncomp, prec, direction))
+
+    def close(self):
+        if getattr(self, 'h', None):
+            check(lib().vh_geom_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,84 @@
+// Device-resident description of one voxel geometry: grid, padded FFT grid,
+// voxel map, Green's-tensor spectrum (one octant), impedance diagonal and the
+// node-incidence matrix Ar in CSR form (rows = nodes) and its transpose.
+#pragma once
+#include "vh_common.h"
+#include "vh_fft.h"
+
+namespace vh {
+
+struct GridDims {
+  int L, M, N;          // voxel grid
+  int Px, Py, Pz;       // padded FFT grid (>= 2L-1, 2M-1, 2N-1)
+  int Hx, Hy, Hz;       // stored spectral octant: P/2 + 1 per axis
+  int nv;               // non-empty voxels
+};
+
+// Sparse matrix in CSR with 32-bit indices, values in both precisions.
+struct DevCsr {
+  int rows = 0, cols = 0, nnz = 0;
+  DevBuf<int> ptr, col;
+  DevBuf<double> v64;
+  DevBuf<float> v32;
+};
+
+struct Geometry {
+  Context* ctx = nullptr;
+  GridDims g{};
+  double dx = 0;
+  int64_t num_curr = 0, num_node = 0;
+  bool dense = false;                 // every voxel of the grid is non-empty
+  std::vector<int64_t> vox;           // host copy: 0-based column-major linear index per voxel
+  DevBuf<int> vmap;                   // L*M*N -> voxel id or -1
+  DevBuf<int> lines;                  // non-empty (y + M z) lines
+  int nlines = 0;
+  FftAxis ax, ay, az;
+  // Green's tensor spectrum of the normalised real-space tensor (ko = 1),
+  // divided by Px*Py*Pz, one octant [c][kz][ky][kx] of Hx*Hy*Hz per component.
+  DevBuf<double2> t64;                // complex, exact
+  DevBuf<float> t32;                  // real part of even comps / imag part of odd comps
+  bool has_tensor = false;
+  double st2[6] = {0, 0, 0, 0, 0, 0}; // self terms G(1,1,1,{1,4,7}) as complex pairs
+  // impedance z = z_real + z_imag per voxel (complex)
+  DevBuf<double2> z64;
+  DevBuf<float2> z32;
+  bool has_z = false;
+  DevCsr Ar;    // num_node x num_curr
+  DevCsr ArT;   // num_curr x num_node
+  bool has_inc = false;
+  // matvec workspaces, grown on demand
+  DevBuf<char> wsA, wsB;
+
+  size_t octant() const { return (size_t)g.Hx * g.Hy * g.Hz; }
+  int64_t n() const { return num_curr + num_node; }
+};
+
+// Matvec y = A x for nrhs vectors (x, y device pointers, leading dimension ld
+// in complex elements).  alpha = j*omega*mu multiplies the Green's product.
+// mode 0: full operator [Z, -Ar'; Ar, 0]; mode 1: the FFT block with the
+// resistive diagonal only (current rows); mode 2: convolution only.
+template <typename R>
+void matvec(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy,
+            double alpha_re, double alpha_im, int mode = 0);
+
+// Build the spectral tensor on the device from a real-space normalised tensor
+// G (L*M*N*7 complex, device, column-major [x][y][z][c]).
+void tensor_from_realspace(Geometry& G, const double2* Greal_dev);
+// Compute the real-space tensor on the device (quadrature + near table).
+void generate_realspace_green(Geometry& G, DevBuf<double2>& out);
+// In-place 3-D FFT of ncomp padded grids (test hook); prec 0 = double, 1 = float.
+void fft3d_test(Geometry& Gm, void* data, int ncomp, int prec, int dir);
+
+// Per-pass timing hooks (filled by matvec when enabled).
+struct PassTimes { float ms[8]; int n; };
+
+// geometry setup helpers
+void geometry_init(Geometry& G, Context* ctx, int L, int M, int N, double dx, int64_t nv,
+                   const int64_t* idx, int pad_mode);
+void geometry_set_incidence_csc(Geometry& G, int64_t num_node, int64_t nnz, const int64_t* colptr,
+                                const int64_t* rowind, const double* val);
+void geometry_set_impedance(Geometry& G, const double* z_interleaved);
+void geometry_set_spectrum_full(Geometry& G, const double* fN, int64_t P0, int64_t P1, int64_t P2,
+                                double scale_re, double scale_im);
+
+}  // namespace vh

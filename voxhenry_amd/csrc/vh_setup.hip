@@ -1,0 +1,189 @@
+// Geometry setup: FFT plans, voxel map, incidence matrix, impedance, and the
+// drop-in path that accepts the reference's padded tensor spectrum.
+#include "vh_geom.h"
+#include <algorithm>
+#include <cmath>
+
+namespace vh {
+
+bool factor_smooth(int n, std::vector<int>& radices) {
+  radices.clear();
+  if (n < 1) return false;
+  int m = n;
+  for (int r : {8, 4, 5, 3, 2, 7}) {
+    while (m % r == 0) { radices.push_back(r); m /= r; }
+  }
+  if (m != 1 || (int)radices.size() > FFT_MAX_STAGES) return false;
+  if (radices.empty()) radices.push_back(1);
+  return true;
+}
+
+int next_smooth(int n) {
+  std::vector<int> r;
+  for (int m = std::max(n, 1);; ++m)
+    if (m >= 2 && factor_smooth(m, r)) return m;
+}
+
+void FftAxis::init(int len) {
+  n = len;
+  VH_REQUIRE(len >= 2 && factor_smooth(len, radices), "FFT length " + std::to_string(len) + " is not {2,3,5,7}-smooth");
+  std::vector<double2> h64(n);
+  std::vector<float2> h32(n);
+  for (int k = 0; k < n; ++k) {
+    // w_n^k = exp(-2 pi i k/n), evaluated in long double then rounded
+    const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)k / (long double)n;
+    const long double c = cosl(a), s = sinl(a);
+    h64[k].x = (double)c; h64[k].y = (double)s;
+    h32[k].x = (float)c; h32[k].y = (float)s;
+  }
+  tw64.alloc(n);
+  tw32.alloc(n);
+  VH_HIP(hipMemcpy(tw64.p, h64.data(), n * sizeof(double2), hipMemcpyHostToDevice));
+  VH_HIP(hipMemcpy(tw32.p, h32.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+}
+
+template <> FftPlan FftAxis::plan<double>() const {
+  FftPlan p{};
+  p.n = n; p.nstage = (int)radices.size();
+  for (int i = 0; i < p.nstage; ++i) p.radix[i] = radices[i];
+  p.tw = tw64.p;
+  return p;
+}
+template <> FftPlan FftAxis::plan<float>() const {
+  FftPlan p{};
+  p.n = n; p.nstage = (int)radices.size();
+  for (int i = 0; i < p.nstage; ++i) p.radix[i] = radices[i];
+  p.tw = tw32.p;
+  return p;
+}
+
+void geometry_init(Geometry& G, Context* ctx, int L, int M, int N, double dx, int64_t nv, const int64_t* idx,
+                   int pad_mode) {
+  VH_REQUIRE(L >= 2 && M >= 2 && N >= 2, "grid dims must be >= 2 (near-field table covers a 2x2x2 block)");
+  VH_REQUIRE(dx > 0, "dx must be positive");
+  const int64_t V = (int64_t)L * M * N;
+  VH_REQUIRE(nv >= 1 && nv <= V, "voxel count out of range");
+  VH_REQUIRE(V < (1LL << 31) && 5 * nv < (1LL << 31), "grid too large for 32-bit voxel indexing");
+  G.ctx = ctx;
+  G.dx = dx;
+  G.g.L = L; G.g.M = M; G.g.N = N;
+  G.g.nv = (int)nv;
+  if (pad_mode == 0) {
+    G.g.Px = 2 * L; G.g.Py = 2 * M; G.g.Pz = 2 * N;
+  } else {
+    G.g.Px = next_smooth(2 * L - 1); G.g.Py = next_smooth(2 * M - 1); G.g.Pz = next_smooth(2 * N - 1);
+  }
+  G.g.Hx = G.g.Px / 2 + 1; G.g.Hy = G.g.Py / 2 + 1; G.g.Hz = G.g.Pz / 2 + 1;
+  G.ax.init(G.g.Px); G.ay.init(G.g.Py); G.az.init(G.g.Pz);
+  G.num_curr = 5 * nv;
+  G.vox.assign(idx, idx + nv);
+  std::vector<int> vmap(V, -1);
+  for (int64_t k = 0; k < nv; ++k) {
+    VH_REQUIRE(idx[k] >= 0 && idx[k] < V, "voxel index out of range");
+    VH_REQUIRE(k == 0 || idx[k] > idx[k - 1], "voxel indices must be strictly increasing (column-major order)");
+    vmap[idx[k]] = (int)k;
+  }
+  G.dense = (nv == V);
+  std::vector<int> lines;
+  for (int64_t line = 0; line < (int64_t)M * N; ++line) {
+    bool any = false;
+    for (int i = 0; i < L && !any; ++i) any = vmap[i + (int64_t)L * line] >= 0;
+    if (any) lines.push_back((int)line);
+  }
+  G.nlines = (int)lines.size();
+  G.vmap.alloc(V);
+  VH_HIP(hipMemcpy(G.vmap.p, vmap.data(), V * sizeof(int), hipMemcpyHostToDevice));
+  G.lines.alloc(std::max<size_t>(lines.size(), 1));
+  if (!lines.empty()) VH_HIP(hipMemcpy(G.lines.p, lines.data(), lines.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+static void upload_csr(DevCsr& D, int rows, int cols, const std::vector<int>& ptr, const std::vector<int>& col,
+                       const std::vector<double>& val) {
+  D.rows = rows; D.cols = cols; D.nnz = (int)col.size();
+  D.ptr.alloc(rows + 1);
+  VH_HIP(hipMemcpy(D.ptr.p, ptr.data(), (rows + 1) * sizeof(int), hipMemcpyHostToDevice));
+  const size_t nz = std::max<size_t>(col.size(), 1);
+  D.col.alloc(nz); D.v64.alloc(nz); D.v32.alloc(nz);
+  if (!col.empty()) {
+    std::vector<float> v32(val.begin(), val.end());
+    VH_HIP(hipMemcpy(D.col.p, col.data(), col.size() * sizeof(int), hipMemcpyHostToDevice));
+    VH_HIP(hipMemcpy(D.v64.p, val.data(), val.size() * sizeof(double), hipMemcpyHostToDevice));
+    VH_HIP(hipMemcpy(D.v32.p, v32.data(), v32.size() * sizeof(float), hipMemcpyHostToDevice));
+  }
+}
+
+void geometry_set_incidence_csc(Geometry& G, int64_t num_node, int64_t nnz, const int64_t* colptr,
+                                const int64_t* rowind, const double* val) {
+  const int64_t nc = G.num_curr;
+  VH_REQUIRE(num_node >= 0 && num_node < (1LL << 31), "num_node out of range");
+  VH_REQUIRE(colptr[0] == 0 && colptr[nc] == nnz, "CSC column pointer inconsistent with nnz");
+  // Ar' in CSR is Ar in CSC: rows of Ar' = columns of Ar.
+  std::vector<int> tp(nc + 1), tc(nnz);
+  std::vector<double> tv(nnz);
+  std::vector<int> cnt(num_node + 1, 0);
+  for (int64_t j = 0; j <= nc; ++j) tp[j] = (int)colptr[j];
+  for (int64_t k = 0; k < nnz; ++k) {
+    VH_REQUIRE(rowind[k] >= 0 && rowind[k] < num_node, "CSC row index out of range");
+    tc[k] = (int)rowind[k];
+    tv[k] = val[k];
+    cnt[rowind[k] + 1]++;
+  }
+  for (int64_t i = 0; i < num_node; ++i) cnt[i + 1] += cnt[i];
+  std::vector<int> rp(cnt.begin(), cnt.end()), rc(nnz);
+  std::vector<double> rv(nnz);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (int64_t j = 0; j < nc; ++j)
+    for (int64_t k = colptr[j]; k < colptr[j + 1]; ++k) {
+      const int r = (int)rowind[k];
+      rc[fill[r]] = (int)j;
+      rv[fill[r]] = val[k];
+      fill[r]++;
+    }
+  G.num_node = num_node;
+  upload_csr(G.Ar, (int)num_node, (int)nc, rp, rc, rv);
+  upload_csr(G.ArT, (int)nc, (int)num_node, tp, tc, tv);
+  G.has_inc = true;
+}
+
+void geometry_set_impedance(Geometry& G, const double* z) {
+  const int nv = G.g.nv;
+  std::vector<float2> z32(nv);
+  for (int k = 0; k < nv; ++k) { z32[k].x = (float)z[2 * k]; z32[k].y = (float)z[2 * k + 1]; }
+  if (G.z64.n != (size_t)nv) { G.z64.alloc(nv); G.z32.alloc(nv); }
+  VH_HIP(hipMemcpy(G.z64.p, z, nv * sizeof(double2), hipMemcpyHostToDevice));
+  VH_HIP(hipMemcpy(G.z32.p, z32.data(), nv * sizeof(float2), hipMemcpyHostToDevice));
+  G.has_z = true;
+}
+
+// Keep one octant of a full padded spectrum (host, column-major [x][y][z][c]).
+void geometry_set_spectrum_full(Geometry& G, const double* fN, int64_t P0, int64_t P1, int64_t P2, double sre,
+                                double sim) {
+  const GridDims& g = G.g;
+  VH_REQUIRE(P0 == g.Px && P1 == g.Py && P2 == g.Pz,
+             "tensor spectrum dims differ from the geometry's padded grid (create the geometry with pad_mode 0)");
+  const size_t O = G.octant();
+  const double invP = 1.0 / ((double)g.Px * g.Py * g.Pz);
+  std::vector<double2> h(7 * O);
+  std::vector<float> h32(7 * O);
+  for (int c = 0; c < 7; ++c)
+    for (int kz = 0; kz < g.Hz; ++kz)
+      for (int ky = 0; ky < g.Hy; ++ky)
+        for (int kx = 0; kx < g.Hx; ++kx) {
+          const size_t src = (size_t)kx + (size_t)g.Px * (ky + (size_t)g.Py * (kz + (size_t)g.Pz * c));
+          const double re = fN[2 * src], im = fN[2 * src + 1];
+          double2 v;
+          v.x = (re * sre - im * sim) * invP;
+          v.y = (re * sim + im * sre) * invP;
+          const size_t dst = (size_t)c * O + kx + (size_t)g.Hx * (ky + (size_t)g.Hy * kz);
+          h[dst] = v;
+          const bool odd = (c == 1 || c == 2 || c == 4);
+          h32[dst] = (float)(odd ? v.y : v.x);
+        }
+  G.t64.alloc(7 * O);
+  G.t32.alloc(7 * O);
+  VH_HIP(hipMemcpy(G.t64.p, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
+  VH_HIP(hipMemcpy(G.t32.p, h32.data(), h32.size() * sizeof(float), hipMemcpyHostToDevice));
+  G.has_tensor = true;
+}
+
+}  // namespace vh
