@@ -1,0 +1,255 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X VoxHenry solve path (one GPU).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+Workload (a "step"): the batched multi-port bus -- 8 parallel copper bars of
+400x4x4 um at 8 um pitch, 1 um voxels in a 400x64x4 grid, 8 ports x 64
+log-spaced frequencies 1e5..1e10 Hz = 512 right-hand sides, every (port,
+frequency) system solved to a true FP64 relative residual <= 1e-8.  The mixed
+path (complex64 inner FFT matvec + preconditioner under FP64 flexible GMRES) is
+the headline `value` (solves/s); the plain FP64 path (pgmres semantics in
+complex128) is timed on the same step for the ratio.  Inputs are synthetic
+(generated geometry), resident in HBM before timing; per-frequency
+preconditioner factors are prepared before the timed region.
+
+Also measured: the bare FFT-matvec microbench -- a fully conductive 128^3
+copper grid (padded to 256^3), 1 um voxels at 1 GHz, currents N(0,1) -- in
+complex128 and complex64, with the per-pass kernel times from HIP events on the
+launch stream.  `roofline` reports its dominant kernel (pass Z: z-FFT, Green's
+product, inverse z-FFT).
+
+The CPU baseline is the FP64 numpy/scipy oracle (oracle/) timed on a bounded
+sample of the same bus workload on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "batched port x freq solves/s at 1e-8 resid, mixed vs FP64; FFT-matvec HBM GB/s"
+PASS_NAMES = ['X', 'Y', 'Z', 'Yi', 'Xi', 'node']
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=2)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--nfreq', type=int, default=64)
+    ap.add_argument('--matvec-reps', type=int, default=20)
+    ap.add_argument('--cube', type=int, default=128)
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--no-fp64', action='store_true')
+    ap.add_argument('--cpu-solves', type=int, default=2)
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pass_bytes(g, nv, c, g_t, octant, nrhs=1):
+    """Algorithmic HBM bytes of each fused pass for nrhs vectors (c bytes per
+    complex, g_t bytes per stored tensor element)."""
+    L, M, N, Px, Py, Pz = g
+    a = Px * M * N
+    b = Px * Py * N
+    return {
+        'X': nrhs * 5 * c * (nv + a),
+        'Y': nrhs * 5 * c * (a + b),
+        'Z': nrhs * 5 * c * (2 * b) + 7 * octant * g_t,
+        'Yi': nrhs * 5 * c * (b + a),
+        'Xi': nrhs * 5 * c * (a + 2 * nv) + nrhs * c * nv,
+    }
+
+
+def b_fft(V, nv, c, tensor_bytes, nrhs=1):
+    """Canonical 3-pass byte model of one FFT-block matvec: b*c*(80V + 15nv) +
+    tensor (stored size) + 8nv."""
+    return nrhs * c * (80 * V + 15 * nv) + tensor_bytes + 8 * nv
+
+
+def matvec_bench(ctx, n, reps):
+    import torch
+    import voxhenry_amd as vh
+    from voxhenry_amd import structures
+    st = structures.cube(n)
+    L = M = N = n
+    G = vh.Geometry(ctx, (L, M, N), st['dx'], np.arange(L * M * N), pad_mode=1)
+    G.generate_green()
+    G.set_impedance(np.full(L * M * N, 1.0 / 5.8e7, dtype=complex))
+    f = 1e9
+    jwmu = 1j * 2 * np.pi * f * vh.MU0
+    nc = G.num_curr
+    rng = np.random.default_rng(0)
+    xh = rng.standard_normal(nc) + 1j * rng.standard_normal(nc)
+    out = {}
+    Px, Py, Pz = G.pad
+    octant = (Px // 2 + 1) * (Py // 2 + 1) * (Pz // 2 + 1)
+    res = {}
+    for prec, name, c, gt, dt in ((vh.VH_C128, 'c128', 16, 16, torch.complex128), (vh.VH_C64, 'c64', 8, 4, torch.complex64)):
+        x = torch.tensor(xh, dtype=dt, device='cuda').reshape(1, -1).contiguous()
+        y = torch.zeros_like(x)
+        G.matvec_timed(x, y, jwmu, prec, 1, 3)       # warm-up (allocates the workspaces)
+        ms, ms_pass = G.matvec_timed_passes(x, y, jwmu, prec, 1, reps)
+        res[name] = y.cpu().numpy().ravel().astype(np.complex128)
+        pb = pass_bytes((L, M, N, Px, Py, Pz), L * M * N, c, gt, octant)
+        bf = b_fft(L * M * N, L * M * N, c, 7 * octant * gt)
+        out[name] = dict(ms=ms, matvecs_per_s=1e3 / ms, B_fft_GB=bf / 1e9, GBs_B_fft=bf / ms / 1e6,
+                         passes={k: dict(ms=ms_pass[i], GB=pb.get(k, 0) / 1e9,
+                                         GBs=(pb[k] / ms_pass[i] / 1e6) if k in pb and ms_pass[i] > 0 else None)
+                                 for i, k in enumerate(PASS_NAMES)})
+        del x, y
+    out['c64_vs_c128_rel'] = float(np.linalg.norm(res['c64'] - res['c128']) / np.linalg.norm(res['c128']))
+    out['grid'] = [L, M, N]
+    out['pad'] = [Px, Py, Pz]
+    G.close()
+    torch.cuda.empty_cache()
+    return out
+
+
+def bus_setup(ctx, nfreq):
+    import voxhenry_amd as vh
+    from voxhenry_amd import structures, solver as vsol
+    freqs = np.logspace(5, 10, nfreq)
+    st = structures.bus(freq=freqs)
+    t0 = time.perf_counter()
+    S = vsol.Solver(ctx, st)
+    S.G.set_impedance(vsol.impedance(st, S.idx, freqs[0]))
+    precs = []
+    for f in freqs:
+        P = vh.Precon(S.G)
+        P.prepare(1j * 2 * np.pi * f * vh.MU0)
+        precs.append(P)
+    return S, freqs, precs, time.perf_counter() - t0
+
+
+def bus_step(S, freqs, precs, method):
+    """One step: every (port, frequency) system of the bus; returns R, L, stats."""
+    import voxhenry_amd as vh
+    R = np.zeros((len(freqs), S.nports, S.nports))
+    Lh = np.zeros_like(R)
+    stats = []
+    for k, f in enumerate(freqs):
+        jwmu = 1j * 2 * np.pi * f * vh.MU0
+        if method == 'mixed':
+            X, st = S.G.solve_mixed(S.rhs, jwmu, precs[k], tol=1e-8)
+        else:
+            X, st = S.G.gmres(S.rhs, jwmu, precs[k], restart=100, tol=1e-8, maxit=10)
+        Y = S.rhs @ X.T
+        Z = np.linalg.inv(Y)
+        R[k] = np.abs(Z.real)
+        Lh[k] = np.abs(Z.imag) / (2 * np.pi * f)
+        stats.append(st)
+    return R, Lh, stats
+
+
+def timed_steps(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(steps):
+        out = fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps, out
+
+
+def cpu_baseline(nsolves):
+    """Oracle (FP64 numpy/scipy) on a bounded sample of the bus workload: setup
+    untimed, then `nsolves` (port, frequency) solves timed."""
+    from oracle import driver, gmres
+    from voxhenry_amd import structures
+    st = structures.bus()
+    geo = dict(st, well_cond=st['well'])
+    t0 = time.perf_counter()
+    prob = driver.Problem(geo)
+    setup = time.perf_counter() - t0
+    fs = np.logspace(5, 10, 64)[[40, 63]][:max(1, nsolves)]
+    t = 0.0
+    done = 0
+    for f in fs:
+        A, P, *_ = prob.operators(f)
+        t1 = time.perf_counter()
+        gmres.pgmres(A, prob.sys['rhs'][:, 0], 100, 1e-8, 10, P)
+        t += time.perf_counter() - t1
+        done += 1
+    return dict(value=done / t, unit='solves/s', cores=1, kind='port',
+                sample='%d bus solves (port 1 at f=%s Hz), oracle pgmres + SuperLU Schur solve, numpy FFT; '
+                       'setup %.1f s untimed' % (done, ','.join('%.3g' % f for f in fs), setup))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print('bench.py runs on one GPU only: this build has no multi-GPU path '
+              '(each (port, frequency) system and each geometry is independent; see DESIGN.md)', file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import voxhenry_amd as vh
+    ctx = vh.Context(0)
+    ctx.use_torch_stream()
+
+    log('[bench] FFT-matvec microbench %d^3' % args.cube)
+    mv = matvec_bench(ctx, args.cube, args.matvec_reps)
+    log('[bench] matvec c128 %.3f ms  c64 %.3f ms' % (mv['c128']['ms'], mv['c64']['ms']))
+
+    log('[bench] bus set-up (%d frequencies)' % args.nfreq)
+    S, freqs, precs, t_setup = bus_setup(ctx, args.nfreq)
+    nsol = len(freqs) * S.nports
+    log('[bench] set-up %.1f s, %d systems per step' % (t_setup, nsol))
+
+    ms_mixed, (Rm, Lm, stm) = timed_steps(lambda: bus_step(S, freqs, precs, 'mixed'), args.steps, args.warmup)
+    log('[bench] mixed step %.3f s' % ms_mixed)
+    fp64 = None
+    if not args.no_fp64:
+        s64, (R64, L64, st64) = timed_steps(lambda: bus_step(S, freqs, precs, 'fp64'), max(1, args.steps // 2), 0)
+        log('[bench] fp64 step %.3f s' % s64)
+        eR = float(np.max(np.abs(Rm - R64) / np.abs(R64)))
+        eL = float(np.max(np.abs(Lm - L64) / np.abs(L64)))
+        fp64 = dict(solves_per_s=nsol / s64, s_per_step=s64, max_rel_err_R_mixed_vs_fp64=eR,
+                    max_rel_err_L_mixed_vs_fp64=eL,
+                    iters_mean=float(np.mean([np.mean(s['niters']) for s in st64])),
+                    relres_max=float(np.max([np.max(s['relres']) for s in st64])),
+                    all_converged=bool(all(np.all(s['flag'] == 0) for s in st64)))
+    mixed = dict(solves_per_s=nsol / ms_mixed, s_per_step=ms_mixed,
+                 outer_iters_mean=float(np.mean([np.mean(s['outer_iters']) for s in stm])),
+                 inner_iters_mean=float(np.mean([np.mean(s['inner_iters']) for s in stm])),
+                 relres_max=float(np.max([np.max(s['relres']) for s in stm])),
+                 all_converged=bool(all(np.all(s['flag'] == 0) for s in stm)))
+
+    z = mv['c64']['passes']['Z']
+    roof = dict(kernel='k_pass_z<float> (z-FFT + Green product + inverse z-FFT), 128^3 grid, complex64',
+                bound='hbm', achieved=z['GBs'], peak=HBM_PEAK_GBS, unit='GB/s',
+                frac=(z['GBs'] / HBM_PEAK_GBS) if z['GBs'] else None, traffic=None)
+
+    cpu = None
+    if not args.no_cpu:
+        log('[bench] CPU baseline (oracle)')
+        cpu = cpu_baseline(args.cpu_solves)
+
+    line = dict(metric=METRIC, value=mixed['solves_per_s'], unit='solves/s', n_gpus=1, steps=args.steps,
+                warmup=args.warmup, ms_per_step=ms_mixed * 1e3, higher_is_better=True, scaling='weak',
+                vs_baseline=None, dtype='c64 inner / f64 outer', data='synthetic',
+                config=dict(workload='bus: 8 bars 400x4x4 um, 400x64x4 grid, 8 ports x %d freqs 1e5-1e10 Hz '
+                                     '= %d systems per step, tol 1e-8 (true FP64 residual)' % (len(freqs), nsol),
+                            parallelism='1 GPU'),
+                mixed=mixed, fp64=fp64,
+                fp64_speedup=(mixed['solves_per_s'] / fp64['solves_per_s']) if fp64 else None,
+                fft_matvec=mv, roofline=roof, cpu_baseline=cpu,
+                setup_s=t_setup)
+    print(json.dumps(line))
+
+
+if __name__ == '__main__':
+    main()

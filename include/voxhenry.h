@@ -74,6 +74,19 @@ int vh_geom_get_green_realspace(vh_geom* g, double* G);
  * port rows already removed (VoxHenry_executer.m:181-192). */
 int vh_geom_set_incidence(vh_geom* g, int64_t num_node, int64_t nnz, const int64_t* colptr, const int64_t* rowind,
                           const double* val);
+/* Build the node numbering and incidence matrix of the geometry itself
+ * (lse_generate_panelIDs.m, lse_compute_Ae_matrix.m), look up the port faces
+ * (lse_assign_exc_grnd_nodes.m), remove the port rows and form the port
+ * right-hand sides rhs = -sum of the excitation rows of Ae (VoxHenry_executer.m:
+ * 165-192).  Port p's excitation faces are rows P[pP_ptr[p]..pP_ptr[p+1]) of
+ * (x, y, z, side) 1-based quadruples (side 1..6 = -x,+x,-y,+y,-z,+z), its ground
+ * faces N[pN_ptr[p]..pN_ptr[p+1]); `well` lists extra grounded faces.
+ * num_node receives the node count after removal. */
+int vh_geom_build_topology(vh_geom* g, int64_t nports, const int64_t* pP_ptr, const int64_t* P, const int64_t* pN_ptr,
+                           const int64_t* N, int64_t nwell, const int64_t* well, int64_t* num_node);
+/* Port right-hand sides from vh_geom_build_topology: nports rows of n = num_curr + num_node. */
+int vh_geom_get_rhs(vh_geom* g, double* rhs);
+
 /* z = z_real + z_imag per non-empty voxel (nvox complex), VoxHenry_executer.m:231-259. */
 int vh_geom_set_impedance(vh_geom* g, const double* z);
 
@@ -88,9 +101,55 @@ int vh_matvec(vh_geom* g, double jwmu_re, double jwmu_im, int prec, int mode, in
               int64_t ldx, void* y, int64_t ldy, int on_device);
 
 /* Timing helper: run the matvec `reps` times on device-resident data and
- * report the average time of one call (ms) and of each fused pass. */
+ * report the average time of one call (ms); if ms_pass is given, a second
+ * sweep records events around each fused pass and reports the average time of
+ * the passes X, Y, Z, Yi, Xi and the node rows (npass = 6). */
 int vh_matvec_timed(vh_geom* g, double jwmu_re, double jwmu_im, int prec, int mode, int64_t nrhs, const void* x,
                     int64_t ldx, void* y, int64_t ldy, int reps, double* ms_total, double* ms_pass, int* npass);
+
+/* ---- preconditioner ---------------------------------------------------------
+ * Replaces lse_sparse_precon_prepare.m (schur_invert + ldlt_decomp + CHOLMOD)
+ * and lse_sparse_precon_multiply.m:61-97.  prepare() computes A_inv =
+ * |1/(z/(s dx) + jwmu g_self)| (s = 1,1,1,6,2), S = Ar diag(A_inv) Ar', its
+ * host LDL^T (nested-dissection order, symbolic analysis kept across calls) and
+ * uploads the factors; apply() computes y = M^-1 x on the device.
+ * stats (optional, 4 entries): nnz(L), forward levels, backward levels, n_node. */
+int vh_precon_create(vh_geom* g, vh_precon** out);
+int vh_precon_destroy(vh_precon* p);
+int vh_precon_prepare(vh_precon* p, double jwmu_re, double jwmu_im, int64_t* stats);
+int vh_precon_apply(vh_precon* p, int prec, int64_t nrhs, const void* x, int64_t ldx, void* y, int64_t ldy,
+                    int on_device);
+int vh_precon_get_ainv(vh_precon* p, double* ainv);
+
+/* Host-only sparse LDL^T solve of a symmetric positive definite matrix given in
+ * CSC with its full pattern (both triangles), the stand-in for CHOLMOD
+ * [LD,p,q] = ldlchol(S) followed by ldlsolve(LD, b) (lse_sparse_precon_prepare.m:154,
+ * lse_sparse_precon_multiply.m:86): nested-dissection order, elimination tree,
+ * up-looking numeric factorisation.  b, x: n x nrhs real, column-major.
+ * xyz (3n) / type (n), optional: voxel-face centre and normal axis of every
+ * node; when given, the ordering cuts along face planes.
+ * stats (optional, 3 entries): nnz(L), forward levels, backward levels. */
+int vh_ldl_solve_host(int64_t n, const int64_t* Ap, const int64_t* Ai, const double* Ax, int64_t nrhs,
+                      const double* b, double* x, const float* xyz, const int* type, int64_t* stats);
+
+/* ---- solvers -----------------------------------------------------------------
+ * vh_gmres replaces pgmres(A, b, restart, tol, maxit, L1) (pgmres.m:70-197,
+ * left preconditioner) or, with flexible != 0, fpgmres(A, b, restart, tol,
+ * maxit, R1) (fpgmres.m:58-152), for nrhs right-hand sides b (stride n) that
+ * share the operator (same frequency).  p may be NULL (no preconditioner).
+ * Outputs per right-hand side: flag, relres, iter[2] = [p outit], niters =
+ * Arnoldi steps; resvec (optional) receives up to resvec_cap relative residuals
+ * per right-hand side (row stride resvec_cap), nresvec their counts. */
+int vh_gmres(vh_geom* g, vh_precon* p, double jwmu_re, double jwmu_im, int prec, int flexible, int64_t nrhs,
+             const void* b, void* x, int on_device, int restart, double tol, int maxit, int* flag, double* relres,
+             int* iter, int* niters, double* resvec, int64_t resvec_cap, int* nresvec);
+
+/* Mixed precision solve: FP64 flexible GMRES (outer, true FP64 residual)
+ * preconditioned by an FP32 left-preconditioned GMRES using the FP32 matvec and
+ * the FP32 preconditioner apply.  b, x complex128 (stride n). */
+int vh_solve_mixed(vh_geom* g, vh_precon* p, double jwmu_re, double jwmu_im, int64_t nrhs, const double* b,
+                   double* x, int on_device, double tol, int outer_restart, int outer_maxit, int inner_restart,
+                   double inner_tol, int* flag, double* relres, int* outer_iters, int* inner_iters);
 
 /* Test hook: in-place 3-D FFT (dir -1 forward, +1 inverse unnormalised) of ncomp
  * device grids of the geometry's padded size. */

@@ -47,9 +47,19 @@ SIGNATURES = {
     'vh_geom_get_green_realspace': [_P, _P],
     'vh_geom_set_incidence': [_P, _I64, _I64, _P, _P, _P],
     'vh_geom_set_impedance': [_P, _P],
+    'vh_geom_build_topology': [_P, _I64, _P, _P, _P, _P, _I64, _P, _P],
+    'vh_geom_get_rhs': [_P, _P],
     'vh_matvec': [_P, _D, _D, _I, _I, _I64, _P, _I64, _P, _I64, _I],
     'vh_matvec_timed': [_P, _D, _D, _I, _I, _I64, _P, _I64, _P, _I64, _I, _P, _P, _P],
     'vh_fft3d': [_P, _P, _I, _I, _I],
+    'vh_precon_create': [_P, C.POINTER(_P)],
+    'vh_precon_destroy': [_P],
+    'vh_precon_prepare': [_P, _D, _D, _P],
+    'vh_precon_apply': [_P, _I, _I64, _P, _I64, _P, _I64, _I],
+    'vh_precon_get_ainv': [_P, _P],
+    'vh_gmres': [_P, _P, _D, _D, _I, _I, _I64, _P, _P, _I, _I, _D, _I, _P, _P, _P, _P, _P, _I64, _P],
+    'vh_ldl_solve_host': [_I64, _P, _P, _P, _I64, _P, _P, _P, _P, _P],
+    'vh_solve_mixed': [_P, _P, _D, _D, _I64, _P, _P, _I, _D, _I, _I, _I, _D, _P, _P, _P, _P],
 }
 _RESTYPES = {'vh_last_error': C.c_char_p, 'vh_version': C.c_int}
 
@@ -57,6 +67,10 @@ _RESTYPES = {'vh_last_error': C.c_char_p, 'vh_version': C.c_int}
 def lib():
     global _lib
     if _lib is None:
+        # torch ships its own HIP runtime (SONAME libamdhip64.so.7) whose users
+        # reference the unversioned name; load it first so this library binds to
+        # the same runtime instead of pulling a second copy into the process.
+        import torch  # noqa: F401
         if not os.path.exists(LIBPATH):
             raise ImportError('libvoxhenry.so not built (%s); run `python -m voxhenry_amd.build` '
                               '-- the solve path has no CPU fallback' % LIBPATH)

@@ -118,6 +118,33 @@ class Geometry:
         check(lib().vh_geom_set_incidence(self.h, A.shape[0], A.nnz, ptr(cp), ptr(ri), ptr(v)))
         self._refresh()
 
+    def build_topology(self, ports_P, ports_N, well=None):
+        """Panel ids, Ar and port right-hand sides built on the host by the library.
+
+        ports_P / ports_N: per port an int array (k,4) of 1-based (x, y, z, side).
+        Returns rhs (nports x n, real)."""
+        def pack(lst):
+            ptrs = np.zeros(len(lst) + 1, dtype=np.int64)
+            rows = []
+            for i, a in enumerate(lst):
+                a = np.asarray(a, dtype=np.int64).reshape(-1, 4)
+                ptrs[i + 1] = ptrs[i] + len(a)
+                rows.append(a)
+            flat = np.ascontiguousarray(np.concatenate(rows) if rows else np.zeros((0, 4), np.int64), dtype=np.int64)
+            return ptrs, flat
+        pp, P = pack(ports_P)
+        pn, N = pack(ports_N)
+        w = np.ascontiguousarray(np.zeros((0, 4), np.int64) if well is None else np.asarray(well, np.int64).reshape(-1, 4))
+        nn = np.zeros(1, dtype=np.int64)
+        check(lib().vh_geom_build_topology(self.h, len(ports_P), ptr(pp), ptr(P), ptr(pn), ptr(N), len(w),
+                                           ptr(w) if len(w) else None, ptr(nn)))
+        self._refresh()
+        rhs = np.zeros((len(ports_P), self.n))
+        if len(ports_P):
+            check(lib().vh_geom_get_rhs(self.h, ptr(rhs)))
+        self.rhs = rhs
+        return rhs
+
     def set_impedance(self, z):
         """z = z_real + z_imag per non-empty voxel (complex, nv)."""
         z = np.ascontiguousarray(np.asarray(z, dtype=np.complex128))
@@ -160,6 +187,59 @@ class Geometry:
                                     ptr(yy), yy.shape[1], reps, ptr(ms), None, None))
         return float(ms[0])
 
+    def matvec_timed_passes(self, x, y, jwmu, prec, mode, reps):
+        """(ms per call, [ms per pass X, Y, Z, Yi, Xi, node]) from HIP events on the launch stream."""
+        xx = x if x.dim() == 2 else x.unsqueeze(0)
+        yy = y if y.dim() == 2 else y.unsqueeze(0)
+        nrhs, ld = xx.shape
+        ms = np.zeros(1)
+        mp = np.zeros(6)
+        npass = np.zeros(1, np.int32)
+        self.ctx.use_torch_stream()
+        check(lib().vh_matvec_timed(self.h, float(np.real(jwmu)), float(np.imag(jwmu)), prec, mode, nrhs, ptr(xx), ld,
+                                    ptr(yy), yy.shape[1], reps, ptr(ms), ptr(mp), ptr(npass)))
+        return float(ms[0]), [float(v) for v in mp]
+
+    # -- solvers ------------------------------------------------------------
+    def gmres(self, b, jwmu, precon=None, restart=100, tol=1e-8, maxit=10, prec=VH_C128, flexible=False,
+              resvec_cap=0):
+        """pgmres (left precon) / fpgmres (flexible) for one or more rhs sharing the operator.
+
+        b: numpy (n,) or (nrhs, n).  Returns x and a dict of per-rhs statistics."""
+        dt = np.complex128 if prec == VH_C128 else np.complex64
+        bb = np.ascontiguousarray(np.atleast_2d(b), dtype=dt)
+        nrhs, n = bb.shape
+        assert n == self.n
+        x = np.zeros_like(bb)
+        flag = np.zeros(nrhs, np.int32)
+        relres = np.zeros(nrhs)
+        it = np.zeros(2 * nrhs, np.int32)
+        nit = np.zeros(nrhs, np.int32)
+        nres = np.zeros(nrhs, np.int32)
+        cap = int(resvec_cap)
+        rv = np.zeros(max(1, nrhs * cap))
+        check(lib().vh_gmres(self.h, precon.h if precon is not None else None, float(np.real(jwmu)),
+                             float(np.imag(jwmu)), prec, int(flexible), nrhs, ptr(bb), ptr(x), 0, restart, tol, maxit,
+                             ptr(flag), ptr(relres), ptr(it), ptr(nit), ptr(rv) if cap else None, cap, ptr(nres)))
+        stats = dict(flag=flag, relres=relres, iter=it.reshape(nrhs, 2), niters=nit, nresvec=nres)
+        if cap:
+            stats['resvec'] = [rv[i * cap:i * cap + min(cap, nres[i])] for i in range(nrhs)]
+        return x.reshape(np.shape(b)), stats
+
+    def solve_mixed(self, b, jwmu, precon, tol=1e-8, outer_restart=8, outer_maxit=4, inner_restart=40,
+                    inner_tol=1e-5):
+        bb = np.ascontiguousarray(np.atleast_2d(b), dtype=np.complex128)
+        nrhs, n = bb.shape
+        x = np.zeros_like(bb)
+        flag = np.zeros(nrhs, np.int32)
+        relres = np.zeros(nrhs)
+        oit = np.zeros(nrhs, np.int32)
+        iit = np.zeros(nrhs, np.int32)
+        check(lib().vh_solve_mixed(self.h, precon.h, float(np.real(jwmu)), float(np.imag(jwmu)), nrhs, ptr(bb),
+                                   ptr(x), 0, tol, outer_restart, outer_maxit, inner_restart, inner_tol, ptr(flag),
+                                   ptr(relres), ptr(oit), ptr(iit)))
+        return x.reshape(np.shape(b)), dict(flag=flag, relres=relres, outer_iters=oit, inner_iters=iit)
+
     def fft3d(self, data, prec, direction):
         self.ctx.use_torch_stream()
         ncomp = data.numel() // (self.pad[0] * self.pad[1] * self.pad[2])
@@ -168,6 +248,55 @@ class Geometry:
     def close(self):
         if getattr(self, 'h', None):
             check(lib().vh_geom_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Precon:
+    """Schur-complement preconditioner of one geometry (vh_precon_*)."""
+
+    def __init__(self, geom):
+        h = C.c_void_p()
+        check(lib().vh_precon_create(geom.h, C.byref(h)))
+        self.h = h
+        self.geom = geom
+        self.stats = None
+
+    def prepare(self, jwmu):
+        st = np.zeros(4, dtype=np.int64)
+        check(lib().vh_precon_prepare(self.h, float(np.real(jwmu)), float(np.imag(jwmu)), ptr(st)))
+        self.stats = dict(nnzL=int(st[0]), levels_fwd=int(st[1]), levels_bwd=int(st[2]), n_node=int(st[3]))
+        return self.stats
+
+    def ainv(self):
+        out = np.zeros(self.geom.num_curr)
+        check(lib().vh_precon_get_ainv(self.h, ptr(out)))
+        return out
+
+    def apply(self, x, prec=VH_C128):
+        if isinstance(x, np.ndarray):
+            dt = np.complex128 if prec == VH_C128 else np.complex64
+            xx = np.ascontiguousarray(np.atleast_2d(x), dtype=dt)
+            nrhs, ld = xx.shape
+            y = np.zeros_like(xx)
+            check(lib().vh_precon_apply(self.h, prec, nrhs, ptr(xx), ld, ptr(y), ld, 0))
+            return y.reshape(np.shape(x))
+        xx = x if x.dim() == 2 else x.unsqueeze(0)
+        nrhs, ld = xx.shape
+        import torch
+        y = torch.zeros_like(xx)
+        self.geom.ctx.use_torch_stream()
+        check(lib().vh_precon_apply(self.h, prec, nrhs, ptr(xx), ld, ptr(y), ld, 1))
+        return y.reshape(x.shape)
+
+    def close(self):
+        if getattr(self, 'h', None):
+            check(lib().vh_precon_destroy(self.h))
             self.h = None
 
     def __del__(self):

@@ -2,9 +2,10 @@
 // catches C++ exceptions and turns them into a status code plus a thread-local
 // message.
 #include "vh_geom.h"
-#include "vh_solver.h"
+#include "vh_krylov.h"
 #include "../../include/voxhenry.h"
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 
@@ -162,6 +163,34 @@ int vh_geom_set_incidence(vh_geom* g, int64_t num_node, int64_t nnz, const int64
   });
 }
 
+int vh_geom_build_topology(vh_geom* g, int64_t nports, const int64_t* pP_ptr, const int64_t* P, const int64_t* pN_ptr,
+                           const int64_t* N, int64_t nwell, const int64_t* well, int64_t* num_node) {
+  return guard([&] {
+    need(g, "g");
+    VH_REQUIRE(nports >= 0 && nports < 100000, "nports out of range");
+    VH_REQUIRE(nports == 0 || (pP_ptr && pN_ptr), "null port pointers");
+    std::vector<std::vector<int64_t>> Pf(nports), Nf(nports);
+    for (int64_t p = 0; p < nports; ++p) {
+      VH_REQUIRE(pP_ptr[p + 1] >= pP_ptr[p] && pN_ptr[p + 1] >= pN_ptr[p], "port pointer arrays not increasing");
+      Pf[p].assign(P + 4 * pP_ptr[p], P + 4 * pP_ptr[p + 1]);
+      Nf[p].assign(N + 4 * pN_ptr[p], N + 4 * pN_ptr[p + 1]);
+    }
+    std::vector<int64_t> w;
+    if (nwell > 0) { need(well, "well"); w.assign(well, well + 4 * nwell); }
+    vh::geometry_build_topology(g->G, Pf, Nf, w, g->G.rhs_h);
+    g->G.nports = (int)nports;
+    if (num_node) *num_node = g->G.num_node;
+  });
+}
+
+int vh_geom_get_rhs(vh_geom* g, double* rhs) {
+  return guard([&] {
+    need(g, "g"); need(rhs, "rhs");
+    VH_REQUIRE(!g->G.rhs_h.empty() || g->G.nports == 0, "no topology built");
+    std::memcpy(rhs, g->G.rhs_h.data(), g->G.rhs_h.size() * sizeof(double));
+  });
+}
+
 int vh_geom_set_impedance(vh_geom* g, const double* z) {
   return guard([&] {
     need(g, "g"); need(z, "z");
@@ -227,8 +256,253 @@ int vh_matvec_timed(vh_geom* g, double jre, double jim, int prec, int mode, int6
     *ms_total = ms / reps;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (npass) *npass = 0;
-    (void)ms_pass;
+    if (ms_pass) {
+      // second sweep with per-pass events (events between passes add no device work)
+      vh::PassTimer tm;
+      tm.create();
+      G.timer = &tm;
+      try {
+        for (int r = 0; r < reps; ++r) {
+          if (prec == VH_C128)
+            vh::matvec<double>(G, (int)nrhs, (const double2*)x, ldx, (double2*)y, ldy, jre, jim, mode);
+          else
+            vh::matvec<float>(G, (int)nrhs, (const float2*)x, ldx, (float2*)y, ldy, jre, jim, mode);
+        }
+        tm.collect();
+      } catch (...) {
+        G.timer = nullptr;
+        tm.destroy();
+        throw;
+      }
+      G.timer = nullptr;
+      for (int k = 0; k < vh::PassTimer::NP; ++k) ms_pass[k] = tm.ms[k] / std::max(1, tm.calls);
+      tm.destroy();
+      if (npass) *npass = vh::PassTimer::NP;
+    } else if (npass) {
+      *npass = 0;
+    }
+  });
+}
+
+int vh_precon_create(vh_geom* g, vh_precon** out) {
+  return guard([&] {
+    need(g, "g"); need(out, "out");
+    auto p = std::make_unique<vh_precon>();
+    p->P.G = &g->G;
+    *out = p.release();
+  });
+}
+
+int vh_precon_destroy(vh_precon* p) {
+  return guard([&] { delete p; });
+}
+
+int vh_precon_prepare(vh_precon* p, double jre, double jim, int64_t* stats) {
+  return guard([&] {
+    need(p, "p");
+    vh::precon_prepare(p->P, jre, jim);
+    if (stats) {
+      stats[0] = p->P.nnzL;
+      stats[1] = (int64_t)p->P.fl_ptr.size() - 1;
+      stats[2] = (int64_t)p->P.bl_ptr.size() - 1;
+      stats[3] = p->P.G->num_node;
+    }
+  });
+}
+
+int vh_precon_get_ainv(vh_precon* p, double* ainv) {
+  return guard([&] {
+    need(p, "p"); need(ainv, "ainv");
+    VH_REQUIRE(p->P.ready, "preconditioner not prepared");
+    std::memcpy(ainv, p->P.ainv_h.data(), p->P.ainv_h.size() * sizeof(double));
+  });
+}
+
+namespace {
+// Host or device staging of nrhs vectors of n complex values of es bytes.
+struct Io {
+  vh::DevBuf<char> bx, by;
+  const void* x = nullptr;
+  void* y = nullptr;
+  void in(const void* hx, int64_t nrhs, int64_t ldx, size_t es, int on_device, hipStream_t st, int64_t ldy) {
+    if (on_device) { x = hx; return; }
+    bx.alloc((size_t)nrhs * ldx * es);
+    VH_HIP(hipMemcpyAsync(bx.p, hx, bx.bytes(), hipMemcpyHostToDevice, st));
+    by.alloc((size_t)nrhs * ldy * es);
+    VH_HIP(hipMemsetAsync(by.p, 0, by.bytes(), st));
+    x = bx.p;
+    y = by.p;
+  }
+};
+}  // namespace
+
+int vh_precon_apply(vh_precon* p, int prec, int64_t nrhs, const void* x, int64_t ldx, void* y, int64_t ldy,
+                    int on_device) {
+  return guard([&] {
+    need(p, "p"); need(x, "x"); need(y, "y");
+    auto& G = *p->P.G;
+    VH_REQUIRE(prec == VH_C128 || prec == VH_C64, "prec must be VH_C128 or VH_C64");
+    VH_REQUIRE(ldx >= G.n() && ldy >= G.n(), "leading dimension smaller than the vector length");
+    VH_REQUIRE(nrhs >= 1 && nrhs < (1 << 16), "nrhs out of range");
+    const size_t es = prec == VH_C128 ? 16 : 8;
+    hipStream_t st = G.ctx->stream;
+    Io io;
+    io.in(x, nrhs, ldx, es, on_device, st, ldy);
+    void* dy = on_device ? y : io.y;
+    if (prec == VH_C128)
+      vh::precon_apply<double>(p->P, (int)nrhs, (const double2*)io.x, ldx, (double2*)dy, ldy);
+    else
+      vh::precon_apply<float>(p->P, (int)nrhs, (const float2*)io.x, ldx, (float2*)dy, ldy);
+    if (!on_device) {
+      VH_HIP(hipMemcpyAsync(y, io.y, (size_t)nrhs * ldy * es, hipMemcpyDeviceToHost, st));
+    }
+    VH_HIP(hipStreamSynchronize(st));
+  });
+}
+
+
+int vh_gmres(vh_geom* g, vh_precon* p, double jre, double jim, int prec, int flexible, int64_t nrhs, const void* b,
+             void* x, int on_device, int restart, double tol, int maxit, int* flag, double* relres, int* iter,
+             int* niters, double* resvec, int64_t resvec_cap, int* nresvec) {
+  return guard([&] {
+    need(g, "g"); need(b, "b"); need(x, "x");
+    auto& G = g->G;
+    VH_REQUIRE(G.has_tensor && G.has_inc && G.has_z, "geometry not complete (tensor, incidence, impedance)");
+    VH_REQUIRE(!p || p->P.ready, "preconditioner not prepared");
+    VH_REQUIRE(prec == VH_C128 || prec == VH_C64, "prec must be VH_C128 or VH_C64");
+    VH_REQUIRE(nrhs >= 1 && nrhs <= 13107, "nrhs out of range");
+    VH_REQUIRE(tol > 0 && restart >= 1 && restart <= 127 && maxit >= 1, "bad solver parameters");
+    const int64_t n = G.n();
+    const size_t es = prec == VH_C128 ? 16 : 8;
+    hipStream_t st = G.ctx->stream;
+    Io io;
+    io.in(b, nrhs, n, es, on_device, st, n);
+    void* dx = on_device ? x : io.y;
+    std::vector<int> sys(nrhs);
+    for (int i = 0; i < nrhs; ++i) sys[i] = i;
+    vh::SolveOpts o;
+    o.restart = restart; o.maxit = maxit; o.tol = tol; o.flexible = flexible != 0; o.final_residual = true;
+    std::vector<vh::SysStats> stats;
+    if (prec == VH_C128) {
+      vh::KrylovWork<double> W;
+      vh::OpFn<double> A = [&](const std::vector<int>& act, const int* sl, const double2* xx, double2* yy) {
+        vh::matvec<double>(G, (int)act.size(), xx, n, yy, n, jre, jim, 0, sl);
+      };
+      vh::OpFn<double> M = [&](const std::vector<int>& act, const int* sl, const double2* xx, double2* yy) {
+        vh::precon_apply<double>(p->P, (int)act.size(), xx, n, yy, n, sl);
+      };
+      vh::gmres_batch<double>(G.ctx, n, A, p ? &M : nullptr, (const double2*)io.x, (double2*)dx, sys, o, W, stats);
+    } else {
+      vh::KrylovWork<float> W;
+      vh::OpFn<float> A = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
+        vh::matvec<float>(G, (int)act.size(), xx, n, yy, n, jre, jim, 0, sl);
+      };
+      vh::OpFn<float> M = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
+        vh::precon_apply<float>(p->P, (int)act.size(), xx, n, yy, n, sl);
+      };
+      vh::gmres_batch<float>(G.ctx, n, A, p ? &M : nullptr, (const float2*)io.x, (float2*)dx, sys, o, W, stats);
+    }
+    if (!on_device) VH_HIP(hipMemcpyAsync(x, io.y, (size_t)nrhs * n * es, hipMemcpyDeviceToHost, st));
+    VH_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nrhs; ++i) {
+      const auto& s = stats[i];
+      if (flag) flag[i] = s.flag;
+      if (relres) relres[i] = s.relres;
+      if (iter) { iter[2 * i] = s.inner; iter[2 * i + 1] = s.outer; }
+      if (niters) niters[i] = s.iters;
+      if (nresvec) nresvec[i] = (int)s.resvec.size();
+      if (resvec)
+        for (int64_t k = 0; k < resvec_cap && k < (int64_t)s.resvec.size(); ++k) resvec[i * resvec_cap + k] = s.resvec[k];
+    }
+  });
+}
+
+int vh_solve_mixed(vh_geom* g, vh_precon* p, double jre, double jim, int64_t nrhs, const double* b, double* x,
+                   int on_device, double tol, int outer_restart, int outer_maxit, int inner_restart, double inner_tol,
+                   int* flag, double* relres, int* outer_iters, int* inner_iters) {
+  return guard([&] {
+    need(g, "g"); need(p, "p"); need(b, "b"); need(x, "x");
+    auto& G = g->G;
+    VH_REQUIRE(G.has_tensor && G.has_inc && G.has_z, "geometry not complete (tensor, incidence, impedance)");
+    VH_REQUIRE(p->P.ready, "preconditioner not prepared");
+    VH_REQUIRE(nrhs >= 1 && nrhs <= 13107, "nrhs out of range");
+    VH_REQUIRE(outer_restart >= 1 && outer_restart <= 127 && inner_restart >= 1 && inner_restart <= 127, "bad restart");
+    const int64_t n = G.n();
+    hipStream_t st = G.ctx->stream;
+    Io io;
+    io.in(b, nrhs, n, 16, on_device, st, n);
+    void* dx = on_device ? (void*)x : io.y;
+    std::vector<int> sys(nrhs);
+    for (int i = 0; i < nrhs; ++i) sys[i] = i;
+    vh::MixedOpts o;
+    o.tol = tol; o.outer_restart = outer_restart; o.outer_maxit = outer_maxit;
+    o.inner_restart = inner_restart; o.inner_tol = inner_tol;
+    vh::OpFn<double> A64 = [&](const std::vector<int>& act, const int* sl, const double2* xx, double2* yy) {
+      vh::matvec<double>(G, (int)act.size(), xx, n, yy, n, jre, jim, 0, sl);
+    };
+    vh::OpFn<float> A32 = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
+      vh::matvec<float>(G, (int)act.size(), xx, n, yy, n, jre, jim, 0, sl);
+    };
+    vh::OpFn<float> M32 = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
+      vh::precon_apply<float>(p->P, (int)act.size(), xx, n, yy, n, sl);
+    };
+    vh::KrylovWork<double> W64;
+    vh::KrylovWork<float> W32;
+    std::vector<vh::SysStats> stats;
+    vh::fgmres_mixed(G.ctx, n, A64, A32, M32, (const double2*)io.x, (double2*)dx, sys, o, W64, W32, stats);
+    if (!on_device) VH_HIP(hipMemcpyAsync(x, io.y, (size_t)nrhs * n * 16, hipMemcpyDeviceToHost, st));
+    VH_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < nrhs; ++i) {
+      if (flag) flag[i] = stats[i].flag;
+      if (relres) relres[i] = stats[i].relres;
+      if (outer_iters) outer_iters[i] = stats[i].iters;
+      if (inner_iters) inner_iters[i] = stats[i].inner_iters;
+    }
+  });
+}
+
+int vh_ldl_solve_host(int64_t n, const int64_t* Ap, const int64_t* Ai, const double* Ax, int64_t nrhs,
+                      const double* b, double* x, const float* xyz, const int* type, int64_t* stats) {
+  return guard([&] {
+    need(Ap, "Ap"); need(Ai, "Ai"); need(Ax, "Ax");
+    VH_REQUIRE(n >= 0 && n < (1LL << 31) && Ap[n] < (1LL << 31), "matrix too large for 32-bit indices");
+    std::vector<int> p(n + 1), i(Ap[n]);
+    for (int64_t k = 0; k <= n; ++k) p[k] = (int)Ap[k];
+    for (int64_t k = 0; k < Ap[n]; ++k) {
+      VH_REQUIRE(Ai[k] >= 0 && Ai[k] < n, "row index out of range");
+      i[k] = (int)Ai[k];
+    }
+    vh::LdlFactor F;
+    std::vector<int> perm = (xyz && type) ? vh::nd_ordering_geometric((int)n, p.data(), i.data(), xyz, type)
+                                          : vh::nd_ordering((int)n, p.data(), i.data());
+    vh::ldl_symbolic((int)n, p.data(), i.data(), perm, F);
+    try {
+      vh::ldl_numeric((int)n, p.data(), i.data(), Ax, F);
+    } catch (const std::runtime_error& e) {
+      throw vh::Error(vh::ERR_NUMERIC, e.what());
+    }
+    std::vector<int> Rp, Ri, lf, lb;
+    std::vector<double> Rx;
+    vh::ldl_levels(F, Rp, Ri, Rx, lf, lb);
+    if (stats) {
+      stats[0] = F.Lp[n];
+      stats[1] = n ? *std::max_element(lf.begin(), lf.end()) + 1 : 0;
+      stats[2] = n ? *std::max_element(lb.begin(), lb.end()) + 1 : 0;
+    }
+    if (b && x) {
+      std::vector<double> w(n);
+      for (int64_t r = 0; r < nrhs; ++r) {
+        const double* br = b + r * n;
+        for (int64_t k = 0; k < n; ++k) w[k] = br[F.perm[k]];
+        for (int64_t j = 0; j < n; ++j)               // L w = b
+          for (int q = F.Lp[j]; q < F.Lp[j + 1]; ++q) w[F.Li[q]] -= F.Lx[q] * w[j];
+        for (int64_t k = 0; k < n; ++k) w[k] /= F.D[k];
+        for (int64_t j = n - 1; j >= 0; --j)          // L' x = w
+          for (int q = F.Lp[j]; q < F.Lp[j + 1]; ++q) w[j] -= F.Lx[q] * w[F.Li[q]];
+        double* xr = x + r * n;
+        for (int64_t k = 0; k < n; ++k) xr[F.perm[k]] = w[k];
+      }
+    }
   });
 }
 

@@ -22,6 +22,19 @@ struct DevCsr {
   DevBuf<float> v32;
 };
 
+// Per-pass timing: when Geometry::timer is set, matvec records an event before
+// and after every pass (X, Y, Z, Yi, Xi, node rows) and accumulates the times.
+struct PassTimer {
+  static constexpr int NP = 6;
+  hipEvent_t ev[NP + 1] = {};
+  double ms[NP] = {0, 0, 0, 0, 0, 0};
+  int calls = 0;
+  bool pending = false;
+  void create();
+  void destroy();
+  void collect();   // waits for the last recorded call and adds its pass times
+};
+
 struct Geometry {
   Context* ctx = nullptr;
   GridDims g{};
@@ -45,9 +58,18 @@ struct Geometry {
   bool has_z = false;
   DevCsr Ar;    // num_node x num_curr
   DevCsr ArT;   // num_curr x num_node
+  // host copies used by the preconditioner set-up
+  std::vector<int> arp_h, arc_h, atp_h, atc_h;
+  std::vector<double> arv_h, atv_h;
+  std::vector<std::complex<double>> zh;
   bool has_inc = false;
-  // matvec workspaces, grown on demand
-  DevBuf<char> wsA, wsB;
+  int64_t num_nodes_full = 0;         // nodes before the port rows were removed
+  std::vector<double> rhs_h;          // port right-hand sides (nports x n), from build_topology
+  int nports = 0;
+  PassTimer* timer = nullptr;
+  // matvec workspaces, grown on demand.  A (pass X output) relies on its
+  // never-written empty lines reading as zero, so each precision has its own.
+  DevBuf<char> wsA64, wsA32, wsB;
 
   size_t octant() const { return (size_t)g.Hx * g.Hy * g.Hz; }
   int64_t n() const { return num_curr + num_node; }
@@ -57,9 +79,10 @@ struct Geometry {
 // in complex elements).  alpha = j*omega*mu multiplies the Green's product.
 // mode 0: full operator [Z, -Ar'; Ar, 0]; mode 1: the FFT block with the
 // resistive diagonal only (current rows); mode 2: convolution only.
+// slots (device, optional): vector k of the batch is x + slots[k]*ldx.
 template <typename R>
 void matvec(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy,
-            double alpha_re, double alpha_im, int mode = 0);
+            double alpha_re, double alpha_im, int mode = 0, const int* slots = nullptr);
 
 // Build the spectral tensor on the device from a real-space normalised tensor
 // G (L*M*N*7 complex, device, column-major [x][y][z][c]).
@@ -69,8 +92,8 @@ void generate_realspace_green(Geometry& G, DevBuf<double2>& out);
 // In-place 3-D FFT of ncomp padded grids (test hook); prec 0 = double, 1 = float.
 void fft3d_test(Geometry& Gm, void* data, int ncomp, int prec, int dir);
 
-// Per-pass timing hooks (filled by matvec when enabled).
-struct PassTimes { float ms[8]; int n; };
+
+
 
 // geometry setup helpers
 void geometry_init(Geometry& G, Context* ctx, int L, int M, int N, double dx, int64_t nv,
@@ -78,6 +101,12 @@ void geometry_init(Geometry& G, Context* ctx, int L, int M, int N, double dx, in
 void geometry_set_incidence_csc(Geometry& G, int64_t num_node, int64_t nnz, const int64_t* colptr,
                                 const int64_t* rowind, const double* val);
 void geometry_set_impedance(Geometry& G, const double* z_interleaved);
+// Panel ids, Ae, port rows removed (Ar, set as the incidence) and the port
+// right-hand sides (rows of length num_curr + num_node).  Face lists hold
+// (x, y, z, side) 1-based quadruples, side 1..6 = -x,+x,-y,+y,-z,+z.
+void geometry_build_topology(Geometry& G, const std::vector<std::vector<int64_t>>& Pfaces,
+                             const std::vector<std::vector<int64_t>>& Nfaces, const std::vector<int64_t>& well,
+                             std::vector<double>& rhs_out);
 void geometry_set_spectrum_full(Geometry& G, const double* fN, int64_t P0, int64_t P1, int64_t P2,
                                 double scale_re, double scale_im);
 

@@ -1,0 +1,94 @@
+// Krylov building blocks (fused Arnoldi reductions) and the batched solvers.
+#pragma once
+#include "vh_solver.h"
+#include <complex>
+#include <functional>
+
+namespace vh {
+
+// Device vector operations on a batch of slot vectors of length n (stride ld).
+template <typename R> struct VecOps {
+  Context* ctx;
+  int64_t n, ld;
+  int kmax;
+  int nchunk;
+  DevBuf<double2> part;
+  VecOps(Context* c, int64_t n, int64_t ld, int kmax, int nslot);
+  void mdot(const cplx<R>* V, int64_t vstride, const cplx<R>* w, int k, int nact, const int* slots, double2* h);
+  void combine(const cplx<R>* V, int64_t vstride, cplx<R>* out, int k, int nact, const int* slots,
+               const double2* coef, double beta, double sign, double2* nrm2);
+  void norm2(const cplx<R>* x, int nact, const int* slots, double2* nrm2);
+  void scale(const cplx<R>* src, cplx<R>* dst, int nact, const int* slots, const double2* nrm2);
+  void sub_from(const cplx<R>* b, int64_t ldb, cplx<R>* y, int nact, const int* slots);
+};
+
+template <typename RD, typename RS>
+void convert_vectors(Context* ctx, const cplx<RS>* src, int64_t lds, cplx<RD>* dst, int64_t ldd, int64_t n, int nact,
+                     const int* slots_src, const int* slots_dst);
+
+// Incremental Givens QR of the (k+1) x k Hessenberg matrix of one system.
+struct Givens {
+  int m = 0, k = 0;
+  double beta0 = 0;
+  std::vector<std::complex<double>> R, H, g, s;
+  std::vector<double> c;
+  void reset(int m, double beta);
+  double push(const std::complex<double>* h, double hnext);
+  void solve(std::vector<std::complex<double>>& y) const;
+  void residual(const std::vector<std::complex<double>>& y, std::vector<std::complex<double>>& res) const;
+};
+
+// y_slot = Op(x_slot) for the systems listed in `act` (host) / `slots` (the
+// same list on the device).
+template <typename R>
+using OpFn = std::function<void(const std::vector<int>& act, const int* slots, const cplx<R>* x, cplx<R>* y)>;
+
+struct SysStats {
+  int flag = 1;               // 0 converged (true residual <= tol), 1 not
+  double relres = 0;          // ||b - A x|| / ||b||
+  int inner = 0, outer = 0;   // iter = [p outit] of pgmres.m:178
+  int iters = 0;              // total Arnoldi steps
+  int inner_iters = 0;        // mixed precision: total inner (FP32) Arnoldi steps
+  int matvecs = 0;            // operator applications (incl. the final residual)
+  std::vector<double> resvec; // resvec of pgmres.m (relative)
+};
+
+struct SolveOpts {
+  int restart = 100, maxit = 10;
+  double tol = 1e-8;
+  bool final_residual = true;     // compute relres with one more matvec
+  bool flexible = false;          // fpgmres semantics (right preconditioner)
+};
+
+// Krylov workspace for nslot systems of length ld with restart m.
+template <typename R> struct KrylovWork {
+  int64_t n = 0, ld = 0;
+  int m = 0, nslot = 0;
+  DevBuf<cplx<R>> V, Z, w, t, r;
+  DevBuf<double2> h, nrm, coef;
+  DevBuf<int> slots_dev, aux_slots;
+  void ensure(int64_t n, int64_t ld, int m, int nslot, bool flexible);
+  static size_t bytes_needed(int64_t ld, int m, int nslot, bool flexible) {
+    return sizeof(cplx<R>) * (size_t)ld * nslot * ((m + 1) + (flexible ? m : 0) + 3);
+  }
+};
+
+// Batched (f)pgmres with the reference's semantics for the systems listed in
+// sys (slot ids); b and x are slot arrays of stride ld (device).
+template <typename R>
+void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, const cplx<R>* b, cplx<R>* x,
+                 const std::vector<int>& sys, const SolveOpts& o, KrylovWork<R>& W, std::vector<SysStats>& st);
+
+// Mixed precision: FP64 flexible GMRES (outer, true residual) preconditioned by
+// an FP32 left-preconditioned GMRES (inner).
+struct MixedOpts {
+  double tol = 1e-8;
+  int outer_restart = 8, outer_maxit = 4;
+  int inner_restart = 40, inner_maxit = 1;
+  double inner_tol = 1e-5;
+};
+void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<float>& A32, const OpFn<float>& M32,
+                  const double2* b, double2* x, const std::vector<int>& sys, const MixedOpts& o,
+                  KrylovWork<double>& W64, KrylovWork<float>& W32, std::vector<SysStats>& st);
+
+}  // namespace vh

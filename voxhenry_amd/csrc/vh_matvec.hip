@@ -33,15 +33,17 @@ template <typename R> __device__ __forceinline__ cplx<R> czero() { cplx<R> c; c.
 template <typename R, int EMAX>
 __global__ __launch_bounds__(FFT_THREADS) void k_pass_x_fwd(
     const cplx<R>* __restrict__ x, int64_t ldx, const int* __restrict__ vmap,
-    const int* __restrict__ lines, int nlines, int T, cplx<R>* __restrict__ A, GridDims g, FftPlan pl) {
+    const int* __restrict__ lines, int nlines, int T, cplx<R>* __restrict__ A, GridDims g, FftPlan pl,
+    const int* __restrict__ slots) {
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* buf = (cplx<R>*)smem;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int rc = blockIdx.y, rhs = rc / 5, comp = rc - rhs * 5;
+  const int slot = slots ? slots[rhs] : rhs;
   const int l0 = tile * T;
   const int Px = g.Px, L = g.L;
-  const cplx<R>* xs = x + (int64_t)rhs * ldx + (int64_t)comp * g.nv;
+  const cplx<R>* xs = x + (int64_t)slot * ldx + (int64_t)comp * g.nv;
   for (int e = tid; e < T * Px; e += nthr) {
     const int p = e / Px, i = e - p * Px;
     cplx<R> v = czero<R>();
@@ -234,13 +236,14 @@ __global__ __launch_bounds__(FFT_THREADS) void k_pass_x_inv(
     const cplx<R>* __restrict__ A, const cplx<R>* __restrict__ x, int64_t ldx, cplx<R>* __restrict__ y, int64_t ldy,
     const int* __restrict__ vmap, const int* __restrict__ lines, int nlines, int T, GridDims g, FftPlan pl,
     const cplx<R>* __restrict__ z, R inv_dx, const int* __restrict__ atp, const int* __restrict__ atc,
-    const R* __restrict__ atv, int64_t num_curr, int mode) {
+    const R* __restrict__ atv, int64_t num_curr, int mode, const int* __restrict__ slots) {
   // mode 0: conv + diag - Ar'phi ; mode 1: conv + diag ; mode 2: conv only
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* buf = (cplx<R>*)smem;
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int rc = blockIdx.y, rhs = rc / 5, comp = rc - rhs * 5;
+  const int slot = slots ? slots[rhs] : rhs;
   const int l0 = tile * T;
   const int Px = g.Px, L = g.L;
   const cplx<R>* As = A + (int64_t)rc * Px * g.Py * g.N;
@@ -257,8 +260,8 @@ __global__ __launch_bounds__(FFT_THREADS) void k_pass_x_inv(
   __syncthreads();
   lds_fft<R, +1, EMAX>(buf, T, Px, 1, pl, tid, nthr, false);
   const R w = inv_dx * (comp < 3 ? R(1) : comp == 3 ? R(1) / R(6) : R(1) / R(2));
-  const cplx<R>* xs = x + (int64_t)rhs * ldx;
-  cplx<R>* ys = y + (int64_t)rhs * ldy;
+  const cplx<R>* xs = x + (int64_t)slot * ldx;
+  cplx<R>* ys = y + (int64_t)slot * ldy;
   for (int e = tid; e < T * L; e += nthr) {
     const int p = e / L, i = e - p * L;
     if (l0 + p >= nlines) continue;
@@ -287,9 +290,10 @@ __global__ __launch_bounds__(FFT_THREADS) void k_pass_x_inv(
 template <typename R>
 __global__ __launch_bounds__(256) void k_node_rows(
     const cplx<R>* __restrict__ x, int64_t ldx, cplx<R>* __restrict__ y, int64_t ldy, const int* __restrict__ arp,
-    const int* __restrict__ arc, const R* __restrict__ arv, int num_node, int64_t num_curr) {
+    const int* __restrict__ arc, const R* __restrict__ arv, int num_node, int64_t num_curr,
+    const int* __restrict__ slots) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int rhs = blockIdx.y;
+  const int rhs = slots ? slots[blockIdx.y] : blockIdx.y;
   if (i >= num_node) return;
   const cplx<R>* xs = x + (int64_t)rhs * ldx;
   cplx<R> s = czero<R>();
@@ -324,19 +328,20 @@ static void ensure_ws(Geometry& G, int nrhs) {
   // must read as zero, so it is cleared once when (re)allocated.
   const size_t a = (size_t)nrhs * 5 * g.Px * g.M * g.N * sizeof(cplx<R>);
   const size_t b = (size_t)nrhs * 5 * g.Px * g.Py * g.N * sizeof(cplx<R>);
-  if (G.wsA.n < a) { G.wsA.alloc(a); VH_HIP(hipMemsetAsync(G.wsA.p, 0, a, G.ctx->stream)); }
+  DevBuf<char>& wsA = sizeof(R) == 8 ? G.wsA64 : G.wsA32;
+  if (wsA.n < a) { wsA.alloc(a); VH_HIP(hipMemsetAsync(wsA.p, 0, a, G.ctx->stream)); }
   if (G.wsB.n < b) G.wsB.alloc(b);
 }
 
 template <typename R>
 static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy,
-                       double are, double aim, int mode) {
+                       double are, double aim, int mode, const int* slots) {
   VH_REQUIRE(G.has_tensor, "matvec: Green's tensor not set");
   VH_REQUIRE(nrhs >= 1, "matvec: nrhs must be >= 1");
   VH_REQUIRE(nrhs * 5 <= 65535, "matvec: too many right-hand sides in one call");
   const GridDims& g = G.g;
   ensure_ws<R>(G, nrhs);
-  cplx<R>* A = (cplx<R>*)G.wsA.p;
+  cplx<R>* A = (cplx<R>*)(sizeof(R) == 8 ? G.wsA64.p : G.wsA32.p);
   cplx<R>* B = (cplx<R>*)G.wsB.p;
   hipStream_t st = G.ctx->stream;
   constexpr int EM = emax_of<R>();
@@ -344,15 +349,19 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
   const size_t budget = 64 * 1024;
   const int want = sizeof(R) == 8 ? 8 : 16;
   FftPlan px = G.ax.plan<R>(), py = G.ay.plan<R>(), pz = G.az.plan<R>();
+  PassTimer* tm = G.timer;
+  if (tm) { tm->collect(); VH_HIP(hipEventRecord(tm->ev[0], st)); }
+  auto mark = [&](int k) { if (tm) VH_HIP(hipEventRecord(tm->ev[k], st)); };
 
   // X
   if (G.nlines > 0) {
     const int T = pick_T(g.Px, 1, elem, EM, budget, 4);
     dim3 grid(ceil_div(G.nlines, T), nrhs * 5);
     hipLaunchKernelGGL((k_pass_x_fwd<R, EM>), grid, dim3(FFT_THREADS), T * g.Px * elem, st, x, ldx, G.vmap.p,
-                       G.lines.p, G.nlines, T, A, g, px);
+                       G.lines.p, G.nlines, T, A, g, px, slots);
     VH_LAUNCHED();
   }
+  mark(1);
   // Y
   {
     const int T = pick_T(g.Py, 1, elem, EM, budget, want);
@@ -361,6 +370,7 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
                        g.Py, T, g, py);
     VH_LAUNCHED();
   }
+  mark(2);
   // Z
   {
     const int T = pick_T(g.Pz, 5, elem, EM, budget, want);
@@ -375,6 +385,7 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
                        alpha);
     VH_LAUNCHED();
   }
+  mark(3);
   // Yi
   {
     const int T = pick_T(g.Py, 1, elem, EM, budget, want);
@@ -383,6 +394,7 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
                        g.M, T, g, py);
     VH_LAUNCHED();
   }
+  mark(4);
   // Xi
   if (G.nlines > 0) {
     const int T = pick_T(g.Px, 1, elem, EM, budget, 4);
@@ -395,26 +407,49 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
     if (mode == 0) VH_REQUIRE(G.has_inc, "matvec: incidence not set");
     hipLaunchKernelGGL((k_pass_x_inv<R, EM>), grid, dim3(FFT_THREADS), T * g.Px * elem, st, B, x, ldx, y, ldy,
                        G.vmap.p, G.lines.p, G.nlines, T, g, px, zz, (R)(1.0 / G.dx), G.ArT.ptr.p, G.ArT.col.p, atv,
-                       G.num_curr, mode);
+                       G.num_curr, mode, slots);
     VH_LAUNCHED();
   }
+  mark(5);
   if (mode == 0 && G.num_node > 0) {
     const R* arv;
     if constexpr (sizeof(R) == 8) arv = G.Ar.v64.p; else arv = G.Ar.v32.p;
     dim3 grid(ceil_div(G.num_node, 256), nrhs);
     hipLaunchKernelGGL((k_node_rows<R>), grid, dim3(256), 0, st, x, ldx, y, ldy, G.Ar.ptr.p, G.Ar.col.p, arv,
-                       (int)G.num_node, G.num_curr);
+                       (int)G.num_node, G.num_curr, slots);
     VH_LAUNCHED();
   }
+  mark(6);
+  if (tm) tm->pending = true;
+}
+
+void PassTimer::create() {
+  for (auto& e : ev) VH_HIP(hipEventCreate(&e));
+}
+void PassTimer::destroy() {
+  for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+}
+void PassTimer::collect() {
+  if (!pending) return;
+  VH_HIP(hipEventSynchronize(ev[NP]));
+  for (int k = 0; k < NP; ++k) {
+    float t = 0;
+    VH_HIP(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+    ms[k] += t;
+  }
+  calls++;
+  pending = false;
 }
 
 template <typename R>
 void matvec(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy, double are, double aim,
-            int mode) {
-  run_passes<R>(G, nrhs, x, ldx, y, ldy, are, aim, mode);
+            int mode, const int* slots) {
+  run_passes<R>(G, nrhs, x, ldx, y, ldy, are, aim, mode, slots);
 }
 
-template void matvec<double>(Geometry&, int, const double2*, int64_t, double2*, int64_t, double, double, int);
-template void matvec<float>(Geometry&, int, const float2*, int64_t, float2*, int64_t, double, double, int);
+template void matvec<double>(Geometry&, int, const double2*, int64_t, double2*, int64_t, double, double, int,
+                             const int*);
+template void matvec<float>(Geometry&, int, const float2*, int64_t, float2*, int64_t, double, double, int,
+                            const int*);
 
 }  // namespace vh

@@ -142,13 +142,19 @@ void geometry_set_incidence_csc(Geometry& G, int64_t num_node, int64_t nnz, cons
   G.num_node = num_node;
   upload_csr(G.Ar, (int)num_node, (int)nc, rp, rc, rv);
   upload_csr(G.ArT, (int)nc, (int)num_node, tp, tc, tv);
+  G.arp_h = std::move(rp); G.arc_h = std::move(rc); G.arv_h = std::move(rv);
+  G.atp_h = std::move(tp); G.atc_h = std::move(tc); G.atv_h = std::move(tv);
   G.has_inc = true;
 }
 
 void geometry_set_impedance(Geometry& G, const double* z) {
   const int nv = G.g.nv;
   std::vector<float2> z32(nv);
-  for (int k = 0; k < nv; ++k) { z32[k].x = (float)z[2 * k]; z32[k].y = (float)z[2 * k + 1]; }
+  G.zh.resize(nv);
+  for (int k = 0; k < nv; ++k) {
+    z32[k].x = (float)z[2 * k]; z32[k].y = (float)z[2 * k + 1];
+    G.zh[k] = std::complex<double>(z[2 * k], z[2 * k + 1]);
+  }
   if (G.z64.n != (size_t)nv) { G.z64.alloc(nv); G.z32.alloc(nv); }
   VH_HIP(hipMemcpy(G.z64.p, z, nv * sizeof(double2), hipMemcpyHostToDevice));
   VH_HIP(hipMemcpy(G.z32.p, z32.data(), nv * sizeof(float2), hipMemcpyHostToDevice));

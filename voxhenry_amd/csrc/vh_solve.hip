@@ -1,0 +1,252 @@
+// Batched restarted GMRES drivers (host orchestration of the device kernels).
+//
+// gmres_batch follows src_iterative_solver/pgmres.m:70-197 / pgmres_iter.m:22-153
+// (left preconditioner) or fpgmres.m:58-152 / fpgmres_iter.m:26-121 (flexible
+// right preconditioner) for every system of the batch:
+//   r = M b (left) | b (flexible); bnorm = ||r|| (left) | ||b||; resvec(1) = ||r||/bnorm
+//   while resvec(end) > tol and outit < maxit:
+//     V(:,1) = r/||r||; for k = 1..m: w = M A V(:,k) | A M V(:,k) ; CGS ; LS ;
+//       stop when ||res|| < tol*bnorm
+//     x += V y | Z y ;  r = V(:,1:k+1) res
+//   relres = ||b - A x|| / ||b||;  flag = relres > tol
+// All systems advance together; one that meets the stopping rule gets its
+// update immediately and leaves the active set, so later steps only work on
+// the systems still running.  One host synchronisation per Arnoldi step brings
+// the new Hessenberg column of every active system to the host.
+#include "vh_krylov.h"
+#include <algorithm>
+#include <cmath>
+
+namespace vh {
+
+using cd = std::complex<double>;
+
+template <typename R>
+void KrylovWork<R>::ensure(int64_t n_, int64_t ld_, int m_, int nslot_, bool flexible) {
+  const bool same = (n_ == n && ld_ == ld && m_ == m && nslot_ <= nslot && (!flexible || Z.p));
+  if (same) return;
+  n = n_; ld = ld_; m = m_; nslot = std::max(nslot_, 1);
+  V.alloc((size_t)(m + 1) * nslot * ld);
+  if (flexible) Z.alloc((size_t)m * nslot * ld);
+  w.alloc((size_t)nslot * ld);
+  t.alloc((size_t)nslot * ld);
+  r.alloc((size_t)nslot * ld);
+  h.alloc((size_t)nslot * (m + 1));
+  coef.alloc((size_t)nslot * (m + 1));
+  nrm.alloc(nslot);
+  slots_dev.alloc(nslot);
+  aux_slots.alloc(nslot);
+}
+
+template struct KrylovWork<double>;
+template struct KrylovWork<float>;
+
+template <typename R>
+void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, const cplx<R>* b, cplx<R>* x,
+                 const std::vector<int>& sys, const SolveOpts& o, KrylovWork<R>& W, std::vector<SysStats>& st) {
+  using C = cplx<R>;
+  hipStream_t strm = ctx->stream;
+  const int nsys = (int)sys.size();
+  if (nsys == 0) return;
+  const int m = o.restart;
+  VH_REQUIRE(m >= 1 && m + 1 <= 128, "restart must be in [1, 127]");
+  VH_REQUIRE(o.maxit >= 1, "maxit must be >= 1");
+  int nslot = 0;
+  for (int s : sys) { VH_REQUIRE(s >= 0, "negative slot"); nslot = std::max(nslot, s + 1); }
+  W.ensure(n, n, m, nslot, o.flexible);
+  VecOps<R> ops(ctx, n, n, m + 1, W.nslot);
+  const int64_t vstr = (int64_t)W.nslot * n;
+  if ((int)st.size() < nslot) st.resize(nslot);
+  const bool left = (M != nullptr) && !o.flexible;
+
+  std::vector<double2> hbuf((size_t)W.nslot * (m + 1));
+  std::vector<double2> nbuf(W.nslot);
+  auto upload = [&](const std::vector<int>& v, int* dst) {
+    VH_HIP(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice, strm));
+  };
+  auto fetch_norms = [&](int cnt) {
+    VH_HIP(hipMemcpyAsync(nbuf.data(), W.nrm.p, cnt * sizeof(double2), hipMemcpyDeviceToHost, strm));
+    VH_HIP(hipStreamSynchronize(strm));
+  };
+
+  std::vector<int> act = sys;
+  upload(act, W.slots_dev.p);
+  for (int s : sys) {
+    VH_HIP(hipMemsetAsync(x + (int64_t)s * n, 0, n * sizeof(C), strm));
+    st[s] = SysStats();
+  }
+  // r = M b (pgmres.m:120-139) or b (fpgmres.m:91-95)
+  if (left) {
+    (*M)(act, W.slots_dev.p, b, W.r.p);
+  } else {
+    convert_vectors<R, R>(ctx, b, n, W.r.p, n, n, nsys, W.slots_dev.p, W.slots_dev.p);
+  }
+  std::vector<double> bnorm(nslot, 0.0), bnorm_true(nslot, 0.0);
+  ops.norm2(W.r.p, nsys, W.slots_dev.p, W.nrm.p);
+  fetch_norms(nsys);
+  std::vector<int> outit(nslot, 0);
+  std::vector<Givens> gv(nslot);
+  {
+    std::vector<int> keep;
+    for (int a = 0; a < nsys; ++a) {
+      const int s = sys[a];
+      bnorm[s] = std::sqrt(nbuf[a].x);
+      st[s].resvec.push_back(bnorm[s] > 0 ? 1.0 : 0.0);
+      if (bnorm[s] > 0 && 1.0 > o.tol) keep.push_back(s);
+    }
+    act.swap(keep);
+  }
+
+  std::vector<cd> y, res;
+  std::vector<double2> cbuf;
+  // x += basis * y for the listed systems (k vectors); optionally r = V res (k+1 vectors)
+  auto finish = [&](const std::vector<int>& L, int k, bool restart_r) {
+    const int nl = (int)L.size();
+    if (nl == 0) return;
+    cbuf.assign((size_t)nl * (m + 1), double2{0, 0});
+    std::vector<double2> rbuf((size_t)nl * (m + 1), double2{0, 0});
+    for (int a = 0; a < nl; ++a) {
+      const int s = L[a];
+      gv[s].solve(y);
+      for (int j = 0; j < k; ++j) { cbuf[(size_t)a * (m + 1) + j].x = y[j].real(); cbuf[(size_t)a * (m + 1) + j].y = y[j].imag(); }
+      if (restart_r) {
+        gv[s].residual(y, res);
+        for (int j = 0; j <= k; ++j) { rbuf[(size_t)a * (m + 1) + j].x = res[j].real(); rbuf[(size_t)a * (m + 1) + j].y = res[j].imag(); }
+      }
+      st[s].inner = k;
+      st[s].outer = outit[s];
+    }
+    upload(L, W.aux_slots.p);
+    VH_HIP(hipMemcpyAsync(W.coef.p, cbuf.data(), cbuf.size() * sizeof(double2), hipMemcpyHostToDevice, strm));
+    const C* basis = o.flexible ? W.Z.p : W.V.p;
+    ops.combine(basis, vstr, x, k, nl, W.aux_slots.p, W.coef.p, 1.0, 1.0, nullptr);      // pgmres_iter.m:136-152
+    if (restart_r) {
+      // the coefficient buffer is read by the kernel above on the same stream, so it can be reused
+      VH_HIP(hipMemcpyAsync(W.coef.p, rbuf.data(), rbuf.size() * sizeof(double2), hipMemcpyHostToDevice, strm));
+      ops.combine(W.V.p, vstr, W.r.p, k + 1, nl, W.aux_slots.p, W.coef.p, 0.0, 1.0, nullptr);  // :153
+    }
+    VH_HIP(hipStreamSynchronize(strm));   // host buffers above are reused
+  };
+
+  while (!act.empty()) {
+    for (int s : act) outit[s]++;
+    int nact = (int)act.size();
+    upload(act, W.slots_dev.p);
+    // V(:,1) = r/||r||  (pgmres_iter.m:55)
+    ops.norm2(W.r.p, nact, W.slots_dev.p, W.nrm.p);
+    ops.scale(W.r.p, W.V.p, nact, W.slots_dev.p, W.nrm.p);
+    fetch_norms(nact);
+    for (int a = 0; a < nact; ++a) gv[act[a]].reset(m, std::sqrt(nbuf[a].x));
+    int k = 0;
+    while (k < m && !act.empty()) {
+      ++k;
+      nact = (int)act.size();
+      const C* vk = W.V.p + (int64_t)(k - 1) * vstr;
+      if (o.flexible) {
+        C* zk = W.Z.p + (int64_t)(k - 1) * vstr;
+        if (M) (*M)(act, W.slots_dev.p, vk, zk);
+        else convert_vectors<R, R>(ctx, vk, n, zk, n, n, nact, W.slots_dev.p, W.slots_dev.p);
+        A(act, W.slots_dev.p, zk, W.w.p);
+      } else {
+        if (M) {
+          A(act, W.slots_dev.p, vk, W.t.p);
+          (*M)(act, W.slots_dev.p, W.t.p, W.w.p);
+        } else {
+          A(act, W.slots_dev.p, vk, W.w.p);
+        }
+      }
+      for (int s : act) { st[s].iters++; st[s].matvecs++; }
+      // classical Gram-Schmidt, one pass (pgmres_iter.m:107-111)
+      ops.mdot(W.V.p, vstr, W.w.p, k, nact, W.slots_dev.p, W.h.p);
+      ops.combine(W.V.p, vstr, W.w.p, k, nact, W.slots_dev.p, W.h.p, 1.0, -1.0, W.nrm.p);
+      ops.scale(W.w.p, W.V.p + (int64_t)k * vstr, nact, W.slots_dev.p, W.nrm.p);
+      VH_HIP(hipMemcpy2DAsync(hbuf.data(), k * sizeof(double2), W.h.p, (m + 1) * sizeof(double2), k * sizeof(double2),
+                              nact, hipMemcpyDeviceToHost, strm));
+      fetch_norms(nact);
+      std::vector<int> conv, keep;
+      std::vector<cd> hc(k);
+      for (int a = 0; a < nact; ++a) {
+        const int s = act[a];
+        for (int j = 0; j < k; ++j) hc[j] = cd(hbuf[(size_t)a * k + j].x, hbuf[(size_t)a * k + j].y);
+        const double hn = std::sqrt(nbuf[a].x);
+        const double rres = gv[s].push(hc.data(), hn);
+        st[s].resvec.push_back(rres / bnorm[s]);
+        if (!(rres >= o.tol * bnorm[s])) conv.push_back(s);   // pgmres_iter.m:126 (NaN also stops)
+        else keep.push_back(s);
+      }
+      if (!conv.empty()) {
+        finish(conv, k, false);
+        act.swap(keep);
+        if (!act.empty()) upload(act, W.slots_dev.p);
+      }
+    }
+    if (!act.empty()) {
+      finish(act, k, true);
+      std::vector<int> keep;
+      for (int s : act) if (outit[s] < o.maxit) keep.push_back(s);
+      act.swap(keep);
+    }
+  }
+
+  // relres = ||b - A x|| / ||b||  (pgmres.m:181-186)
+  if (o.final_residual) {
+    upload(sys, W.slots_dev.p);
+    A(sys, W.slots_dev.p, x, W.t.p);
+    ops.sub_from(b, n, W.t.p, nsys, W.slots_dev.p);
+    ops.norm2(W.t.p, nsys, W.slots_dev.p, W.nrm.p);
+    fetch_norms(nsys);
+    std::vector<double2> bn(nsys);
+    ops.norm2(b, nsys, W.slots_dev.p, W.nrm.p);
+    VH_HIP(hipMemcpyAsync(bn.data(), W.nrm.p, nsys * sizeof(double2), hipMemcpyDeviceToHost, strm));
+    VH_HIP(hipStreamSynchronize(strm));
+    for (int a = 0; a < nsys; ++a) {
+      const int s = sys[a];
+      const double nb = std::sqrt(bn[a].x);
+      st[s].relres = nb > 0 ? std::sqrt(nbuf[a].x) / nb : 0.0;
+      st[s].flag = (st[s].relres <= o.tol) ? 0 : 1;
+      st[s].matvecs++;
+    }
+  } else {
+    for (int s : sys) { st[s].relres = st[s].resvec.back(); st[s].flag = st[s].relres <= o.tol ? 0 : 1; }
+  }
+}
+
+template void gmres_batch<double>(Context*, int64_t, const OpFn<double>&, const OpFn<double>*, const double2*, double2*,
+                                  const std::vector<int>&, const SolveOpts&, KrylovWork<double>&,
+                                  std::vector<SysStats>&);
+template void gmres_batch<float>(Context*, int64_t, const OpFn<float>&, const OpFn<float>*, const float2*, float2*,
+                                 const std::vector<int>&, const SolveOpts&, KrylovWork<float>&, std::vector<SysStats>&);
+
+void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<float>& A32, const OpFn<float>& M32,
+                  const double2* b, double2* x, const std::vector<int>& sys, const MixedOpts& o,
+                  KrylovWork<double>& W64, KrylovWork<float>& W32, std::vector<SysStats>& st) {
+  int nslot = 0;
+  for (int s : sys) nslot = std::max(nslot, s + 1);
+  DevBuf<float2> v32((size_t)nslot * n), z32((size_t)nslot * n);
+  std::vector<SysStats> inner_st;
+  std::vector<int> inner_iters(nslot, 0);
+  SolveOpts io;
+  io.restart = o.inner_restart;
+  io.maxit = o.inner_maxit;
+  io.tol = o.inner_tol;
+  io.final_residual = false;
+  io.flexible = false;
+  // preconditioner of the outer iteration: z = (FP32 GMRES solve of A z = v)
+  OpFn<double> inner = [&](const std::vector<int>& act, const int* slots, const double2* v, double2* z) {
+    const int na = (int)act.size();
+    convert_vectors<float, double>(ctx, v, n, v32.p, n, n, na, slots, slots);
+    gmres_batch<float>(ctx, n, A32, &M32, v32.p, z32.p, act, io, W32, inner_st);
+    convert_vectors<double, float>(ctx, z32.p, n, z, n, n, na, slots, slots);
+    for (int s : act) inner_iters[s] += inner_st[s].iters;
+  };
+  SolveOpts oo;
+  oo.restart = o.outer_restart;
+  oo.maxit = o.outer_maxit;
+  oo.tol = o.tol;
+  oo.final_residual = true;
+  oo.flexible = true;
+  gmres_batch<double>(ctx, n, A64, &inner, b, x, sys, oo, W64, st);
+  for (int s : sys) st[s].inner_iters = inner_iters[s];
+}
+
+}  // namespace vh

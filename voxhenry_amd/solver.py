@@ -1,0 +1,107 @@
+"""Frequency sweep of one structure on the device, mirroring the solve part of
+VoxHenry_executer.m:84-328:
+
+* set-up (once): non-empty voxels (pre_define_structure_params.m:5-16), the
+  Green's tensor (GPU, lse_generate_circulant_tensor), panel ids / Ar / port
+  right-hand sides (library host code);
+* per frequency: tolerance rule (:219-223), impedance z_real + z_imag (:231-259),
+  preconditioner prepare (:273), one batched solve over all ports (:276-298),
+  Y(:,port) = sum(rhs(:,kk).*x) (:311-316), Z = inv(Y), R = |Re Z|,
+  L = |Im Z|/(2 pi f) (:327-328).
+
+method 'fp64' runs pgmres with the reference semantics in complex128 on the
+GPU; 'mixed' runs the FP64 flexible GMRES with the FP32 inner solver.
+"""
+import time
+
+import numpy as np
+
+from .core import Geometry, Precon, MU0
+from ._lib import VH_C128
+
+
+def reference_tol(f):
+    """VoxHenry_executer.m:219-223."""
+    return 1e-12 if f < 1e6 else 1e-8
+
+
+def nonempty(structure):
+    """0-based column-major indices of the non-empty voxels (idxS)."""
+    s = np.abs(structure['sigma'].ravel(order='F')) > 1e-12
+    if structure.get('lambdaL') is not None:
+        s |= np.abs(structure['lambdaL'].ravel(order='F')) > 1e-12
+    return np.flatnonzero(s)
+
+
+def impedance(structure, idx, f):
+    """z = z_real + z_imag per non-empty voxel (VoxHenry_executer.m:231-259)."""
+    sig = structure['sigma'].ravel(order='F')[idx]
+    lam = structure.get('lambdaL')
+    with np.errstate(divide='ignore', invalid='ignore'):
+        if lam is None:
+            return (1.0 / sig).astype(complex)
+        lam = lam.ravel(order='F')[idx]
+        w = 2 * np.pi * f
+        den = (w * MU0 * sig * lam ** 2) ** 2 + 1
+        z = sig * (w * MU0 * lam ** 2) ** 2 / den + 1j * w * MU0 * lam ** 2 / den
+        normal = (np.abs(lam) <= 1e-12) & (np.abs(sig) > 1e-12)
+        z[normal] = 1.0 / sig[normal]
+    return z
+
+
+class Solver:
+    def __init__(self, ctx, structure, pad_mode=1):
+        self.ctx = ctx
+        self.structure = structure
+        self.idx = nonempty(structure)
+        dims = structure['sigma'].shape
+        t0 = time.perf_counter()
+        self.G = Geometry(ctx, dims, structure['dx'], self.idx, pad_mode=pad_mode)
+        self.G.generate_green()
+        t1 = time.perf_counter()
+        self.rhs = self.G.build_topology(structure['ports_P'], structure['ports_N'], structure.get('well'))
+        t2 = time.perf_counter()
+        self.P = Precon(self.G)
+        self.nports = len(structure['ports_P'])
+        self.times = dict(tensor=t1 - t0, topology=t2 - t1)
+        self._z_f = None
+
+    def prepare(self, f):
+        """Impedance and preconditioner for frequency f (executer :225-273)."""
+        if self.structure.get('lambdaL') is not None or self._z_f is None:
+            self.G.set_impedance(impedance(self.structure, self.idx, f))
+            self._z_f = f
+        jwmu = 1j * 2 * np.pi * f * MU0
+        t = time.perf_counter()
+        self.P.prepare(jwmu)
+        self.times['precon'] = self.times.get('precon', 0.0) + time.perf_counter() - t
+        return jwmu
+
+    def solve(self, f, method='fp64', tol=None, restart=100, maxit=10, **kw):
+        """All ports at frequency f; returns X (nports x n) and statistics."""
+        tol = reference_tol(f) if tol is None else tol
+        jwmu = self.prepare(f)
+        t = time.perf_counter()
+        if method == 'fp64':
+            X, st = self.G.gmres(self.rhs, jwmu, self.P, restart=restart, tol=tol, maxit=maxit, prec=VH_C128)
+        elif method == 'mixed':
+            X, st = self.G.solve_mixed(self.rhs, jwmu, self.P, tol=tol, **kw)
+        else:
+            raise ValueError(method)
+        st['solve_time'] = time.perf_counter() - t
+        return X, st
+
+    def sweep(self, freqs, method='fp64', tol=None, **kw):
+        out = dict(freq=np.asarray(freqs, float), Y=[], Z=[], R=[], L=[], stats=[])
+        for f in freqs:
+            X, st = self.solve(f, method, tol, **kw)
+            Y = self.rhs @ X.T                                   # Y[kk, port] = sum(rhs_kk .* x_port)
+            Z = np.linalg.inv(Y)
+            out['Y'].append(Y)
+            out['Z'].append(Z)
+            out['R'].append(np.abs(Z.real))
+            out['L'].append(np.abs(Z.imag) / (2 * np.pi * f))
+            out['stats'].append(st)
+        for k in ('Y', 'Z', 'R', 'L'):
+            out[k] = np.array(out[k])
+        return out
