@@ -151,6 +151,23 @@ int vh_solve_mixed(vh_geom* g, vh_precon* p, double jwmu_re, double jwmu_im, int
                    double* x, int on_device, double tol, int outer_restart, int outer_maxit, int inner_restart,
                    double inner_tol, int* flag, double* relres, int* outer_iters, int* inner_iters);
 
+/* Batched solve of nsys (port, frequency) systems of one geometry: right-hand
+ * side s is b + s*n (complex128), its j*omega*mu is jwmu[s] (complex) and its
+ * preconditioner precons[pidx[s]] (prepared for that frequency, or for a
+ * nearby one: any preconditioner gives the same solution).
+ * method 0: FP64 left-preconditioned GMRES with the pgmres semantics
+ *           (restart, maxit, tol) for every system;
+ * method 1: mixed precision, FP64 flexible GMRES (outer_restart, maxit) with
+ *           the FP32 inner GMRES (inner_restart, inner_tol).
+ * All systems run in lock step and converged ones leave the batch; if the
+ * Krylov workspaces exceed hbm_budget bytes (0: 80% of free memory) the batch
+ * is processed in chunks.  Outputs per system: flag, relres, iters (outer
+ * Arnoldi steps), inner_iters (FP32 steps, method 1). */
+int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64_t nsys, const double* jwmu,
+                   const int* pidx, const double* b, double* x, int on_device, int method, double tol, int restart,
+                   int maxit, int inner_restart, double inner_tol, double hbm_budget, int* flag, double* relres,
+                   int* iters, int* inner_iters);
+
 /* Test hook: in-place 3-D FFT (dir -1 forward, +1 inverse unnormalised) of ncomp
  * device grids of the geometry's padded size. */
 int vh_fft3d(vh_geom* g, void* data, int ncomp, int prec, int dir);

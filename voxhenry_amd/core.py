@@ -240,6 +240,35 @@ class Geometry:
                                    ptr(relres), ptr(oit), ptr(iit)))
         return x.reshape(np.shape(b)), dict(flag=flag, relres=relres, outer_iters=oit, inner_iters=iit)
 
+    def solve_batch(self, B, jwmu, precons, pidx, method='mixed', tol=1e-8, restart=None, maxit=None,
+                    inner_restart=40, inner_tol=1e-5, hbm_budget=0.0):
+        """Many (port, frequency) systems in one batched solve (vh_solve_batch).
+
+        B: (nsys, n) rhs; jwmu: (nsys,) complex; precons: list of Precon;
+        pidx: (nsys,) index into precons.  method 'fp64' (pgmres semantics,
+        default restart 100 / maxit 10) or 'mixed' (default outer restart 8 /
+        maxit 4)."""
+        m = 0 if method == 'fp64' else 1
+        if restart is None:
+            restart = 100 if m == 0 else 8
+        if maxit is None:
+            maxit = 10 if m == 0 else 4
+        BB = np.ascontiguousarray(np.atleast_2d(B), dtype=np.complex128)
+        nsys, n = BB.shape
+        assert n == self.n
+        jw = np.ascontiguousarray(np.asarray(jwmu, dtype=np.complex128).ravel())
+        pi = np.ascontiguousarray(np.asarray(pidx, dtype=np.int32).ravel())
+        hs = (C.c_void_p * len(precons))(*[p.h.value for p in precons])
+        X = np.zeros_like(BB)
+        flag = np.zeros(nsys, np.int32)
+        relres = np.zeros(nsys)
+        its = np.zeros(nsys, np.int32)
+        iits = np.zeros(nsys, np.int32)
+        check(lib().vh_solve_batch(self.h, len(precons), C.cast(hs, C.c_void_p), nsys, ptr(jw), ptr(pi), ptr(BB),
+                                   ptr(X), 0, m, tol, restart, maxit, inner_restart, inner_tol, float(hbm_budget),
+                                   ptr(flag), ptr(relres), ptr(its), ptr(iits)))
+        return X, dict(flag=flag, relres=relres, iters=its, inner_iters=iits)
+
     def fft3d(self, data, prec, direction):
         self.ctx.use_torch_stream()
         ncomp = data.numel() // (self.pad[0] * self.pad[1] * self.pad[2])

@@ -302,9 +302,9 @@ int vh_precon_prepare(vh_precon* p, double jre, double jim, int64_t* stats) {
     need(p, "p");
     vh::precon_prepare(p->P, jre, jim);
     if (stats) {
-      stats[0] = p->P.nnzL;
-      stats[1] = (int64_t)p->P.fl_ptr.size() - 1;
-      stats[2] = (int64_t)p->P.bl_ptr.size() - 1;
+      stats[0] = p->P.sym->nnzL;
+      stats[1] = (int64_t)p->P.sym->fl_ptr.size() - 1;
+      stats[2] = (int64_t)p->P.sym->bl_ptr.size() - 1;
       stats[3] = p->P.G->num_node;
     }
   });
@@ -502,6 +502,104 @@ int vh_ldl_solve_host(int64_t n, const int64_t* Ap, const int64_t* Ai, const dou
         double* xr = x + r * n;
         for (int64_t k = 0; k < n; ++k) xr[F.perm[k]] = w[k];
       }
+    }
+  });
+}
+
+int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64_t nsys, const double* jwmu,
+                   const int* pidx, const double* b, double* x, int on_device, int method, double tol, int restart,
+                   int maxit, int inner_restart, double inner_tol, double hbm_budget, int* flag, double* relres,
+                   int* iters, int* inner_iters) {
+  return guard([&] {
+    need(g, "g"); need(precons, "precons"); need(jwmu, "jwmu"); need(pidx, "pidx"); need(b, "b"); need(x, "x");
+    auto& G = g->G;
+    VH_REQUIRE(G.has_tensor && G.has_inc && G.has_z, "geometry not complete (tensor, incidence, impedance)");
+    VH_REQUIRE(nprecon >= 1 && nsys >= 1 && nsys <= 13107, "nprecon / nsys out of range");
+    VH_REQUIRE(method == 0 || method == 1, "method must be 0 (fp64) or 1 (mixed)");
+    VH_REQUIRE(restart >= 1 && restart <= 127 && inner_restart >= 1 && inner_restart <= 127 && maxit >= 1,
+               "bad restart / maxit");
+    for (int64_t k = 0; k < nprecon; ++k) VH_REQUIRE(precons[k] && precons[k]->P.ready, "preconditioner not prepared");
+    for (int64_t s = 0; s < nsys; ++s) VH_REQUIRE(pidx[s] >= 0 && pidx[s] < nprecon, "pidx out of range");
+    const int64_t n = G.n();
+    hipStream_t st = G.ctx->stream;
+    Io io;
+    io.in(b, nsys, n, 16, on_device, st, n);
+    double2* dx = (double2*)(on_device ? (void*)x : io.y);
+    const double2* db = (const double2*)io.x;
+    // per-slot j*omega*mu on the device
+    vh::DevBuf<double2> alphas(nsys);
+    VH_HIP(hipMemcpyAsync(alphas.p, jwmu, nsys * sizeof(double2), hipMemcpyHostToDevice, st));
+    // group lists for the preconditioner apply
+    vh::DevBuf<int> gslots(nsys);
+    std::vector<int> gbuf(nsys);
+    auto grouped = [&](const std::vector<int>& act, auto apply) {
+      // sort the active systems by preconditioner, apply each group with its own slot list
+      std::vector<std::vector<int>> by(nprecon);
+      for (int s : act) by[pidx[s]].push_back(s);
+      int off = 0;
+      for (int64_t k = 0; k < nprecon; ++k) {
+        if (by[k].empty()) continue;
+        std::copy(by[k].begin(), by[k].end(), gbuf.begin() + off);
+        off += (int)by[k].size();
+      }
+      VH_HIP(hipMemcpyAsync(gslots.p, gbuf.data(), off * sizeof(int), hipMemcpyHostToDevice, st));
+      off = 0;
+      for (int64_t k = 0; k < nprecon; ++k) {
+        if (by[k].empty()) continue;
+        apply(precons[k]->P, (int)by[k].size(), gslots.p + off);
+        off += (int)by[k].size();
+      }
+      VH_HIP(hipStreamSynchronize(st));   // gbuf is reused by the next call
+    };
+    // HBM budget -> chunk size
+    size_t freeb = 0, totalb = 0;
+    VH_HIP(hipMemGetInfo(&freeb, &totalb));
+    const double budget = hbm_budget > 0 ? hbm_budget : 0.8 * (double)freeb;
+    double per_sys;
+    if (method == 0) per_sys = (double)vh::KrylovWork<double>::bytes_needed(n, restart, 1, false);
+    else per_sys = (double)vh::KrylovWork<double>::bytes_needed(n, restart, 1, true) +
+                   (double)vh::KrylovWork<float>::bytes_needed(n, inner_restart, 1, false) + 2.0 * n * 8;
+    per_sys += 5.0 * n * 16;   // matvec workspaces (A + B grids are ~4-5 vectors per system)
+    int chunk = (int)std::max(1.0, std::min((double)nsys, std::floor(budget / per_sys)));
+    std::vector<vh::SysStats> stats;
+    vh::KrylovWork<double> W64;
+    vh::KrylovWork<float> W32;
+    vh::OpFn<double> A64 = [&](const std::vector<int>& act, const int* sl, const double2* xx, double2* yy) {
+      vh::matvec<double>(G, (int)act.size(), xx, n, yy, n, 0.0, 0.0, 0, sl, alphas.p);
+    };
+    vh::OpFn<float> A32 = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
+      vh::matvec<float>(G, (int)act.size(), xx, n, yy, n, 0.0, 0.0, 0, sl, alphas.p);
+    };
+    vh::OpFn<double> M64 = [&](const std::vector<int>& act, const int*, const double2* xx, double2* yy) {
+      grouped(act, [&](vh::Precon& P, int cnt, const int* sl) { vh::precon_apply<double>(P, cnt, xx, n, yy, n, sl); });
+    };
+    vh::OpFn<float> M32 = [&](const std::vector<int>& act, const int*, const float2* xx, float2* yy) {
+      grouped(act, [&](vh::Precon& P, int cnt, const int* sl) { vh::precon_apply<float>(P, cnt, xx, n, yy, n, sl); });
+    };
+    stats.resize(nsys);
+    for (int64_t s0 = 0; s0 < nsys; s0 += chunk) {
+      std::vector<int> sys;
+      for (int64_t s = s0; s < std::min<int64_t>(nsys, s0 + chunk); ++s) sys.push_back((int)s);
+      std::vector<vh::SysStats> cs;
+      if (method == 0) {
+        vh::SolveOpts o;
+        o.restart = restart; o.maxit = maxit; o.tol = tol; o.final_residual = true; o.flexible = false;
+        vh::gmres_batch<double>(G.ctx, n, A64, &M64, db, dx, sys, o, W64, cs);
+      } else {
+        vh::MixedOpts o;
+        o.tol = tol; o.outer_restart = restart; o.outer_maxit = maxit;
+        o.inner_restart = inner_restart; o.inner_tol = inner_tol;
+        vh::fgmres_mixed(G.ctx, n, A64, A32, M32, db, dx, sys, o, W64, W32, cs);
+      }
+      for (int s : sys) stats[s] = cs[s];
+    }
+    if (!on_device) VH_HIP(hipMemcpyAsync(x, io.y, (size_t)nsys * n * 16, hipMemcpyDeviceToHost, st));
+    VH_HIP(hipStreamSynchronize(st));
+    for (int64_t s = 0; s < nsys; ++s) {
+      if (flag) flag[s] = stats[s].flag;
+      if (relres) relres[s] = stats[s].relres;
+      if (iters) iters[s] = stats[s].iters;
+      if (inner_iters) inner_iters[s] = stats[s].inner_iters;
     }
   });
 }

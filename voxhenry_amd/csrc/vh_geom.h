@@ -4,6 +4,7 @@
 #pragma once
 #include "vh_common.h"
 #include "vh_fft.h"
+#include <memory>
 
 namespace vh {
 
@@ -34,6 +35,8 @@ struct PassTimer {
   void destroy();
   void collect();   // waits for the last recorded call and adds its pass times
 };
+
+struct PreconSymbolic;
 
 struct Geometry {
   Context* ctx = nullptr;
@@ -67,6 +70,7 @@ struct Geometry {
   std::vector<double> rhs_h;          // port right-hand sides (nports x n), from build_topology
   int nports = 0;
   PassTimer* timer = nullptr;
+  std::shared_ptr<PreconSymbolic> precon_sym;   // shared by all preconditioners of this geometry
   // matvec workspaces, grown on demand.  A (pass X output) relies on its
   // never-written empty lines reading as zero, so each precision has its own.
   DevBuf<char> wsA64, wsA32, wsB;
@@ -80,9 +84,11 @@ struct Geometry {
 // mode 0: full operator [Z, -Ar'; Ar, 0]; mode 1: the FFT block with the
 // resistive diagonal only (current rows); mode 2: convolution only.
 // slots (device, optional): vector k of the batch is x + slots[k]*ldx.
+// alphas (device, optional): j*omega*mu per vector slot (overrides alpha).
 template <typename R>
 void matvec(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy,
-            double alpha_re, double alpha_im, int mode = 0, const int* slots = nullptr);
+            double alpha_re, double alpha_im, int mode = 0, const int* slots = nullptr,
+            const double2* alphas = nullptr);
 
 // Build the spectral tensor on the device from a real-space normalised tensor
 // G (L*M*N*7 complex, device, column-major [x][y][z][c]).

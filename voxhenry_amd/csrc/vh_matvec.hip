@@ -180,7 +180,9 @@ __device__ __forceinline__ void apply_point<float>(float2* f, const TensorRef<fl
 
 template <typename R, int EMAX>
 __global__ __launch_bounds__(FFT_THREADS) void k_pass_z(
-    cplx<R>* __restrict__ B, int nrhs, int T, GridDims g, FftPlan pl, TensorRef<R> tens, cplx<R> alpha) {
+    cplx<R>* __restrict__ B, int nrhs, int T, GridDims g, FftPlan pl, TensorRef<R> tens, cplx<R> alpha0,
+    const double2* __restrict__ alphas, const int* __restrict__ slots) {
+  // alphas (optional): j*omega*mu per vector slot; otherwise alpha0 for all
   extern __shared__ __align__(16) unsigned char smem[];
   cplx<R>* buf = (cplx<R>*)smem;
   const int tid = threadIdx.x, nthr = blockDim.x;
@@ -193,6 +195,11 @@ __global__ __launch_bounds__(FFT_THREADS) void k_pass_z(
   const int64_t plane = (int64_t)Px * g.Py;     // elements per z-plane of one rc
   const size_t O = (size_t)g.Hx * g.Hy * g.Hz;
   for (int rhs = 0; rhs < nrhs; ++rhs) {
+    cplx<R> alpha = alpha0;
+    if (alphas) {
+      const double2 a = alphas[slots ? slots[rhs] : rhs];
+      alpha.x = (R)a.x; alpha.y = (R)a.y;
+    }
     cplx<R>* base = B + (int64_t)rhs * 5 * plane * N + (int64_t)ky * Px + kx0;
     for (int e = tid; e < T5 * Pz; e += nthr) {
       const int kz = e / T5, p = e - kz * T5;
@@ -335,7 +342,7 @@ static void ensure_ws(Geometry& G, int nrhs) {
 
 template <typename R>
 static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy,
-                       double are, double aim, int mode, const int* slots) {
+                       double are, double aim, int mode, const int* slots, const double2* alphas) {
   VH_REQUIRE(G.has_tensor, "matvec: Green's tensor not set");
   VH_REQUIRE(nrhs >= 1, "matvec: nrhs must be >= 1");
   VH_REQUIRE(nrhs * 5 <= 65535, "matvec: too many right-hand sides in one call");
@@ -382,7 +389,7 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
     TensorRef<R> tr;
     if constexpr (sizeof(R) == 8) tr.t = G.t64.p; else tr.t = G.t32.p;
     hipLaunchKernelGGL((k_pass_z<R, EM>), grid, dim3(FFT_THREADS), 5 * T * g.Pz * elem, st, B, nrhs, T, g, pz, tr,
-                       alpha);
+                       alpha, alphas, slots);
     VH_LAUNCHED();
   }
   mark(3);
@@ -443,13 +450,13 @@ void PassTimer::collect() {
 
 template <typename R>
 void matvec(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy, double are, double aim,
-            int mode, const int* slots) {
-  run_passes<R>(G, nrhs, x, ldx, y, ldy, are, aim, mode, slots);
+            int mode, const int* slots, const double2* alphas) {
+  run_passes<R>(G, nrhs, x, ldx, y, ldy, are, aim, mode, slots, alphas);
 }
 
 template void matvec<double>(Geometry&, int, const double2*, int64_t, double2*, int64_t, double, double, int,
-                             const int*);
+                             const int*, const double2*);
 template void matvec<float>(Geometry&, int, const float2*, int64_t, float2*, int64_t, double, double, int,
-                            const int*);
+                            const int*, const double2*);
 
 }  // namespace vh

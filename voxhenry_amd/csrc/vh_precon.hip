@@ -154,6 +154,98 @@ static void build_schedule(const std::vector<int>& lev, std::vector<int>& lptr, 
   for (int i = 0; i < n; ++i) rows[fill[lev[i]]++] = i;
 }
 
+static void build_symbolic(Geometry& G, bool geometric, PreconSymbolic& Y) {
+  const int nv = G.g.nv;
+  const int nn = (int)G.num_node;
+  const auto& arp = G.arp_h; const auto& arc = G.arc_h;
+  const auto& atp = G.atp_h; const auto& atc = G.atc_h; const auto& atv = G.atv_h;
+  std::vector<int> Sp(nn + 1, 0), Si;
+  std::vector<int> mark(nn, -1);
+  Si.reserve((size_t)nn * 16);
+  for (int i = 0; i < nn; ++i) {
+    for (int p = arp[i]; p < arp[i + 1]; ++p) {
+      const int k = arc[p];
+      for (int q = atp[k]; q < atp[k + 1]; ++q) {
+        const int j = atc[q];
+        if (mark[j] != i) { mark[j] = i; Si.push_back(j); }
+      }
+    }
+    std::sort(Si.begin() + Sp[i], Si.end());
+    Sp[i + 1] = (int)Si.size();
+  }
+  Y.Sp = std::move(Sp);
+  Y.Si = std::move(Si);
+  // face coordinates of every node from the Jx/Jy/Jz columns of its voxels:
+  // the -1 entry is the face at the voxel's lower side, +1 the upper side
+  std::vector<float> xyz(3 * (size_t)nn, 0.f);
+  std::vector<int> type(nn, -1);
+  const int L = G.g.L, M = G.g.M;
+  for (int v = 0; v < nv; ++v) {
+    const int64_t lin = G.vox[v];
+    const float c[3] = {(float)(lin % L), (float)((lin / L) % M), (float)(lin / ((int64_t)L * M))};
+    for (int d = 0; d < 3; ++d) {
+      const int64_t row = (int64_t)d * nv + v;
+      for (int q = atp[row]; q < atp[row + 1]; ++q) {
+        const int node = atc[q];
+        float* pxyz = &xyz[3 * (size_t)node];
+        for (int e = 0; e < 3; ++e) pxyz[e] = c[e] + 0.5f;
+        pxyz[d] = c[d] + (atv[q] > 0 ? 1.f : 0.f);
+        type[node] = d;
+      }
+    }
+  }
+  for (int i = 0; i < nn && geometric; ++i) geometric = type[i] >= 0;
+  std::vector<int> perm = geometric ? nd_ordering_geometric(nn, Y.Sp.data(), Y.Si.data(), xyz.data(), type.data())
+                                    : nd_ordering(nn, Y.Sp.data(), Y.Si.data());
+  ldl_symbolic(nn, Y.Sp.data(), Y.Si.data(), perm, Y.F);
+  Y.nnzL = (int64_t)Y.F.Lp[nn];
+}
+
+// CSR copy of the L pattern, CSC->CSR position map and level schedules
+// (needs the row indices of L, i.e. one numeric factorisation).
+static void build_levels(Geometry& G, const LdlFactor& F, PreconSymbolic& Y) {
+  const int n = F.n;
+  std::vector<int> Rp(n + 1, 0);
+  for (int j = 0; j < n; ++j)
+    for (int p = F.Lp[j]; p < F.Lp[j + 1]; ++p) Rp[F.Li[p] + 1]++;
+  for (int i = 0; i < n; ++i) Rp[i + 1] += Rp[i];
+  std::vector<int> Ri(Rp[n]);
+  Y.csc2csr.assign(Rp[n], 0);
+  std::vector<int> fill(Rp.begin(), Rp.end() - 1);
+  for (int j = 0; j < n; ++j)
+    for (int p = F.Lp[j]; p < F.Lp[j + 1]; ++p) {
+      const int i = F.Li[p];
+      Ri[fill[i]] = j;
+      Y.csc2csr[p] = fill[i]++;
+    }
+  std::vector<int> lev_f(n, 0), lev_b(n, 0);
+  for (int i = 0; i < n; ++i) {
+    int l = 0;
+    for (int p = Rp[i]; p < Rp[i + 1]; ++p) l = std::max(l, lev_f[Ri[p]] + 1);
+    lev_f[i] = l;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    int l = 0;
+    for (int p = F.Lp[i]; p < F.Lp[i + 1]; ++p) l = std::max(l, lev_b[F.Li[p]] + 1);
+    lev_b[i] = l;
+  }
+  build_schedule(lev_f, Y.fl_ptr, Y.fl_rows);
+  build_schedule(lev_b, Y.bl_ptr, Y.bl_rows);
+  upload(Y.d_fl_rows, Y.fl_rows);
+  upload(Y.d_bl_rows, Y.bl_rows);
+  upload(Y.d_fl_ptr, Y.fl_ptr);
+  upload(Y.d_bl_ptr, Y.bl_ptr);
+  upload(Y.d_perm, F.perm);
+  upload(Y.Rp, Rp);
+  upload(Y.Ri, Ri);
+  upload(Y.Cp, F.Lp);
+  upload(Y.Ci, F.Li);
+  // Ar' with permuted node columns
+  std::vector<int> tcp(G.atc_h.size());
+  for (size_t q = 0; q < G.atc_h.size(); ++q) tcp[q] = F.iperm[G.atc_h[q]];
+  upload(Y.ArTp_col, tcp);
+}
+
 void precon_prepare(Precon& P, double jre, double jim) {
   Geometry& G = *P.G;
   VH_REQUIRE(G.has_inc && G.has_z, "preconditioner: incidence and impedance must be set first");
@@ -174,87 +266,50 @@ void precon_prepare(Precon& P, double jre, double jim) {
     ainv[3 * nv + v] = d2; ainv[4 * nv + v] = d3;
   }
   P.ainv_h = ainv;
-  // S = Ar diag(ainv) Ar' (:102), full symmetric CSR
+  bool fresh = false;
+  if (!G.precon_sym) {
+    G.precon_sym = std::make_shared<PreconSymbolic>();
+    build_symbolic(G, P.geometric_order, *G.precon_sym);
+    fresh = true;
+  }
+  P.sym = G.precon_sym;
+  PreconSymbolic& Y = *P.sym;
+  // S = Ar diag(A_inv) Ar' (:102) on the shared pattern
   const auto& arp = G.arp_h; const auto& arc = G.arc_h; const auto& arv = G.arv_h;
   const auto& atp = G.atp_h; const auto& atc = G.atc_h; const auto& atv = G.atv_h;
-  if (!P.symbolic) {
-    std::vector<int> Sp(nn + 1, 0), Si;
-    std::vector<int> mark(nn, -1);
-    Si.reserve((size_t)nn * 16);
-    for (int i = 0; i < nn; ++i) {
-      for (int p = arp[i]; p < arp[i + 1]; ++p) {
-        const int k = arc[p];
-        for (int q = atp[k]; q < atp[k + 1]; ++q) {
-          const int j = atc[q];
-          if (mark[j] != i) { mark[j] = i; Si.push_back(j); }
-        }
-      }
-      std::sort(Si.begin() + Sp[i], Si.end());
-      Sp[i + 1] = (int)Si.size();
-    }
-    P.Sp = std::move(Sp);
-    P.Si = std::move(Si);
-    // face coordinates of every node from the Jx/Jy/Jz columns of its voxels:
-    // the -1 entry is the face at the voxel's lower side, +1 the upper side
-    std::vector<float> xyz(3 * (size_t)nn, 0.f);
-    std::vector<int> type(nn, -1);
-    const int L = G.g.L, M = G.g.M;
-    for (int v = 0; v < nv; ++v) {
-      const int64_t lin = G.vox[v];
-      const float c[3] = {(float)(lin % L), (float)((lin / L) % M), (float)(lin / ((int64_t)L * M))};
-      for (int d = 0; d < 3; ++d) {
-        const int64_t row = (int64_t)d * nv + v;
-        for (int q = atp[row]; q < atp[row + 1]; ++q) {
-          const int node = atc[q];
-          float* pxyz = &xyz[3 * (size_t)node];
-          for (int e = 0; e < 3; ++e) pxyz[e] = c[e] + 0.5f;
-          pxyz[d] = c[d] + (atv[q] > 0 ? 1.f : 0.f);
-          type[node] = d;
-        }
-      }
-    }
-    bool geometric = P.geometric_order;
-    for (int i = 0; i < nn && geometric; ++i) geometric = type[i] >= 0;
-    std::vector<int> perm = geometric ? nd_ordering_geometric(nn, P.Sp.data(), P.Si.data(), xyz.data(), type.data())
-                                      : nd_ordering(nn, P.Sp.data(), P.Si.data());
-    ldl_symbolic(nn, P.Sp.data(), P.Si.data(), perm, P.F);
-    P.symbolic = true;
-  }
-  std::vector<double> Sx(P.Si.size(), 0.0), acc(nn, 0.0);
+  std::vector<double> Sx(Y.Si.size(), 0.0), acc(nn, 0.0);
   for (int i = 0; i < nn; ++i) {
     for (int p = arp[i]; p < arp[i + 1]; ++p) {
       const int k = arc[p];
       const double a = arv[p] * ainv[k];
       for (int q = atp[k]; q < atp[k + 1]; ++q) acc[atc[q]] += a * atv[q];
     }
-    for (int p = P.Sp[i]; p < P.Sp[i + 1]; ++p) { Sx[p] = acc[P.Si[p]]; acc[P.Si[p]] = 0.0; }
+    for (int p = Y.Sp[i]; p < Y.Sp[i + 1]; ++p) { Sx[p] = acc[Y.Si[p]]; acc[Y.Si[p]] = 0.0; }
   }
-  ldl_numeric(nn, P.Sp.data(), P.Si.data(), Sx.data(), P.F);
-  // device copies
-  std::vector<int> Rp, Ri, lev_f, lev_b;
-  std::vector<double> Rx;
-  ldl_levels(P.F, Rp, Ri, Rx, lev_f, lev_b);
-  if (!P.levels_built) {
-    build_schedule(lev_f, P.fl_ptr, P.fl_rows);
-    build_schedule(lev_b, P.bl_ptr, P.bl_rows);
-    upload(P.d_fl_rows, P.fl_rows);
-    upload(P.d_bl_rows, P.bl_rows);
-    upload(P.d_fl_ptr, P.fl_ptr);
-    upload(P.d_bl_ptr, P.bl_ptr);
-    upload(P.d_perm, P.F.perm);
-    upload(P.Rp, Rp); upload(P.Ri, Ri);
-    upload(P.Cp, P.F.Lp); upload(P.Ci, P.F.Li);
-    // Ar' with permuted node columns
-    std::vector<int> tcp(atc.size());
-    for (size_t q = 0; q < atc.size(); ++q) tcp[q] = P.F.iperm[atc[q]];
-    upload(P.ArTp_col, tcp);
-    P.levels_built = true;
+  // numeric factorisation on a private copy of the symbolic data
+  P.F.n = Y.F.n;
+  P.F.perm = Y.F.perm; P.F.iperm = Y.F.iperm; P.F.parent = Y.F.parent; P.F.Lp = Y.F.Lp;
+  P.F.Li.assign(Y.nnzL, 0);
+  P.F.Lx.assign(Y.nnzL, 0.0);
+  P.F.D.assign(nn, 0.0);
+  try {
+    ldl_numeric(nn, Y.Sp.data(), Y.Si.data(), Sx.data(), P.F);
+  } catch (const std::runtime_error& e) {
+    throw Error(ERR_NUMERIC, e.what());
   }
+  if (fresh) {
+    Y.F.Li = P.F.Li;
+    build_levels(G, P.F, Y);
+  }
+  std::vector<double> Rx(Y.nnzL);
+  for (int64_t p = 0; p < Y.nnzL; ++p) Rx[Y.csc2csr[p]] = P.F.Lx[p];
   upload(P.Rx64, Rx); upload_f32(P.Rx32, Rx);
   upload(P.Cx64, P.F.Lx); upload_f32(P.Cx32, P.F.Lx);
   upload(P.D64, P.F.D); upload_f32(P.D32, P.F.D);
   upload(P.ainv64, ainv); upload_f32(P.ainv32, ainv);
-  P.nnzL = (int64_t)P.F.Lp[nn];
+  // the values are on the device; the host copies of L are not needed any more
+  P.F.Li.clear(); P.F.Li.shrink_to_fit();
+  P.F.Lx.clear(); P.F.Lx.shrink_to_fit();
   P.ready = true;
 }
 
@@ -310,20 +365,21 @@ void precon_apply(Precon& P, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y
   if (nn > 0) {
     const int64_t items = (int64_t)nn * nrhs;
     hipLaunchKernelGGL((k_pc_gather<R>), dim3(ceil_div(items, 256)), dim3(256), 0, st, x, ldx, w, nrhs, nn,
-                       P.d_perm.p, G.Ar.ptr.p, G.Ar.col.p, arv, ainv, nc, slots);
+                       P.sym->d_perm.p, G.Ar.ptr.p, G.Ar.col.p, arv, ainv, nc, slots);
     VH_LAUNCHED();
     const R* Rx = d ? (const R*)P.Rx64.p : (const R*)P.Rx32.p;
     const R* Cx = d ? (const R*)P.Cx64.p : (const R*)P.Cx32.p;
     const R* D = d ? (const R*)P.D64.p : (const R*)P.D32.p;
-    run_levels<R>(P, P.fl_ptr, P.d_fl_ptr, P.d_fl_rows, P.Rp.p, P.Ri.p, Rx, D, w, nrhs, false, st);
-    run_levels<R>(P, P.bl_ptr, P.d_bl_ptr, P.d_bl_rows, P.Cp.p, P.Ci.p, Cx, D, w, nrhs, true, st);
+    PreconSymbolic& Y = *P.sym;
+    run_levels<R>(P, Y.fl_ptr, Y.d_fl_ptr, Y.d_fl_rows, Y.Rp.p, Y.Ri.p, Rx, D, w, nrhs, false, st);
+    run_levels<R>(P, Y.bl_ptr, Y.d_bl_ptr, Y.d_bl_rows, Y.Cp.p, Y.Ci.p, Cx, D, w, nrhs, true, st);
     hipLaunchKernelGGL((k_pc_scatter_nodes<R>), dim3(ceil_div(items, 256)), dim3(256), 0, st, w, y, ldy, nrhs, nn,
-                       P.d_perm.p, nc, slots);
+                       P.sym->d_perm.p, nc, slots);
     VH_LAUNCHED();
   }
   const R* atv = d ? (const R*)G.ArT.v64.p : (const R*)G.ArT.v32.p;
   hipLaunchKernelGGL((k_pc_currents<R>), dim3(ceil_div(nc, 256), nrhs), dim3(256), 0, st, x, ldx, w, y, ldy, nrhs, nc,
-                     G.ArT.ptr.p, P.ArTp_col.p, atv, ainv, slots);
+                     G.ArT.ptr.p, P.sym->ArTp_col.p, atv, ainv, slots);
   VH_LAUNCHED();
 }
 

@@ -91,6 +91,50 @@ class Solver:
         st['solve_time'] = time.perf_counter() - t
         return X, st
 
+    def precons_for(self, freqs):
+        """One prepared preconditioner per frequency (symbolic analysis shared)."""
+        out = []
+        for f in freqs:
+            P = Precon(self.G)
+            if self.structure.get('lambdaL') is not None:
+                self.G.set_impedance(impedance(self.structure, self.idx, f))
+            P.prepare(1j * 2 * np.pi * f * MU0)
+            out.append(P)
+        return out
+
+    def sweep_batched(self, freqs, method='mixed', tol=1e-8, precons=None, pidx_of_freq=None, **kw):
+        """Every (port, frequency) system in one batched solve (vh_solve_batch).
+
+        precons[pidx_of_freq[k]] preconditions frequency k (default: one
+        preconditioner per frequency).  Only normal conductors: the impedance
+        diagonal is frequency independent."""
+        if self.structure.get('lambdaL') is not None:
+            raise ValueError('batched sweep needs frequency-independent impedance (normal conductors)')
+        freqs = np.asarray(freqs, float)
+        if self._z_f is None:
+            self.G.set_impedance(impedance(self.structure, self.idx, freqs[0]))
+            self._z_f = freqs[0]
+        if precons is None:
+            precons = self.precons_for(freqs)
+            pidx_of_freq = np.arange(len(freqs))
+        npo = self.nports
+        B = np.repeat(self.rhs[None], len(freqs), axis=0).reshape(-1, self.G.n)
+        jw = np.repeat(1j * 2 * np.pi * freqs * MU0, npo)
+        pidx = np.repeat(np.asarray(pidx_of_freq), npo)
+        t = time.perf_counter()
+        X, st = self.G.solve_batch(B, jw, precons, pidx, method=method, tol=tol, **kw)
+        st['solve_time'] = time.perf_counter() - t
+        X = X.reshape(len(freqs), npo, -1)
+        out = dict(freq=freqs, Y=[], Z=[], R=[], L=[], stats=st)
+        for k, f in enumerate(freqs):
+            Y = self.rhs @ X[k].T
+            Z = np.linalg.inv(Y)
+            out['Y'].append(Y); out['Z'].append(Z)
+            out['R'].append(np.abs(Z.real)); out['L'].append(np.abs(Z.imag) / (2 * np.pi * f))
+        for k in ('Y', 'Z', 'R', 'L'):
+            out[k] = np.array(out[k])
+        return out
+
     def sweep(self, freqs, method='fp64', tol=None, **kw):
         out = dict(freq=np.asarray(freqs, float), Y=[], Z=[], R=[], L=[], stats=[])
         for f in freqs:
