@@ -303,8 +303,8 @@ int vh_precon_prepare(vh_precon* p, double jre, double jim, int64_t* stats) {
     vh::precon_prepare(p->P, jre, jim);
     if (stats) {
       stats[0] = p->P.sym->nnzL;
-      stats[1] = (int64_t)p->P.sym->fl_ptr.size() - 1;
-      stats[2] = (int64_t)p->P.sym->bl_ptr.size() - 1;
+      stats[1] = (int64_t)p->P.sym->f_lptr.size() - 1;
+      stats[2] = (int64_t)p->P.sym->b_lptr.size() - 1;
       stats[3] = p->P.G->num_node;
     }
   });

@@ -53,11 +53,17 @@ static int bfs_levels(const Graph& g, int start, const std::vector<int>& mark, i
 
 struct ND {
   const Graph& g;
-  std::vector<int> mark, level, order, out;
+  std::vector<int> mark, level, order, out, blocks;   // blocks: sizes of the emitted leaf / separator blocks
   int stamp = 0;
   static constexpr int LEAF = 48;
 
   explicit ND(const Graph& gg) : g(gg), mark(gg.n, -1), level(gg.n, -1) { out.reserve(gg.n); }
+
+  void emit(const std::vector<int>& v) {
+    if (v.empty()) return;
+    out.insert(out.end(), v.begin(), v.end());
+    blocks.push_back((int)v.size());
+  }
 
   void run(std::vector<int> verts) {
     // explicit stack to avoid deep recursion on long chains
@@ -68,12 +74,12 @@ struct ND {
       Item it = std::move(stack.back());
       stack.pop_back();
       if (it.phase == 1) {           // emit the separator after both halves
-        out.insert(out.end(), it.sep.begin(), it.sep.end());
+        emit(it.sep);
         continue;
       }
       std::vector<int>& vs = it.v;
       if ((int)vs.size() <= LEAF) {
-        out.insert(out.end(), vs.begin(), vs.end());
+        emit(vs);
         continue;
       }
       ++stamp;
@@ -100,7 +106,7 @@ struct ND {
         start = far;
       }
       if (ecc < 2) {
-        out.insert(out.end(), vs.begin(), vs.end());
+        emit(vs);
         continue;
       }
       // level sizes; separator = the smallest level in the middle band
@@ -137,13 +143,14 @@ struct ND {
 
 }  // namespace
 
-std::vector<int> nd_ordering(int n, const int* Ap, const int* Ai) {
+std::vector<int> nd_ordering(int n, const int* Ap, const int* Ai, std::vector<int>* blocks) {
   Graph g{n, Ap, Ai};
   ND nd(g);
   std::vector<int> all(n);
   std::iota(all.begin(), all.end(), 0);
   nd.run(std::move(all));
   if ((int)nd.out.size() != n) throw std::runtime_error("nested dissection lost vertices");
+  if (blocks) *blocks = nd.blocks;
   return nd.out;
 }
 
@@ -154,9 +161,15 @@ std::vector<int> nd_ordering(int n, const int* Ap, const int* Ai) {
 // the faces normal to axis d at the integer plane c separate every node with
 // coordinate < c from every node with coordinate > c.  Each step cuts the
 // longest extent of the current node set at its median plane.
-std::vector<int> nd_ordering_geometric(int n, const int* Ap, const int* Ai, const float* xyz, const int* type) {
-  std::vector<int> out;
+std::vector<int> nd_ordering_geometric(int n, const int* Ap, const int* Ai, const float* xyz, const int* type,
+                                       std::vector<int>* blocks) {
+  std::vector<int> out, bl;
   out.reserve(n);
+  auto emit = [&](const std::vector<int>& v) {
+    if (v.empty()) return;
+    out.insert(out.end(), v.begin(), v.end());
+    bl.push_back((int)v.size());
+  };
   struct Item { std::vector<int> v; int phase; std::vector<int> sep; };
   std::vector<Item> stack;
   {
@@ -170,9 +183,9 @@ std::vector<int> nd_ordering_geometric(int n, const int* Ap, const int* Ai, cons
   while (!stack.empty()) {
     Item it = std::move(stack.back());
     stack.pop_back();
-    if (it.phase == 1) { out.insert(out.end(), it.sep.begin(), it.sep.end()); continue; }
+    if (it.phase == 1) { emit(it.sep); continue; }
     std::vector<int>& vs = it.v;
-    if ((int)vs.size() <= LEAF) { out.insert(out.end(), vs.begin(), vs.end()); continue; }
+    if ((int)vs.size() <= LEAF) { emit(vs); continue; }
     float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
     for (int v : vs)
       for (int d = 0; d < 3; ++d) { lo[d] = std::min(lo[d], xyz[3 * v + d]); hi[d] = std::max(hi[d], xyz[3 * v + d]); }
@@ -204,9 +217,11 @@ std::vector<int> nd_ordering_geometric(int n, const int* Ap, const int* Ai, cons
       ND nd(g);
       nd.run(vs);
       out.insert(out.end(), nd.out.begin(), nd.out.end());
+      bl.insert(bl.end(), nd.blocks.begin(), nd.blocks.end());
     }
   }
   if ((int)out.size() != n) throw std::runtime_error("geometric dissection lost vertices");
+  if (blocks) *blocks = bl;
   return out;
 }
 

@@ -16,11 +16,14 @@ struct LdlFactor {
 
 // Nested-dissection ordering of the graph of a symmetric pattern (CSR/CSC,
 // diagonal entries allowed).  Returns perm (new -> old).
-std::vector<int> nd_ordering(int n, const int* Ap, const int* Ai);
+// blocks (optional) receives the sizes of the consecutive blocks of the
+// ordering (leaves and separators, in elimination order).
+std::vector<int> nd_ordering(int n, const int* Ap, const int* Ai, std::vector<int>* blocks = nullptr);
 // Nested dissection by panel planes for node graphs of voxel grids: xyz (3n)
 // are face-centre coordinates in voxel units, type[i] the axis the face is
 // normal to.
-std::vector<int> nd_ordering_geometric(int n, const int* Ap, const int* Ai, const float* xyz, const int* type);
+std::vector<int> nd_ordering_geometric(int n, const int* Ap, const int* Ai, const float* xyz, const int* type,
+                                       std::vector<int>* blocks = nullptr);
 // Symbolic factorisation of the full symmetric pattern under perm.
 void ldl_symbolic(int n, const int* Ap, const int* Ai, const std::vector<int>& perm, LdlFactor& F);
 // Numeric factorisation (same pattern as the symbolic call).

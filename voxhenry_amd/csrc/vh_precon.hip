@@ -40,64 +40,98 @@ __global__ void k_pc_gather(const cplx<R>* __restrict__ x, int64_t ldx, cplx<R>*
   w[t] = s;
 }
 
-// One wide level: rows[0..nrows), thread per (row, rhs).
-// forward (BWD = 0): w_i -= sum_j L(i,j) w_j            (CSR of L)
-// backward (BWD = 1): w_i = w_i / D_i - sum_k L(k,i) w_k (CSC of L = CSR of L')
-template <typename R, int BWD>
-__global__ void k_trsv_level(const int* __restrict__ rows, int nrows, const int* __restrict__ ptr,
-                             const int* __restrict__ idx, const R* __restrict__ val, const R* __restrict__ D,
-                             cplx<R>* __restrict__ w, int nrhs) {
+// Supernodal solves with inverted diagonal blocks.  Block k owns rows and
+// columns [c0, c1).  Forward (L y = b), blocks in increasing level order:
+//   t_r = b_r - sum_{j < c0} L(r,j) y_j   (k_sn_update_fwd, rows of the level)
+//   y_k = Linv_k t_k                      (k_sn_gemv_fwd)
+// Backward (L' x = D^-1 y), blocks in decreasing level order:
+//   u_r = y_r / D_r - sum_{i >= c1} L(i,r) x_i   (k_sn_update_bwd)
+//   x_k = Linv_k' u_k                             (k_sn_gemv_bwd)
+// w holds b, t and finally x; w2 holds y then u.  Vectors are [row][rhs].
+template <typename R>
+__global__ void k_sn_update_fwd(const int* __restrict__ rows, int nrows, const int* __restrict__ rp,
+                                const int* __restrict__ split, const int* __restrict__ ri, const R* __restrict__ rx,
+                                cplx<R>* __restrict__ w, const cplx<R>* __restrict__ w2, int nrhs) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (int64_t)nrows * nrhs) return;
-  const int q = (int)(t / nrhs), r = (int)(t - (int64_t)q * nrhs);
-  const int i = rows[q];
-  cplx<R> s = w[(int64_t)i * nrhs + r];
-  if (BWD) { const R inv = R(1) / D[i]; s.x *= inv; s.y *= inv; }
-  for (int p = ptr[i]; p < ptr[i + 1]; ++p) {
-    const R l = val[p];
-    const cplx<R> v = w[(int64_t)idx[p] * nrhs + r];
+  const int q = (int)(t / nrhs), c = (int)(t - (int64_t)q * nrhs);
+  const int r = rows[q];
+  cplx<R> s = w[(int64_t)r * nrhs + c];
+  for (int p = rp[r]; p < split[r]; ++p) {
+    const R l = rx[p];
+    const cplx<R> v = w2[(int64_t)ri[p] * nrhs + c];
     s.x -= l * v.x; s.y -= l * v.y;
   }
-  w[(int64_t)i * nrhs + r] = s;
+  w[(int64_t)r * nrhs + c] = s;
 }
 
-// A run of narrow levels [l0, l1): block b owns right-hand sides
-// [b*rpb, min(nrhs,(b+1)*rpb)); one wave per (row, rhs) item with the row's
-// entries split over the 64 lanes.
-template <typename R, int BWD>
-__global__ __launch_bounds__(256) void k_trsv_chain(const int* __restrict__ lptr, const int* __restrict__ rows, int l0,
-                                                    int l1, const int* __restrict__ ptr, const int* __restrict__ idx,
-                                                    const R* __restrict__ val, const R* __restrict__ D,
-                                                    cplx<R>* __restrict__ w, int nrhs, int rpb) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-  const int r0 = blockIdx.x * rpb;
-  const int nr = min(rpb, nrhs - r0);
-  for (int l = l0; l < l1; ++l) {
-    const int a = lptr[l], b = lptr[l + 1];
-    const int items = (b - a) * nr;
-    for (int it = wave; it < items; it += nwave) {
-      const int q = it / nr, r = r0 + (it - q * nr);
-      const int i = rows[a + q];
-      R sx = 0, sy = 0;
-      for (int p = ptr[i] + lane; p < ptr[i + 1]; p += 64) {
-        const R lv = val[p];
-        const cplx<R> v = w[(int64_t)idx[p] * nrhs + r];
-        sx += lv * v.x; sy += lv * v.y;
-      }
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) {
-        sx += __shfl_xor(sx, o, 64);
-        sy += __shfl_xor(sy, o, 64);
-      }
-      if (lane == 0) {
-        cplx<R> s = w[(int64_t)i * nrhs + r];
-        if (BWD) { const R inv = R(1) / D[i]; s.x *= inv; s.y *= inv; }
-        s.x -= sx; s.y -= sy;
-        w[(int64_t)i * nrhs + r] = s;
-      }
-    }
-    __syncthreads();
+template <typename R>
+__global__ void k_sn_gemv_fwd(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of_row,
+                              const int* __restrict__ sn_ptr, const int64_t* __restrict__ sn_off,
+                              const R* __restrict__ Li, const cplx<R>* __restrict__ w, cplx<R>* __restrict__ w2,
+                              int nrhs) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nrows * nrhs) return;
+  const int q = (int)(t / nrhs), c = (int)(t - (int64_t)q * nrhs);
+  const int r = rows[q];
+  const int k = sn_of_row[r];
+  const int c0 = sn_ptr[k];
+  const int i = r - c0;
+  const R* row = Li + sn_off[k] + (int64_t)i * (i + 1) / 2;
+  const cplx<R>* tt = w + (int64_t)c0 * nrhs + c;
+  R sx = 0, sy = 0;
+  for (int j = 0; j <= i; ++j) {
+    const R l = row[j];
+    const cplx<R> v = tt[(int64_t)j * nrhs];
+    sx += l * v.x; sy += l * v.y;
   }
+  cplx<R> o; o.x = sx; o.y = sy;
+  w2[(int64_t)r * nrhs + c] = o;
+}
+
+template <typename R>
+__global__ void k_sn_update_bwd(const int* __restrict__ rows, int nrows, const int* __restrict__ cp,
+                                const int* __restrict__ split, const int* __restrict__ ci, const R* __restrict__ cx,
+                                const R* __restrict__ D, const cplx<R>* __restrict__ w, cplx<R>* __restrict__ w2,
+                                int nrhs) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nrows * nrhs) return;
+  const int q = (int)(t / nrhs), c = (int)(t - (int64_t)q * nrhs);
+  const int r = rows[q];
+  cplx<R> s = w2[(int64_t)r * nrhs + c];
+  const R inv = R(1) / D[r];
+  s.x *= inv; s.y *= inv;
+  for (int p = split[r]; p < cp[r + 1]; ++p) {
+    const R l = cx[p];
+    const cplx<R> v = w[(int64_t)ci[p] * nrhs + c];
+    s.x -= l * v.x; s.y -= l * v.y;
+  }
+  w2[(int64_t)r * nrhs + c] = s;
+}
+
+template <typename R>
+__global__ void k_sn_gemv_bwd(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of_row,
+                              const int* __restrict__ sn_ptr, const int64_t* __restrict__ sn_off,
+                              const R* __restrict__ LiT, const cplx<R>* __restrict__ w2, cplx<R>* __restrict__ w,
+                              int nrhs) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)nrows * nrhs) return;
+  const int q = (int)(t / nrhs), c = (int)(t - (int64_t)q * nrhs);
+  const int r = rows[q];
+  const int k = sn_of_row[r];
+  const int c0 = sn_ptr[k], sz = sn_ptr[k + 1] - c0;
+  const int i = r - c0;
+  // row i of the transpose holds Linv(i..sz-1, i): offset i*sz - i(i-1)/2
+  const R* row = LiT + sn_off[k] + (int64_t)i * sz - (int64_t)i * (i - 1) / 2;
+  const cplx<R>* uu = w2 + (int64_t)r * nrhs + c;
+  R sx = 0, sy = 0;
+  for (int j = 0; j < sz - i; ++j) {
+    const R l = row[j];
+    const cplx<R> v = uu[(int64_t)j * nrhs];
+    sx += l * v.x; sy += l * v.y;
+  }
+  cplx<R> o; o.x = sx; o.y = sy;
+  w[(int64_t)r * nrhs + c] = o;
 }
 
 // y_r[nc + perm[i]] = w[i][r]
@@ -142,18 +176,6 @@ template <typename T> static void upload_f32(DevBuf<float>& d, const std::vector
   upload(d, f);
 }
 
-static void build_schedule(const std::vector<int>& lev, std::vector<int>& lptr, std::vector<int>& rows) {
-  const int n = (int)lev.size();
-  int nl = 0;
-  for (int l : lev) nl = std::max(nl, l + 1);
-  lptr.assign(nl + 1, 0);
-  for (int l : lev) lptr[l + 1]++;
-  for (int l = 0; l < nl; ++l) lptr[l + 1] += lptr[l];
-  rows.assign(n, 0);
-  std::vector<int> fill(lptr.begin(), lptr.end() - 1);
-  for (int i = 0; i < n; ++i) rows[fill[lev[i]]++] = i;
-}
-
 static void build_symbolic(Geometry& G, bool geometric, PreconSymbolic& Y) {
   const int nv = G.g.nv;
   const int nn = (int)G.num_node;
@@ -195,14 +217,30 @@ static void build_symbolic(Geometry& G, bool geometric, PreconSymbolic& Y) {
     }
   }
   for (int i = 0; i < nn && geometric; ++i) geometric = type[i] >= 0;
-  std::vector<int> perm = geometric ? nd_ordering_geometric(nn, Y.Sp.data(), Y.Si.data(), xyz.data(), type.data())
-                                    : nd_ordering(nn, Y.Sp.data(), Y.Si.data());
+  std::vector<int> perm = geometric
+      ? nd_ordering_geometric(nn, Y.Sp.data(), Y.Si.data(), xyz.data(), type.data(), &Y.blocks)
+      : nd_ordering(nn, Y.Sp.data(), Y.Si.data(), &Y.blocks);
   ldl_symbolic(nn, Y.Sp.data(), Y.Si.data(), perm, Y.F);
   Y.nnzL = (int64_t)Y.F.Lp[nn];
 }
 
-// CSR copy of the L pattern, CSC->CSR position map and level schedules
-// (needs the row indices of L, i.e. one numeric factorisation).
+// Rows of the blocks of each level, ordered by level.
+static void level_rows(const std::vector<int>& lev, const std::vector<int>& sn_ptr, std::vector<int>& lptr,
+                       std::vector<int>& rows) {
+  const int nsn = (int)lev.size();
+  int nl = 0;
+  for (int l : lev) nl = std::max(nl, l + 1);
+  lptr.assign(nl + 1, 0);
+  for (int k = 0; k < nsn; ++k) lptr[lev[k] + 1] += sn_ptr[k + 1] - sn_ptr[k];
+  for (int l = 0; l < nl; ++l) lptr[l + 1] += lptr[l];
+  rows.assign(lptr[nl], 0);
+  std::vector<int> fill(lptr.begin(), lptr.end() - 1);
+  for (int k = 0; k < nsn; ++k)
+    for (int r = sn_ptr[k]; r < sn_ptr[k + 1]; ++r) rows[fill[lev[k]]++] = r;
+}
+
+// CSR copy of the L pattern, CSC->CSR map, blocks, their levels and the
+// packed offsets of the dense inverses (needs the row indices of L).
 static void build_levels(Geometry& G, const LdlFactor& F, PreconSymbolic& Y) {
   const int n = F.n;
   std::vector<int> Rp(n + 1, 0);
@@ -218,32 +256,103 @@ static void build_levels(Geometry& G, const LdlFactor& F, PreconSymbolic& Y) {
       Ri[fill[i]] = j;
       Y.csc2csr[p] = fill[i]++;
     }
-  std::vector<int> lev_f(n, 0), lev_b(n, 0);
-  for (int i = 0; i < n; ++i) {
-    int l = 0;
-    for (int p = Rp[i]; p < Rp[i + 1]; ++p) l = std::max(l, lev_f[Ri[p]] + 1);
-    lev_f[i] = l;
+  // blocks
+  const int nsn = (int)Y.blocks.size();
+  Y.sn_ptr.assign(nsn + 1, 0);
+  for (int k = 0; k < nsn; ++k) Y.sn_ptr[k + 1] = Y.sn_ptr[k] + Y.blocks[k];
+  VH_REQUIRE(Y.sn_ptr[nsn] == n, "dissection blocks do not cover the Schur matrix");
+  Y.sn_of_row.assign(n, 0);
+  for (int k = 0; k < nsn; ++k)
+    for (int r = Y.sn_ptr[k]; r < Y.sn_ptr[k + 1]; ++r) Y.sn_of_row[r] = k;
+  Y.sn_off.assign(nsn + 1, 0);
+  for (int k = 0; k < nsn; ++k) {
+    const int64_t sz = Y.blocks[k];
+    Y.sn_off[k + 1] = Y.sn_off[k] + sz * (sz + 1) / 2;
   }
-  for (int i = n - 1; i >= 0; --i) {
-    int l = 0;
-    for (int p = F.Lp[i]; p < F.Lp[i + 1]; ++p) l = std::max(l, lev_b[F.Li[p]] + 1);
-    lev_b[i] = l;
+  Y.dense = Y.sn_off[nsn];
+  // splits: CSR row r entries with column < c0 come first; CSC column r entries
+  // with row >= c1 come last (indices are sorted)
+  Y.row_split.assign(n, 0);
+  Y.col_split.assign(n, 0);
+  for (int r = 0; r < n; ++r) {
+    const int k = Y.sn_of_row[r];
+    const int c0 = Y.sn_ptr[k], c1 = Y.sn_ptr[k + 1];
+    int p = Rp[r];
+    while (p < Rp[r + 1] && Ri[p] < c0) ++p;
+    Y.row_split[r] = p;
+    int q = F.Lp[r];
+    while (q < F.Lp[r + 1] && F.Li[q] < c1) ++q;
+    Y.col_split[r] = q;
   }
-  build_schedule(lev_f, Y.fl_ptr, Y.fl_rows);
-  build_schedule(lev_b, Y.bl_ptr, Y.bl_rows);
-  upload(Y.d_fl_rows, Y.fl_rows);
-  upload(Y.d_bl_rows, Y.bl_rows);
-  upload(Y.d_fl_ptr, Y.fl_ptr);
-  upload(Y.d_bl_ptr, Y.bl_ptr);
+  // block levels: forward from the blocks a block's rows reach, backward from
+  // the blocks its columns reach
+  std::vector<int> lf(nsn, 0), lb(nsn, 0);
+  for (int k = 0; k < nsn; ++k) {
+    int l = 0;
+    for (int r = Y.sn_ptr[k]; r < Y.sn_ptr[k + 1]; ++r)
+      for (int p = Rp[r]; p < Y.row_split[r]; ++p) l = std::max(l, lf[Y.sn_of_row[Ri[p]]] + 1);
+    lf[k] = l;
+  }
+  for (int k = nsn - 1; k >= 0; --k) {
+    int l = 0;
+    for (int r = Y.sn_ptr[k]; r < Y.sn_ptr[k + 1]; ++r)
+      for (int q = Y.col_split[r]; q < F.Lp[r + 1]; ++q) l = std::max(l, lb[Y.sn_of_row[F.Li[q]]] + 1);
+    lb[k] = l;
+  }
+  level_rows(lf, Y.sn_ptr, Y.f_lptr, Y.f_rows);
+  level_rows(lb, Y.sn_ptr, Y.b_lptr, Y.b_rows);
   upload(Y.d_perm, F.perm);
   upload(Y.Rp, Rp);
   upload(Y.Ri, Ri);
   upload(Y.Cp, F.Lp);
   upload(Y.Ci, F.Li);
+  upload(Y.d_sn_ptr, Y.sn_ptr);
+  upload(Y.d_sn_of_row, Y.sn_of_row);
+  upload(Y.d_row_split, Y.row_split);
+  upload(Y.d_col_split, Y.col_split);
+  upload(Y.d_f_rows, Y.f_rows);
+  upload(Y.d_b_rows, Y.b_rows);
+  upload(Y.d_sn_off, Y.sn_off);
   // Ar' with permuted node columns
   std::vector<int> tcp(G.atc_h.size());
   for (size_t q = 0; q < G.atc_h.size(); ++q) tcp[q] = F.iperm[G.atc_h[q]];
   upload(Y.ArTp_col, tcp);
+}
+
+// Dense inverses of the unit lower diagonal blocks of L (packed row-major
+// lower, and the transposes packed row-major upper).
+static void block_inverses(const PreconSymbolic& Y, const LdlFactor& F, std::vector<double>& Li,
+                           std::vector<double>& LiT) {
+  const int nsn = (int)Y.blocks.size();
+  Li.assign(Y.dense, 0.0);
+  LiT.assign(Y.dense, 0.0);
+  std::vector<double> B, X;
+  for (int k = 0; k < nsn; ++k) {
+    const int c0 = Y.sn_ptr[k], sz = Y.blocks[k];
+    B.assign((size_t)sz * sz, 0.0);
+    for (int j = 0; j < sz; ++j) {
+      B[(size_t)j * sz + j] = 1.0;
+      for (int q = F.Lp[c0 + j]; q < Y.col_split[c0 + j]; ++q) B[(size_t)(F.Li[q] - c0) * sz + j] = F.Lx[q];
+    }
+    // X = B^-1 by forward substitution, column by column (B unit lower)
+    X.assign((size_t)sz * sz, 0.0);
+    for (int c = 0; c < sz; ++c) {
+      X[(size_t)c * sz + c] = 1.0;
+      for (int r = c + 1; r < sz; ++r) {
+        double acc = 0.0;
+        const double* br = &B[(size_t)r * sz];
+        for (int m = c; m < r; ++m) acc += br[m] * X[(size_t)m * sz + c];
+        X[(size_t)r * sz + c] = -acc;
+      }
+    }
+    const int64_t off = Y.sn_off[k];
+    for (int r = 0; r < sz; ++r)
+      for (int c = 0; c <= r; ++c) Li[off + (int64_t)r * (r + 1) / 2 + c] = X[(size_t)r * sz + c];
+    for (int c = 0; c < sz; ++c) {
+      const int64_t base = off + (int64_t)c * sz - (int64_t)c * (c - 1) / 2;
+      for (int r = c; r < sz; ++r) LiT[base + (r - c)] = X[(size_t)r * sz + c];
+    }
+  }
 }
 
 void precon_prepare(Precon& P, double jre, double jim) {
@@ -303,50 +412,18 @@ void precon_prepare(Precon& P, double jre, double jim) {
   }
   std::vector<double> Rx(Y.nnzL);
   for (int64_t p = 0; p < Y.nnzL; ++p) Rx[Y.csc2csr[p]] = P.F.Lx[p];
+  std::vector<double> Li, LiT;
+  block_inverses(Y, P.F, Li, LiT);
   upload(P.Rx64, Rx); upload_f32(P.Rx32, Rx);
   upload(P.Cx64, P.F.Lx); upload_f32(P.Cx32, P.F.Lx);
   upload(P.D64, P.F.D); upload_f32(P.D32, P.F.D);
   upload(P.ainv64, ainv); upload_f32(P.ainv32, ainv);
+  upload(P.Li64, Li); upload_f32(P.Li32, Li);
+  upload(P.LiT64, LiT); upload_f32(P.LiT32, LiT);
   // the values are on the device; the host copies of L are not needed any more
   P.F.Li.clear(); P.F.Li.shrink_to_fit();
   P.F.Lx.clear(); P.F.Lx.shrink_to_fit();
   P.ready = true;
-}
-
-template <typename R>
-static void run_levels(Precon& P, const std::vector<int>& lptr, const DevBuf<int>& dptr, const DevBuf<int>& drows,
-                       const int* ptr, const int* idx, const R* val, const R* D, cplx<R>* w, int nrhs, bool bwd,
-                       hipStream_t st) {
-  const int nl = (int)lptr.size() - 1;
-  const int64_t wide = P.wide_items;
-  int l = 0;
-  while (l < nl) {
-    const int64_t items = (int64_t)(lptr[l + 1] - lptr[l]) * nrhs;
-    if (items >= wide) {
-      const int64_t nb = ceil_div(items, 256);
-      if (bwd)
-        hipLaunchKernelGGL((k_trsv_level<R, 1>), dim3(nb), dim3(256), 0, st, drows.p + lptr[l], lptr[l + 1] - lptr[l],
-                           ptr, idx, val, D, w, nrhs);
-      else
-        hipLaunchKernelGGL((k_trsv_level<R, 0>), dim3(nb), dim3(256), 0, st, drows.p + lptr[l], lptr[l + 1] - lptr[l],
-                           ptr, idx, val, D, w, nrhs);
-      VH_LAUNCHED();
-      ++l;
-    } else {
-      int l1 = l + 1;
-      while (l1 < nl && (int64_t)(lptr[l1 + 1] - lptr[l1]) * nrhs < wide) ++l1;
-      const int rpb = std::min(nrhs, P.chain_rpb);
-      const int nb = (int)ceil_div(nrhs, rpb);
-      if (bwd)
-        hipLaunchKernelGGL((k_trsv_chain<R, 1>), dim3(nb), dim3(256), 0, st, dptr.p, drows.p, l, l1, ptr, idx, val, D,
-                           w, nrhs, rpb);
-      else
-        hipLaunchKernelGGL((k_trsv_chain<R, 0>), dim3(nb), dim3(256), 0, st, dptr.p, drows.p, l, l1, ptr, idx, val, D,
-                           w, nrhs, rpb);
-      VH_LAUNCHED();
-      l = l1;
-    }
-  }
 }
 
 template <typename R>
@@ -356,30 +433,55 @@ void precon_apply(Precon& P, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y
   hipStream_t st = G.ctx->stream;
   const int nn = (int)G.num_node;
   const int64_t nc = G.num_curr;
-  const size_t need = (size_t)nn * nrhs * sizeof(cplx<R>);
+  const size_t need = 2 * (size_t)nn * nrhs * sizeof(cplx<R>);
   if (P.ws.n < need) P.ws.alloc(need);
   cplx<R>* w = (cplx<R>*)P.ws.p;
+  cplx<R>* w2 = w + (size_t)nn * nrhs;
   const bool d = sizeof(R) == 8;
   const R* arv = d ? (const R*)G.Ar.v64.p : (const R*)G.Ar.v32.p;
   const R* ainv = d ? (const R*)P.ainv64.p : (const R*)P.ainv32.p;
   if (nn > 0) {
+    PreconSymbolic& Y = *P.sym;
     const int64_t items = (int64_t)nn * nrhs;
     hipLaunchKernelGGL((k_pc_gather<R>), dim3(ceil_div(items, 256)), dim3(256), 0, st, x, ldx, w, nrhs, nn,
-                       P.sym->d_perm.p, G.Ar.ptr.p, G.Ar.col.p, arv, ainv, nc, slots);
+                       Y.d_perm.p, G.Ar.ptr.p, G.Ar.col.p, arv, ainv, nc, slots);
     VH_LAUNCHED();
     const R* Rx = d ? (const R*)P.Rx64.p : (const R*)P.Rx32.p;
     const R* Cx = d ? (const R*)P.Cx64.p : (const R*)P.Cx32.p;
     const R* D = d ? (const R*)P.D64.p : (const R*)P.D32.p;
-    PreconSymbolic& Y = *P.sym;
-    run_levels<R>(P, Y.fl_ptr, Y.d_fl_ptr, Y.d_fl_rows, Y.Rp.p, Y.Ri.p, Rx, D, w, nrhs, false, st);
-    run_levels<R>(P, Y.bl_ptr, Y.d_bl_ptr, Y.d_bl_rows, Y.Cp.p, Y.Ci.p, Cx, D, w, nrhs, true, st);
+    const R* Li = d ? (const R*)P.Li64.p : (const R*)P.Li32.p;
+    const R* LiT = d ? (const R*)P.LiT64.p : (const R*)P.LiT32.p;
+    const int nlf = (int)Y.f_lptr.size() - 1;
+    for (int l = 0; l < nlf; ++l) {
+      const int a = Y.f_lptr[l], nr = Y.f_lptr[l + 1] - a;
+      const dim3 grid(ceil_div((int64_t)nr * nrhs, 256));
+      if (l > 0) {
+        hipLaunchKernelGGL((k_sn_update_fwd<R>), grid, dim3(256), 0, st, Y.d_f_rows.p + a, nr, Y.Rp.p,
+                           Y.d_row_split.p, Y.Ri.p, Rx, w, w2, nrhs);
+        VH_LAUNCHED();
+      }
+      hipLaunchKernelGGL((k_sn_gemv_fwd<R>), grid, dim3(256), 0, st, Y.d_f_rows.p + a, nr, Y.d_sn_of_row.p,
+                         Y.d_sn_ptr.p, Y.d_sn_off.p, Li, w, w2, nrhs);
+      VH_LAUNCHED();
+    }
+    const int nlb = (int)Y.b_lptr.size() - 1;
+    for (int l = 0; l < nlb; ++l) {
+      const int a = Y.b_lptr[l], nr = Y.b_lptr[l + 1] - a;
+      const dim3 grid(ceil_div((int64_t)nr * nrhs, 256));
+      hipLaunchKernelGGL((k_sn_update_bwd<R>), grid, dim3(256), 0, st, Y.d_b_rows.p + a, nr, Y.Cp.p,
+                         Y.d_col_split.p, Y.Ci.p, Cx, D, w, w2, nrhs);
+      VH_LAUNCHED();
+      hipLaunchKernelGGL((k_sn_gemv_bwd<R>), grid, dim3(256), 0, st, Y.d_b_rows.p + a, nr, Y.d_sn_of_row.p,
+                         Y.d_sn_ptr.p, Y.d_sn_off.p, LiT, w2, w, nrhs);
+      VH_LAUNCHED();
+    }
     hipLaunchKernelGGL((k_pc_scatter_nodes<R>), dim3(ceil_div(items, 256)), dim3(256), 0, st, w, y, ldy, nrhs, nn,
-                       P.sym->d_perm.p, nc, slots);
+                       Y.d_perm.p, nc, slots);
     VH_LAUNCHED();
   }
   const R* atv = d ? (const R*)G.ArT.v64.p : (const R*)G.ArT.v32.p;
   hipLaunchKernelGGL((k_pc_currents<R>), dim3(ceil_div(nc, 256), nrhs), dim3(256), 0, st, x, ldx, w, y, ldy, nrhs, nc,
-                     G.ArT.ptr.p, P.sym->ArTp_col.p, atv, ainv, slots);
+                     G.ArT.ptr.p, nn > 0 ? P.sym->ArTp_col.p : nullptr, atv, ainv, slots);
   VH_LAUNCHED();
 }
 

@@ -13,10 +13,17 @@ struct PreconSymbolic {
   std::vector<int> Sp, Si;            // pattern of S (full symmetric CSR)
   LdlFactor F;                        // perm/iperm/parent/Lp/Li (values unused)
   std::vector<int> csc2csr;           // position of each CSC entry of L in its CSR copy
-  std::vector<int> fl_ptr, fl_rows, bl_ptr, bl_rows;
   int64_t nnzL = 0;
-  DevBuf<int> d_perm, d_fl_rows, d_bl_rows, d_fl_ptr, d_bl_ptr;
-  DevBuf<int> Rp, Ri, Cp, Ci, ArTp_col;
+  // supernodes = the dissection blocks: columns [sn_ptr[k], sn_ptr[k+1])
+  std::vector<int> blocks, sn_ptr, sn_of_row;
+  std::vector<int64_t> sn_off;        // offset of block k in the packed dense inverses
+  int64_t dense = 0;                  // total packed entries, sum s(s+1)/2
+  std::vector<int> row_split;         // CSR: first entry of row r in its own block's columns
+  std::vector<int> col_split;         // CSC: first entry of column r below its own block
+  std::vector<int> f_lptr, f_rows, b_lptr, b_rows;   // rows of the blocks of each level
+  DevBuf<int> d_perm, Rp, Ri, Cp, Ci, ArTp_col;
+  DevBuf<int> d_sn_ptr, d_sn_of_row, d_row_split, d_col_split, d_f_rows, d_b_rows;
+  DevBuf<int64_t> d_sn_off;
 };
 
 struct Precon {
@@ -26,12 +33,10 @@ struct Precon {
   std::shared_ptr<PreconSymbolic> sym;
   LdlFactor F;                        // numeric values (Lx, D) of this frequency
   std::vector<double> ainv_h;
-  // tuning
-  int64_t wide_items = 2048;          // level launched on the whole grid above this many (row, rhs) items
-  int chain_rpb = 16;                 // right-hand sides per workgroup in narrow-level chains
-  // device values
-  DevBuf<double> Rx64, Cx64, D64, ainv64;
-  DevBuf<float> Rx32, Cx32, D32, ainv32;
+  // device values: L (CSR and CSC), D, A_inv, inverses of the diagonal blocks
+  // (row-major lower: Linv; the transpose row-major: LinvT)
+  DevBuf<double> Rx64, Cx64, D64, ainv64, Li64, LiT64;
+  DevBuf<float> Rx32, Cx32, D32, ainv32, Li32, LiT32;
   DevBuf<char> ws;
 };
 
