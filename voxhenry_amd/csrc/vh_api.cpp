@@ -561,37 +561,64 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
                    (double)vh::KrylovWork<float>::bytes_needed(n, inner_restart, 1, false) + 2.0 * n * 8;
     per_sys += 5.0 * n * 16;   // matvec workspaces (A + B grids are ~4-5 vectors per system)
     int chunk = (int)std::max(1.0, std::min((double)nsys, std::floor(budget / per_sys)));
-    std::vector<vh::SysStats> stats;
+    std::vector<vh::SysStats> stats(nsys);
     vh::KrylovWork<double> W64;
     vh::KrylovWork<float> W32;
+    // each chunk is solved with chunk-relative slots 0..cnt-1 (workspaces sized
+    // for the chunk); coff is the chunk's first system
+    int64_t coff = 0;
     vh::OpFn<double> A64 = [&](const std::vector<int>& act, const int* sl, const double2* xx, double2* yy) {
-      vh::matvec<double>(G, (int)act.size(), xx, n, yy, n, 0.0, 0.0, 0, sl, alphas.p);
+      vh::matvec<double>(G, (int)act.size(), xx, n, yy, n, 0.0, 0.0, 0, sl, alphas.p + coff);
     };
     vh::OpFn<float> A32 = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
-      vh::matvec<float>(G, (int)act.size(), xx, n, yy, n, 0.0, 0.0, 0, sl, alphas.p);
+      vh::matvec<float>(G, (int)act.size(), xx, n, yy, n, 0.0, 0.0, 0, sl, alphas.p + coff);
+    };
+    auto grouped_c = [&](const std::vector<int>& act, auto apply) {
+      std::vector<int> shifted(act.size());
+      for (size_t i = 0; i < act.size(); ++i) shifted[i] = act[i];
+      std::vector<std::vector<int>> by(nprecon);
+      for (int s : shifted) by[pidx[coff + s]].push_back(s);
+      int off = 0;
+      for (int64_t k = 0; k < nprecon; ++k) {
+        if (by[k].empty()) continue;
+        std::copy(by[k].begin(), by[k].end(), gbuf.begin() + off);
+        off += (int)by[k].size();
+      }
+      VH_HIP(hipMemcpyAsync(gslots.p, gbuf.data(), off * sizeof(int), hipMemcpyHostToDevice, st));
+      off = 0;
+      for (int64_t k = 0; k < nprecon; ++k) {
+        if (by[k].empty()) continue;
+        apply(precons[k]->P, (int)by[k].size(), gslots.p + off);
+        off += (int)by[k].size();
+      }
+      VH_HIP(hipStreamSynchronize(st));   // gbuf is reused by the next call
     };
     vh::OpFn<double> M64 = [&](const std::vector<int>& act, const int*, const double2* xx, double2* yy) {
-      grouped(act, [&](vh::Precon& P, int cnt, const int* sl) { vh::precon_apply<double>(P, cnt, xx, n, yy, n, sl); });
+      grouped_c(act, [&](vh::Precon& P, int cnt, const int* sl) { vh::precon_apply<double>(P, cnt, xx, n, yy, n, sl); });
     };
     vh::OpFn<float> M32 = [&](const std::vector<int>& act, const int*, const float2* xx, float2* yy) {
-      grouped(act, [&](vh::Precon& P, int cnt, const int* sl) { vh::precon_apply<float>(P, cnt, xx, n, yy, n, sl); });
+      grouped_c(act, [&](vh::Precon& P, int cnt, const int* sl) { vh::precon_apply<float>(P, cnt, xx, n, yy, n, sl); });
     };
-    stats.resize(nsys);
+    (void)grouped;
     for (int64_t s0 = 0; s0 < nsys; s0 += chunk) {
-      std::vector<int> sys;
-      for (int64_t s = s0; s < std::min<int64_t>(nsys, s0 + chunk); ++s) sys.push_back((int)s);
+      coff = s0;
+      const int cnt = (int)std::min<int64_t>(chunk, nsys - s0);
+      std::vector<int> sys(cnt);
+      for (int i = 0; i < cnt; ++i) sys[i] = i;
       std::vector<vh::SysStats> cs;
+      const double2* bc = db + s0 * n;
+      double2* xc = dx + s0 * n;
       if (method == 0) {
         vh::SolveOpts o;
         o.restart = restart; o.maxit = maxit; o.tol = tol; o.final_residual = true; o.flexible = false;
-        vh::gmres_batch<double>(G.ctx, n, A64, &M64, db, dx, sys, o, W64, cs);
+        vh::gmres_batch<double>(G.ctx, n, A64, &M64, bc, xc, sys, o, W64, cs);
       } else {
         vh::MixedOpts o;
         o.tol = tol; o.outer_restart = restart; o.outer_maxit = maxit;
         o.inner_restart = inner_restart; o.inner_tol = inner_tol;
-        vh::fgmres_mixed(G.ctx, n, A64, A32, M32, db, dx, sys, o, W64, W32, cs);
+        vh::fgmres_mixed(G.ctx, n, A64, A32, M32, bc, xc, sys, o, W64, W32, cs);
       }
-      for (int s : sys) stats[s] = cs[s];
+      for (int i = 0; i < cnt; ++i) stats[s0 + i] = cs[i];
     }
     if (!on_device) VH_HIP(hipMemcpyAsync(x, io.y, (size_t)nsys * n * 16, hipMemcpyDeviceToHost, st));
     VH_HIP(hipStreamSynchronize(st));
