@@ -155,3 +155,40 @@ def test_golden_solution_residual(ctx):
     relres = np.linalg.norm(r) / np.linalg.norm(b)
     say('golden relres', relres)
     assert 1e-10 < relres < 3e-9
+
+
+# Grids whose padded lengths exercise every radix of the pass kernels (2, 3,
+# 4, 5, 7, 8, 16), one- to four-stage plans, long x lines (Px = 800 needs 64
+# threads per line) and sparse voxel sets (the voxel-map gather and scatter).
+RADIX_SHAPES = [((400, 4, 4), False), ((9, 33, 17), True), ((64, 3, 40), False), ((5, 6, 2), True),
+                ((128, 20, 3), True), ((16, 16, 16), False)]
+
+
+@pytest.mark.parametrize('dims,hollow', RADIX_SHAPES)
+def test_matvec_radix_shapes(ctx, dims, hollow):
+    import voxhenry_amd as vh
+    say('matvec shapes', dims, hollow)
+    L, M, N = dims
+    geo = problems.bar(L, M, N)
+    if hollow:
+        rng = np.random.default_rng(11)
+        mask = rng.random((L, M, N)) < 0.3
+        mask[0, :, :] = False
+        mask[-1, :, :] = False
+        geo['sigma'][mask] = 0.0
+    s = topology.build_system(geo)
+    f = 1e9
+    jwmu = 1j * 2 * np.pi * f * MU0
+    fN2, _ = green.circulant_tensor(L, M, N, geo['dx'])
+    G = problems.device_geometry(ctx, geo, s, pad_mode=1, f=f)
+    zr, zi = topology.impedance(geo, s['idxS'], f)
+    rng = np.random.default_rng(5)
+    n = s['num_curr'] + s['num_node']
+    X = rng.standard_normal((3, n)) + 1j * rng.standard_normal((3, n))
+    Y = G.matvec(X, jwmu, vh.VH_C128, 0)
+    Y32 = G.matvec(X.astype(np.complex64), jwmu, vh.VH_C64, 0)
+    for k in range(3):
+        ref = omv.lse_matvect_mult(X[k], jwmu * fN2, s['Ar'], zr, zi, geo['dx'], s['idxS5'])
+        assert rel(Y[k], ref) < TOL64, (k, rel(Y[k], ref))
+        assert rel(Y32[k].astype(complex), ref) < TOL32, (k, rel(Y32[k].astype(complex), ref))
+    G.close()

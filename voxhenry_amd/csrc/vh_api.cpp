@@ -529,28 +529,11 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
     // per-slot j*omega*mu on the device
     vh::DevBuf<double2> alphas(nsys);
     VH_HIP(hipMemcpyAsync(alphas.p, jwmu, nsys * sizeof(double2), hipMemcpyHostToDevice, st));
-    // group lists for the preconditioner apply
+    // group lists for the preconditioner apply (pinned staging, rewritten only
+    // after the stream synchronisation that ends each apply)
     vh::DevBuf<int> gslots(nsys);
-    std::vector<int> gbuf(nsys);
-    auto grouped = [&](const std::vector<int>& act, auto apply) {
-      // sort the active systems by preconditioner, apply each group with its own slot list
-      std::vector<std::vector<int>> by(nprecon);
-      for (int s : act) by[pidx[s]].push_back(s);
-      int off = 0;
-      for (int64_t k = 0; k < nprecon; ++k) {
-        if (by[k].empty()) continue;
-        std::copy(by[k].begin(), by[k].end(), gbuf.begin() + off);
-        off += (int)by[k].size();
-      }
-      VH_HIP(hipMemcpyAsync(gslots.p, gbuf.data(), off * sizeof(int), hipMemcpyHostToDevice, st));
-      off = 0;
-      for (int64_t k = 0; k < nprecon; ++k) {
-        if (by[k].empty()) continue;
-        apply(precons[k]->P, (int)by[k].size(), gslots.p + off);
-        off += (int)by[k].size();
-      }
-      VH_HIP(hipStreamSynchronize(st));   // gbuf is reused by the next call
-    };
+    vh::PinnedBuf<int> gbuf;
+    gbuf.alloc(nsys);
     // HBM budget -> chunk size
     size_t freeb = 0, totalb = 0;
     VH_HIP(hipMemGetInfo(&freeb, &totalb));
@@ -573,18 +556,18 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
     vh::OpFn<float> A32 = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
       vh::matvec<float>(G, (int)act.size(), xx, n, yy, n, 0.0, 0.0, 0, sl, alphas.p + coff);
     };
+    // apply the preconditioners to the active systems of the chunk (slots are
+    // chunk-relative), one group per preconditioner
     auto grouped_c = [&](const std::vector<int>& act, auto apply) {
-      std::vector<int> shifted(act.size());
-      for (size_t i = 0; i < act.size(); ++i) shifted[i] = act[i];
       std::vector<std::vector<int>> by(nprecon);
-      for (int s : shifted) by[pidx[coff + s]].push_back(s);
+      for (int s : act) by[pidx[coff + s]].push_back(s);
       int off = 0;
       for (int64_t k = 0; k < nprecon; ++k) {
         if (by[k].empty()) continue;
-        std::copy(by[k].begin(), by[k].end(), gbuf.begin() + off);
+        std::copy(by[k].begin(), by[k].end(), gbuf.p + off);
         off += (int)by[k].size();
       }
-      VH_HIP(hipMemcpyAsync(gslots.p, gbuf.data(), off * sizeof(int), hipMemcpyHostToDevice, st));
+      VH_HIP(hipMemcpyAsync(gslots.p, gbuf.p, off * sizeof(int), hipMemcpyHostToDevice, st));
       off = 0;
       for (int64_t k = 0; k < nprecon; ++k) {
         if (by[k].empty()) continue;
@@ -599,7 +582,6 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
     vh::OpFn<float> M32 = [&](const std::vector<int>& act, const int*, const float2* xx, float2* yy) {
       grouped_c(act, [&](vh::Precon& P, int cnt, const int* sl) { vh::precon_apply<float>(P, cnt, xx, n, yy, n, sl); });
     };
-    (void)grouped;
     for (int64_t s0 = 0; s0 < nsys; s0 += chunk) {
       coff = s0;
       const int cnt = (int)std::min<int64_t>(chunk, nsys - s0);

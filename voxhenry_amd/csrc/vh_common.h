@@ -9,6 +9,7 @@
 #include <stdexcept>
 #include <vector>
 #include <complex>
+#include <cstdlib>
 
 namespace vh {
 
@@ -60,6 +61,14 @@ template <typename C> __host__ __device__ inline C cconjmul(C a, C b) {  // conj
 }
 template <typename C, typename S> __host__ __device__ inline C cscale(C a, S s) { a.x *= s; a.y *= s; return a; }
 
+// Debug aid: with VH_POISON_ALLOC=1 in the environment every new device
+// allocation is filled with 0xFF bytes (NaN in floating point), so a read of
+// memory that was never written shows up in the results.
+inline bool poison_alloc() {
+  static const bool on = [] { const char* e = std::getenv("VH_POISON_ALLOC"); return e && e[0] == '1'; }();
+  return on;
+}
+
 // RAII device allocation.
 template <typename T> struct DevBuf {
   T* p = nullptr;
@@ -80,9 +89,41 @@ template <typename T> struct DevBuf {
       throw Error(ERR_ALLOC, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
     }
     n = count;
+    if (poison_alloc()) {
+      e = hipMemset(p, 0xFF, count * sizeof(T));
+      if (e != hipSuccess) throw Error(ERR_HIP, std::string("poison fill failed: ") + hipGetErrorString(e));
+      e = hipDeviceSynchronize();
+      if (e != hipSuccess) throw Error(ERR_HIP, std::string("poison fill failed: ") + hipGetErrorString(e));
+    }
   }
   void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
   size_t bytes() const { return n * sizeof(T); }
+};
+
+// RAII pinned (page-locked) host allocation for the small host<->device
+// transfers of the solver loops: asynchronous copies from and to it are
+// ordered on the stream and complete at the next stream synchronisation.
+template <typename T> struct PinnedBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+  ~PinnedBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    hipError_t e = hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error(ERR_ALLOC, "hipHostMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
+    }
+    n = count;
+  }
+  void ensure(size_t count) { if (n < count) alloc(count); }
+  void release() { if (p) (void)hipHostFree(p); p = nullptr; n = 0; }
+  T& operator[](size_t i) { return p[i]; }
+  const T& operator[](size_t i) const { return p[i]; }
 };
 
 struct Context {

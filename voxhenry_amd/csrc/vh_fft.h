@@ -93,6 +93,42 @@ template <int DIR, typename C> struct Dft<8, DIR, C> {
   }
 };
 
+// Radix 16 as 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2; DFT4 over n1, twiddle
+// w16^(n2 k1), DFT4 over n2.
+template <int DIR, typename C> struct Dft<16, DIR, C> {
+  __device__ __forceinline__ static void run(C* v) {
+    using R = decltype(v[0].x);
+    const R c1 = R(0.92387953251128675612818318939678829), s1 = R(0.38268343236508977172845998403039887);
+    const R h = R(0.70710678118654752440084436210484904);
+    C y[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      C t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+      Dft<4, DIR, C>::run(t);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) y[n2][k1] = t[k1];
+    }
+    // twiddles w16^m, m = n2*k1 in {1,2,3,4,6,9}: cos(m pi/8) + DIR*i*sin(m pi/8)
+    auto tw = [&](C a, R c, R s) { C r; const R sd = DIR < 0 ? -s : s; r.x = a.x * c - a.y * sd; r.y = a.x * sd + a.y * c; return r; };
+    y[1][1] = tw(y[1][1], c1, s1);
+    y[1][2] = tw(y[1][2], h, h);
+    y[1][3] = tw(y[1][3], s1, c1);
+    y[2][1] = tw(y[2][1], h, h);
+    y[2][2] = rot90<DIR>(y[2][2]);
+    y[2][3] = tw(y[2][3], -h, h);
+    y[3][1] = tw(y[3][1], s1, c1);
+    y[3][2] = tw(y[3][2], -h, h);
+    y[3][3] = tw(y[3][3], -c1, -s1);
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      C t[4] = {y[0][k1], y[1][k1], y[2][k1], y[3][k1]};
+      Dft<4, DIR, C>::run(t);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+  }
+};
+
 // Odd radix P in {3,5,7}: X_k = v0 + sum_q cos(th) s_q + DIR*i sum_q sin(th) d_q,
 // s_q = v_q + v_{P-q}, d_q = v_q - v_{P-q}, th = 2 pi q k / P.
 template <int P, int DIR, typename C> struct DftOdd {

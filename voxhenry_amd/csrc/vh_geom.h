@@ -4,9 +4,12 @@
 #pragma once
 #include "vh_common.h"
 #include "vh_fft.h"
+#include "vh_fft2.h"
 #include <memory>
 
 namespace vh {
+
+constexpr int TENSOR_STRIDE = 8;   // stored components per spectral point (7 + 1 pad)
 
 struct GridDims {
   int L, M, N;          // voxel grid
@@ -48,9 +51,11 @@ struct Geometry {
   DevBuf<int> vmap;                   // L*M*N -> voxel id or -1
   DevBuf<int> lines;                  // non-empty (y + M z) lines
   int nlines = 0;
-  FftAxis ax, ay, az;
+  FftAxis ax, ay, az;                 // Green's tensor transforms
+  FftAxis2 fx, fy, fz;                // matvec passes
   // Green's tensor spectrum of the normalised real-space tensor (ko = 1),
-  // divided by Px*Py*Pz, one octant [c][kz][ky][kx] of Hx*Hy*Hz per component.
+  // divided by Px*Py*Pz: one octant [kz][ky][kx] of Hx*Hy*Hz points, the 7
+  // components of a point contiguous (padded to TENSOR_STRIDE).
   DevBuf<double2> t64;                // complex, exact
   DevBuf<float> t32;                  // real part of even comps / imag part of odd comps
   bool has_tensor = false;

@@ -217,9 +217,9 @@ __global__ void k_extract_octant(const double2* __restrict__ F, double2* __restr
   const int kz = (int)(o / ((int64_t)g.Hx * g.Hy));
   const int64_t P = (int64_t)g.Px * g.Py * g.Pz;
   const double2 v = cscale(F[kx + (int64_t)g.Px * (ky + (int64_t)g.Py * kz) + P * c], invP);
-  t64[id] = v;
+  t64[o * TENSOR_STRIDE + c] = v;
   const bool odd = (c == 1 || c == 2 || c == 4);
-  t32[id] = (float)(odd ? v.y : v.x);
+  t32[o * TENSOR_STRIDE + c] = (float)(odd ? v.y : v.x);
 }
 
 // Near block (offsets in {0,1}^3), assembled on the host exactly as
@@ -324,8 +324,10 @@ void tensor_from_realspace(Geometry& Gm, const double2* Greal) {
   VH_LAUNCHED();
   fft3d_inplace<double, -1>(Gm, Gp.p, 7);
   const size_t O = Gm.octant();
-  Gm.t64.alloc(7 * O);
-  Gm.t32.alloc(7 * O);
+  Gm.t64.alloc(TENSOR_STRIDE * O);
+  Gm.t32.alloc(TENSOR_STRIDE * O);
+  VH_HIP(hipMemsetAsync(Gm.t64.p, 0, Gm.t64.bytes(), st));
+  VH_HIP(hipMemsetAsync(Gm.t32.p, 0, Gm.t32.bytes(), st));
   hipLaunchKernelGGL(k_extract_octant, dim3(ceil_div(7 * O, 256)), dim3(256), 0, st, Gp.p, Gm.t64.p, Gm.t32.p, g,
                      1.0 / (double)P);
   VH_LAUNCHED();

@@ -67,6 +67,9 @@ template <typename R> struct KrylovWork {
   DevBuf<cplx<R>> V, Z, w, t, r;
   DevBuf<double2> h, nrm, coef;
   DevBuf<int> slots_dev, aux_slots;
+  // pinned host staging: slot lists, LS coefficients, Hessenberg columns, norms
+  PinnedBuf<int> h_slots, h_aux;
+  PinnedBuf<double2> h_coef, h_rcoef, h_h, h_nrm, h_bn;
   void ensure(int64_t n, int64_t ld, int m, int nslot, bool flexible);
   static size_t bytes_needed(int64_t ld, int m, int nslot, bool flexible) {
     return sizeof(cplx<R>) * (size_t)ld * nslot * ((m + 1) + (flexible ? m : 0) + 3);

@@ -27,98 +27,105 @@ __device__ __forceinline__ int xcd_tile(int bid, int nblk) {
 
 template <typename R> __device__ __forceinline__ cplx<R> czero() { cplx<R> c; c.x = 0; c.y = 0; return c; }
 
+// Pass shapes: threads per workgroup and the bound E on elements per thread
+// and pencil.  X/Xi and Y/Yi hold one pencil element set per thread; Z holds
+// the five components' spectra of its pencils in registers between the
+// forward and the inverse transform, hence the smaller E.
+constexpr int NT_XY = 256;
+constexpr int NT_Z = 512;
+template <typename R> constexpr int e_xy() { return sizeof(R) == 8 ? 8 : 16; }
+template <typename R> constexpr int e_z() { return sizeof(R) == 8 ? 4 : 8; }
+template <int N> constexpr int ilog2() { return N <= 1 ? 0 : 1 + ilog2<N / 2>(); }
+
 // ---------------------------------------------------------------------------
-// pass X: gather + forward FFT along x.  One block = T lines of one (rhs, comp).
+// pass X: gather + forward FFT along x.  A workgroup transforms T lines of one
+// (rhs, comp); the tpp threads of a line are consecutive, so the first stage's
+// loads of a line are contiguous.  LDS: line t at t*LP, element i at
+// i + i/16 (one pad slot every 16 elements keeps the stride-R writes of the
+// early stages on distinct banks).
 // ---------------------------------------------------------------------------
-template <typename R, int EMAX>
-__global__ __launch_bounds__(FFT_THREADS) void k_pass_x_fwd(
-    const cplx<R>* __restrict__ x, int64_t ldx, const int* __restrict__ vmap,
-    const int* __restrict__ lines, int nlines, int T, cplx<R>* __restrict__ A, GridDims g, FftPlan pl,
-    const int* __restrict__ slots) {
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+static inline int lpad_host(int i) { return i + (i >> 4); }
+
+template <typename R, int E, int NT>
+__global__ __launch_bounds__(NT) void k_pass_x_fwd(
+    const cplx<R>* __restrict__ x, int64_t ldx, const int* __restrict__ vmap, const int* __restrict__ lines,
+    int nlines, int ltpp, cplx<R>* __restrict__ A, GridDims g, Plan2 pl, const int* __restrict__ slots, int dense) {
+  using C = cplx<R>;
   extern __shared__ __align__(16) unsigned char smem[];
-  cplx<R>* buf = (cplx<R>*)smem;
-  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int tid = threadIdx.x;
+  const int tpp = 1 << ltpp, T = NT >> ltpp;
+  const int jl = tid & (tpp - 1), t = tid >> ltpp;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int rc = blockIdx.y, rhs = rc / 5, comp = rc - rhs * 5;
   const int slot = slots ? slots[rhs] : rhs;
-  const int l0 = tile * T;
-  const int Px = g.Px, L = g.L;
-  const cplx<R>* xs = x + (int64_t)slot * ldx + (int64_t)comp * g.nv;
-  for (int e = tid; e < T * Px; e += nthr) {
-    const int p = e / Px, i = e - p * Px;
-    cplx<R> v = czero<R>();
-    if (i < L && l0 + p < nlines) {
-      const int line = lines[l0 + p];
-      const int vid = vmap[i + L * line];
-      if (vid >= 0) v = xs[vid];
+  const int l = tile * T + t;
+  const bool valid = l < nlines;
+  const int line = valid ? lines[l] : 0;
+  const int L = g.L, Px = g.Px;
+  const C* xs = x + (int64_t)slot * ldx + (int64_t)comp * g.nv;
+  const int* vm = vmap + (int64_t)L * line;
+  const int64_t lbase = (int64_t)L * line;
+  C* lds = (C*)smem + t * (lpad(Px - 1) + 1);
+  auto lx = [](int i) { return lpad(i); };
+  auto gld = [&](int i) -> C {
+    C v = czero<R>();
+    if (valid && i < L) {
+      if (dense) v = xs[lbase + i];
+      else { const int vid = vm[i]; if (vid >= 0) v = xs[vid]; }
     }
-    buf[e] = v;
-  }
-  __syncthreads();
-  lds_fft<R, -1, EMAX>(buf, T, Px, 1, pl, tid, nthr, false);
-  cplx<R>* As = A + (int64_t)rc * Px * g.M * g.N;
-  for (int e = tid; e < T * Px; e += nthr) {
-    const int p = e / Px, i = e - p * Px;
-    if (l0 + p < nlines) As[(int64_t)lines[l0 + p] * Px + i] = buf[e];
-  }
+    return v;
+  };
+  C* As = A + ((int64_t)rc * g.M * g.N + line) * Px;
+  auto gst = [&](int o, C v) { if (valid) As[o] = v; };
+  fft_pencil<R, -1, E>(pl, lds, lx, jl, tpp, gld, gst);
 }
 
 // ---------------------------------------------------------------------------
-// pass Y (forward M -> Py or inverse Py -> M): one block = T consecutive kx of
-// one z plane of one (rhs, comp).
+// pass Y (forward M -> Py or inverse Py -> M): a workgroup transforms T
+// consecutive kx of one z plane of one (rhs, comp); consecutive threads take
+// consecutive kx, so every load and store moves T contiguous elements.  in and
+// out may alias (the inverse works in place): all loads of a workgroup precede
+// its first store.
 // ---------------------------------------------------------------------------
-template <typename R, int DIR, int EMAX>
-__global__ __launch_bounds__(FFT_THREADS) void k_pass_y(
-    const cplx<R>* in, int in_rows, int nin, cplx<R>* out, int out_rows, int nout, int T, GridDims g, FftPlan pl) {
-  // in/out may alias (inverse pass works in place): each block reads its whole
-  // tile before the FFT and writes it back afterwards.
+template <typename R, int DIR, int E, int NT>
+__global__ __launch_bounds__(NT) void k_pass_y(const cplx<R>* in, int in_rows, int nin, cplx<R>* out, int out_rows,
+                                               int nout, int ltpp, GridDims g, Plan2 pl) {
+  using C = cplx<R>;
   extern __shared__ __align__(16) unsigned char smem[];
-  cplx<R>* buf = (cplx<R>*)smem;
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const int tiles_x = (g.Px + T - 1) / T;
+  const int tid = threadIdx.x;
+  const int lT = ilog2<NT>() - ltpp;
+  const int T = 1 << lT, tpp = 1 << ltpp;
+  const int t = tid & (T - 1), jl = tid >> lT;
+  const int tiles_x = (g.Px + T - 1) >> lT;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = tile % tiles_x, z = tile / tiles_x;
-  const int kx0 = tx * T;
+  const int kx = (tx << lT) + t;
+  const bool valid = kx < g.Px;
   const int rc = blockIdx.y;
-  const int Px = g.Px, Py = pl.n;
-  const cplx<R>* is = in + ((int64_t)rc * g.N + z) * in_rows * Px;
-  for (int e = tid; e < T * Py; e += nthr) {
-    const int y = e / T, t = e - y * T;
-    const int kx = kx0 + t;
-    cplx<R> v = czero<R>();
-    if (y < nin && kx < Px) v = is[(int64_t)y * Px + kx];
-    buf[e] = v;
-  }
-  __syncthreads();
-  lds_fft<R, DIR, EMAX>(buf, T, 1, T, pl, tid, nthr, true);
-  cplx<R>* os = out + ((int64_t)rc * g.N + z) * out_rows * Px;
-  for (int e = tid; e < T * nout; e += nthr) {
-    const int y = e / T, t = e - y * T;
-    const int kx = kx0 + t;
-    if (kx < Px) os[(int64_t)y * Px + kx] = buf[e];
-  }
+  const int Px = g.Px;
+  const C* is = in + ((int64_t)rc * g.N + z) * in_rows * Px + kx;
+  C* os = out + ((int64_t)rc * g.N + z) * out_rows * Px + kx;
+  C* lds = (C*)smem + t;
+  auto lx = [T](int i) { return i * T; };
+  auto gld = [&](int i) -> C { return (valid && i < nin) ? is[i * Px] : czero<R>(); };
+  auto gst = [&](int o, C v) { if (valid && o < nout) os[o * Px] = v; };
+  fft_pencil<R, DIR, E>(pl, lds, lx, jl, tpp, gld, gst);
 }
 
 // ---------------------------------------------------------------------------
 // pass Z: forward FFT along z, Green's product, inverse FFT, in place.
-// One block = T consecutive kx of one ky column, looping over the rhs.
-// LDS holds 5T pencils: element (pencil p = c*T + t, kz) at kz*5T + p.
+// A workgroup owns T consecutive kx of one ky column and loops over the rhs.
+// The five components are transformed one after the other through the same
+// LDS array; each leaves its spectrum in registers (last forward stage), the
+// 5x5 product is applied in registers, and the inverse transforms start from
+// registers.  Columns ky and Py-ky read the same tensor octant rows, so the
+// tile order visits them back to back.
 // ---------------------------------------------------------------------------
 struct TensorIdx {
-  int idx;        // octant index
+  int idx;        // octant point index
   float sx, sy, sz;
 };
-
-__device__ __forceinline__ TensorIdx tensor_index(const GridDims& g, int kx, int ky, int kz) {
-  TensorIdx t;
-  int ax = kx, ay = ky, az = kz;
-  t.sx = 1.f; t.sy = 1.f; t.sz = 1.f;
-  if (ax >= g.Hx) { ax = g.Px - ax; t.sx = -1.f; }
-  if (ay >= g.Hy) { ay = g.Py - ay; t.sy = -1.f; }
-  if (az >= g.Hz) { az = g.Pz - az; t.sz = -1.f; }
-  t.idx = ax + g.Hx * (ay + g.Hy * az);
-  return t;
-}
 
 // 5x5 block product of lse_matvect_mult.m:57-86 with complex tensor entries.
 template <typename C>
@@ -155,142 +162,173 @@ template <> struct TensorRef<double> { const double2* t; };
 template <> struct TensorRef<float> { const float* t; };
 
 template <typename R>
-__device__ __forceinline__ void apply_point(cplx<R>* f, const TensorRef<R>& T, size_t O, const TensorIdx& ti,
-                                            cplx<R> alpha);
+__device__ __forceinline__ void apply_point(cplx<R>* f, const TensorRef<R>& T, const TensorIdx& ti, cplx<R> alpha);
 template <>
-__device__ __forceinline__ void apply_point<double>(double2* f, const TensorRef<double>& T, size_t O,
-                                                    const TensorIdx& ti, double2 alpha) {
+__device__ __forceinline__ void apply_point<double>(double2* f, const TensorRef<double>& T, const TensorIdx& ti,
+                                                    double2 alpha) {
+  const double2* p = T.t + (size_t)ti.idx * TENSOR_STRIDE;
   double2 G[7];
 #pragma unroll
-  for (int c = 0; c < 7; ++c) G[c] = T.t[c * O + ti.idx];
+  for (int c = 0; c < 7; ++c) G[c] = p[c];
   G[1] = cscale(G[1], (double)ti.sx);
   G[2] = cscale(G[2], (double)ti.sy);
   G[4] = cscale(G[4], (double)ti.sz);
   green_product(f, G, alpha);
 }
 template <>
-__device__ __forceinline__ void apply_point<float>(float2* f, const TensorRef<float>& T, size_t O,
-                                                   const TensorIdx& ti, float2 alpha) {
-  float g[7];
-#pragma unroll
-  for (int c = 0; c < 7; ++c) g[c] = T.t[c * O + ti.idx];
-  g[1] *= ti.sx; g[2] *= ti.sy; g[4] *= ti.sz;
+__device__ __forceinline__ void apply_point<float>(float2* f, const TensorRef<float>& T, const TensorIdx& ti,
+                                                   float2 alpha) {
+  const float4* p = (const float4*)(T.t + (size_t)ti.idx * TENSOR_STRIDE);
+  const float4 a = p[0], b = p[1];
+  float g[7] = {a.x, a.y * ti.sx, a.z * ti.sy, a.w, b.x * ti.sz, b.y, b.z};
   green_product_real(f, g, alpha);
 }
 
-template <typename R, int EMAX>
-__global__ __launch_bounds__(FFT_THREADS) void k_pass_z(
-    cplx<R>* __restrict__ B, int nrhs, int T, GridDims g, FftPlan pl, TensorRef<R> tens, cplx<R> alpha0,
-    const double2* __restrict__ alphas, const int* __restrict__ slots) {
-  // alphas (optional): j*omega*mu per vector slot; otherwise alpha0 for all
+template <typename C, int BL, int RL>
+__device__ __forceinline__ void copy_regs(C (&d)[BL][RL], const C (&s)[BL][RL]) {
+#pragma unroll
+  for (int m = 0; m < BL; ++m)
+#pragma unroll
+    for (int r = 0; r < RL; ++r) d[m][r] = s[m][r];
+}
+
+template <typename R, int E, int NT, int RL>
+__global__ __launch_bounds__(NT) void k_pass_z(cplx<R>* __restrict__ B, int nrhs, int ltpp, GridDims g, Plan2 pf,
+                                               Plan2 pi, TensorRef<R> tens, cplx<R> alpha0,
+                                               const double2* __restrict__ alphas, const int* __restrict__ slots) {
+  using C = cplx<R>;
+  constexpr int BL = (E + RL - 1) / RL;
   extern __shared__ __align__(16) unsigned char smem[];
-  cplx<R>* buf = (cplx<R>*)smem;
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  const int tiles_x = (g.Px + T - 1) / T;
+  const int tid = threadIdx.x;
+  const int lT = ilog2<NT>() - ltpp;
+  const int T = 1 << lT, tpp = 1 << ltpp;
+  const int t = tid & (T - 1), jl = tid >> lT;
+  const int Px = g.Px, Py = g.Py, Pz = g.Pz, N = g.N;
+  const int tiles_x = (Px + T - 1) >> lT;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int tx = tile % tiles_x, ky = tile / tiles_x;
-  const int kx0 = tx * T;
-  const int Px = g.Px, Pz = g.Pz, N = g.N;
-  const int T5 = 5 * T;
-  const int64_t plane = (int64_t)Px * g.Py;     // elements per z-plane of one rc
-  const size_t O = (size_t)g.Hx * g.Hy * g.Hz;
+  const int tx = tile % tiles_x, u = tile / tiles_x;
+  const int v = u - 1;
+  const int ky = u == 0 ? 0 : ((v & 1) ? Py - 1 - (v >> 1) : 1 + (v >> 1));
+  const int kx = (tx << lT) + t;
+  const bool valid = kx < Px;
+  TensorIdx ti;
+  const int ax = kx < g.Hx ? kx : Px - kx;
+  const int ay = ky < g.Hy ? ky : Py - ky;
+  ti.sx = kx < g.Hx ? 1.f : -1.f;
+  ti.sy = ky < g.Hy ? 1.f : -1.f;
+  const int axy = valid ? ax + g.Hx * ay : 0;
+  const int HxHy = g.Hx * g.Hy;
+  const int plane = Px * Py;
+  const int nbL = pf.st[pf.nstage - 1].nb;
+  C* lds = (C*)smem + t;
+  auto lx = [T](int i) { return i * T; };
   for (int rhs = 0; rhs < nrhs; ++rhs) {
-    cplx<R> alpha = alpha0;
+    C alpha = alpha0;
     if (alphas) {
       const double2 a = alphas[slots ? slots[rhs] : rhs];
       alpha.x = (R)a.x; alpha.y = (R)a.y;
     }
-    cplx<R>* base = B + (int64_t)rhs * 5 * plane * N + (int64_t)ky * Px + kx0;
-    for (int e = tid; e < T5 * Pz; e += nthr) {
-      const int kz = e / T5, p = e - kz * T5;
-      const int c = p / T, t = p - c * T;
-      cplx<R> v = czero<R>();
-      if (kz < N && kx0 + t < Px) v = base[((int64_t)c * N + kz) * plane + t];
-      buf[e] = v;
+    C* base = B + (int64_t)rhs * 5 * N * plane + (int64_t)ky * Px + kx;
+    C h0[BL][RL], h1[BL][RL], h2[BL][RL], h3[BL][RL], h4[BL][RL];
+#pragma unroll 1
+    for (int c = 0; c < 5; ++c) {
+      const C* bc = base + (int64_t)c * N * plane;
+      auto gld = [&](int i) -> C { return (valid && i < N) ? bc[(int64_t)i * plane] : czero<R>(); };
+      C o[BL][RL];
+      fft_pencil_to_regs<R, -1, E, RL>(pf, lds, lx, jl, tpp, gld, o);
+      switch (c) {
+        case 0: copy_regs(h0, o); break;
+        case 1: copy_regs(h1, o); break;
+        case 2: copy_regs(h2, o); break;
+        case 3: copy_regs(h3, o); break;
+        default: copy_regs(h4, o); break;
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    lds_fft<R, -1, EMAX>(buf, T5, 1, T5, pl, tid, nthr, true);
-    for (int e = tid; e < T * Pz; e += nthr) {
-      const int kz = e / T, t = e - kz * T;
-      const int kx = kx0 + t;
-      if (kx < Px) {
-        cplx<R>* f0 = buf + kz * T5 + t;
-        cplx<R> f[5];
 #pragma unroll
-        for (int c = 0; c < 5; ++c) f[c] = f0[c * T];
-        const TensorIdx ti = tensor_index(g, kx, ky, kz);
-        apply_point<R>(f, tens, O, ti, alpha);
+    for (int m = 0; m < BL; ++m) {
+      const int j = jl + m * tpp;
+      if (j < nbL && valid) {
 #pragma unroll
-        for (int c = 0; c < 5; ++c) f0[c * T] = f[c];
+        for (int r = 0; r < RL; ++r) {
+          const int kz = j + r * nbL;
+          const bool lo = kz < g.Hz;
+          TensorIdx tk = ti;
+          tk.sz = lo ? 1.f : -1.f;
+          tk.idx = axy + HxHy * (lo ? kz : Pz - kz);
+          C f[5] = {h0[m][r], h1[m][r], h2[m][r], h3[m][r], h4[m][r]};
+          apply_point<R>(f, tens, tk, alpha);
+          h0[m][r] = f[0]; h1[m][r] = f[1]; h2[m][r] = f[2]; h3[m][r] = f[3]; h4[m][r] = f[4];
+        }
       }
     }
-    __syncthreads();
-    lds_fft<R, +1, EMAX>(buf, T5, 1, T5, pl, tid, nthr, true);
-    for (int e = tid; e < T5 * N; e += nthr) {
-      const int kz = e / T5, p = e - kz * T5;
-      const int c = p / T, t = p - c * T;
-      if (kx0 + t < Px) base[((int64_t)c * N + kz) * plane + t] = buf[e];
+#pragma unroll 1
+    for (int c = 0; c < 5; ++c) {
+      C* bc = base + (int64_t)c * N * plane;
+      auto gst = [&](int o, C val) { if (valid && o < N) bc[(int64_t)o * plane] = val; };
+      C in[BL][RL];
+      switch (c) {
+        case 0: copy_regs(in, h0); break;
+        case 1: copy_regs(in, h1); break;
+        case 2: copy_regs(in, h2); break;
+        case 3: copy_regs(in, h3); break;
+        default: copy_regs(in, h4); break;
+      }
+      fft_pencil_from_regs<R, +1, E, RL>(pi, lds, lx, jl, tpp, in, gst);
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
 // ---------------------------------------------------------------------------
 // pass Xi: inverse FFT along x, crop, diagonal, -Ar' phi, scatter.
 // ---------------------------------------------------------------------------
-template <typename R, int EMAX>
-__global__ __launch_bounds__(FFT_THREADS) void k_pass_x_inv(
-    const cplx<R>* __restrict__ A, const cplx<R>* __restrict__ x, int64_t ldx, cplx<R>* __restrict__ y, int64_t ldy,
-    const int* __restrict__ vmap, const int* __restrict__ lines, int nlines, int T, GridDims g, FftPlan pl,
+template <typename R, int E, int NT>
+__global__ __launch_bounds__(NT) void k_pass_x_inv(
+    const cplx<R>* __restrict__ B, const cplx<R>* __restrict__ x, int64_t ldx, cplx<R>* __restrict__ y, int64_t ldy,
+    const int* __restrict__ vmap, const int* __restrict__ lines, int nlines, int ltpp, GridDims g, Plan2 pl,
     const cplx<R>* __restrict__ z, R inv_dx, const int* __restrict__ atp, const int* __restrict__ atc,
-    const R* __restrict__ atv, int64_t num_curr, int mode, const int* __restrict__ slots) {
+    const R* __restrict__ atv, int64_t num_curr, int mode, const int* __restrict__ slots, int dense) {
   // mode 0: conv + diag - Ar'phi ; mode 1: conv + diag ; mode 2: conv only
+  using C = cplx<R>;
   extern __shared__ __align__(16) unsigned char smem[];
-  cplx<R>* buf = (cplx<R>*)smem;
-  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int tid = threadIdx.x;
+  const int tpp = 1 << ltpp, T = NT >> ltpp;
+  const int jl = tid & (tpp - 1), t = tid >> ltpp;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int rc = blockIdx.y, rhs = rc / 5, comp = rc - rhs * 5;
   const int slot = slots ? slots[rhs] : rhs;
-  const int l0 = tile * T;
-  const int Px = g.Px, L = g.L;
-  const cplx<R>* As = A + (int64_t)rc * Px * g.Py * g.N;
-  for (int e = tid; e < T * Px; e += nthr) {
-    const int p = e / Px, i = e - p * Px;
-    cplx<R> v = czero<R>();
-    if (l0 + p < nlines) {
-      const int line = lines[l0 + p];
-      const int yy = line % g.M, zz = line / g.M;
-      v = As[((int64_t)zz * g.Py + yy) * Px + i];
-    }
-    buf[e] = v;
-  }
-  __syncthreads();
-  lds_fft<R, +1, EMAX>(buf, T, Px, 1, pl, tid, nthr, false);
+  const int l = tile * T + t;
+  const bool valid = l < nlines;
+  const int line = valid ? lines[l] : 0;
+  const int L = g.L, Px = g.Px;
+  const int yy = line % g.M, zz = line / g.M;
+  const C* Bs = B + (((int64_t)rc * g.N + zz) * g.Py + yy) * Px;
+  C* lds = (C*)smem + t * (lpad(Px - 1) + 1);
+  auto lx = [](int i) { return lpad(i); };
+  auto gld = [&](int i) -> C { return valid ? Bs[i] : czero<R>(); };
   const R w = inv_dx * (comp < 3 ? R(1) : comp == 3 ? R(1) / R(6) : R(1) / R(2));
-  const cplx<R>* xs = x + (int64_t)slot * ldx;
-  cplx<R>* ys = y + (int64_t)slot * ldy;
-  for (int e = tid; e < T * L; e += nthr) {
-    const int p = e / L, i = e - p * L;
-    if (l0 + p >= nlines) continue;
-    const int vid = vmap[i + L * lines[l0 + p]];
-    if (vid < 0) continue;
+  const C* xs = x + (int64_t)slot * ldx;
+  C* ys = y + (int64_t)slot * ldy;
+  const int* vm = vmap + (int64_t)L * line;
+  const int64_t lbase = (int64_t)L * line;
+  auto gst = [&](int o, C acc) {
+    if (!valid || o >= L) return;
+    const int vid = dense ? (int)(lbase + o) : vm[o];
+    if (vid < 0) return;
     const int64_t row = (int64_t)comp * g.nv + vid;
-    cplx<R> acc = buf[p * Px + i];
-    if (mode <= 1) {
-      const cplx<R> J = xs[row];
-      const cplx<R> zz = z[vid];
-      acc = cadd(acc, cscale(cmul(zz, J), w));
-    }
+    if (mode <= 1) acc = cadd(acc, cscale(cmul(z[vid], xs[row]), w));
     if (mode == 0) {
-      cplx<R> s = czero<R>();
+      C s = czero<R>();
       for (int k = atp[row]; k < atp[row + 1]; ++k) {
-        const cplx<R> ph = xs[num_curr + atc[k]];
+        const C ph = xs[num_curr + atc[k]];
         s.x += atv[k] * ph.x; s.y += atv[k] * ph.y;
       }
       acc = csub(acc, s);
     }
     ys[row] = acc;
-  }
+  };
+  fft_pencil<R, +1, E>(pl, lds, lx, jl, tpp, gld, gst);
 }
 
 // Node rows: y_node = Ar * J.
@@ -314,20 +352,6 @@ __global__ __launch_bounds__(256) void k_node_rows(
 // ---------------------------------------------------------------------------
 // host launch logic
 // ---------------------------------------------------------------------------
-template <typename R> constexpr int emax_of() { return sizeof(R) == 8 ? 16 : 32; }
-
-// Pick the number of pencils per block: T*n*npen_factor fits EMAX*threads and
-// the LDS budget; prefer >= 128 contiguous bytes per row.
-static int pick_T(int n, int factor, size_t elem, int emax, size_t lds_budget, int want) {
-  int T = want;
-  while (T > 1 && ((int64_t)T * n * factor > (int64_t)emax * FFT_THREADS ||
-                   (size_t)T * n * factor * elem > lds_budget))
-    T >>= 1;
-  if ((int64_t)T * n * factor > (int64_t)emax * FFT_THREADS || (size_t)T * n * factor * elem > lds_budget)
-    throw Error(ERR_ARG, "padded FFT length " + std::to_string(n) + " exceeds the LDS pencil kernels");
-  return T;
-}
-
 template <typename R>
 static void ensure_ws(Geometry& G, int nrhs) {
   const GridDims& g = G.g;
@@ -338,6 +362,25 @@ static void ensure_ws(Geometry& G, int nrhs) {
   DevBuf<char>& wsA = sizeof(R) == 8 ? G.wsA64 : G.wsA32;
   if (wsA.n < a) { wsA.alloc(a); VH_HIP(hipMemsetAsync(wsA.p, 0, a, G.ctx->stream)); }
   if (G.wsB.n < b) G.wsB.alloc(b);
+}
+
+static int log2i(int v) { int l = 0; while ((1 << l) < v) ++l; return l; }
+
+// Threads per pencil for a pass, or an error naming the unsupported length.
+static int pass_tpp(int n, int E, int NT, const char* axis) {
+  const int tpp = fft_tpp(n, E, NT);
+  if (tpp < 0)
+    throw Error(ERR_ARG, std::string("padded FFT length ") + std::to_string(n) + " along " + axis +
+                         " exceeds the pencil kernels (max " + std::to_string(E * NT) + ")");
+  return tpp;
+}
+
+template <typename R, int RL>
+static void launch_z(dim3 grid, size_t lds, hipStream_t st, cplx<R>* B, int nrhs, int ltpp, const GridDims& g,
+                     const Plan2& pf, const Plan2& pi, TensorRef<R> tr, cplx<R> alpha, const double2* alphas,
+                     const int* slots) {
+  hipLaunchKernelGGL((k_pass_z<R, e_z<R>(), NT_Z, RL>), grid, dim3(NT_Z), lds, st, B, nrhs, ltpp, g, pf, pi, tr,
+                     alpha, alphas, slots);
 }
 
 template <typename R>
@@ -351,70 +394,81 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
   cplx<R>* A = (cplx<R>*)(sizeof(R) == 8 ? G.wsA64.p : G.wsA32.p);
   cplx<R>* B = (cplx<R>*)G.wsB.p;
   hipStream_t st = G.ctx->stream;
-  constexpr int EM = emax_of<R>();
+  constexpr int EXY = e_xy<R>(), EZ = e_z<R>();
   const size_t elem = sizeof(cplx<R>);
-  const size_t budget = 64 * 1024;
-  const int want = sizeof(R) == 8 ? 8 : 16;
-  FftPlan px = G.ax.plan<R>(), py = G.ay.plan<R>(), pz = G.az.plan<R>();
+  const int dense = G.dense ? 1 : 0;
   PassTimer* tm = G.timer;
   if (tm) { tm->collect(); VH_HIP(hipEventRecord(tm->ev[0], st)); }
   auto mark = [&](int k) { if (tm) VH_HIP(hipEventRecord(tm->ev[k], st)); };
 
+  const int tx = pass_tpp(g.Px, EXY, NT_XY, "x");
+  const int ty = pass_tpp(g.Py, EXY, NT_XY, "y");
+  const int tz = pass_tpp(g.Pz, EZ, NT_Z, "z");
+  const Plan2 px = G.fx.plan<R>(rmax_for(EXY), false);
+  const Plan2 py = G.fy.plan<R>(rmax_for(EXY), false);
+  const Plan2 pzf = G.fz.plan<R>(rmax_for(EZ), false);
+  const Plan2 pzi = G.fz.plan<R>(rmax_for(EZ), true);
+  const int Tx = NT_XY / tx, Ty = NT_XY / ty, Tz = NT_Z / tz;
+  const size_t lds_x = (size_t)Tx * (lpad_host(g.Px - 1) + 1) * elem;
+
   // X
   if (G.nlines > 0) {
-    const int T = pick_T(g.Px, 1, elem, EM, budget, 4);
-    dim3 grid(ceil_div(G.nlines, T), nrhs * 5);
-    hipLaunchKernelGGL((k_pass_x_fwd<R, EM>), grid, dim3(FFT_THREADS), T * g.Px * elem, st, x, ldx, G.vmap.p,
-                       G.lines.p, G.nlines, T, A, g, px, slots);
+    dim3 grid(ceil_div(G.nlines, Tx), nrhs * 5);
+    hipLaunchKernelGGL((k_pass_x_fwd<R, EXY, NT_XY>), grid, dim3(NT_XY), lds_x, st, x, ldx, G.vmap.p, G.lines.p,
+                       G.nlines, log2i(tx), A, g, px, slots, dense);
     VH_LAUNCHED();
   }
   mark(1);
   // Y
   {
-    const int T = pick_T(g.Py, 1, elem, EM, budget, want);
-    dim3 grid(ceil_div(g.Px, T) * g.N, nrhs * 5);
-    hipLaunchKernelGGL((k_pass_y<R, -1, EM>), grid, dim3(FFT_THREADS), T * g.Py * elem, st, A, g.M, g.M, B, g.Py,
-                       g.Py, T, g, py);
+    dim3 grid(ceil_div(g.Px, Ty) * g.N, nrhs * 5);
+    hipLaunchKernelGGL((k_pass_y<R, -1, EXY, NT_XY>), grid, dim3(NT_XY), (size_t)Ty * g.Py * elem, st, A, g.M, g.M,
+                       B, g.Py, g.Py, log2i(ty), g, py);
     VH_LAUNCHED();
   }
   mark(2);
   // Z
   {
-    const int T = pick_T(g.Pz, 5, elem, EM, budget, want);
-    dim3 grid(ceil_div(g.Px, T) * g.Py);
+    dim3 grid(ceil_div(g.Px, Tz) * g.Py);
     cplx<R> alpha;
-    const double scale = 1.0;
-    alpha.x = (R)(are * scale);
-    alpha.y = (R)(aim * scale);
+    alpha.x = (R)are;
+    alpha.y = (R)aim;
     TensorRef<R> tr;
     if constexpr (sizeof(R) == 8) tr.t = G.t64.p; else tr.t = G.t32.p;
-    hipLaunchKernelGGL((k_pass_z<R, EM>), grid, dim3(FFT_THREADS), 5 * T * g.Pz * elem, st, B, nrhs, T, g, pz, tr,
-                       alpha, alphas, slots);
+    const size_t lds = (size_t)Tz * g.Pz * elem;
+    const int ltz = log2i(tz);
+    switch (G.fz.last_radix(rmax_for(EZ))) {
+      case 2: launch_z<R, 2>(grid, lds, st, B, nrhs, ltz, g, pzf, pzi, tr, alpha, alphas, slots); break;
+      case 3: launch_z<R, 3>(grid, lds, st, B, nrhs, ltz, g, pzf, pzi, tr, alpha, alphas, slots); break;
+      case 4: launch_z<R, 4>(grid, lds, st, B, nrhs, ltz, g, pzf, pzi, tr, alpha, alphas, slots); break;
+      case 5: launch_z<R, 5>(grid, lds, st, B, nrhs, ltz, g, pzf, pzi, tr, alpha, alphas, slots); break;
+      case 7: launch_z<R, 7>(grid, lds, st, B, nrhs, ltz, g, pzf, pzi, tr, alpha, alphas, slots); break;
+      case 8: launch_z<R, 8>(grid, lds, st, B, nrhs, ltz, g, pzf, pzi, tr, alpha, alphas, slots); break;
+      default: throw Error(ERR_INTERNAL, "pass Z: unexpected last radix");
+    }
     VH_LAUNCHED();
   }
   mark(3);
-  // Yi
+  // Yi (in place in B: rows y < M of every (rc, z) slab)
   {
-    const int T = pick_T(g.Py, 1, elem, EM, budget, want);
-    dim3 grid(ceil_div(g.Px, T) * g.N, nrhs * 5);
-    hipLaunchKernelGGL((k_pass_y<R, +1, EM>), grid, dim3(FFT_THREADS), T * g.Py * elem, st, B, g.Py, g.Py, B, g.Py,
-                       g.M, T, g, py);
+    dim3 grid(ceil_div(g.Px, Ty) * g.N, nrhs * 5);
+    hipLaunchKernelGGL((k_pass_y<R, +1, EXY, NT_XY>), grid, dim3(NT_XY), (size_t)Ty * g.Py * elem, st, B, g.Py,
+                       g.Py, B, g.Py, g.M, log2i(ty), g, py);
     VH_LAUNCHED();
   }
   mark(4);
   // Xi
   if (G.nlines > 0) {
-    const int T = pick_T(g.Px, 1, elem, EM, budget, 4);
-    dim3 grid(ceil_div(G.nlines, T), nrhs * 5);
+    dim3 grid(ceil_div(G.nlines, Tx), nrhs * 5);
     const R* atv;
     if constexpr (sizeof(R) == 8) atv = G.ArT.v64.p; else atv = G.ArT.v32.p;
     const cplx<R>* zz;
     if constexpr (sizeof(R) == 8) zz = G.z64.p; else zz = G.z32.p;
     if (mode <= 1) VH_REQUIRE(G.has_z, "matvec: impedance not set");
     if (mode == 0) VH_REQUIRE(G.has_inc, "matvec: incidence not set");
-    hipLaunchKernelGGL((k_pass_x_inv<R, EM>), grid, dim3(FFT_THREADS), T * g.Px * elem, st, B, x, ldx, y, ldy,
-                       G.vmap.p, G.lines.p, G.nlines, T, g, px, zz, (R)(1.0 / G.dx), G.ArT.ptr.p, G.ArT.col.p, atv,
-                       G.num_curr, mode, slots);
+    hipLaunchKernelGGL((k_pass_x_inv<R, EXY, NT_XY>), grid, dim3(NT_XY), lds_x, st, B, x, ldx, y, ldy, G.vmap.p,
+                       G.lines.p, G.nlines, log2i(tx), g, px, zz, (R)(1.0 / G.dx), G.ArT.ptr.p, G.ArT.col.p, atv,
+                       G.num_curr, mode, slots, dense);
     VH_LAUNCHED();
   }
   mark(5);

@@ -1,6 +1,7 @@
 // Geometry setup: FFT plans, voxel map, incidence matrix, impedance, and the
 // drop-in path that accepts the reference's padded tensor spectrum.
 #include "vh_geom.h"
+#include "vh_fft2.h"
 #include <algorithm>
 #include <cmath>
 
@@ -57,6 +58,85 @@ template <> FftPlan FftAxis::plan<float>() const {
   return p;
 }
 
+// Radix sequence of a v2 plan: the power-of-two part split into as few stages
+// of at most rmax as possible, as evenly as possible (larger first), followed
+// by the radices 7, 5, 3.
+static std::vector<int> radices2(int n, int rmax) {
+  int m = n, e = 0;
+  while (m % 2 == 0) { m /= 2; ++e; }
+  std::vector<int> odd;
+  for (int r : {7, 5, 3})
+    while (m % r == 0) { odd.push_back(r); m /= r; }
+  VH_REQUIRE(m == 1, "FFT length " + std::to_string(n) + " is not {2,3,5,7}-smooth");
+  const int lm = rmax == 16 ? 4 : rmax == 8 ? 3 : 2;
+  std::vector<int> rad;
+  if (e > 0) {
+    const int s = (e + lm - 1) / lm;
+    for (int i = 0; i < s; ++i) rad.push_back(1 << (e / s + (i < e % s ? 1 : 0)));
+  }
+  rad.insert(rad.end(), odd.begin(), odd.end());
+  if (rad.empty()) rad.push_back(1);
+  return rad;
+}
+
+void FftAxis2::init(int len) {
+  n = len;
+  const int rm[3] = {16, 8, 4};
+  for (int a = 0; a < 3; ++a)
+    for (int rev = 0; rev < 2; ++rev) {
+      Entry& E = e[a][rev];
+      E.radices = radices2(len, rm[a]);
+      if (rev) std::reverse(E.radices.begin(), E.radices.end());
+      VH_REQUIRE((int)E.radices.size() <= F2_MAX_STAGES, "FFT length has too many stages");
+      VH_REQUIRE(len >= 2 && E.radices[0] >= 2, "FFT length must be >= 2");
+      std::vector<double2> h64;
+      std::vector<float2> h32;
+      E.st.clear();
+      int ns = 1;
+      for (int r : E.radices) {
+        Stage2 S{};
+        S.rad = r; S.ns = ns; S.nb = len / r;
+        S.ns_rcp = ns > 1 ? (uint32_t)(((1ULL << 32) + ns - 1) / ns) : 0u;
+        S.tw_off = (int)h64.size();
+        for (int k = 0; k < ns; ++k)
+          for (int q = 1; q < r; ++q) {
+            // w_{ns r}^{k q} = exp(-2 pi i k q / (ns r)), in long double, then rounded
+            const long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)(k * q) /
+                                  (long double)(ns * r);
+            double2 w; w.x = (double)cosl(a); w.y = (double)sinl(a);
+            float2 f; f.x = (float)cosl(a); f.y = (float)sinl(a);
+            h64.push_back(w); h32.push_back(f);
+          }
+        E.st.push_back(S);
+        ns *= r;
+      }
+      E.dst.alloc(E.st.size());
+      VH_HIP(hipMemcpy(E.dst.p, E.st.data(), E.st.size() * sizeof(Stage2), hipMemcpyHostToDevice));
+      const size_t nt = std::max<size_t>(h64.size(), 1);
+      E.tw64.alloc(nt); E.tw32.alloc(nt);
+      if (!h64.empty()) {
+        VH_HIP(hipMemcpy(E.tw64.p, h64.data(), h64.size() * sizeof(double2), hipMemcpyHostToDevice));
+        VH_HIP(hipMemcpy(E.tw32.p, h32.data(), h32.size() * sizeof(float2), hipMemcpyHostToDevice));
+      }
+    }
+}
+
+static int rmax_slot(int rmax) { return rmax >= 16 ? 0 : rmax >= 8 ? 1 : 2; }
+
+template <typename R> Plan2 FftAxis2::plan(int rmax, bool reversed) const {
+  const Entry& E = e[rmax_slot(rmax)][reversed ? 1 : 0];
+  Plan2 p{};
+  p.n = n;
+  p.nstage = (int)E.st.size();
+  p.st = E.dst.p;
+  if constexpr (sizeof(R) == 8) p.tw = E.tw64.p; else p.tw = E.tw32.p;
+  return p;
+}
+template Plan2 FftAxis2::plan<double>(int, bool) const;
+template Plan2 FftAxis2::plan<float>(int, bool) const;
+
+int FftAxis2::last_radix(int rmax) const { return e[rmax_slot(rmax)][0].radices.back(); }
+
 void geometry_init(Geometry& G, Context* ctx, int L, int M, int N, double dx, int64_t nv, const int64_t* idx,
                    int pad_mode) {
   VH_REQUIRE(L >= 2 && M >= 2 && N >= 2, "grid dims must be >= 2 (near-field table covers a 2x2x2 block)");
@@ -75,6 +155,7 @@ void geometry_init(Geometry& G, Context* ctx, int L, int M, int N, double dx, in
   }
   G.g.Hx = G.g.Px / 2 + 1; G.g.Hy = G.g.Py / 2 + 1; G.g.Hz = G.g.Pz / 2 + 1;
   G.ax.init(G.g.Px); G.ay.init(G.g.Py); G.az.init(G.g.Pz);
+  G.fx.init(G.g.Px); G.fy.init(G.g.Py); G.fz.init(G.g.Pz);
   G.num_curr = 5 * nv;
   G.vox.assign(idx, idx + nv);
   std::vector<int> vmap(V, -1);
@@ -169,8 +250,8 @@ void geometry_set_spectrum_full(Geometry& G, const double* fN, int64_t P0, int64
              "tensor spectrum dims differ from the geometry's padded grid (create the geometry with pad_mode 0)");
   const size_t O = G.octant();
   const double invP = 1.0 / ((double)g.Px * g.Py * g.Pz);
-  std::vector<double2> h(7 * O);
-  std::vector<float> h32(7 * O);
+  std::vector<double2> h(TENSOR_STRIDE * O, double2{0, 0});
+  std::vector<float> h32(TENSOR_STRIDE * O, 0.f);
   for (int c = 0; c < 7; ++c)
     for (int kz = 0; kz < g.Hz; ++kz)
       for (int ky = 0; ky < g.Hy; ++ky)
@@ -180,13 +261,13 @@ void geometry_set_spectrum_full(Geometry& G, const double* fN, int64_t P0, int64
           double2 v;
           v.x = (re * sre - im * sim) * invP;
           v.y = (re * sim + im * sre) * invP;
-          const size_t dst = (size_t)c * O + kx + (size_t)g.Hx * (ky + (size_t)g.Hy * kz);
+          const size_t dst = (kx + (size_t)g.Hx * (ky + (size_t)g.Hy * kz)) * TENSOR_STRIDE + c;
           h[dst] = v;
           const bool odd = (c == 1 || c == 2 || c == 4);
           h32[dst] = (float)(odd ? v.y : v.x);
         }
-  G.t64.alloc(7 * O);
-  G.t32.alloc(7 * O);
+  G.t64.alloc(TENSOR_STRIDE * O);
+  G.t32.alloc(TENSOR_STRIDE * O);
   VH_HIP(hipMemcpy(G.t64.p, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
   VH_HIP(hipMemcpy(G.t32.p, h32.data(), h32.size() * sizeof(float), hipMemcpyHostToDevice));
   G.has_tensor = true;

@@ -36,6 +36,13 @@ void KrylovWork<R>::ensure(int64_t n_, int64_t ld_, int m_, int nslot_, bool fle
   nrm.alloc(nslot);
   slots_dev.alloc(nslot);
   aux_slots.alloc(nslot);
+  h_slots.alloc(nslot);
+  h_aux.alloc(nslot);
+  h_coef.alloc((size_t)nslot * (m + 1));
+  h_rcoef.alloc((size_t)nslot * (m + 1));
+  h_h.alloc((size_t)nslot * (m + 1));
+  h_nrm.alloc(nslot);
+  h_bn.alloc(nslot);
 }
 
 template struct KrylovWork<double>;
@@ -59,13 +66,18 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
   if ((int)st.size() < nslot) st.resize(nslot);
   const bool left = (M != nullptr) && !o.flexible;
 
-  std::vector<double2> hbuf((size_t)W.nslot * (m + 1));
-  std::vector<double2> nbuf(W.nslot);
+  // All host<->device transfers of the loop go through pinned staging buffers
+  // of the workspace; each staging buffer is rewritten only after the stream
+  // has been synchronised since its last copy.
+  const double2* nbuf = W.h_nrm.p;
+  const double2* hbuf = W.h_h.p;
   auto upload = [&](const std::vector<int>& v, int* dst) {
-    VH_HIP(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice, strm));
+    int* stage = dst == W.aux_slots.p ? W.h_aux.p : W.h_slots.p;
+    std::copy(v.begin(), v.end(), stage);
+    VH_HIP(hipMemcpyAsync(dst, stage, v.size() * sizeof(int), hipMemcpyHostToDevice, strm));
   };
   auto fetch_norms = [&](int cnt) {
-    VH_HIP(hipMemcpyAsync(nbuf.data(), W.nrm.p, cnt * sizeof(double2), hipMemcpyDeviceToHost, strm));
+    VH_HIP(hipMemcpyAsync(W.h_nrm.p, W.nrm.p, cnt * sizeof(double2), hipMemcpyDeviceToHost, strm));
     VH_HIP(hipStreamSynchronize(strm));
   };
 
@@ -98,13 +110,14 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
   }
 
   std::vector<cd> y, res;
-  std::vector<double2> cbuf;
   // x += basis * y for the listed systems (k vectors); optionally r = V res (k+1 vectors)
   auto finish = [&](const std::vector<int>& L, int k, bool restart_r) {
     const int nl = (int)L.size();
     if (nl == 0) return;
-    cbuf.assign((size_t)nl * (m + 1), double2{0, 0});
-    std::vector<double2> rbuf((size_t)nl * (m + 1), double2{0, 0});
+    double2* cbuf = W.h_coef.p;
+    double2* rbuf = W.h_rcoef.p;
+    std::fill(cbuf, cbuf + (size_t)nl * (m + 1), double2{0, 0});
+    std::fill(rbuf, rbuf + (size_t)nl * (m + 1), double2{0, 0});
     for (int a = 0; a < nl; ++a) {
       const int s = L[a];
       gv[s].solve(y);
@@ -117,12 +130,12 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
       st[s].outer = outit[s];
     }
     upload(L, W.aux_slots.p);
-    VH_HIP(hipMemcpyAsync(W.coef.p, cbuf.data(), cbuf.size() * sizeof(double2), hipMemcpyHostToDevice, strm));
+    VH_HIP(hipMemcpyAsync(W.coef.p, cbuf, (size_t)nl * (m + 1) * sizeof(double2), hipMemcpyHostToDevice, strm));
     const C* basis = o.flexible ? W.Z.p : W.V.p;
     ops.combine(basis, vstr, x, k, nl, W.aux_slots.p, W.coef.p, 1.0, 1.0, nullptr);      // pgmres_iter.m:136-152
     if (restart_r) {
-      // the coefficient buffer is read by the kernel above on the same stream, so it can be reused
-      VH_HIP(hipMemcpyAsync(W.coef.p, rbuf.data(), rbuf.size() * sizeof(double2), hipMemcpyHostToDevice, strm));
+      // the device coefficient buffer is read by the kernel above on the same stream, so it can be reused
+      VH_HIP(hipMemcpyAsync(W.coef.p, rbuf, (size_t)nl * (m + 1) * sizeof(double2), hipMemcpyHostToDevice, strm));
       ops.combine(W.V.p, vstr, W.r.p, k + 1, nl, W.aux_slots.p, W.coef.p, 0.0, 1.0, nullptr);  // :153
     }
     VH_HIP(hipStreamSynchronize(strm));   // host buffers above are reused
@@ -160,14 +173,13 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
       ops.mdot(W.V.p, vstr, W.w.p, k, nact, W.slots_dev.p, W.h.p);
       ops.combine(W.V.p, vstr, W.w.p, k, nact, W.slots_dev.p, W.h.p, 1.0, -1.0, W.nrm.p);
       ops.scale(W.w.p, W.V.p + (int64_t)k * vstr, nact, W.slots_dev.p, W.nrm.p);
-      VH_HIP(hipMemcpy2DAsync(hbuf.data(), k * sizeof(double2), W.h.p, (m + 1) * sizeof(double2), k * sizeof(double2),
-                              nact, hipMemcpyDeviceToHost, strm));
+      VH_HIP(hipMemcpyAsync(W.h_h.p, W.h.p, (size_t)nact * (m + 1) * sizeof(double2), hipMemcpyDeviceToHost, strm));
       fetch_norms(nact);
       std::vector<int> conv, keep;
       std::vector<cd> hc(k);
       for (int a = 0; a < nact; ++a) {
         const int s = act[a];
-        for (int j = 0; j < k; ++j) hc[j] = cd(hbuf[(size_t)a * k + j].x, hbuf[(size_t)a * k + j].y);
+        for (int j = 0; j < k; ++j) hc[j] = cd(hbuf[(size_t)a * (m + 1) + j].x, hbuf[(size_t)a * (m + 1) + j].y);
         const double hn = std::sqrt(nbuf[a].x);
         const double rres = gv[s].push(hc.data(), hn);
         st[s].resvec.push_back(rres / bnorm[s]);
@@ -195,9 +207,9 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
     ops.sub_from(b, n, W.t.p, nsys, W.slots_dev.p);
     ops.norm2(W.t.p, nsys, W.slots_dev.p, W.nrm.p);
     fetch_norms(nsys);
-    std::vector<double2> bn(nsys);
+    const double2* bn = W.h_bn.p;
     ops.norm2(b, nsys, W.slots_dev.p, W.nrm.p);
-    VH_HIP(hipMemcpyAsync(bn.data(), W.nrm.p, nsys * sizeof(double2), hipMemcpyDeviceToHost, strm));
+    VH_HIP(hipMemcpyAsync(W.h_bn.p, W.nrm.p, nsys * sizeof(double2), hipMemcpyDeviceToHost, strm));
     VH_HIP(hipStreamSynchronize(strm));
     for (int a = 0; a < nsys; ++a) {
       const int s = sys[a];
