@@ -49,7 +49,45 @@ def parse():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
     ap.add_argument('--cpu-solves', type=int, default=2)
+    ap.add_argument('--probe-pass-z', action='store_true',
+                    help='only the 128^3 complex64 matvec (3 calls) and a reference elementwise kernel of known '
+                         'traffic (1 GiB read + 1 GiB written), for hardware-counter runs')
     return ap.parse_args()
+
+
+TRAFFIC_FILE = os.path.join(ROOT, 'profiles', 'pass_z_cube128_c64_traffic.md')
+
+
+def recorded_traffic():
+    """HBM bytes per launch of the roofline kernel, from the committed
+    FETCH_SIZE / WRITE_SIZE summary (None when absent)."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith('traffic_bytes_per_launch:'):
+                return int(line.split(':')[1])
+    except OSError:
+        pass
+    return None
+
+
+def probe_pass_z(ctx, n=128):
+    import torch
+    import voxhenry_amd as vh
+    G = vh.Geometry(ctx, (n, n, n), 1e-6, np.arange(n ** 3), pad_mode=1)
+    G.generate_green()
+    best, ms = G.autotune(vh.VH_C64, nrhs=1, reps=3)
+    log('[probe] pass-Z shape %d (ms per shape %s)' % (best, np.round(ms, 3)))
+    x = torch.zeros((1, G.num_curr), dtype=torch.complex64, device='cuda')
+    y = torch.zeros_like(x)
+    for _ in range(3):
+        G.matvec_timed(x, y, 1j * 1e4, vh.VH_C64, 2, 1)
+    a = torch.empty(1 << 28, dtype=torch.float32, device='cuda')
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    log('[probe] done')
 
 
 def log(*a):
@@ -86,6 +124,7 @@ def matvec_bench(ctx, n, reps):
     G = vh.Geometry(ctx, (L, M, N), st['dx'], np.arange(L * M * N), pad_mode=1)
     G.generate_green()
     G.set_impedance(np.full(L * M * N, 1.0 / 5.8e7, dtype=complex))
+    tuned = {name: G.autotune(p, nrhs=1, reps=3)[0] for name, p in (('c128', vh.VH_C128), ('c64', vh.VH_C64))}
     f = 1e9
     jwmu = 1j * 2 * np.pi * f * vh.MU0
     nc = G.num_curr
@@ -109,6 +148,7 @@ def matvec_bench(ctx, n, reps):
                                  for i, k in enumerate(PASS_NAMES)})
         del x, y
     out['c64_vs_c128_rel'] = float(np.linalg.norm(res['c64'] - res['c128']) / np.linalg.norm(res['c128']))
+    out['zshape'] = tuned
     out['grid'] = [L, M, N]
     out['pad'] = [Px, Py, Pz]
     G.close()
@@ -124,6 +164,8 @@ def bus_setup(ctx, nfreq):
     t0 = time.perf_counter()
     S = vsol.Solver(ctx, st)
     S.G.set_impedance(vsol.impedance(st, S.idx, freqs[0]))
+    # pass shapes tuned for the batch widths of the solve (untimed set-up)
+    S.zshape = {p: S.G.autotune(p, nrhs=64, reps=2)[0] for p in (vh.VH_C64, vh.VH_C128)}
     precs = []
     for f in freqs:
         P = vh.Precon(S.G)
@@ -132,27 +174,39 @@ def bus_setup(ctx, nfreq):
     return S, freqs, precs, time.perf_counter() - t0
 
 
-def bus_step(S, freqs, precs, method):
-    """One step: every (port, frequency) system of the bus in one batched solve
-    (per-system j*omega*mu, the preconditioner of its own frequency); returns
-    R, L (per frequency) and the solver statistics."""
+def bus_rhs(S, freqs):
+    """Device-resident right-hand sides of one step: the port vectors repeated
+    for every frequency (system s = frequency s // nports, port s % nports),
+    with each system's j*omega*mu and preconditioner index."""
+    import torch
     import voxhenry_amd as vh
     npo = S.nports
-    B = np.repeat(S.rhs[None], len(freqs), axis=0).reshape(-1, S.G.n)
+    rhs = torch.tensor(S.rhs, dtype=torch.complex128, device='cuda')
+    B = rhs.repeat(len(freqs), 1).contiguous()
     jw = np.repeat(1j * 2 * np.pi * np.asarray(freqs) * vh.MU0, npo)
     pidx = np.repeat(np.arange(len(freqs)), npo)
+    # the port vectors are nonzero on the port node rows only
+    nz = torch.tensor(np.flatnonzero(np.any(S.rhs != 0, axis=0)), device='cuda')
+    Pnz = rhs.index_select(1, nz).transpose(0, 1).contiguous()     # (nnz, nports)
+    return (nz, Pnz), B, jw, pidx
+
+
+def bus_step(S, freqs, precs, method, data, **opts):
+    """One step: every (port, frequency) system of the bus in one batched solve
+    (per-system j*omega*mu, the preconditioner of its own frequency), then the
+    port admittances Y(f) = P^T X(f) on the device; returns the port
+    impedance matrices Z(f) and the solver statistics."""
+    import torch
+    (nz, Pnz), B, jw, pidx = data
+    npo = S.nports
     t = time.perf_counter()
-    X, st = S.G.solve_batch(B, jw, precs, pidx, method=method, tol=1e-8)
+    X, st = S.G.solve_batch(B, jw, precs, pidx, method=method, tol=1e-8, **opts)
+    Xp = X.index_select(1, nz).reshape(len(freqs), npo, -1)      # port rows of every solution
+    Yh = torch.matmul(Xp, Pnz).cpu().numpy()                    # Y[k] = X_k P^T (nports x nports)
     log('[bench]   %s step solve %.3f s, iters mean %.1f, inner mean %.1f, max relres %.2e'
         % (method, time.perf_counter() - t, st['iters'].mean(), st['inner_iters'].mean(), st['relres'].max()))
-    X = X.reshape(len(freqs), npo, -1)
-    R = np.zeros((len(freqs), npo, npo))
-    Lh = np.zeros_like(R)
-    for k, f in enumerate(freqs):
-        Z = np.linalg.inv(S.rhs @ X[k].T)
-        R[k] = np.abs(Z.real)
-        Lh[k] = np.abs(Z.imag) / (2 * np.pi * f)
-    return R, Lh, st
+    Z = np.stack([np.linalg.inv(Yh[k].T) for k in range(len(freqs))])
+    return Z, st
 
 
 def timed_steps(fn, steps, warmup):
@@ -202,6 +256,9 @@ def main():
     import voxhenry_amd as vh
     ctx = vh.Context(0)
     ctx.use_torch_stream()
+    if args.probe_pass_z:
+        probe_pass_z(ctx)
+        return
 
     log('[bench] FFT-matvec microbench %d^3' % args.cube)
     mv = matvec_bench(ctx, args.cube, args.matvec_reps)
@@ -212,16 +269,25 @@ def main():
     nsol = len(freqs) * S.nports
     log('[bench] set-up %.1f s, %d systems per step' % (t_setup, nsol))
 
-    ms_mixed, (Rm, Lm, stm) = timed_steps(lambda: bus_step(S, freqs, precs, 'mixed'), args.steps, args.warmup)
+    data = bus_rhs(S, freqs)
+    ms_mixed, (Zm, stm) = timed_steps(lambda: bus_step(S, freqs, precs, 'mixed', data), args.steps, args.warmup)
     log('[bench] mixed step %.3f s' % ms_mixed)
     fp64 = None
     if not args.no_fp64:
-        s64, (R64, L64, st64) = timed_steps(lambda: bus_step(S, freqs, precs, 'fp64'), max(1, args.steps // 2), 0)
+        s64, (Z64, st64) = timed_steps(lambda: bus_step(S, freqs, precs, 'fp64', data), max(1, args.steps // 2),
+                                       max(1, args.warmup))
         log('[bench] fp64 step %.3f s' % s64)
-        eR = float(np.max(np.abs(Rm - R64) / np.abs(R64)))
-        eL = float(np.max(np.abs(Lm - L64) / np.abs(L64)))
-        fp64 = dict(solves_per_s=nsol / s64, s_per_step=s64, max_rel_err_R_mixed_vs_fp64=eR,
-                    max_rel_err_L_mixed_vs_fp64=eL, iters_mean=float(st64['iters'].mean()),
+        w = 2 * np.pi * np.asarray(freqs)[:, None]
+        dm, d64 = np.diagonal(Zm, axis1=1, axis2=2), np.diagonal(Z64, axis1=1, axis2=2)
+        fp64 = dict(solves_per_s=nsol / s64, s_per_step=s64,
+                    # self resistance / inductance of every port, mixed vs FP64, relative
+                    max_rel_err_R_mixed_vs_fp64=float(np.max(np.abs(dm.real - d64.real) / np.abs(d64.real))),
+                    max_rel_err_L_mixed_vs_fp64=float(np.max(np.abs(dm.imag - d64.imag) / np.abs(d64.imag))),
+                    # whole port impedance matrix, normwise per frequency
+                    max_rel_err_Z_mixed_vs_fp64=float(max(np.linalg.norm(Zm[k] - Z64[k]) / np.linalg.norm(Z64[k])
+                                                          for k in range(len(freqs)))),
+                    L_self_nH_at_fmin=float(d64[0].imag.mean() / w[0, 0] * 1e9),
+                    iters_mean=float(st64['iters'].mean()), iters_max=int(st64['iters'].max()),
                     relres_max=float(st64['relres'].max()), all_converged=bool(np.all(st64['flag'] == 0)))
     mixed = dict(solves_per_s=nsol / ms_mixed, s_per_step=ms_mixed, outer_iters_mean=float(stm['iters'].mean()),
                  inner_iters_mean=float(stm['inner_iters'].mean()), relres_max=float(stm['relres'].max()),
@@ -230,7 +296,7 @@ def main():
     z = mv['c64']['passes']['Z']
     roof = dict(kernel='k_pass_z<float> (z-FFT + Green product + inverse z-FFT), 128^3 grid, complex64',
                 bound='hbm', achieved=z['GBs'], peak=HBM_PEAK_GBS, unit='GB/s',
-                frac=(z['GBs'] / HBM_PEAK_GBS) if z['GBs'] else None, traffic=None)
+                frac=(z['GBs'] / HBM_PEAK_GBS) if z['GBs'] else None, traffic=recorded_traffic())
 
     cpu = None
     if not args.no_cpu:

@@ -107,6 +107,17 @@ int vh_matvec(vh_geom* g, double jwmu_re, double jwmu_im, int prec, int mode, in
 int vh_matvec_timed(vh_geom* g, double jwmu_re, double jwmu_im, int prec, int mode, int64_t nrhs, const void* x,
                     int64_t ldx, void* y, int64_t ldy, int reps, double* ms_total, double* ms_pass, int* npass);
 
+/* Autotuner of the FFT pass shapes (no counterpart in the reference, whose
+ * FFT is MATLAB's fftn): times every compiled pass-Z shape (threads per
+ * workgroup x elements per thread, which fixes the radices and the kx tile
+ * width) on nrhs device vectors of this geometry (convolution only, `reps`
+ * calls each) and keeps the fastest for precision prec.  ms_shape (optional,
+ * nshape entries) receives the average time of one matvec per shape;
+ * *best receives the chosen index. */
+int vh_geom_autotune(vh_geom* g, int prec, int64_t nrhs, int reps, int* best, double* ms_shape, int* nshape);
+/* Select a pass-Z shape by index (as reported by vh_geom_autotune). */
+int vh_geom_set_zshape(vh_geom* g, int prec, int shape);
+
 /* ---- preconditioner ---------------------------------------------------------
  * Replaces lse_sparse_precon_prepare.m (schur_invert + ldlt_decomp + CHOLMOD)
  * and lse_sparse_precon_multiply.m:61-97.  prepare() computes A_inv =

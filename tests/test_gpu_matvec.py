@@ -192,3 +192,33 @@ def test_matvec_radix_shapes(ctx, dims, hollow):
         assert rel(Y[k], ref) < TOL64, (k, rel(Y[k], ref))
         assert rel(Y32[k].astype(complex), ref) < TOL32, (k, rel(Y32[k].astype(complex), ref))
     G.close()
+
+
+@pytest.mark.parametrize('dims', [(16, 16, 16), (400, 4, 4), (9, 33, 17)])
+def test_matvec_every_zshape(ctx, dims):
+    """Every compiled pass-Z shape (the autotuner's choices) gives the same
+    operator as the oracle; the autotuner itself leaves a valid choice."""
+    import voxhenry_amd as vh
+    say('zshapes', dims)
+    L, M, N = dims
+    geo = problems.bar(L, M, N)
+    s = topology.build_system(geo)
+    f = 3e9
+    jwmu = 1j * 2 * np.pi * f * MU0
+    fN2, _ = green.circulant_tensor(L, M, N, geo['dx'])
+    G = problems.device_geometry(ctx, geo, s, pad_mode=1, f=f)
+    zr, zi = topology.impedance(geo, s['idxS'], f)
+    rng = np.random.default_rng(9)
+    n = s['num_curr'] + s['num_node']
+    X = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    ref = [omv.lse_matvect_mult(X[k], jwmu * fN2, s['Ar'], zr, zi, geo['dx'], s['idxS5']) for k in range(2)]
+    for prec, nsh, tol in ((vh.VH_C128, 2, TOL64), (vh.VH_C64, 4, TOL32)):
+        for sh in range(nsh):
+            G.set_zshape(prec, sh)
+            Y = G.matvec(X if prec == vh.VH_C128 else X.astype(np.complex64), jwmu, prec, 0)
+            for k in range(2):
+                e = rel(Y[k].astype(complex), ref[k])
+                assert e < tol, (prec, sh, k, e)
+        best, ms = G.autotune(prec, nrhs=2, reps=1)
+        assert 0 <= best < nsh and len(ms) == nsh and np.all(ms > 0)
+    G.close()

@@ -244,30 +244,53 @@ class Geometry:
                     inner_restart=40, inner_tol=1e-5, hbm_budget=0.0):
         """Many (port, frequency) systems in one batched solve (vh_solve_batch).
 
-        B: (nsys, n) rhs; jwmu: (nsys,) complex; precons: list of Precon;
-        pidx: (nsys,) index into precons.  method 'fp64' (pgmres semantics,
-        default restart 100 / maxit 10) or 'mixed' (default outer restart 8 /
-        maxit 4)."""
+        B: (nsys, n) rhs, a numpy array (host) or a complex128 torch tensor on
+        the device (then X is a device tensor too); jwmu: (nsys,) complex;
+        precons: list of Precon; pidx: (nsys,) index into precons.  method
+        'fp64' (pgmres semantics, default restart 100 / maxit 10) or 'mixed'
+        (default outer restart 8 / maxit 4)."""
         m = 0 if method == 'fp64' else 1
         if restart is None:
             restart = 100 if m == 0 else 8
         if maxit is None:
             maxit = 10 if m == 0 else 4
-        BB = np.ascontiguousarray(np.atleast_2d(B), dtype=np.complex128)
+        on_dev = not isinstance(B, np.ndarray)
+        if on_dev:
+            torch = _torch()
+            assert B.dtype == torch.complex128 and B.is_cuda and B.dim() == 2
+            BB = B.contiguous()
+            self.ctx.use_torch_stream()
+        else:
+            BB = np.ascontiguousarray(np.atleast_2d(B), dtype=np.complex128)
         nsys, n = BB.shape
         assert n == self.n
         jw = np.ascontiguousarray(np.asarray(jwmu, dtype=np.complex128).ravel())
         pi = np.ascontiguousarray(np.asarray(pidx, dtype=np.int32).ravel())
         hs = (C.c_void_p * len(precons))(*[p.h.value for p in precons])
-        X = np.zeros_like(BB)
+        X = _torch().zeros_like(BB) if on_dev else np.zeros_like(BB)
         flag = np.zeros(nsys, np.int32)
         relres = np.zeros(nsys)
         its = np.zeros(nsys, np.int32)
         iits = np.zeros(nsys, np.int32)
         check(lib().vh_solve_batch(self.h, len(precons), C.cast(hs, C.c_void_p), nsys, ptr(jw), ptr(pi), ptr(BB),
-                                   ptr(X), 0, m, tol, restart, maxit, inner_restart, inner_tol, float(hbm_budget),
+                                   ptr(X), int(on_dev), m, tol, restart, maxit, inner_restart, inner_tol,
+                                   float(hbm_budget),
                                    ptr(flag), ptr(relres), ptr(its), ptr(iits)))
         return X, dict(flag=flag, relres=relres, iters=its, inner_iters=iits)
+
+    def autotune(self, prec, nrhs=1, reps=3):
+        """Time every compiled pass-Z shape on nrhs vectors of this geometry
+        and keep the fastest (vh_geom_autotune); returns (best index, ms per
+        shape)."""
+        self.ctx.use_torch_stream()
+        best = C.c_int(0)
+        ms = np.zeros(16)
+        nsh = C.c_int(16)
+        check(lib().vh_geom_autotune(self.h, prec, int(nrhs), int(reps), C.byref(best), ptr(ms), C.byref(nsh)))
+        return best.value, ms[:nsh.value]
+
+    def set_zshape(self, prec, shape):
+        check(lib().vh_geom_set_zshape(self.h, prec, int(shape)))
 
     def fft3d(self, data, prec, direction):
         self.ctx.use_torch_stream()

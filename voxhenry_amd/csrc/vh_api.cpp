@@ -17,6 +17,18 @@ struct vh_geom_s {
 struct vh_precon_s { vh::Precon P; };
 
 namespace {
+// Caller host memory is pageable: copies to and from it are blocking copies
+// made after the context stream has drained (device to host) or before any
+// work that reads the buffer is queued (host to device).
+vh::KrylovCache& krylov_cache(vh::Geometry& G) {
+  if (!G.krylov) G.krylov = std::make_shared<vh::KrylovCache>();
+  return *G.krylov;
+}
+void h2d(void* dst, const void* src, size_t bytes) { VH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); }
+void d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  VH_HIP(hipStreamSynchronize(st));
+  VH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+}
 thread_local std::string g_last_error;
 
 template <typename F> int guard(F&& f) {
@@ -216,7 +228,7 @@ int vh_matvec(vh_geom* g, double jre, double jim, int prec, int mode, int64_t nr
     if (!on_device) {
       bx.alloc((size_t)nrhs * ldx * es);
       by.alloc((size_t)nrhs * ldy * es);
-      VH_HIP(hipMemcpyAsync(bx.p, x, bx.bytes(), hipMemcpyHostToDevice, st));
+      h2d(bx.p, x, bx.bytes());
       VH_HIP(hipMemsetAsync(by.p, 0, by.bytes(), st));
       dx = bx.p;
       dy = by.p;
@@ -226,9 +238,72 @@ int vh_matvec(vh_geom* g, double jre, double jim, int prec, int mode, int64_t nr
     else
       vh::matvec<float>(G, (int)nrhs, (const float2*)dx, ldx, (float2*)dy, ldy, jre, jim, mode);
     if (!on_device) {
-      VH_HIP(hipMemcpyAsync(y, by.p, by.bytes(), hipMemcpyDeviceToHost, st));
-      VH_HIP(hipStreamSynchronize(st));
+      d2h(y, by.p, by.bytes(), st);
     }
+  });
+}
+
+int vh_geom_autotune(vh_geom* g, int prec, int64_t nrhs, int reps, int* best, double* ms_shape, int* nshape) {
+  return guard([&] {
+    need(g, "g"); need(best, "best");
+    auto& G = g->G;
+    VH_REQUIRE(prec == VH_C128 || prec == VH_C64, "prec must be VH_C128 or VH_C64");
+    VH_REQUIRE(nrhs >= 1 && nrhs <= 13107 && reps >= 1, "nrhs / reps out of range");
+    VH_REQUIRE(G.has_tensor, "autotune: Green's tensor not set");
+    const int p = prec == VH_C128 ? 0 : 1;
+    const int ns = vh::matvec_num_zshapes(p);
+    const int64_t n = G.num_curr;
+    const size_t es = prec == VH_C128 ? 16 : 8;
+    vh::DevBuf<char> x((size_t)nrhs * n * es), y((size_t)nrhs * n * es);
+    hipStream_t st = G.ctx->stream;
+    VH_HIP(hipMemsetAsync(x.p, 0, x.bytes(), st));
+    hipEvent_t e0, e1;
+    VH_HIP(hipEventCreate(&e0));
+    VH_HIP(hipEventCreate(&e1));
+    auto run = [&] {
+      if (prec == VH_C128)
+        vh::matvec<double>(G, (int)nrhs, (const double2*)x.p, n, (double2*)y.p, n, 0.0, 1.0, 2);
+      else
+        vh::matvec<float>(G, (int)nrhs, (const float2*)x.p, n, (float2*)y.p, n, 0.0, 1.0, 2);
+    };
+    const int keep = G.zshape[prec == VH_C128 ? 1 : 0];
+    int arg = 0;
+    double tbest = 1e300;
+    try {
+      for (int i = 0; i < ns; ++i) {
+        G.zshape[prec == VH_C128 ? 1 : 0] = i;
+        run();   // warm-up
+        VH_HIP(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) run();
+        VH_HIP(hipEventRecord(e1, st));
+        VH_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        VH_HIP(hipEventElapsedTime(&ms, e0, e1));
+        const double t = ms / reps;
+        if (ms_shape && nshape && i < *nshape) ms_shape[i] = t;
+        if (t < tbest) { tbest = t; arg = i; }
+      }
+    } catch (...) {
+      G.zshape[prec == VH_C128 ? 1 : 0] = keep;
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    G.zshape[prec == VH_C128 ? 1 : 0] = arg;
+    *best = arg;
+    if (nshape) *nshape = ns;
+  });
+}
+
+int vh_geom_set_zshape(vh_geom* g, int prec, int shape) {
+  return guard([&] {
+    need(g, "g");
+    VH_REQUIRE(prec == VH_C128 || prec == VH_C64, "prec must be VH_C128 or VH_C64");
+    const int p = prec == VH_C128 ? 0 : 1;
+    VH_REQUIRE(shape >= 0 && shape < vh::matvec_num_zshapes(p), "shape index out of range");
+    g->G.zshape[prec == VH_C128 ? 1 : 0] = shape;
   });
 }
 
@@ -327,7 +402,7 @@ struct Io {
   void in(const void* hx, int64_t nrhs, int64_t ldx, size_t es, int on_device, hipStream_t st, int64_t ldy) {
     if (on_device) { x = hx; return; }
     bx.alloc((size_t)nrhs * ldx * es);
-    VH_HIP(hipMemcpyAsync(bx.p, hx, bx.bytes(), hipMemcpyHostToDevice, st));
+    h2d(bx.p, hx, bx.bytes());
     by.alloc((size_t)nrhs * ldy * es);
     VH_HIP(hipMemsetAsync(by.p, 0, by.bytes(), st));
     x = bx.p;
@@ -354,7 +429,7 @@ int vh_precon_apply(vh_precon* p, int prec, int64_t nrhs, const void* x, int64_t
     else
       vh::precon_apply<float>(p->P, (int)nrhs, (const float2*)io.x, ldx, (float2*)dy, ldy);
     if (!on_device) {
-      VH_HIP(hipMemcpyAsync(y, io.y, (size_t)nrhs * ldy * es, hipMemcpyDeviceToHost, st));
+      d2h(y, io.y, (size_t)nrhs * ldy * es, st);
     }
     VH_HIP(hipStreamSynchronize(st));
   });
@@ -383,8 +458,10 @@ int vh_gmres(vh_geom* g, vh_precon* p, double jre, double jim, int prec, int fle
     vh::SolveOpts o;
     o.restart = restart; o.maxit = maxit; o.tol = tol; o.flexible = flexible != 0; o.final_residual = true;
     std::vector<vh::SysStats> stats;
+    vh::KrylovCache& K = krylov_cache(G);
     if (prec == VH_C128) {
-      vh::KrylovWork<double> W;
+      K.layout(n, (int)nrhs, restart, o.flexible, 0, false, false);
+      vh::KrylovWork<double>& W = K.w64;
       vh::OpFn<double> A = [&](const std::vector<int>& act, const int* sl, const double2* xx, double2* yy) {
         vh::matvec<double>(G, (int)act.size(), xx, n, yy, n, jre, jim, 0, sl);
       };
@@ -393,7 +470,8 @@ int vh_gmres(vh_geom* g, vh_precon* p, double jre, double jim, int prec, int fle
       };
       vh::gmres_batch<double>(G.ctx, n, A, p ? &M : nullptr, (const double2*)io.x, (double2*)dx, sys, o, W, stats);
     } else {
-      vh::KrylovWork<float> W;
+      K.layout(n, (int)nrhs, 0, false, restart, o.flexible, false);
+      vh::KrylovWork<float>& W = K.w32;
       vh::OpFn<float> A = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
         vh::matvec<float>(G, (int)act.size(), xx, n, yy, n, jre, jim, 0, sl);
       };
@@ -402,7 +480,7 @@ int vh_gmres(vh_geom* g, vh_precon* p, double jre, double jim, int prec, int fle
       };
       vh::gmres_batch<float>(G.ctx, n, A, p ? &M : nullptr, (const float2*)io.x, (float2*)dx, sys, o, W, stats);
     }
-    if (!on_device) VH_HIP(hipMemcpyAsync(x, io.y, (size_t)nrhs * n * es, hipMemcpyDeviceToHost, st));
+    if (!on_device) d2h(x, io.y, (size_t)nrhs * n * es, st);
     VH_HIP(hipStreamSynchronize(st));
     for (int i = 0; i < nrhs; ++i) {
       const auto& s = stats[i];
@@ -446,11 +524,11 @@ int vh_solve_mixed(vh_geom* g, vh_precon* p, double jre, double jim, int64_t nrh
     vh::OpFn<float> M32 = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
       vh::precon_apply<float>(p->P, (int)act.size(), xx, n, yy, n, sl);
     };
-    vh::KrylovWork<double> W64;
-    vh::KrylovWork<float> W32;
     std::vector<vh::SysStats> stats;
-    vh::fgmres_mixed(G.ctx, n, A64, A32, M32, (const double2*)io.x, (double2*)dx, sys, o, W64, W32, stats);
-    if (!on_device) VH_HIP(hipMemcpyAsync(x, io.y, (size_t)nrhs * n * 16, hipMemcpyDeviceToHost, st));
+    vh::KrylovCache& K = krylov_cache(G);
+    K.layout(n, (int)nrhs, outer_restart, true, inner_restart, false, true);
+    vh::fgmres_mixed(G.ctx, n, A64, A32, M32, (const double2*)io.x, (double2*)dx, sys, o, K, stats);
+    if (!on_device) d2h(x, io.y, (size_t)nrhs * n * 16, st);
     VH_HIP(hipStreamSynchronize(st));
     for (int i = 0; i < nrhs; ++i) {
       if (flag) flag[i] = stats[i].flag;
@@ -528,27 +606,41 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
     const double2* db = (const double2*)io.x;
     // per-slot j*omega*mu on the device
     vh::DevBuf<double2> alphas(nsys);
-    VH_HIP(hipMemcpyAsync(alphas.p, jwmu, nsys * sizeof(double2), hipMemcpyHostToDevice, st));
-    // group lists for the preconditioner apply (pinned staging, rewritten only
-    // after the stream synchronisation that ends each apply)
-    vh::DevBuf<int> gslots(nsys);
-    vh::PinnedBuf<int> gbuf;
-    gbuf.alloc(nsys);
-    // HBM budget -> chunk size
+    h2d(alphas.p, jwmu, nsys * sizeof(double2));
+    // factor tables of the preconditioners and the preconditioner of every system
+    vh::PreconSymbolic* sym = G.precon_sym.get();
+    std::vector<vh::PcTab<double>> t64(nprecon);
+    std::vector<vh::PcTab<float>> t32(nprecon);
+    for (int64_t k = 0; k < nprecon; ++k) {
+      VH_REQUIRE(precons[k]->P.G == &G && precons[k]->P.sym.get() == sym,
+                 "every preconditioner must belong to this geometry");
+      t64[k] = precons[k]->P.tab64_h;
+      t32[k] = precons[k]->P.tab32_h;
+    }
+    vh::DevBuf<vh::PcTab<double>> tabs64(nprecon);
+    vh::DevBuf<vh::PcTab<float>> tabs32(nprecon);
+    vh::DevBuf<int> dpidx(nsys);
+    VH_HIP(hipMemcpy(tabs64.p, t64.data(), nprecon * sizeof(t64[0]), hipMemcpyHostToDevice));
+    VH_HIP(hipMemcpy(tabs32.p, t32.data(), nprecon * sizeof(t32[0]), hipMemcpyHostToDevice));
+    VH_HIP(hipMemcpy(dpidx.p, pidx, nsys * sizeof(int), hipMemcpyHostToDevice));
+    // HBM budget -> chunk size.  The geometry's workspace arena counts as
+    // available (it is reused), so the chunk size, and with it the arena,
+    // is the same on every call with the same sizes.
+    vh::KrylovCache& K = krylov_cache(G);
     size_t freeb = 0, totalb = 0;
     VH_HIP(hipMemGetInfo(&freeb, &totalb));
-    const double budget = hbm_budget > 0 ? hbm_budget : 0.8 * (double)freeb;
-    double per_sys;
-    if (method == 0) per_sys = (double)vh::KrylovWork<double>::bytes_needed(n, restart, 1, false);
-    else per_sys = (double)vh::KrylovWork<double>::bytes_needed(n, restart, 1, true) +
-                   (double)vh::KrylovWork<float>::bytes_needed(n, inner_restart, 1, false) + 2.0 * n * 8;
+    const double budget = hbm_budget > 0 ? hbm_budget : 0.8 * ((double)freeb + (double)K.arena.n);
+    const bool mixed = method == 1;
+    double per_sys = (double)vh::KrylovCache::bytes(n, 1, restart, mixed, mixed ? inner_restart : 0, false, mixed);
     per_sys += 5.0 * n * 16;   // matvec workspaces (A + B grids are ~4-5 vectors per system)
     int chunk = (int)std::max(1.0, std::min((double)nsys, std::floor(budget / per_sys)));
     std::vector<vh::SysStats> stats(nsys);
-    vh::KrylovWork<double> W64;
-    vh::KrylovWork<float> W32;
-    // each chunk is solved with chunk-relative slots 0..cnt-1 (workspaces sized
-    // for the chunk); coff is the chunk's first system
+    if (vh::trace_on()) fprintf(stderr, "[vh] solve_batch nsys %ld chunk %d budget %.1f GB per_sys %.3f GB\n",
+                                (long)nsys, chunk, budget / 1e9, per_sys / 1e9);
+    K.layout(n, chunk, restart, mixed, mixed ? inner_restart : 0, false, mixed);
+    vh::KrylovWork<double>& W64 = K.w64;
+    // each chunk is solved with chunk-relative slots 0..cnt-1 (workspaces laid
+    // out for the chunk); coff is the chunk's first system
     int64_t coff = 0;
     vh::OpFn<double> A64 = [&](const std::vector<int>& act, const int* sl, const double2* xx, double2* yy) {
       vh::matvec<double>(G, (int)act.size(), xx, n, yy, n, 0.0, 0.0, 0, sl, alphas.p + coff);
@@ -556,31 +648,13 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
     vh::OpFn<float> A32 = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
       vh::matvec<float>(G, (int)act.size(), xx, n, yy, n, 0.0, 0.0, 0, sl, alphas.p + coff);
     };
-    // apply the preconditioners to the active systems of the chunk (slots are
-    // chunk-relative), one group per preconditioner
-    auto grouped_c = [&](const std::vector<int>& act, auto apply) {
-      std::vector<std::vector<int>> by(nprecon);
-      for (int s : act) by[pidx[coff + s]].push_back(s);
-      int off = 0;
-      for (int64_t k = 0; k < nprecon; ++k) {
-        if (by[k].empty()) continue;
-        std::copy(by[k].begin(), by[k].end(), gbuf.p + off);
-        off += (int)by[k].size();
-      }
-      VH_HIP(hipMemcpyAsync(gslots.p, gbuf.p, off * sizeof(int), hipMemcpyHostToDevice, st));
-      off = 0;
-      for (int64_t k = 0; k < nprecon; ++k) {
-        if (by[k].empty()) continue;
-        apply(precons[k]->P, (int)by[k].size(), gslots.p + off);
-        off += (int)by[k].size();
-      }
-      VH_HIP(hipStreamSynchronize(st));   // gbuf is reused by the next call
+    // preconditioner of every active system of the chunk in one pass (the
+    // system's preconditioner is dpidx[coff + chunk-relative slot])
+    vh::OpFn<double> M64 = [&](const std::vector<int>& act, const int* sl, const double2* xx, double2* yy) {
+      vh::precon_apply_cols<double>(G, *sym, tabs64.p, dpidx.p + coff, (int)act.size(), xx, n, yy, n, sl);
     };
-    vh::OpFn<double> M64 = [&](const std::vector<int>& act, const int*, const double2* xx, double2* yy) {
-      grouped_c(act, [&](vh::Precon& P, int cnt, const int* sl) { vh::precon_apply<double>(P, cnt, xx, n, yy, n, sl); });
-    };
-    vh::OpFn<float> M32 = [&](const std::vector<int>& act, const int*, const float2* xx, float2* yy) {
-      grouped_c(act, [&](vh::Precon& P, int cnt, const int* sl) { vh::precon_apply<float>(P, cnt, xx, n, yy, n, sl); });
+    vh::OpFn<float> M32 = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
+      vh::precon_apply_cols<float>(G, *sym, tabs32.p, dpidx.p + coff, (int)act.size(), xx, n, yy, n, sl);
     };
     for (int64_t s0 = 0; s0 < nsys; s0 += chunk) {
       coff = s0;
@@ -598,11 +672,11 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
         vh::MixedOpts o;
         o.tol = tol; o.outer_restart = restart; o.outer_maxit = maxit;
         o.inner_restart = inner_restart; o.inner_tol = inner_tol;
-        vh::fgmres_mixed(G.ctx, n, A64, A32, M32, bc, xc, sys, o, W64, W32, cs);
+        vh::fgmres_mixed(G.ctx, n, A64, A32, M32, bc, xc, sys, o, K, cs);
       }
       for (int i = 0; i < cnt; ++i) stats[s0 + i] = cs[i];
     }
-    if (!on_device) VH_HIP(hipMemcpyAsync(x, io.y, (size_t)nsys * n * 16, hipMemcpyDeviceToHost, st));
+    if (!on_device) d2h(x, io.y, (size_t)nsys * n * 16, st);
     VH_HIP(hipStreamSynchronize(st));
     for (int64_t s = 0; s < nsys; ++s) {
       if (flag) flag[s] = stats[s].flag;

@@ -69,6 +69,12 @@ inline bool poison_alloc() {
   return on;
 }
 
+// Debug aid: with VH_TRACE=1 the solvers print host-side phase times to stderr.
+inline bool trace_on() {
+  static const bool on = [] { const char* e = std::getenv("VH_TRACE"); return e && e[0] == '1'; }();
+  return on;
+}
+
 // RAII device allocation.
 template <typename T> struct DevBuf {
   T* p = nullptr;

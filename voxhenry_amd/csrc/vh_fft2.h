@@ -221,6 +221,7 @@ struct FftAxis2 {
   // Plan whose power-of-two radices are at most rmax (16, 8 or 4).
   template <typename R> Plan2 plan(int rmax, bool reversed) const;
   int last_radix(int rmax) const;   // last radix of the forward-order plan
+  const std::vector<Stage2>& stages(int rmax) const;   // forward-order stages (host copy)
 };
 
 // Threads per pencil: smallest power of two t <= nt_max with ceil(n/t) <= E.

@@ -40,6 +40,7 @@ struct PassTimer {
 };
 
 struct PreconSymbolic;
+struct KrylovCache;
 
 struct Geometry {
   Context* ctx = nullptr;
@@ -75,7 +76,9 @@ struct Geometry {
   std::vector<double> rhs_h;          // port right-hand sides (nports x n), from build_topology
   int nports = 0;
   PassTimer* timer = nullptr;
+  int zshape[2] = {0, 0};             // pass-Z shape index per precision [FP32, FP64] (autotuned)
   std::shared_ptr<PreconSymbolic> precon_sym;   // shared by all preconditioners of this geometry
+  std::shared_ptr<KrylovCache> krylov;          // solver workspaces kept across calls
   // matvec workspaces, grown on demand.  A (pass X output) relies on its
   // never-written empty lines reading as zero, so each precision has its own.
   DevBuf<char> wsA64, wsA32, wsB;
@@ -118,6 +121,8 @@ void geometry_set_impedance(Geometry& G, const double* z_interleaved);
 void geometry_build_topology(Geometry& G, const std::vector<std::vector<int64_t>>& Pfaces,
                              const std::vector<std::vector<int64_t>>& Nfaces, const std::vector<int64_t>& well,
                              std::vector<double>& rhs_out);
+// Number of compiled pass-Z shapes of a precision (prec 0 = FP64, 1 = FP32).
+int matvec_num_zshapes(int prec);
 void geometry_set_spectrum_full(Geometry& G, const double* fN, int64_t P0, int64_t P1, int64_t P2,
                                 double scale_re, double scale_im);
 

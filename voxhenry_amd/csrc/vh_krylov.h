@@ -58,22 +58,33 @@ struct SolveOpts {
   double tol = 1e-8;
   bool final_residual = true;     // compute relres with one more matvec
   bool flexible = false;          // fpgmres semantics (right preconditioner)
+  bool reorth = false;            // classical Gram-Schmidt with a second pass (CGS2)
+  const std::vector<double>* tol_slot = nullptr;   // per-slot tolerance (overrides tol)
 };
 
-// Krylov workspace for nslot systems of length ld with restart m.
+// Krylov workspace for nslot systems of length ld with restart m.  The large
+// vectors (basis V, flexible basis Z, w, t, r) are views into an arena owned
+// by a KrylovCache; the small per-system arrays are owned here.
 template <typename R> struct KrylovWork {
   int64_t n = 0, ld = 0;
   int m = 0, nslot = 0;
-  DevBuf<cplx<R>> V, Z, w, t, r;
-  DevBuf<double2> h, nrm, coef;
+  bool flexible = false;
+  cplx<R>* V = nullptr;   // (m+1) basis vectors, vector j of slot s at V + j*nslot*ld + s*ld
+  cplx<R>* Z = nullptr;   // m preconditioned basis vectors (flexible only)
+  cplx<R>* w = nullptr;
+  cplx<R>* t = nullptr;
+  cplx<R>* r = nullptr;
+  DevBuf<double2> h, h2, nrm, coef;
   DevBuf<int> slots_dev, aux_slots;
   // pinned host staging: slot lists, LS coefficients, Hessenberg columns, norms
   PinnedBuf<int> h_slots, h_aux;
-  PinnedBuf<double2> h_coef, h_rcoef, h_h, h_nrm, h_bn;
-  void ensure(int64_t n, int64_t ld, int m, int nslot, bool flexible);
+  PinnedBuf<double2> h_coef, h_rcoef, h_h, h_h2, h_nrm, h_bn;
   static size_t bytes_needed(int64_t ld, int m, int nslot, bool flexible) {
     return sizeof(cplx<R>) * (size_t)ld * nslot * ((m + 1) + (flexible ? m : 0) + 3);
   }
+  // Points the large vectors at base (bytes_needed bytes) and sizes the small arrays.
+  void bind(char* base, int64_t n, int64_t ld, int m, int nslot, bool flexible);
+  void unbind() { V = Z = w = t = r = nullptr; m = 0; nslot = 0; }
 };
 
 // Batched (f)pgmres with the reference's semantics for the systems listed in
@@ -88,10 +99,27 @@ struct MixedOpts {
   double tol = 1e-8;
   int outer_restart = 8, outer_maxit = 4;
   int inner_restart = 40, inner_maxit = 1;
-  double inner_tol = 1e-5;
+  double inner_tol = 1e-5;          // tightest inner tolerance (the first inner solve)
+  bool adaptive_inner = true;       // relax the inner tolerance as the outer residual falls
 };
+// Workspaces of the solvers, kept by a geometry across calls.  All large
+// vectors live in one device arena that only grows: releasing and allocating
+// a basis of hundreds of systems on every call would cost more than the solve
+// (the driver clears a reused allocation).  layout() carves the FP64 Krylov
+// workspace (m64 > 0), the FP32 one (m32 > 0) and, for the mixed solver, the
+// FP32 copies of the outer vector and the inner solution.
+struct KrylovCache {
+  DevBuf<char> arena;
+  KrylovWork<double> w64;
+  KrylovWork<float> w32;
+  float2* v32 = nullptr;
+  float2* z32 = nullptr;
+  static size_t bytes(int64_t n, int nslot, int m64, bool flex64, int m32, bool flex32, bool mixed_vectors);
+  void layout(int64_t n, int nslot, int m64, bool flex64, int m32, bool flex32, bool mixed_vectors);
+};
+
 void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<float>& A32, const OpFn<float>& M32,
                   const double2* b, double2* x, const std::vector<int>& sys, const MixedOpts& o,
-                  KrylovWork<double>& W64, KrylovWork<float>& W32, std::vector<SysStats>& st);
+                  KrylovCache& K, std::vector<SysStats>& st);   // K laid out for the slots of sys
 
 }  // namespace vh

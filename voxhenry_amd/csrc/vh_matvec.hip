@@ -30,11 +30,9 @@ template <typename R> __device__ __forceinline__ cplx<R> czero() { cplx<R> c; c.
 // Pass shapes: threads per workgroup and the bound E on elements per thread
 // and pencil.  X/Xi and Y/Yi hold one pencil element set per thread; Z holds
 // the five components' spectra of its pencils in registers between the
-// forward and the inverse transform, hence the smaller E.
+// forward and the inverse transform, hence smaller bounds (ZShape below).
 constexpr int NT_XY = 256;
-constexpr int NT_Z = 512;
 template <typename R> constexpr int e_xy() { return sizeof(R) == 8 ? 8 : 16; }
-template <typename R> constexpr int e_z() { return sizeof(R) == 8 ? 4 : 8; }
 template <int N> constexpr int ilog2() { return N <= 1 ? 0 : 1 + ilog2<N / 2>(); }
 
 // ---------------------------------------------------------------------------
@@ -115,7 +113,7 @@ __global__ __launch_bounds__(NT) void k_pass_y(const cplx<R>* in, int in_rows, i
 
 // ---------------------------------------------------------------------------
 // pass Z: forward FFT along z, Green's product, inverse FFT, in place.
-// A workgroup owns T consecutive kx of one ky column and loops over the rhs.
+// A workgroup owns T consecutive kx of one ky column of one rhs.
 // The five components are transformed one after the other through the same
 // LDS array; each leaves its spectrum in registers (last forward stage), the
 // 5x5 product is applied in registers, and the inverse transforms start from
@@ -193,7 +191,7 @@ __device__ __forceinline__ void copy_regs(C (&d)[BL][RL], const C (&s)[BL][RL]) 
 }
 
 template <typename R, int E, int NT, int RL>
-__global__ __launch_bounds__(NT) void k_pass_z(cplx<R>* __restrict__ B, int nrhs, int ltpp, GridDims g, Plan2 pf,
+__global__ __launch_bounds__(NT) void k_pass_z(cplx<R>* __restrict__ B, int ltpp, GridDims g, Plan2 pf,
                                                Plan2 pi, TensorRef<R> tens, cplx<R> alpha0,
                                                const double2* __restrict__ alphas, const int* __restrict__ slots) {
   using C = cplx<R>;
@@ -222,7 +220,8 @@ __global__ __launch_bounds__(NT) void k_pass_z(cplx<R>* __restrict__ B, int nrhs
   const int nbL = pf.st[pf.nstage - 1].nb;
   C* lds = (C*)smem + t;
   auto lx = [T](int i) { return i * T; };
-  for (int rhs = 0; rhs < nrhs; ++rhs) {
+  {
+    const int rhs = blockIdx.y;
     C alpha = alpha0;
     if (alphas) {
       const double2 a = alphas[slots ? slots[rhs] : rhs];
@@ -367,20 +366,93 @@ static void ensure_ws(Geometry& G, int nrhs) {
 static int log2i(int v) { int l = 0; while ((1 << l) < v) ++l; return l; }
 
 // Threads per pencil for a pass, or an error naming the unsupported length.
-static int pass_tpp(int n, int E, int NT, const char* axis) {
-  const int tpp = fft_tpp(n, E, NT);
+// Starts from the smallest power of two that keeps ceil(n/tpp) <= E and halves
+// it while some stage has fewer butterflies than threads and every stage's
+// butterflies are still covered (ceil(E/R) * tpp >= n/R), so short
+// transforms such as a single radix-7 stage use one thread per pencil.
+static int pass_tpp(const FftAxis2& ax, int E, int NT, const char* axis) {
+  int tpp = fft_tpp(ax.n, E, NT);
   if (tpp < 0)
-    throw Error(ERR_ARG, std::string("padded FFT length ") + std::to_string(n) + " along " + axis +
+    throw Error(ERR_ARG, std::string("padded FFT length ") + std::to_string(ax.n) + " along " + axis +
                          " exceeds the pencil kernels (max " + std::to_string(E * NT) + ")");
+  const auto& st = ax.stages(rmax_for(E));
+  while (tpp > 1) {
+    const int h = tpp / 2;
+    bool idle = false, covered = true;
+    for (const Stage2& S : st) {
+      if (S.nb < tpp) idle = true;
+      if ((E + S.rad - 1) / S.rad * h < S.nb) covered = false;
+    }
+    if (!idle || !covered) break;
+    tpp = h;
+  }
   return tpp;
 }
 
-template <typename R, int RL>
-static void launch_z(dim3 grid, size_t lds, hipStream_t st, cplx<R>* B, int nrhs, int ltpp, const GridDims& g,
-                     const Plan2& pf, const Plan2& pi, TensorRef<R> tr, cplx<R> alpha, const double2* alphas,
-                     const int* slots) {
-  hipLaunchKernelGGL((k_pass_z<R, e_z<R>(), NT_Z, RL>), grid, dim3(NT_Z), lds, st, B, nrhs, ltpp, g, pf, pi, tr,
-                     alpha, alphas, slots);
+// Z-pass shapes (threads per workgroup, element bound per thread) the
+// autotuner chooses from; entry 0 is the default.  Fewer elements per thread
+// hold fewer spectra in registers (higher occupancy) at the price of more
+// stages (radix <= rmax_for(E)) and narrower kx tiles.
+struct ZShape { int nt, e; };
+constexpr ZShape kZShapes32[] = {{512, 8}, {256, 8}, {512, 4}, {256, 4}};
+constexpr ZShape kZShapes64[] = {{512, 4}, {256, 4}};
+template <typename R> constexpr int num_zshapes() {
+  return sizeof(R) == 8 ? (int)(sizeof(kZShapes64) / sizeof(ZShape)) : (int)(sizeof(kZShapes32) / sizeof(ZShape));
+}
+template <typename R> static ZShape zshape(int i) { return sizeof(R) == 8 ? kZShapes64[i] : kZShapes32[i]; }
+
+// A one-stage plan (Pz = RL) with one thread per pencil holds exactly RL
+// elements per component: its kernel is built with E = RL, so no register is
+// reserved for butterflies that do not exist.
+template <typename R, int NT, int E, int RL>
+static void launch_z_k(dim3 grid, size_t lds, hipStream_t st, cplx<R>* B, int ltpp, const GridDims& g,
+                       const Plan2& pf, const Plan2& pi, TensorRef<R> tr, cplx<R> alpha, const double2* alphas,
+                       const int* slots) {
+  if (pf.nstage == 1 && ltpp == 0)
+    hipLaunchKernelGGL((k_pass_z<R, RL, NT, RL>), grid, dim3(NT), lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas,
+                       slots);
+  else
+    hipLaunchKernelGGL((k_pass_z<R, E, NT, RL>), grid, dim3(NT), lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas,
+                       slots);
+}
+
+template <typename R, int NT, int E>
+static void launch_z_rl(int rl, dim3 grid, size_t lds, hipStream_t st, cplx<R>* B, int ltpp, const GridDims& g,
+                        const Plan2& pf, const Plan2& pi, TensorRef<R> tr, cplx<R> alpha, const double2* alphas,
+                        const int* slots) {
+  switch (rl) {
+    case 2: launch_z_k<R, NT, E, 2>(grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
+    case 3: launch_z_k<R, NT, E, 3>(grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
+    case 4: launch_z_k<R, NT, E, 4>(grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
+    case 5: launch_z_k<R, NT, E, 5>(grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
+    case 7: launch_z_k<R, NT, E, 7>(grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
+    case 8:
+      if constexpr (rmax_for(E) >= 8) {
+        launch_z_k<R, NT, E, 8>(grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots);
+        break;
+      }
+      [[fallthrough]];
+    default: throw Error(ERR_INTERNAL, "pass Z: unexpected last radix");
+  }
+}
+
+template <typename R>
+static void launch_z(int shape, int rl, dim3 grid, size_t lds, hipStream_t st, cplx<R>* B, int ltpp,
+                     const GridDims& g, const Plan2& pf, const Plan2& pi, TensorRef<R> tr, cplx<R> alpha,
+                     const double2* alphas, const int* slots) {
+  if constexpr (sizeof(R) == 8) {
+    switch (shape) {
+      case 0: launch_z_rl<R, 512, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
+      default: launch_z_rl<R, 256, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
+    }
+  } else {
+    switch (shape) {
+      case 0: launch_z_rl<R, 512, 8>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
+      case 1: launch_z_rl<R, 256, 8>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
+      case 2: launch_z_rl<R, 512, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
+      default: launch_z_rl<R, 256, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
+    }
+  }
 }
 
 template <typename R>
@@ -394,22 +466,26 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
   cplx<R>* A = (cplx<R>*)(sizeof(R) == 8 ? G.wsA64.p : G.wsA32.p);
   cplx<R>* B = (cplx<R>*)G.wsB.p;
   hipStream_t st = G.ctx->stream;
-  constexpr int EXY = e_xy<R>(), EZ = e_z<R>();
+  constexpr int EXY = e_xy<R>();
+  const int zsh = std::min(std::max(G.zshape[sizeof(R) == 8 ? 1 : 0], 0), num_zshapes<R>() - 1);
+  const ZShape Zs = zshape<R>(zsh);
+  const int EZ = Zs.e, NTZ = Zs.nt;
   const size_t elem = sizeof(cplx<R>);
   const int dense = G.dense ? 1 : 0;
   PassTimer* tm = G.timer;
   if (tm) { tm->collect(); VH_HIP(hipEventRecord(tm->ev[0], st)); }
   auto mark = [&](int k) { if (tm) VH_HIP(hipEventRecord(tm->ev[k], st)); };
 
-  const int tx = pass_tpp(g.Px, EXY, NT_XY, "x");
-  const int ty = pass_tpp(g.Py, EXY, NT_XY, "y");
-  const int tz = pass_tpp(g.Pz, EZ, NT_Z, "z");
+  const int tx = pass_tpp(G.fx, EXY, NT_XY, "x");
+  const int ty = pass_tpp(G.fy, EXY, NT_XY, "y");
+  const int tz = pass_tpp(G.fz, EZ, NTZ, "z");
   const Plan2 px = G.fx.plan<R>(rmax_for(EXY), false);
   const Plan2 py = G.fy.plan<R>(rmax_for(EXY), false);
   const Plan2 pzf = G.fz.plan<R>(rmax_for(EZ), false);
   const Plan2 pzi = G.fz.plan<R>(rmax_for(EZ), true);
-  const int Tx = NT_XY / tx, Ty = NT_XY / ty, Tz = NT_Z / tz;
-  const size_t lds_x = (size_t)Tx * (lpad_host(g.Px - 1) + 1) * elem;
+  const int Tx = NT_XY / tx, Ty = NT_XY / ty, Tz = NTZ / tz;
+  const size_t lds_x = px.nstage > 1 ? (size_t)Tx * (lpad_host(g.Px - 1) + 1) * elem : 0;
+  const size_t lds_y = py.nstage > 1 ? (size_t)Ty * g.Py * elem : 0;
 
   // X
   if (G.nlines > 0) {
@@ -422,37 +498,29 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
   // Y
   {
     dim3 grid(ceil_div(g.Px, Ty) * g.N, nrhs * 5);
-    hipLaunchKernelGGL((k_pass_y<R, -1, EXY, NT_XY>), grid, dim3(NT_XY), (size_t)Ty * g.Py * elem, st, A, g.M, g.M,
+    hipLaunchKernelGGL((k_pass_y<R, -1, EXY, NT_XY>), grid, dim3(NT_XY), lds_y, st, A, g.M, g.M,
                        B, g.Py, g.Py, log2i(ty), g, py);
     VH_LAUNCHED();
   }
   mark(2);
   // Z
   {
-    dim3 grid(ceil_div(g.Px, Tz) * g.Py);
+    dim3 grid(ceil_div(g.Px, Tz) * g.Py, nrhs);
     cplx<R> alpha;
     alpha.x = (R)are;
     alpha.y = (R)aim;
     TensorRef<R> tr;
     if constexpr (sizeof(R) == 8) tr.t = G.t64.p; else tr.t = G.t32.p;
-    const size_t lds = (size_t)Tz * g.Pz * elem;
+    const size_t lds = pzf.nstage > 1 ? (size_t)Tz * g.Pz * elem : 0;   // one stage: registers only
     const int ltz = log2i(tz);
-    switch (G.fz.last_radix(rmax_for(EZ))) {
-      case 2: launch_z<R, 2>(grid, lds, st, B, nrhs, ltz, g, pzf, pzi, tr, alpha, alphas, slots); break;
-      case 3: launch_z<R, 3>(grid, lds, st, B, nrhs, ltz, g, pzf, pzi, tr, alpha, alphas, slots); break;
-      case 4: launch_z<R, 4>(grid, lds, st, B, nrhs, ltz, g, pzf, pzi, tr, alpha, alphas, slots); break;
-      case 5: launch_z<R, 5>(grid, lds, st, B, nrhs, ltz, g, pzf, pzi, tr, alpha, alphas, slots); break;
-      case 7: launch_z<R, 7>(grid, lds, st, B, nrhs, ltz, g, pzf, pzi, tr, alpha, alphas, slots); break;
-      case 8: launch_z<R, 8>(grid, lds, st, B, nrhs, ltz, g, pzf, pzi, tr, alpha, alphas, slots); break;
-      default: throw Error(ERR_INTERNAL, "pass Z: unexpected last radix");
-    }
+    launch_z<R>(zsh, G.fz.last_radix(rmax_for(EZ)), grid, lds, st, B, ltz, g, pzf, pzi, tr, alpha, alphas, slots);
     VH_LAUNCHED();
   }
   mark(3);
   // Yi (in place in B: rows y < M of every (rc, z) slab)
   {
     dim3 grid(ceil_div(g.Px, Ty) * g.N, nrhs * 5);
-    hipLaunchKernelGGL((k_pass_y<R, +1, EXY, NT_XY>), grid, dim3(NT_XY), (size_t)Ty * g.Py * elem, st, B, g.Py,
+    hipLaunchKernelGGL((k_pass_y<R, +1, EXY, NT_XY>), grid, dim3(NT_XY), lds_y, st, B, g.Py,
                        g.Py, B, g.Py, g.M, log2i(ty), g, py);
     VH_LAUNCHED();
   }
@@ -483,6 +551,8 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
   mark(6);
   if (tm) tm->pending = true;
 }
+
+int matvec_num_zshapes(int prec) { return prec == 0 ? num_zshapes<double>() : num_zshapes<float>(); }
 
 void PassTimer::create() {
   for (auto& e : ev) VH_HIP(hipEventCreate(&e));

@@ -3,13 +3,13 @@
 // Replaces lse_sparse_precon_prepare.m:33-104,149-159 (A_inv, S, ldlchol) and
 // lse_sparse_precon_multiply.m:84-88 (apply):
 //     bb = b - Ar*(A_inv .* a);   y(q) = LD \ bb(q);   x = (a + Ar'*y) .* A_inv
-// S = Ar*diag(A_inv)*Ar' is factored on the host (vh_ldl.cpp); the permuted
-// L, L' and D go to the device.  The solves run level by level: a level with
-// many rows is one grid launch (one thread per row and right-hand side); a run
-// of consecutive narrow levels is one launch in which each workgroup owns a
-// group of right-hand sides and walks the levels with a barrier in between.
-// The work vector is stored node-major, right-hand side fastest ([node][rhs]),
-// so a level touches contiguous memory for all right-hand sides of a row.
+// S = Ar*diag(A_inv)*Ar' is factored on the host (vh_ldl.cpp) after a nested
+// dissection ordering whose separator blocks become dense supernodes; L (CSR
+// and CSC copies), D and the inverses of the diagonal blocks go to the device.
+// The triangular solves run level by level over the block dependency levels,
+// one launch per level and kernel for every column of the batch: columns may
+// belong to different frequencies (different factor values, same pattern).
+// Work vectors are [node][column], column fastest.
 #include "vh_solver.h"
 #include <algorithm>
 #include <cmath>
@@ -19,26 +19,67 @@ namespace vh {
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-// w[i][r] = b_r[perm[i]] - sum_k Ar(perm[i],k) * Ainv[k] * a_r[k]
+// All kernels work on a batch of ncol columns; column c is the vector in slot
+// slots[c] (c when slots is null) and uses the factor values of table entry
+// grp_of_slot[slot] (entry 0 when grp_of_slot is null).  Work vectors are
+// [column][row] (one contiguous vector per column: the entries of a row of L
+// run through contiguous block-column ranges, so lanes over entries read
+// consecutive addresses), always FP64 with FP64 accumulation: with
+// FP32 factors the apply is then a fixed linear operator up to the rounding of
+// its output (FP32 work vectors would add rounding noise amplified by the
+// conditioning of L at every level, which stalls an FP32 Krylov solve).
 template <typename R>
-__global__ void k_pc_gather(const cplx<R>* __restrict__ x, int64_t ldx, cplx<R>* __restrict__ w, int nrhs, int nn,
-                            const int* __restrict__ perm, const int* __restrict__ arp, const int* __restrict__ arc,
-                            const R* __restrict__ arv, const R* __restrict__ ainv, int64_t nc,
-                            const int* __restrict__ slots) {
+__device__ __forceinline__ const PcTab<R>& col_tab(const PcTab<R>* tab, const int* grp_of_slot, int slot) {
+  return tab[grp_of_slot ? grp_of_slot[slot] : 0];
+}
+
+// w[i][c] = b_c[perm[i]] - sum_k Ar(perm[i],k) * Ainv[k] * a_c[k]
+template <typename R, typename WT>
+__global__ __launch_bounds__(256) void k_pc_gather(const cplx<R>* __restrict__ x, int64_t ldx, cplx<WT>* __restrict__ w,
+                                                   int ncol, int nn, const int* __restrict__ perm,
+                                                   const int* __restrict__ arp, const int* __restrict__ arc,
+                                                   const R* __restrict__ arv, const PcTab<R>* __restrict__ tab,
+                                                   const int* __restrict__ grp_of_slot, int64_t nc,
+                                                   const int* __restrict__ slots) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)nn * nrhs) return;
-  const int i = (int)(t / nrhs), r = (int)(t - (int64_t)i * nrhs);
+  if (t >= (int64_t)nn * ncol) return;
+  const int c = (int)(t / nn), i = (int)(t - (int64_t)c * nn);
+  const int slot = slots ? slots[c] : c;
+  const R* ainv = col_tab(tab, grp_of_slot, slot).ainv;
   const int node = perm[i];
-  const cplx<R>* xr = x + (int64_t)(slots ? slots[r] : r) * ldx;
-  cplx<R> s = xr[nc + node];
+  const cplx<R>* xr = x + (int64_t)slot * ldx;
+  const cplx<R> b = xr[nc + node];
+  double2 s; s.x = b.x; s.y = b.y;
   for (int p = arp[node]; p < arp[node + 1]; ++p) {
     const int k = arc[p];
-    const R c = arv[p] * ainv[k];
+    const double cf = (double)arv[p] * (double)ainv[k];
     const cplx<R> a = xr[k];
-    s.x -= c * a.x; s.y -= c * a.y;
+    s.x -= cf * a.x; s.y -= cf * a.y;
   }
-  w[t] = s;
+  cplx<WT> o; o.x = (WT)s.x; o.y = (WT)s.y;
+  w[t] = o;
 }
+
+// Lane layout of the level kernels: a wave serves 64/LPR work items (row of
+// the level, column) with LPR lanes each (item = lane mod NI, part = lane div
+// NI); the parts of an item stride over its row's entries, and the partial
+// sums meet with xor-shuffles over multiples of NI.  Items are numbered
+// column-major: item = column * nrows + row of the level.
+template <int LPR> struct Lanes {
+  static constexpr int NI = 64 / LPR;
+  int64_t item;
+  int part;
+  __device__ __forceinline__ Lanes() {
+    const int lane = threadIdx.x & 63;
+    item = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * NI + (lane % NI);
+    part = lane / NI;
+  }
+  template <typename C> __device__ __forceinline__ C sum(C v) const {
+#pragma unroll
+    for (int o = NI; o < 64; o <<= 1) { v.x += __shfl_xor(v.x, o, 64); v.y += __shfl_xor(v.y, o, 64); }
+    return v;
+  }
+};
 
 // Supernodal solves with inverted diagonal blocks.  Block k owns rows and
 // columns [c0, c1).  Forward (L y = b), blocks in increasing level order:
@@ -47,121 +88,169 @@ __global__ void k_pc_gather(const cplx<R>* __restrict__ x, int64_t ldx, cplx<R>*
 // Backward (L' x = D^-1 y), blocks in decreasing level order:
 //   u_r = y_r / D_r - sum_{i >= c1} L(i,r) x_i   (k_sn_update_bwd)
 //   x_k = Linv_k' u_k                             (k_sn_gemv_bwd)
-// w holds b, t and finally x; w2 holds y then u.  Vectors are [row][rhs].
-template <typename R>
-__global__ void k_sn_update_fwd(const int* __restrict__ rows, int nrows, const int* __restrict__ rp,
-                                const int* __restrict__ split, const int* __restrict__ ri, const R* __restrict__ rx,
-                                cplx<R>* __restrict__ w, const cplx<R>* __restrict__ w2, int nrhs) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)nrows * nrhs) return;
-  const int q = (int)(t / nrhs), c = (int)(t - (int64_t)q * nrhs);
-  const int r = rows[q];
-  cplx<R> s = w[(int64_t)r * nrhs + c];
-  for (int p = rp[r]; p < split[r]; ++p) {
-    const R l = rx[p];
-    const cplx<R> v = w2[(int64_t)ri[p] * nrhs + c];
-    s.x -= l * v.x; s.y -= l * v.y;
+// w holds b, t and finally x; w2 holds y then u.
+template <typename R, typename WT, int LPR>
+__global__ __launch_bounds__(256) void k_sn_update_fwd(const int* __restrict__ rows, int nrows,
+                                                       const int* __restrict__ rp, const int* __restrict__ split,
+                                                       const int* __restrict__ ri, const PcTab<R>* __restrict__ tab,
+                                                       const int* __restrict__ grp_of_slot,
+                                                       const int* __restrict__ slots, int ncol, int nn,
+                                                       cplx<WT>* __restrict__ w, const cplx<WT>* __restrict__ w2) {
+  const Lanes<LPR> ln;
+  const bool valid = ln.item < (int64_t)nrows * ncol;
+  double2 s; s.x = 0; s.y = 0;
+  int r = 0, c = 0;
+  if (valid) {
+    c = (int)(ln.item / nrows);
+    r = rows[ln.item - (int64_t)c * nrows];
+    const R* rx = col_tab(tab, grp_of_slot, slots ? slots[c] : c).Rx;
+    const cplx<WT>* wc2 = w2 + (int64_t)c * nn;
+    for (int p = rp[r] + ln.part; p < split[r]; p += LPR) {
+      const double l = rx[p];
+      const cplx<WT> v = wc2[ri[p]];
+      s.x += l * v.x; s.y += l * v.y;
+    }
   }
-  w[(int64_t)r * nrhs + c] = s;
-}
-
-template <typename R>
-__global__ void k_sn_gemv_fwd(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of_row,
-                              const int* __restrict__ sn_ptr, const int64_t* __restrict__ sn_off,
-                              const R* __restrict__ Li, const cplx<R>* __restrict__ w, cplx<R>* __restrict__ w2,
-                              int nrhs) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)nrows * nrhs) return;
-  const int q = (int)(t / nrhs), c = (int)(t - (int64_t)q * nrhs);
-  const int r = rows[q];
-  const int k = sn_of_row[r];
-  const int c0 = sn_ptr[k];
-  const int i = r - c0;
-  const R* row = Li + sn_off[k] + (int64_t)i * (i + 1) / 2;
-  const cplx<R>* tt = w + (int64_t)c0 * nrhs + c;
-  R sx = 0, sy = 0;
-  for (int j = 0; j <= i; ++j) {
-    const R l = row[j];
-    const cplx<R> v = tt[(int64_t)j * nrhs];
-    sx += l * v.x; sy += l * v.y;
+  s = ln.sum(s);
+  if (valid && ln.part == 0) {
+    cplx<WT>& o = w[(int64_t)c * nn + r];
+    o.x = (WT)((double)o.x - s.x); o.y = (WT)((double)o.y - s.y);
   }
-  cplx<R> o; o.x = sx; o.y = sy;
-  w2[(int64_t)r * nrhs + c] = o;
 }
 
-template <typename R>
-__global__ void k_sn_update_bwd(const int* __restrict__ rows, int nrows, const int* __restrict__ cp,
-                                const int* __restrict__ split, const int* __restrict__ ci, const R* __restrict__ cx,
-                                const R* __restrict__ D, const cplx<R>* __restrict__ w, cplx<R>* __restrict__ w2,
-                                int nrhs) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)nrows * nrhs) return;
-  const int q = (int)(t / nrhs), c = (int)(t - (int64_t)q * nrhs);
-  const int r = rows[q];
-  cplx<R> s = w2[(int64_t)r * nrhs + c];
-  const R inv = R(1) / D[r];
-  s.x *= inv; s.y *= inv;
-  for (int p = split[r]; p < cp[r + 1]; ++p) {
-    const R l = cx[p];
-    const cplx<R> v = w[(int64_t)ci[p] * nrhs + c];
-    s.x -= l * v.x; s.y -= l * v.y;
+template <typename R, typename WT, int LPR>
+__global__ __launch_bounds__(256) void k_sn_gemv_fwd(const int* __restrict__ rows, int nrows,
+                                                     const int* __restrict__ sn_of_row, const int* __restrict__ sn_ptr,
+                                                     const int64_t* __restrict__ sn_off,
+                                                     const PcTab<R>* __restrict__ tab,
+                                                     const int* __restrict__ grp_of_slot,
+                                                     const int* __restrict__ slots, int ncol, int nn,
+                                                     const cplx<WT>* __restrict__ w, cplx<WT>* __restrict__ w2) {
+  const Lanes<LPR> ln;
+  const bool valid = ln.item < (int64_t)nrows * ncol;
+  double2 s; s.x = 0; s.y = 0;
+  int r = 0, c = 0;
+  if (valid) {
+    c = (int)(ln.item / nrows);
+    r = rows[ln.item - (int64_t)c * nrows];
+    const int k = sn_of_row[r];
+    const int c0 = sn_ptr[k];
+    const int i = r - c0;
+    const R* row = col_tab(tab, grp_of_slot, slots ? slots[c] : c).Li + sn_off[k] + (int64_t)i * (i + 1) / 2;
+    const cplx<WT>* tt = w + (int64_t)c * nn + c0;
+    for (int j = ln.part; j <= i; j += LPR) {
+      const double l = row[j];
+      const cplx<WT> v = tt[j];
+      s.x += l * v.x; s.y += l * v.y;
+    }
   }
-  w2[(int64_t)r * nrhs + c] = s;
+  s = ln.sum(s);
+  if (valid && ln.part == 0) { cplx<WT> o; o.x = (WT)s.x; o.y = (WT)s.y; w2[(int64_t)c * nn + r] = o; }
 }
 
-template <typename R>
-__global__ void k_sn_gemv_bwd(const int* __restrict__ rows, int nrows, const int* __restrict__ sn_of_row,
-                              const int* __restrict__ sn_ptr, const int64_t* __restrict__ sn_off,
-                              const R* __restrict__ LiT, const cplx<R>* __restrict__ w2, cplx<R>* __restrict__ w,
-                              int nrhs) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)nrows * nrhs) return;
-  const int q = (int)(t / nrhs), c = (int)(t - (int64_t)q * nrhs);
-  const int r = rows[q];
-  const int k = sn_of_row[r];
-  const int c0 = sn_ptr[k], sz = sn_ptr[k + 1] - c0;
-  const int i = r - c0;
-  // row i of the transpose holds Linv(i..sz-1, i): offset i*sz - i(i-1)/2
-  const R* row = LiT + sn_off[k] + (int64_t)i * sz - (int64_t)i * (i - 1) / 2;
-  const cplx<R>* uu = w2 + (int64_t)r * nrhs + c;
-  R sx = 0, sy = 0;
-  for (int j = 0; j < sz - i; ++j) {
-    const R l = row[j];
-    const cplx<R> v = uu[(int64_t)j * nrhs];
-    sx += l * v.x; sy += l * v.y;
+template <typename R, typename WT, int LPR>
+__global__ __launch_bounds__(256) void k_sn_update_bwd(const int* __restrict__ rows, int nrows,
+                                                       const int* __restrict__ cp, const int* __restrict__ split,
+                                                       const int* __restrict__ ci, const PcTab<R>* __restrict__ tab,
+                                                       const int* __restrict__ grp_of_slot,
+                                                       const int* __restrict__ slots, int ncol, int nn,
+                                                       const cplx<WT>* __restrict__ w, cplx<WT>* __restrict__ w2) {
+  const Lanes<LPR> ln;
+  const bool valid = ln.item < (int64_t)nrows * ncol;
+  double2 s; s.x = 0; s.y = 0;
+  int r = 0, c = 0;
+  const R* D = nullptr;
+  if (valid) {
+    c = (int)(ln.item / nrows);
+    r = rows[ln.item - (int64_t)c * nrows];
+    const PcTab<R>& T = col_tab(tab, grp_of_slot, slots ? slots[c] : c);
+    const R* cx = T.Cx;
+    D = T.D;
+    const cplx<WT>* wc = w + (int64_t)c * nn;
+    for (int p = split[r] + ln.part; p < cp[r + 1]; p += LPR) {
+      const double l = cx[p];
+      const cplx<WT> v = wc[ci[p]];
+      s.x += l * v.x; s.y += l * v.y;
+    }
   }
-  cplx<R> o; o.x = sx; o.y = sy;
-  w[(int64_t)r * nrhs + c] = o;
+  s = ln.sum(s);
+  if (valid && ln.part == 0) {
+    cplx<WT>& o = w2[(int64_t)c * nn + r];
+    const double inv = 1.0 / (double)D[r];
+    o.x = (WT)((double)o.x * inv - s.x);
+    o.y = (WT)((double)o.y * inv - s.y);
+  }
 }
 
-// y_r[nc + perm[i]] = w[i][r]
-template <typename R>
-__global__ void k_pc_scatter_nodes(const cplx<R>* __restrict__ w, cplx<R>* __restrict__ y, int64_t ldy, int nrhs,
-                                   int nn, const int* __restrict__ perm, int64_t nc, const int* __restrict__ slots) {
+template <typename R, typename WT, int LPR>
+__global__ __launch_bounds__(256) void k_sn_gemv_bwd(const int* __restrict__ rows, int nrows,
+                                                     const int* __restrict__ sn_of_row, const int* __restrict__ sn_ptr,
+                                                     const int64_t* __restrict__ sn_off,
+                                                     const PcTab<R>* __restrict__ tab,
+                                                     const int* __restrict__ grp_of_slot,
+                                                     const int* __restrict__ slots, int ncol, int nn,
+                                                     const cplx<WT>* __restrict__ w2, cplx<WT>* __restrict__ w) {
+  const Lanes<LPR> ln;
+  const bool valid = ln.item < (int64_t)nrows * ncol;
+  double2 s; s.x = 0; s.y = 0;
+  int r = 0, c = 0;
+  if (valid) {
+    c = (int)(ln.item / nrows);
+    r = rows[ln.item - (int64_t)c * nrows];
+    const int k = sn_of_row[r];
+    const int c0 = sn_ptr[k], sz = sn_ptr[k + 1] - c0;
+    const int i = r - c0;
+    // row i of the transpose holds Linv(i..sz-1, i): offset i*sz - i(i-1)/2
+    const R* row = col_tab(tab, grp_of_slot, slots ? slots[c] : c).LiT + sn_off[k] + (int64_t)i * sz -
+                   (int64_t)i * (i - 1) / 2;
+    const cplx<WT>* uu = w2 + (int64_t)c * nn + r;
+    for (int j = ln.part; j < sz - i; j += LPR) {
+      const double l = row[j];
+      const cplx<WT> v = uu[j];
+      s.x += l * v.x; s.y += l * v.y;
+    }
+  }
+  s = ln.sum(s);
+  if (valid && ln.part == 0) { cplx<WT> o; o.x = (WT)s.x; o.y = (WT)s.y; w[(int64_t)c * nn + r] = o; }
+}
+
+// y_c[nc + perm[i]] = w[i][c]
+template <typename R, typename WT>
+__global__ __launch_bounds__(256) void k_pc_scatter_nodes(const cplx<WT>* __restrict__ w, cplx<R>* __restrict__ y,
+                                                          int64_t ldy, int ncol, int nn, const int* __restrict__ perm,
+                                                          int64_t nc, const int* __restrict__ slots) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)nn * nrhs) return;
-  const int i = (int)(t / nrhs), r = (int)(t - (int64_t)i * nrhs);
-  y[(int64_t)(slots ? slots[r] : r) * ldy + nc + perm[i]] = w[t];
+  if (t >= (int64_t)nn * ncol) return;
+  const int c = (int)(t / nn), i = (int)(t - (int64_t)c * nn);
+  cplx<R> o; o.x = (R)w[t].x; o.y = (R)w[t].y;
+  y[(int64_t)(slots ? slots[c] : c) * ldy + nc + perm[i]] = o;
 }
 
-// y_r[k] = (a_r[k] + sum_i ArTp(k,i) w[i][r]) * Ainv[k]   (ArTp columns are permuted node ids)
-template <typename R>
-__global__ void k_pc_currents(const cplx<R>* __restrict__ x, int64_t ldx, const cplx<R>* __restrict__ w,
-                              cplx<R>* __restrict__ y, int64_t ldy, int nrhs, int64_t nc, const int* __restrict__ tp,
-                              const int* __restrict__ tc, const R* __restrict__ tv, const R* __restrict__ ainv,
-                              const int* __restrict__ slots) {
+// y_c[k] = (a_c[k] + sum_i ArTp(k,i) w[i][c]) * Ainv[k]   (ArTp columns are permuted node ids)
+template <typename R, typename WT>
+__global__ __launch_bounds__(256) void k_pc_currents(const cplx<R>* __restrict__ x, int64_t ldx,
+                                                     const cplx<WT>* __restrict__ w, cplx<R>* __restrict__ y,
+                                                     int64_t ldy, int ncol, int nn, int64_t nc,
+                                                     const int* __restrict__ tp,
+                                                     const int* __restrict__ tc, const R* __restrict__ tv,
+                                                     const PcTab<R>* __restrict__ tab,
+                                                     const int* __restrict__ grp_of_slot,
+                                                     const int* __restrict__ slots) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y;
-  const int slot = slots ? slots[r] : r;
+  const int c = blockIdx.y;
+  const int slot = slots ? slots[c] : c;
   if (k >= nc) return;
-  cplx<R> s = x[(int64_t)slot * ldx + k];
-  for (int p = tp[k]; p < tp[k + 1]; ++p) {
-    const cplx<R> v = w[(int64_t)tc[p] * nrhs + r];
-    s.x += tv[p] * v.x; s.y += tv[p] * v.y;
+  const cplx<R> a = x[(int64_t)slot * ldx + k];
+  double sx = a.x, sy = a.y;
+  if (tc) {
+    for (int p = tp[k]; p < tp[k + 1]; ++p) {
+      const cplx<WT> v = w[(int64_t)c * nn + tc[p]];
+      sx += (double)tv[p] * v.x; sy += (double)tv[p] * v.y;
+    }
   }
-  const R ai = ainv[k];
-  s.x *= ai; s.y *= ai;
-  y[(int64_t)slot * ldy + k] = s;
+  const double ai = col_tab(tab, grp_of_slot, slot).ainv[k];
+  cplx<R> o; o.x = (R)(sx * ai); o.y = (R)(sy * ai);
+  y[(int64_t)slot * ldy + k] = o;
 }
 
 // ---------------------------------------------------------------------------
@@ -301,6 +390,40 @@ static void build_levels(Geometry& G, const LdlFactor& F, PreconSymbolic& Y) {
   }
   level_rows(lf, Y.sn_ptr, Y.f_lptr, Y.f_rows);
   level_rows(lb, Y.sn_ptr, Y.b_lptr, Y.b_rows);
+  // lanes per item: the power of two nearest above half the mean row length, in [1, 64]
+  auto lpr = [](int64_t tot, int64_t cnt) {
+    const double avg = cnt > 0 ? (double)tot / (double)cnt : 0.0;
+    int l = 1;
+    while (l < 64 && 2.0 * l < avg) l <<= 1;
+    return l;
+  };
+  const int nlf = (int)Y.f_lptr.size() - 1, nlb = (int)Y.b_lptr.size() - 1;
+  Y.f_lpr_u.assign(nlf, 1); Y.f_lpr_g.assign(nlf, 1); Y.f_upd_nnz.assign(nlf, 0);
+  Y.b_lpr_u.assign(nlb, 1); Y.b_lpr_g.assign(nlb, 1);
+  for (int l = 0; l < nlf; ++l) {
+    int64_t u = 0, g = 0;
+    for (int q = Y.f_lptr[l]; q < Y.f_lptr[l + 1]; ++q) {
+      const int r = Y.f_rows[q];
+      u += Y.row_split[r] - Rp[r];
+      g += r - Y.sn_ptr[Y.sn_of_row[r]] + 1;
+    }
+    const int64_t cnt = Y.f_lptr[l + 1] - Y.f_lptr[l];
+    Y.f_upd_nnz[l] = u;
+    Y.f_lpr_u[l] = lpr(u, cnt);
+    Y.f_lpr_g[l] = lpr(g, cnt);
+  }
+  for (int l = 0; l < nlb; ++l) {
+    int64_t u = 0, g = 0;
+    for (int q = Y.b_lptr[l]; q < Y.b_lptr[l + 1]; ++q) {
+      const int r = Y.b_rows[q];
+      const int k = Y.sn_of_row[r];
+      u += F.Lp[r + 1] - Y.col_split[r];
+      g += Y.sn_ptr[k + 1] - r;
+    }
+    const int64_t cnt = Y.b_lptr[l + 1] - Y.b_lptr[l];
+    Y.b_lpr_u[l] = lpr(u, cnt);
+    Y.b_lpr_g[l] = lpr(g, cnt);
+  }
   upload(Y.d_perm, F.perm);
   upload(Y.Rp, Rp);
   upload(Y.Ri, Ri);
@@ -420,71 +543,108 @@ void precon_prepare(Precon& P, double jre, double jim) {
   upload(P.ainv64, ainv); upload_f32(P.ainv32, ainv);
   upload(P.Li64, Li); upload_f32(P.Li32, Li);
   upload(P.LiT64, LiT); upload_f32(P.LiT32, LiT);
+  P.tab64_h = PcTab<double>{P.Rx64.p, P.Cx64.p, P.D64.p, P.ainv64.p, P.Li64.p, P.LiT64.p};
+  P.tab32_h = PcTab<float>{P.Rx32.p, P.Cx32.p, P.D32.p, P.ainv32.p, P.Li32.p, P.LiT32.p};
+  P.tab64.alloc(1);
+  P.tab32.alloc(1);
+  VH_HIP(hipMemcpy(P.tab64.p, &P.tab64_h, sizeof(PcTab<double>), hipMemcpyHostToDevice));
+  VH_HIP(hipMemcpy(P.tab32.p, &P.tab32_h, sizeof(PcTab<float>), hipMemcpyHostToDevice));
   // the values are on the device; the host copies of L are not needed any more
   P.F.Li.clear(); P.F.Li.shrink_to_fit();
   P.F.Lx.clear(); P.F.Lx.shrink_to_fit();
   P.ready = true;
 }
 
-template <typename R>
-void precon_apply(Precon& P, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy, const int* slots) {
-  VH_REQUIRE(P.ready, "preconditioner not prepared");
-  Geometry& G = *P.G;
+#define VH_LPR_LAUNCH(lpr, KERNEL, grid, ...)                                                          \
+  switch (lpr) {                                                                                      \
+    case 1: hipLaunchKernelGGL((KERNEL<R, WT, 1>), grid, dim3(256), 0, st, __VA_ARGS__); break;       \
+    case 2: hipLaunchKernelGGL((KERNEL<R, WT, 2>), grid, dim3(256), 0, st, __VA_ARGS__); break;       \
+    case 4: hipLaunchKernelGGL((KERNEL<R, WT, 4>), grid, dim3(256), 0, st, __VA_ARGS__); break;       \
+    case 8: hipLaunchKernelGGL((KERNEL<R, WT, 8>), grid, dim3(256), 0, st, __VA_ARGS__); break;       \
+    case 16: hipLaunchKernelGGL((KERNEL<R, WT, 16>), grid, dim3(256), 0, st, __VA_ARGS__); break;     \
+    case 32: hipLaunchKernelGGL((KERNEL<R, WT, 32>), grid, dim3(256), 0, st, __VA_ARGS__); break;     \
+    default: hipLaunchKernelGGL((KERNEL<R, WT, 64>), grid, dim3(256), 0, st, __VA_ARGS__); break;     \
+  }
+
+// grid for nitems (row, column) items of lpr lanes each, 256-thread blocks
+static dim3 lane_grid(int64_t nitems, int lpr) { return dim3((unsigned)ceil_div(nitems * lpr, 256)); }
+
+template <typename R, typename WT>
+static void apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, const int* grp_of_slot, int ncol,
+                       const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy, const int* slots) {
+  VH_REQUIRE(ncol >= 1, "preconditioner: no columns");
   hipStream_t st = G.ctx->stream;
   const int nn = (int)G.num_node;
   const int64_t nc = G.num_curr;
-  const size_t need = 2 * (size_t)nn * nrhs * sizeof(cplx<R>);
-  if (P.ws.n < need) P.ws.alloc(need);
-  cplx<R>* w = (cplx<R>*)P.ws.p;
-  cplx<R>* w2 = w + (size_t)nn * nrhs;
+  const size_t need = 2 * (size_t)nn * ncol * sizeof(cplx<WT>);
+  if (Y.ws.n < need) Y.ws.alloc(need);
+  cplx<WT>* w = (cplx<WT>*)Y.ws.p;
+  cplx<WT>* w2 = w + (size_t)nn * ncol;
   const bool d = sizeof(R) == 8;
   const R* arv = d ? (const R*)G.Ar.v64.p : (const R*)G.Ar.v32.p;
-  const R* ainv = d ? (const R*)P.ainv64.p : (const R*)P.ainv32.p;
   if (nn > 0) {
-    PreconSymbolic& Y = *P.sym;
-    const int64_t items = (int64_t)nn * nrhs;
-    hipLaunchKernelGGL((k_pc_gather<R>), dim3(ceil_div(items, 256)), dim3(256), 0, st, x, ldx, w, nrhs, nn,
-                       Y.d_perm.p, G.Ar.ptr.p, G.Ar.col.p, arv, ainv, nc, slots);
+    const int64_t items = (int64_t)nn * ncol;
+    hipLaunchKernelGGL((k_pc_gather<R, WT>), dim3(ceil_div(items, 256)), dim3(256), 0, st, x, ldx, w, ncol, nn,
+                       Y.d_perm.p, G.Ar.ptr.p, G.Ar.col.p, arv, tab, grp_of_slot, nc, slots);
     VH_LAUNCHED();
-    const R* Rx = d ? (const R*)P.Rx64.p : (const R*)P.Rx32.p;
-    const R* Cx = d ? (const R*)P.Cx64.p : (const R*)P.Cx32.p;
-    const R* D = d ? (const R*)P.D64.p : (const R*)P.D32.p;
-    const R* Li = d ? (const R*)P.Li64.p : (const R*)P.Li32.p;
-    const R* LiT = d ? (const R*)P.LiT64.p : (const R*)P.LiT32.p;
     const int nlf = (int)Y.f_lptr.size() - 1;
     for (int l = 0; l < nlf; ++l) {
       const int a = Y.f_lptr[l], nr = Y.f_lptr[l + 1] - a;
-      const dim3 grid(ceil_div((int64_t)nr * nrhs, 256));
-      if (l > 0) {
-        hipLaunchKernelGGL((k_sn_update_fwd<R>), grid, dim3(256), 0, st, Y.d_f_rows.p + a, nr, Y.Rp.p,
-                           Y.d_row_split.p, Y.Ri.p, Rx, w, w2, nrhs);
+      const int64_t it = (int64_t)nr * ncol;
+      if (Y.f_upd_nnz[l] > 0) {
+        VH_LPR_LAUNCH(Y.f_lpr_u[l], k_sn_update_fwd, lane_grid(it, Y.f_lpr_u[l]), Y.d_f_rows.p + a, nr, Y.Rp.p,
+                      Y.d_row_split.p, Y.Ri.p, tab, grp_of_slot, slots, ncol, nn, w, w2);
         VH_LAUNCHED();
       }
-      hipLaunchKernelGGL((k_sn_gemv_fwd<R>), grid, dim3(256), 0, st, Y.d_f_rows.p + a, nr, Y.d_sn_of_row.p,
-                         Y.d_sn_ptr.p, Y.d_sn_off.p, Li, w, w2, nrhs);
+      VH_LPR_LAUNCH(Y.f_lpr_g[l], k_sn_gemv_fwd, lane_grid(it, Y.f_lpr_g[l]), Y.d_f_rows.p + a, nr,
+                    Y.d_sn_of_row.p, Y.d_sn_ptr.p, Y.d_sn_off.p, tab, grp_of_slot, slots, ncol, nn, w, w2);
       VH_LAUNCHED();
     }
     const int nlb = (int)Y.b_lptr.size() - 1;
     for (int l = 0; l < nlb; ++l) {
       const int a = Y.b_lptr[l], nr = Y.b_lptr[l + 1] - a;
-      const dim3 grid(ceil_div((int64_t)nr * nrhs, 256));
-      hipLaunchKernelGGL((k_sn_update_bwd<R>), grid, dim3(256), 0, st, Y.d_b_rows.p + a, nr, Y.Cp.p,
-                         Y.d_col_split.p, Y.Ci.p, Cx, D, w, w2, nrhs);
+      const int64_t it = (int64_t)nr * ncol;
+      VH_LPR_LAUNCH(Y.b_lpr_u[l], k_sn_update_bwd, lane_grid(it, Y.b_lpr_u[l]), Y.d_b_rows.p + a, nr, Y.Cp.p,
+                    Y.d_col_split.p, Y.Ci.p, tab, grp_of_slot, slots, ncol, nn, w, w2);
       VH_LAUNCHED();
-      hipLaunchKernelGGL((k_sn_gemv_bwd<R>), grid, dim3(256), 0, st, Y.d_b_rows.p + a, nr, Y.d_sn_of_row.p,
-                         Y.d_sn_ptr.p, Y.d_sn_off.p, LiT, w2, w, nrhs);
+      VH_LPR_LAUNCH(Y.b_lpr_g[l], k_sn_gemv_bwd, lane_grid(it, Y.b_lpr_g[l]), Y.d_b_rows.p + a, nr,
+                    Y.d_sn_of_row.p, Y.d_sn_ptr.p, Y.d_sn_off.p, tab, grp_of_slot, slots, ncol, nn, w2, w);
       VH_LAUNCHED();
     }
-    hipLaunchKernelGGL((k_pc_scatter_nodes<R>), dim3(ceil_div(items, 256)), dim3(256), 0, st, w, y, ldy, nrhs, nn,
+    hipLaunchKernelGGL((k_pc_scatter_nodes<R, WT>), dim3(ceil_div(items, 256)), dim3(256), 0, st, w, y, ldy, ncol, nn,
                        Y.d_perm.p, nc, slots);
     VH_LAUNCHED();
   }
   const R* atv = d ? (const R*)G.ArT.v64.p : (const R*)G.ArT.v32.p;
-  hipLaunchKernelGGL((k_pc_currents<R>), dim3(ceil_div(nc, 256), nrhs), dim3(256), 0, st, x, ldx, w, y, ldy, nrhs, nc,
-                     G.ArT.ptr.p, nn > 0 ? P.sym->ArTp_col.p : nullptr, atv, ainv, slots);
+  hipLaunchKernelGGL((k_pc_currents<R, WT>), dim3(ceil_div(nc, 256), ncol), dim3(256), 0, st, x, ldx, w, y, ldy, ncol, nn, nc,
+                     G.ArT.ptr.p, nn > 0 ? Y.ArTp_col.p : nullptr, atv, tab, grp_of_slot, slots);
   VH_LAUNCHED();
 }
 
+template <typename R>
+void precon_apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, const int* grp_of_slot, int ncol,
+                       const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy, const int* slots) {
+  // FP32 work vectors only on request (VH_PC_WORK32=1): they halve the work
+  // vector traffic but make the FP32 apply noisier.
+  static const bool work32 = [] { const char* e = std::getenv("VH_PC_WORK32"); return e && e[0] == '1'; }();
+  if (sizeof(R) == 4 && work32)
+    apply_cols<R, float>(G, Y, tab, grp_of_slot, ncol, x, ldx, y, ldy, slots);
+  else
+    apply_cols<R, double>(G, Y, tab, grp_of_slot, ncol, x, ldx, y, ldy, slots);
+}
+
+template <typename R>
+void precon_apply(Precon& P, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy, const int* slots) {
+  VH_REQUIRE(P.ready, "preconditioner not prepared");
+  const PcTab<R>* tab;
+  if constexpr (sizeof(R) == 8) tab = P.tab64.p; else tab = P.tab32.p;
+  precon_apply_cols<R>(*P.G, *P.sym, tab, nullptr, nrhs, x, ldx, y, ldy, slots);
+}
+
+template void precon_apply_cols<double>(Geometry&, PreconSymbolic&, const PcTab<double>*, const int*, int,
+                                        const double2*, int64_t, double2*, int64_t, const int*);
+template void precon_apply_cols<float>(Geometry&, PreconSymbolic&, const PcTab<float>*, const int*, int,
+                                       const float2*, int64_t, float2*, int64_t, const int*);
 template void precon_apply<double>(Precon&, int, const double2*, int64_t, double2*, int64_t, const int*);
 template void precon_apply<float>(Precon&, int, const float2*, int64_t, float2*, int64_t, const int*);
 

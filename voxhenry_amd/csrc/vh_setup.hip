@@ -136,6 +136,7 @@ template Plan2 FftAxis2::plan<double>(int, bool) const;
 template Plan2 FftAxis2::plan<float>(int, bool) const;
 
 int FftAxis2::last_radix(int rmax) const { return e[rmax_slot(rmax)][0].radices.back(); }
+const std::vector<Stage2>& FftAxis2::stages(int rmax) const { return e[rmax_slot(rmax)][0].st; }
 
 void geometry_init(Geometry& G, Context* ctx, int L, int M, int N, double dx, int64_t nv, const int64_t* idx,
                    int pad_mode) {

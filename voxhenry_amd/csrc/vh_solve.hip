@@ -15,6 +15,7 @@
 // the new Hessenberg column of every active system to the host.
 #include "vh_krylov.h"
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 
 namespace vh {
@@ -22,16 +23,20 @@ namespace vh {
 using cd = std::complex<double>;
 
 template <typename R>
-void KrylovWork<R>::ensure(int64_t n_, int64_t ld_, int m_, int nslot_, bool flexible) {
-  const bool same = (n_ == n && ld_ == ld && m_ == m && nslot_ <= nslot && (!flexible || Z.p));
-  if (same) return;
-  n = n_; ld = ld_; m = m_; nslot = std::max(nslot_, 1);
-  V.alloc((size_t)(m + 1) * nslot * ld);
-  if (flexible) Z.alloc((size_t)m * nslot * ld);
-  w.alloc((size_t)nslot * ld);
-  t.alloc((size_t)nslot * ld);
-  r.alloc((size_t)nslot * ld);
+void KrylovWork<R>::bind(char* base, int64_t n_, int64_t ld_, int m_, int nslot_, bool flexible_) {
+  const bool resize = (m_ != m || nslot_ != nslot || !h.p);
+  n = n_; ld = ld_; m = m_; nslot = std::max(nslot_, 1); flexible = flexible_;
+  const size_t vec = (size_t)nslot * ld;
+  cplx<R>* p = (cplx<R>*)base;
+  V = p; p += (size_t)(m + 1) * vec;
+  Z = nullptr;
+  if (flexible) { Z = p; p += (size_t)m * vec; }
+  w = p; p += vec;
+  t = p; p += vec;
+  r = p;
+  if (!resize) return;
   h.alloc((size_t)nslot * (m + 1));
+  h2.alloc((size_t)nslot * (m + 1));
   coef.alloc((size_t)nslot * (m + 1));
   nrm.alloc(nslot);
   slots_dev.alloc(nslot);
@@ -41,8 +46,45 @@ void KrylovWork<R>::ensure(int64_t n_, int64_t ld_, int m_, int nslot_, bool fle
   h_coef.alloc((size_t)nslot * (m + 1));
   h_rcoef.alloc((size_t)nslot * (m + 1));
   h_h.alloc((size_t)nslot * (m + 1));
+  h_h2.alloc((size_t)nslot * (m + 1));
   h_nrm.alloc(nslot);
   h_bn.alloc(nslot);
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t KrylovCache::bytes(int64_t n, int nslot, int m64, bool flex64, int m32, bool flex32, bool mixed_vectors) {
+  size_t b = 0;
+  if (m64 > 0) b += align256(KrylovWork<double>::bytes_needed(n, m64, nslot, flex64));
+  if (m32 > 0) b += align256(KrylovWork<float>::bytes_needed(n, m32, nslot, flex32));
+  if (mixed_vectors) b += 2 * align256(sizeof(float2) * (size_t)nslot * n);
+  return b;
+}
+
+void KrylovCache::layout(int64_t n, int nslot, int m64, bool flex64, int m32, bool flex32, bool mixed_vectors) {
+  const size_t need = bytes(n, nslot, m64, flex64, m32, flex32, mixed_vectors);
+  if (arena.n < need) {
+    w64.unbind(); w32.unbind();
+    arena.alloc(need);
+  }
+  char* p = arena.p;
+  if (m64 > 0) {
+    w64.bind(p, n, n, m64, nslot, flex64);
+    p += align256(KrylovWork<double>::bytes_needed(n, m64, nslot, flex64));
+  } else {
+    w64.unbind();
+  }
+  if (m32 > 0) {
+    w32.bind(p, n, n, m32, nslot, flex32);
+    p += align256(KrylovWork<float>::bytes_needed(n, m32, nslot, flex32));
+  } else {
+    w32.unbind();
+  }
+  v32 = z32 = nullptr;
+  if (mixed_vectors) {
+    v32 = (float2*)p; p += align256(sizeof(float2) * (size_t)nslot * n);
+    z32 = (float2*)p;
+  }
 }
 
 template struct KrylovWork<double>;
@@ -60,11 +102,18 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
   VH_REQUIRE(o.maxit >= 1, "maxit must be >= 1");
   int nslot = 0;
   for (int s : sys) { VH_REQUIRE(s >= 0, "negative slot"); nslot = std::max(nslot, s + 1); }
-  W.ensure(n, n, m, nslot, o.flexible);
+  using clk = std::chrono::steady_clock;
+  const auto t_start = clk::now();
+  double t_wait = 0, t_host = 0;
+  int nsteps = 0;
+  VH_REQUIRE(W.V && W.ld == n && W.m == m && W.nslot >= nslot && (!o.flexible || W.Z),
+             "internal: Krylov workspace not laid out for this solve");
+  const double t_alloc = std::chrono::duration<double>(clk::now() - t_start).count();
   VecOps<R> ops(ctx, n, n, m + 1, W.nslot);
   const int64_t vstr = (int64_t)W.nslot * n;
   if ((int)st.size() < nslot) st.resize(nslot);
   const bool left = (M != nullptr) && !o.flexible;
+  auto tol_of = [&](int s) { return o.tol_slot ? (*o.tol_slot)[s] : o.tol; };
 
   // All host<->device transfers of the loop go through pinned staging buffers
   // of the workspace; each staging buffer is rewritten only after the stream
@@ -78,7 +127,9 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
   };
   auto fetch_norms = [&](int cnt) {
     VH_HIP(hipMemcpyAsync(W.h_nrm.p, W.nrm.p, cnt * sizeof(double2), hipMemcpyDeviceToHost, strm));
+    const auto t0 = clk::now();
     VH_HIP(hipStreamSynchronize(strm));
+    t_wait += std::chrono::duration<double>(clk::now() - t0).count();
   };
 
   std::vector<int> act = sys;
@@ -89,12 +140,12 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
   }
   // r = M b (pgmres.m:120-139) or b (fpgmres.m:91-95)
   if (left) {
-    (*M)(act, W.slots_dev.p, b, W.r.p);
+    (*M)(act, W.slots_dev.p, b, W.r);
   } else {
-    convert_vectors<R, R>(ctx, b, n, W.r.p, n, n, nsys, W.slots_dev.p, W.slots_dev.p);
+    convert_vectors<R, R>(ctx, b, n, W.r, n, n, nsys, W.slots_dev.p, W.slots_dev.p);
   }
   std::vector<double> bnorm(nslot, 0.0), bnorm_true(nslot, 0.0);
-  ops.norm2(W.r.p, nsys, W.slots_dev.p, W.nrm.p);
+  ops.norm2(W.r, nsys, W.slots_dev.p, W.nrm.p);
   fetch_norms(nsys);
   std::vector<int> outit(nslot, 0);
   std::vector<Givens> gv(nslot);
@@ -104,7 +155,7 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
       const int s = sys[a];
       bnorm[s] = std::sqrt(nbuf[a].x);
       st[s].resvec.push_back(bnorm[s] > 0 ? 1.0 : 0.0);
-      if (bnorm[s] > 0 && 1.0 > o.tol) keep.push_back(s);
+      if (bnorm[s] > 0 && 1.0 > tol_of(s)) keep.push_back(s);
     }
     act.swap(keep);
   }
@@ -131,12 +182,12 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
     }
     upload(L, W.aux_slots.p);
     VH_HIP(hipMemcpyAsync(W.coef.p, cbuf, (size_t)nl * (m + 1) * sizeof(double2), hipMemcpyHostToDevice, strm));
-    const C* basis = o.flexible ? W.Z.p : W.V.p;
+    const C* basis = o.flexible ? W.Z : W.V;
     ops.combine(basis, vstr, x, k, nl, W.aux_slots.p, W.coef.p, 1.0, 1.0, nullptr);      // pgmres_iter.m:136-152
     if (restart_r) {
       // the device coefficient buffer is read by the kernel above on the same stream, so it can be reused
       VH_HIP(hipMemcpyAsync(W.coef.p, rbuf, (size_t)nl * (m + 1) * sizeof(double2), hipMemcpyHostToDevice, strm));
-      ops.combine(W.V.p, vstr, W.r.p, k + 1, nl, W.aux_slots.p, W.coef.p, 0.0, 1.0, nullptr);  // :153
+      ops.combine(W.V, vstr, W.r, k + 1, nl, W.aux_slots.p, W.coef.p, 0.0, 1.0, nullptr);  // :153
     }
     VH_HIP(hipStreamSynchronize(strm));   // host buffers above are reused
   };
@@ -146,46 +197,59 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
     int nact = (int)act.size();
     upload(act, W.slots_dev.p);
     // V(:,1) = r/||r||  (pgmres_iter.m:55)
-    ops.norm2(W.r.p, nact, W.slots_dev.p, W.nrm.p);
-    ops.scale(W.r.p, W.V.p, nact, W.slots_dev.p, W.nrm.p);
+    ops.norm2(W.r, nact, W.slots_dev.p, W.nrm.p);
+    ops.scale(W.r, W.V, nact, W.slots_dev.p, W.nrm.p);
     fetch_norms(nact);
     for (int a = 0; a < nact; ++a) gv[act[a]].reset(m, std::sqrt(nbuf[a].x));
     int k = 0;
     while (k < m && !act.empty()) {
       ++k;
       nact = (int)act.size();
-      const C* vk = W.V.p + (int64_t)(k - 1) * vstr;
+      const C* vk = W.V + (int64_t)(k - 1) * vstr;
       if (o.flexible) {
-        C* zk = W.Z.p + (int64_t)(k - 1) * vstr;
+        C* zk = W.Z + (int64_t)(k - 1) * vstr;
         if (M) (*M)(act, W.slots_dev.p, vk, zk);
         else convert_vectors<R, R>(ctx, vk, n, zk, n, n, nact, W.slots_dev.p, W.slots_dev.p);
-        A(act, W.slots_dev.p, zk, W.w.p);
+        A(act, W.slots_dev.p, zk, W.w);
       } else {
         if (M) {
-          A(act, W.slots_dev.p, vk, W.t.p);
-          (*M)(act, W.slots_dev.p, W.t.p, W.w.p);
+          A(act, W.slots_dev.p, vk, W.t);
+          (*M)(act, W.slots_dev.p, W.t, W.w);
         } else {
-          A(act, W.slots_dev.p, vk, W.w.p);
+          A(act, W.slots_dev.p, vk, W.w);
         }
       }
       for (int s : act) { st[s].iters++; st[s].matvecs++; }
-      // classical Gram-Schmidt, one pass (pgmres_iter.m:107-111)
-      ops.mdot(W.V.p, vstr, W.w.p, k, nact, W.slots_dev.p, W.h.p);
-      ops.combine(W.V.p, vstr, W.w.p, k, nact, W.slots_dev.p, W.h.p, 1.0, -1.0, W.nrm.p);
-      ops.scale(W.w.p, W.V.p + (int64_t)k * vstr, nact, W.slots_dev.p, W.nrm.p);
+      // classical Gram-Schmidt, one pass (pgmres_iter.m:107-111); with reorth a
+      // second pass re-projects w (the FP32 inner solve: one pass loses
+      // orthogonality once the residual nears sqrt(eps))
+      ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p);
+      ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, 1.0, -1.0, o.reorth ? nullptr : W.nrm.p);
+      if (o.reorth) {
+        ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h2.p);
+        ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h2.p, 1.0, -1.0, W.nrm.p);
+        VH_HIP(hipMemcpyAsync(W.h_h2.p, W.h2.p, (size_t)nact * (m + 1) * sizeof(double2), hipMemcpyDeviceToHost,
+                              strm));
+      }
+      ops.scale(W.w, W.V + (int64_t)k * vstr, nact, W.slots_dev.p, W.nrm.p);
       VH_HIP(hipMemcpyAsync(W.h_h.p, W.h.p, (size_t)nact * (m + 1) * sizeof(double2), hipMemcpyDeviceToHost, strm));
       fetch_norms(nact);
+      ++nsteps;
+      const auto th = clk::now();
       std::vector<int> conv, keep;
       std::vector<cd> hc(k);
       for (int a = 0; a < nact; ++a) {
         const int s = act[a];
         for (int j = 0; j < k; ++j) hc[j] = cd(hbuf[(size_t)a * (m + 1) + j].x, hbuf[(size_t)a * (m + 1) + j].y);
+        if (o.reorth)
+          for (int j = 0; j < k; ++j) hc[j] += cd(W.h_h2[(size_t)a * (m + 1) + j].x, W.h_h2[(size_t)a * (m + 1) + j].y);
         const double hn = std::sqrt(nbuf[a].x);
         const double rres = gv[s].push(hc.data(), hn);
         st[s].resvec.push_back(rres / bnorm[s]);
-        if (!(rres >= o.tol * bnorm[s])) conv.push_back(s);   // pgmres_iter.m:126 (NaN also stops)
+        if (!(rres >= tol_of(s) * bnorm[s])) conv.push_back(s);   // pgmres_iter.m:126 (NaN also stops)
         else keep.push_back(s);
       }
+      t_host += std::chrono::duration<double>(clk::now() - th).count();
       if (!conv.empty()) {
         finish(conv, k, false);
         act.swap(keep);
@@ -200,12 +264,16 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
     }
   }
 
+  if (trace_on())
+    fprintf(stderr, "[vh] gmres<%s> nsys %d m %d: %d steps, alloc %.3f s, gpu wait %.3f s, host LS %.3f s, total %.3f s\n",
+            sizeof(R) == 8 ? "f64" : "f32", nsys, m, nsteps, t_alloc, t_wait, t_host,
+            std::chrono::duration<double>(clk::now() - t_start).count());
   // relres = ||b - A x|| / ||b||  (pgmres.m:181-186)
   if (o.final_residual) {
     upload(sys, W.slots_dev.p);
-    A(sys, W.slots_dev.p, x, W.t.p);
-    ops.sub_from(b, n, W.t.p, nsys, W.slots_dev.p);
-    ops.norm2(W.t.p, nsys, W.slots_dev.p, W.nrm.p);
+    A(sys, W.slots_dev.p, x, W.t);
+    ops.sub_from(b, n, W.t, nsys, W.slots_dev.p);
+    ops.norm2(W.t, nsys, W.slots_dev.p, W.nrm.p);
     fetch_norms(nsys);
     const double2* bn = W.h_bn.p;
     ops.norm2(b, nsys, W.slots_dev.p, W.nrm.p);
@@ -219,7 +287,7 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
       st[s].matvecs++;
     }
   } else {
-    for (int s : sys) { st[s].relres = st[s].resvec.back(); st[s].flag = st[s].relres <= o.tol ? 0 : 1; }
+    for (int s : sys) { st[s].relres = st[s].resvec.back(); st[s].flag = st[s].relres <= tol_of(s) ? 0 : 1; }
   }
 }
 
@@ -230,11 +298,16 @@ template void gmres_batch<float>(Context*, int64_t, const OpFn<float>&, const Op
                                  const std::vector<int>&, const SolveOpts&, KrylovWork<float>&, std::vector<SysStats>&);
 
 void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<float>& A32, const OpFn<float>& M32,
-                  const double2* b, double2* x, const std::vector<int>& sys, const MixedOpts& o,
-                  KrylovWork<double>& W64, KrylovWork<float>& W32, std::vector<SysStats>& st) {
+                  const double2* b, double2* x, const std::vector<int>& sys, const MixedOpts& o, KrylovCache& K,
+                  std::vector<SysStats>& st) {
   int nslot = 0;
   for (int s : sys) nslot = std::max(nslot, s + 1);
-  DevBuf<float2> v32((size_t)nslot * n), z32((size_t)nslot * n);
+  VH_REQUIRE(K.v32 && K.z32 && K.w64.nslot >= nslot && K.w32.nslot >= nslot,
+             "internal: mixed-precision workspace not laid out for this solve");
+  float2* v32 = K.v32;
+  float2* z32 = K.z32;
+  KrylovWork<double>& W64 = K.w64;
+  KrylovWork<float>& W32 = K.w32;
   std::vector<SysStats> inner_st;
   std::vector<int> inner_iters(nslot, 0);
   SolveOpts io;
@@ -243,12 +316,26 @@ void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<f
   io.tol = o.inner_tol;
   io.final_residual = false;
   io.flexible = false;
+  io.reorth = true;
+  // Inner tolerance per system: the outer residual only has to fall from its
+  // current value r (relative) to tol, so an inner solve that reduces its
+  // own residual by about tol / r suffices (inexact Krylov: the admissible
+  // inner error grows as the outer residual falls).  The target keeps a
+  // factor 20 of margin below tol (R and L derived from low-frequency
+  // solutions amplify the residual by |Z| / |Im Z|).  Clamped to
+  // [inner_tol, 0.1]; the first inner solve (r = 1) uses inner_tol.
+  std::vector<double> itol(nslot, o.inner_tol);
+  io.tol_slot = &itol;
   // preconditioner of the outer iteration: z = (FP32 GMRES solve of A z = v)
   OpFn<double> inner = [&](const std::vector<int>& act, const int* slots, const double2* v, double2* z) {
     const int na = (int)act.size();
-    convert_vectors<float, double>(ctx, v, n, v32.p, n, n, na, slots, slots);
-    gmres_batch<float>(ctx, n, A32, &M32, v32.p, z32.p, act, io, W32, inner_st);
-    convert_vectors<double, float>(ctx, z32.p, n, z, n, n, na, slots, slots);
+    for (int s : act) {
+      const double r = st[s].resvec.empty() ? 1.0 : st[s].resvec.back();
+      itol[s] = o.adaptive_inner && r > 0 ? std::min(0.1, std::max(o.inner_tol, 0.05 * o.tol / r)) : o.inner_tol;
+    }
+    convert_vectors<float, double>(ctx, v, n, v32, n, n, na, slots, slots);
+    gmres_batch<float>(ctx, n, A32, &M32, v32, z32, act, io, W32, inner_st);
+    convert_vectors<double, float>(ctx, z32, n, z, n, n, na, slots, slots);
     for (int s : act) inner_iters[s] += inner_st[s].iters;
   };
   SolveOpts oo;

@@ -6,6 +6,16 @@
 
 namespace vh {
 
+// Device pointers to the factor values of one preconditioner, in one precision.
+template <typename R> struct PcTab {
+  const R* Rx;     // L in CSR (row copy)
+  const R* Cx;     // L in CSC
+  const R* D;
+  const R* ainv;
+  const R* Li;     // inverses of the diagonal blocks, packed row-major lower
+  const R* LiT;    // their transposes, packed row-major upper
+};
+
 // Pattern data of the Schur factor, shared by every frequency of a geometry:
 // S pattern, ordering, elimination tree and L pattern, level schedules, and
 // their device copies.
@@ -21,6 +31,11 @@ struct PreconSymbolic {
   std::vector<int> row_split;         // CSR: first entry of row r in its own block's columns
   std::vector<int> col_split;         // CSC: first entry of column r below its own block
   std::vector<int> f_lptr, f_rows, b_lptr, b_rows;   // rows of the blocks of each level
+  // lanes per (row, column) item of each level's kernels, from the mean row
+  // length (update: entries outside the block; gemv: entries of the inverse)
+  std::vector<int> f_lpr_u, f_lpr_g, b_lpr_u, b_lpr_g;
+  std::vector<int64_t> f_upd_nnz;     // update entries of each forward level (0: no update launch)
+  DevBuf<char> ws;                    // apply workspace (two [node][column] vectors), shared
   DevBuf<int> d_perm, Rp, Ri, Cp, Ci, ArTp_col;
   DevBuf<int> d_sn_ptr, d_sn_of_row, d_row_split, d_col_split, d_f_rows, d_b_rows;
   DevBuf<int64_t> d_sn_off;
@@ -37,7 +52,10 @@ struct Precon {
   // (row-major lower: Linv; the transpose row-major: LinvT)
   DevBuf<double> Rx64, Cx64, D64, ainv64, Li64, LiT64;
   DevBuf<float> Rx32, Cx32, D32, ainv32, Li32, LiT32;
-  DevBuf<char> ws;
+  DevBuf<PcTab<double>> tab64;        // one-entry tables for the single apply
+  DevBuf<PcTab<float>> tab32;
+  PcTab<double> tab64_h{};
+  PcTab<float> tab32_h{};
 };
 
 // A_inv, S, LDL^T for jwmu = j*omega*mu (lse_sparse_precon_prepare.m).
@@ -46,5 +64,11 @@ void precon_prepare(Precon& P, double jwmu_re, double jwmu_im);
 template <typename R>
 void precon_apply(Precon& P, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy,
                   const int* slots = nullptr);
+// Several preconditioners of one geometry (sharing Y) applied in one pass to
+// ncol columns: column c is slot slots[c] and uses tab[grp_of_slot[slot]]
+// (tab[0] when grp_of_slot is null); tab is a device array.
+template <typename R>
+void precon_apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, const int* grp_of_slot, int ncol,
+                       const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy, const int* slots);
 
 }  // namespace vh
