@@ -36,6 +36,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "batched port x freq solves/s at 1e-8 resid, mixed vs FP64; FFT-matvec HBM GB/s"
 PASS_NAMES = ['X', 'Y', 'Z', 'Yi', 'Xi', 'node']
+# pass-Z shapes (threads, elements per thread) in the library's FP32 table order
+ZSHAPES32 = [(512, 8), (256, 8), (512, 4), (256, 4)]
 
 
 def parse():
@@ -294,7 +296,9 @@ def main():
                  all_converged=bool(np.all(stm['flag'] == 0)))
 
     z = mv['c64']['passes']['Z']
-    roof = dict(kernel='k_pass_z<float> (z-FFT + Green product + inverse z-FFT), 128^3 grid, complex64',
+    nt, e = ZSHAPES32[mv['zshape']['c64']]
+    kname = ('k_pass_z_p2<float, 256, %d, true>' % nt) if e == 8 else ('k_pass_z<float, %d, %d, 4>' % (e, nt))
+    roof = dict(kernel=kname + ' (z-FFT + Green product + inverse z-FFT), 128^3 grid, complex64',
                 bound='hbm', achieved=z['GBs'], peak=HBM_PEAK_GBS, unit='GB/s',
                 frac=(z['GBs'] / HBM_PEAK_GBS) if z['GBs'] else None, traffic=recorded_traffic())
 

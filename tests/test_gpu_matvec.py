@@ -161,7 +161,8 @@ def test_golden_solution_residual(ctx):
 # 4, 5, 7, 8, 16), one- to four-stage plans, long x lines (Px = 800 needs 64
 # threads per line) and sparse voxel sets (the voxel-map gather and scatter).
 RADIX_SHAPES = [((400, 4, 4), False), ((9, 33, 17), True), ((64, 3, 40), False), ((5, 6, 2), True),
-                ((128, 20, 3), True), ((16, 16, 16), False)]
+                ((128, 20, 3), True), ((16, 16, 16), False), ((4, 4, 128), True), ((6, 5, 64), False),
+                ((3, 4, 100), False)]
 
 
 @pytest.mark.parametrize('dims,hollow', RADIX_SHAPES)
@@ -194,7 +195,7 @@ def test_matvec_radix_shapes(ctx, dims, hollow):
     G.close()
 
 
-@pytest.mark.parametrize('dims', [(16, 16, 16), (400, 4, 4), (9, 33, 17)])
+@pytest.mark.parametrize('dims', [(16, 16, 16), (400, 4, 4), (9, 33, 17), (4, 4, 128), (5, 4, 100)])
 def test_matvec_every_zshape(ctx, dims):
     """Every compiled pass-Z shape (the autotuner's choices) gives the same
     operator as the oracle; the autotuner itself leaves a valid choice."""

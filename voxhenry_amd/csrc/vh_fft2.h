@@ -231,4 +231,58 @@ inline int fft_tpp(int n, int E, int nt_max) {
   return ((n + t - 1) / t > E) ? -1 : t;
 }
 
+// ---------------------------------------------------------------------------
+// Compile-time plans for power-of-two lengths.  P2Plan<N, RMAX, REV> has the
+// radix sequence of the run-time plan built for (N, RMAX) -- the power-of-two
+// exponent split into as few stages of at most RMAX as possible, larger
+// stages first -- in forward (REV = false) or reversed order, so the
+// run-time twiddle tables of that plan serve it unchanged.
+// ---------------------------------------------------------------------------
+constexpr int clog2(int v) { return v <= 1 ? 0 : 1 + clog2(v / 2); }
+
+template <int N, int RMAX, bool REV> struct P2Plan {
+  static constexpr int L = clog2(N), LM = clog2(RMAX);
+  static constexpr int S = (L + LM - 1) / LM;
+  // log2 radix of stage s of the forward order
+  static constexpr int lr_fwd(int s) { return L / S + (s < L % S ? 1 : 0); }
+  static constexpr int rad(int s) { return 1 << lr_fwd(REV ? S - 1 - s : s); }
+  static constexpr int ns(int s) { return s == 0 ? 1 : ns(s - 1) * rad(s - 1); }
+  static constexpr int tw_off(int s) { return s == 0 ? 0 : tw_off(s - 1) + ns(s - 1) * (rad(s - 1) - 1); }
+};
+
+// One stage of a compile-time plan on TPP threads per pencil (TPP divides
+// N/RAD): butterflies j = jl + m*TPP take elements j + r*N/RAD (in v, or read
+// through ld) and write (j div NS)*NS*RAD + (j mod NS) + r*NS through st.
+template <int N, int RAD, int TPP> struct SStage {
+  static constexpr int NB = N / RAD;
+  static_assert(NB % TPP == 0, "threads per pencil must divide the butterflies of every stage");
+  static constexpr int BM = NB / TPP;
+};
+
+template <typename R, int DIR, int N, int RAD, int NS, int TPP, class ST>
+__device__ __forceinline__ void sstage_regs(const cplx<R>* __restrict__ tws, int jl,
+                                            cplx<R> (&v)[SStage<N, RAD, TPP>::BM][RAD], ST& st) {
+  constexpr int BM = SStage<N, RAD, TPP>::BM;
+#pragma unroll
+  for (int m = 0; m < BM; ++m) {
+    const int j = jl + m * TPP;
+    const int k = j & (NS - 1);
+    const int q = j >> clog2(NS);
+    butterfly<R, DIR, RAD>(v[m], tws + k * (RAD - 1), NS > 1);
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) st(q * NS * RAD + k + r * NS, v[m][r]);
+  }
+}
+
+template <typename R, int DIR, int N, int RAD, int NS, int TPP, class LD, class ST>
+__device__ __forceinline__ void sstage(const cplx<R>* __restrict__ tws, int jl, LD& ld, ST& st) {
+  constexpr int NB = SStage<N, RAD, TPP>::NB, BM = SStage<N, RAD, TPP>::BM;
+  cplx<R> v[BM][RAD];
+#pragma unroll
+  for (int m = 0; m < BM; ++m)
+#pragma unroll
+    for (int r = 0; r < RAD; ++r) v[m][r] = ld(jl + m * TPP + r * NB);
+  sstage_regs<R, DIR, N, RAD, NS, TPP>(tws, jl, v, st);
+}
+
 }  // namespace vh

@@ -77,6 +77,7 @@ struct Geometry {
   int nports = 0;
   PassTimer* timer = nullptr;
   int zshape[2] = {0, 0};             // pass-Z shape index per precision [FP32, FP64] (autotuned)
+  bool use_p2z = true;                // compile-time pass-Z kernels for power-of-two Pz
   std::shared_ptr<PreconSymbolic> precon_sym;   // shared by all preconditioners of this geometry
   std::shared_ptr<KrylovCache> krylov;          // solver workspaces kept across calls
   // matvec workspaces, grown on demand.  A (pass X output) relies on its

@@ -13,6 +13,7 @@
 // plus Ar*J for the node rows.  rc = rhs*5 + component.  Lines of the voxel grid
 // without any conductor are skipped in X and Xi (their A rows stay zero).
 #include "vh_geom.h"
+#include <type_traits>
 
 namespace vh {
 
@@ -280,6 +281,166 @@ __global__ __launch_bounds__(NT) void k_pass_z(cplx<R>* __restrict__ B, int ltpp
 }
 
 // ---------------------------------------------------------------------------
+// pass Z for power-of-two Pz (16..512), compile-time plan: radices, thread
+// mapping and twiddle offsets are constants, the middle stages ping-pong
+// between two LDS buffers (no barrier between a stage's reads and writes),
+// and the global loads of the next component are issued before the current
+// component is transformed.  HALF (N = Pz/2, the usual padding) makes the
+// upper half of the first stage's inputs and of the last stage's outputs
+// compile-time zero / unused.  Same data flow and results as k_pass_z.
+// ---------------------------------------------------------------------------
+template <typename R, int DIR, class P, int PZ, int TPP, int T, int S, int LAST> struct MidStages {
+  __device__ __forceinline__ static void run(const cplx<R>* __restrict__ tw, int jl, cplx<R>* const (&bufs)[2], int t) {
+    if constexpr (S < LAST) {
+      const cplx<R>* in = bufs[(S - 1) & 1];
+      cplx<R>* out = bufs[S & 1];
+      auto ld = [&](int i) { return in[i * T + t]; };
+      auto st = [&](int o, cplx<R> v) { out[o * T + t] = v; };
+      sstage<R, DIR, PZ, P::rad(S), P::ns(S), TPP>(tw + P::tw_off(S), jl, ld, st);
+      __syncthreads();
+      MidStages<R, DIR, P, PZ, TPP, T, S + 1, LAST>::run(tw, jl, bufs, t);
+    }
+  }
+};
+
+template <typename R> constexpr int e_p2() { return sizeof(R) == 8 ? 4 : 8; }
+
+template <typename R, int PZ, int NT, bool HALF>
+__global__ __launch_bounds__(NT) void k_pass_z_p2(cplx<R>* __restrict__ B, GridDims g, const cplx<R>* __restrict__ twf,
+                                                  const cplx<R>* __restrict__ twi, TensorRef<R> tens, cplx<R> alpha0,
+                                                  const double2* __restrict__ alphas, const int* __restrict__ slots) {
+  using C = cplx<R>;
+  constexpr int E = e_p2<R>();
+  constexpr int TPP = PZ / E, T = NT / TPP;
+  static_assert(T >= 1 && (T & (T - 1)) == 0, "pencils per workgroup must be a power of two");
+  using PF = P2Plan<PZ, E, false>;
+  using PI = P2Plan<PZ, E, true>;
+  constexpr int S = PF::S;
+  static_assert(S >= 2, "one-stage lengths use k_pass_z");
+  constexpr int R0 = PF::rad(0);
+  constexpr int BM0 = SStage<PZ, R0, TPP>::BM, NB0 = SStage<PZ, R0, TPP>::NB;
+  constexpr int RL = PF::rad(S - 1), NSL = PF::ns(S - 1);
+  constexpr int BML = SStage<PZ, RL, TPP>::BM, NBL = SStage<PZ, RL, TPP>::NB;
+  static_assert(PI::rad(0) == RL, "inverse plan starts with the last forward radix");
+  extern __shared__ __align__(16) unsigned char smem[];
+  C* const bufs[2] = {(C*)smem, (C*)smem + T * PZ};
+  const int tid = threadIdx.x;
+  const int t = tid & (T - 1), jl = tid >> clog2(T);
+  const int Px = g.Px, Py = g.Py, N = g.N;
+  const int tiles_x = (Px + T - 1) / T;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = tile % tiles_x, u = tile / tiles_x;
+  const int v = u - 1;
+  const int ky = u == 0 ? 0 : ((v & 1) ? Py - 1 - (v >> 1) : 1 + (v >> 1));
+  const int kx = tx * T + t;
+  const bool valid = kx < Px;
+  TensorIdx ti;
+  const int ax = kx < g.Hx ? kx : Px - kx;
+  const int ay = ky < g.Hy ? ky : Py - ky;
+  ti.sx = kx < g.Hx ? 1.f : -1.f;
+  ti.sy = ky < g.Hy ? 1.f : -1.f;
+  const int axy = valid ? ax + g.Hx * ay : 0;
+  const int HxHy = g.Hx * g.Hy;
+  const int rhs = blockIdx.y;
+  C alpha = alpha0;
+  if (alphas) {
+    const double2 a = alphas[slots ? slots[rhs] : rhs];
+    alpha.x = (R)a.x; alpha.y = (R)a.y;
+  }
+  const int64_t plane = (int64_t)Px * Py;
+  C* base = B + (int64_t)rhs * 5 * N * plane + (int64_t)ky * Px + kx;
+  auto gload = [&](int c, C (&p)[BM0][R0]) {
+    const C* bc = base + (int64_t)c * N * plane;
+#pragma unroll
+    for (int m = 0; m < BM0; ++m)
+#pragma unroll
+      for (int r = 0; r < R0; ++r) {
+        const int i = jl + m * TPP + r * NB0;
+        if constexpr (HALF) p[m][r] = (r < R0 / 2 && valid) ? bc[(int64_t)i * plane] : czero<R>();   // i < Pz/2 iff r < R0/2
+        else p[m][r] = (valid && i < N) ? bc[(int64_t)i * plane] : czero<R>();
+      }
+  };
+  C held[5][BML][RL];
+  C pre[BM0][R0];
+  gload(0, pre);
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    C cur[BM0][R0];
+#pragma unroll
+    for (int m = 0; m < BM0; ++m)
+#pragma unroll
+      for (int r = 0; r < R0; ++r) cur[m][r] = pre[m][r];
+    if (c < 4) gload(c + 1, pre);
+    {
+      auto st = [&](int o, C val) { bufs[0][o * T + t] = val; };
+      sstage_regs<R, -1, PZ, R0, 1, TPP>(twf, jl, cur, st);
+    }
+    __syncthreads();
+    MidStages<R, -1, PF, PZ, TPP, T, 1, S - 1>::run(twf, jl, bufs, t);
+    {
+      const C* in = bufs[(S - 2) & 1];
+#pragma unroll
+      for (int m = 0; m < BML; ++m)
+#pragma unroll
+        for (int r = 0; r < RL; ++r) held[c][m][r] = in[(jl + m * TPP + r * NBL) * T + t];
+      // last stage: Ns * RL = Pz, so j < Ns and j mod Ns = j
+#pragma unroll
+      for (int m = 0; m < BML; ++m)
+        butterfly<R, -1, RL>(held[c][m], twf + PF::tw_off(S - 1) + (jl + m * TPP) * (RL - 1), NSL > 1);
+    }
+    __syncthreads();
+  }
+  if (valid) {
+#pragma unroll
+    for (int m = 0; m < BML; ++m)
+#pragma unroll
+      for (int r = 0; r < RL; ++r) {
+        const int kz = jl + m * TPP + r * NBL;
+        const bool lo = kz < g.Hz;
+        TensorIdx tk = ti;
+        tk.sz = lo ? 1.f : -1.f;
+        tk.idx = axy + HxHy * (lo ? kz : g.Pz - kz);
+        C f[5] = {held[0][m][r], held[1][m][r], held[2][m][r], held[3][m][r], held[4][m][r]};
+        apply_point<R>(f, tens, tk, alpha);
+#pragma unroll
+        for (int c = 0; c < 5; ++c) held[c][m][r] = f[c];
+      }
+  }
+  constexpr int RI = PI::rad(S - 1), NSI = PI::ns(S - 1);
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    {
+      auto st = [&](int o, C val) { bufs[0][o * T + t] = val; };
+      sstage_regs<R, +1, PZ, RL, 1, TPP>(twi, jl, held[c], st);
+    }
+    __syncthreads();
+    MidStages<R, +1, PI, PZ, TPP, T, 1, S - 1>::run(twi, jl, bufs, t);
+    {
+      // last inverse stage: Ns * RI = Pz, so output o = j + r*Ns with j < Ns
+      const C* in = bufs[(S - 2) & 1];
+      C* bc = base + (int64_t)c * N * plane;
+      constexpr int BMI = SStage<PZ, RI, TPP>::BM, NBI = SStage<PZ, RI, TPP>::NB;
+      static_assert(NBI == NSI, "last stage");
+#pragma unroll
+      for (int m = 0; m < BMI; ++m) {
+        const int j = jl + m * TPP;
+        C w[RI];
+#pragma unroll
+        for (int r = 0; r < RI; ++r) w[r] = in[(j + r * NBI) * T + t];
+        butterfly<R, +1, RI>(w, twi + PI::tw_off(S - 1) + j * (RI - 1), NSI > 1);
+#pragma unroll
+        for (int r = 0; r < RI; ++r) {
+          const int o = j + r * NSI;
+          if constexpr (HALF) { if (r < RI / 2 && valid) bc[(int64_t)o * plane] = w[r]; }   // o < Pz/2 iff r < RI/2
+          else { if (valid && o < N) bc[(int64_t)o * plane] = w[r]; }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // pass Xi: inverse FFT along x, crop, diagonal, -Ar' phi, scatter.
 // ---------------------------------------------------------------------------
 template <typename R, int E, int NT>
@@ -436,22 +597,48 @@ static void launch_z_rl(int rl, dim3 grid, size_t lds, hipStream_t st, cplx<R>* 
   }
 }
 
+// Power-of-two Pz in [16, 512] with the precision's default element bound:
+// the compile-time kernel (returns false when it does not apply).
+template <typename R, int NT>
+static bool launch_z_p2(int pz, int nrhs, hipStream_t st, cplx<R>* B, const GridDims& g, const Plan2& pf,
+                        const Plan2& pi, TensorRef<R> tr, cplx<R> alpha, const double2* alphas, const int* slots) {
+  auto go = [&](auto pzc) {
+    constexpr int PZ = decltype(pzc)::value;
+    constexpr int TPP = PZ / e_p2<R>(), T = NT / TPP;
+    const dim3 grid((unsigned)(ceil_div(g.Px, T) * g.Py), nrhs);
+    const size_t lds = 2 * (size_t)T * PZ * sizeof(cplx<R>);
+    if (2 * g.N == PZ)
+      hipLaunchKernelGGL((k_pass_z_p2<R, PZ, NT, true>), grid, dim3(NT), lds, st, B, g, (const cplx<R>*)pf.tw,
+                         (const cplx<R>*)pi.tw, tr, alpha, alphas, slots);
+    else
+      hipLaunchKernelGGL((k_pass_z_p2<R, PZ, NT, false>), grid, dim3(NT), lds, st, B, g, (const cplx<R>*)pf.tw,
+                         (const cplx<R>*)pi.tw, tr, alpha, alphas, slots);
+    return true;
+  };
+  switch (pz) {
+    case 16: return go(std::integral_constant<int, 16>());
+    case 32: return go(std::integral_constant<int, 32>());
+    case 64: return go(std::integral_constant<int, 64>());
+    case 128: return go(std::integral_constant<int, 128>());
+    case 256: return go(std::integral_constant<int, 256>());
+    case 512: return go(std::integral_constant<int, 512>());
+    default: return false;
+  }
+}
+
 template <typename R>
 static void launch_z(int shape, int rl, dim3 grid, size_t lds, hipStream_t st, cplx<R>* B, int ltpp,
                      const GridDims& g, const Plan2& pf, const Plan2& pi, TensorRef<R> tr, cplx<R> alpha,
                      const double2* alphas, const int* slots) {
+  const ZShape z = zshape<R>(shape);
   if constexpr (sizeof(R) == 8) {
-    switch (shape) {
-      case 0: launch_z_rl<R, 512, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
-      default: launch_z_rl<R, 256, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
-    }
+    if (z.nt == 512) launch_z_rl<R, 512, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots);
+    else launch_z_rl<R, 256, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots);
   } else {
-    switch (shape) {
-      case 0: launch_z_rl<R, 512, 8>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
-      case 1: launch_z_rl<R, 256, 8>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
-      case 2: launch_z_rl<R, 512, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
-      default: launch_z_rl<R, 256, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
-    }
+    if (z.nt == 512 && z.e == 8) launch_z_rl<R, 512, 8>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots);
+    else if (z.e == 8) launch_z_rl<R, 256, 8>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots);
+    else if (z.nt == 512) launch_z_rl<R, 512, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots);
+    else launch_z_rl<R, 256, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots);
   }
 }
 
@@ -513,7 +700,13 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
     if constexpr (sizeof(R) == 8) tr.t = G.t64.p; else tr.t = G.t32.p;
     const size_t lds = pzf.nstage > 1 ? (size_t)Tz * g.Pz * elem : 0;   // one stage: registers only
     const int ltz = log2i(tz);
-    launch_z<R>(zsh, G.fz.last_radix(rmax_for(EZ)), grid, lds, st, B, ltz, g, pzf, pzi, tr, alpha, alphas, slots);
+    bool done = false;
+    if (EZ == e_p2<R>() && G.use_p2z) {
+      if (NTZ == 512) done = launch_z_p2<R, 512>(g.Pz, nrhs, st, B, g, pzf, pzi, tr, alpha, alphas, slots);
+      else done = launch_z_p2<R, 256>(g.Pz, nrhs, st, B, g, pzf, pzi, tr, alpha, alphas, slots);
+    }
+    if (!done)
+      launch_z<R>(zsh, G.fz.last_radix(rmax_for(EZ)), grid, lds, st, B, ltz, g, pzf, pzi, tr, alpha, alphas, slots);
     VH_LAUNCHED();
   }
   mark(3);
