@@ -51,6 +51,9 @@ def parse():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
     ap.add_argument('--cpu-solves', type=int, default=2)
+    ap.add_argument('--no-configs', action='store_true', help='skip the bar / spiral / uneven-batch lines')
+    ap.add_argument('--uneven-count', type=int, default=8, help='geometries of the uneven batch to run')
+    ap.add_argument('--uneven-nfreq', type=int, default=16)
     ap.add_argument('--probe-pass-z', action='store_true',
                     help='only the 128^3 complex64 matvec (3 calls) and a reference elementwise kernel of known '
                          'traffic (1 GiB read + 1 GiB written), for hardware-counter runs')
@@ -211,6 +214,79 @@ def bus_step(S, freqs, precs, method, data, **opts):
     return Z, st
 
 
+def sweep_config(ctx, st, freqs, tol=1e-8, methods=('mixed', 'fp64'), reps=1):
+    """Batched (port x frequency) solve of one geometry, device-resident:
+    set-up (tensor, topology, preconditioners) untimed, then one warm-up and
+    `reps` timed sweeps per method; returns solves/s per method, R/L of the
+    FP64 sweep and the mixed-vs-FP64 errors (self terms, and normwise Z)."""
+    import torch
+    from voxhenry_amd import solver as vsol
+    t0 = time.perf_counter()
+    S = vsol.Solver(ctx, st)
+    precs = S.precons_for(freqs)
+    pidx = np.arange(len(freqs))
+    setup = time.perf_counter() - t0
+    out = dict(nv=int(len(S.idx)), n=int(S.G.n), nports=S.nports, nfreq=len(freqs), setup_s=setup,
+               grid=list(st['sigma'].shape))
+    res = {}
+    for m in methods:
+        S.sweep_batched(freqs, method=m, tol=tol, precons=precs, pidx_of_freq=pidx, device=True)
+        t = time.perf_counter()
+        for _ in range(reps):
+            r = S.sweep_batched(freqs, method=m, tol=tol, precons=precs, pidx_of_freq=pidx, device=True)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t) / reps
+        res[m] = r
+        nsys = len(freqs) * S.nports
+        out[m] = dict(solves_per_s=nsys / dt, s=dt, all_converged=bool(np.all(r['stats']['flag'] == 0)),
+                      relres_max=float(r['stats']['relres'].max()), iters_max=int(r['stats']['iters'].max()))
+    if 'mixed' in res and 'fp64' in res:
+        a, b = res['mixed'], res['fp64']
+        ii = np.arange(S.nports)
+        out['mixed_vs_fp64'] = dict(
+            R_self=float(np.max(np.abs(a['R'][:, ii, ii] - b['R'][:, ii, ii]) / np.abs(b['R'][:, ii, ii]))),
+            L_self=float(np.max(np.abs(a['L'][:, ii, ii] - b['L'][:, ii, ii]) / np.abs(b['L'][:, ii, ii]))),
+            Z_norm=float(max(np.linalg.norm(a['Z'][k] - b['Z'][k]) / np.linalg.norm(b['Z'][k])
+                             for k in range(len(freqs)))))
+        out['speedup_mixed_over_fp64'] = out['mixed']['solves_per_s'] / out['fp64']['solves_per_s']
+    if 'fp64' in res:
+        out['R_ohm'] = [float(v) for v in res['fp64']['R'][:, 0, 0]]
+        out['L_H'] = [float(v) for v in res['fp64']['L'][:, 0, 0]]
+    for P in precs:
+        P.close()
+    S.G.close()
+    torch.cuda.empty_cache()
+    return out
+
+
+def uneven_config(ctx, count, nfreq, seed=0):
+    """A prefix of the seeded uneven batch (bars, round wires, loops, two-fluid
+    strips), each geometry's port x frequency systems in one batched solve;
+    aggregate solves/s over the geometries (set-up untimed)."""
+    from voxhenry_amd import structures
+    freqs = np.logspace(6, 10, nfreq)
+    tot = {'mixed': 0.0, 'fp64': 0.0}
+    nsys = 0
+    per = []
+    worst = dict(R_self=0.0, L_self=0.0, Z_norm=0.0)
+    for kind, st in structures.uneven_batch(seed, count):
+        r = sweep_config(ctx, st, freqs)
+        nsys += r['nfreq'] * r['nports']
+        for m in tot:
+            tot[m] += r[m]['s']
+        for k in worst:
+            worst[k] = max(worst[k], r['mixed_vs_fp64'][k])
+        per.append(dict(kind=kind, grid=r['grid'], nv=r['nv'], setup_s=round(r['setup_s'], 2),
+                        mixed_solves_per_s=round(r['mixed']['solves_per_s'], 1),
+                        fp64_solves_per_s=round(r['fp64']['solves_per_s'], 1),
+                        converged=r['mixed']['all_converged'] and r['fp64']['all_converged']))
+        log('[bench]   uneven %s %s: mixed %.1f fp64 %.1f solves/s' % (kind, r['grid'], r['mixed']['solves_per_s'],
+                                                                     r['fp64']['solves_per_s']))
+    return dict(seed=seed, geometries=count, nfreq=nfreq, systems=nsys,
+                mixed_solves_per_s=nsys / tot['mixed'], fp64_solves_per_s=nsys / tot['fp64'],
+                speedup_mixed_over_fp64=tot['fp64'] / tot['mixed'], mixed_vs_fp64_max=worst, per_geometry=per)
+
+
 def timed_steps(fn, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -224,10 +300,22 @@ def timed_steps(fn, steps, warmup):
     return (time.perf_counter() - t0) / steps, out
 
 
+def cpu_model():
+    try:
+        for line in open('/proc/cpuinfo'):
+            if line.startswith('model name'):
+                return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return 'unknown'
+
+
 def cpu_baseline(nsolves):
-    """Oracle (FP64 numpy/scipy) on a bounded sample of the bus workload: setup
-    untimed, then `nsolves` (port, frequency) solves timed."""
-    from oracle import driver, gmres
+    """Oracle (FP64 numpy/scipy restatement of the reference's algorithm) on a
+    bounded sample of the bus workload: set-up untimed, then `nsolves`
+    (port, frequency) solves timed with one core and with the host's cores
+    (FFT workers; at most 16, the share of one GPU's host)."""
+    from oracle import driver, gmres, matvec as omv
     from voxhenry_amd import structures
     st = structures.bus()
     geo = dict(st, well_cond=st['well'])
@@ -235,17 +323,25 @@ def cpu_baseline(nsolves):
     prob = driver.Problem(geo)
     setup = time.perf_counter() - t0
     fs = np.logspace(5, 10, 64)[[40, 63]][:max(1, nsolves)]
-    t = 0.0
-    done = 0
-    for f in fs:
-        A, P, *_ = prob.operators(f)
-        t1 = time.perf_counter()
-        gmres.pgmres(A, prob.sys['rhs'][:, 0], 100, 1e-8, 10, P)
-        t += time.perf_counter() - t1
-        done += 1
-    return dict(value=done / t, unit='solves/s', cores=1, kind='port',
-                sample='%d bus solves (port 1 at f=%s Hz), oracle pgmres + SuperLU Schur solve, numpy FFT; '
-                       'setup %.1f s untimed' % (done, ','.join('%.3g' % f for f in fs), setup))
+    ops = [prob.operators(f) for f in fs]
+    cores = max(1, min(16, os.cpu_count() or 1))
+
+    def run(workers):
+        omv.FFT_WORKERS = workers
+        t = 0.0
+        for A, P, *_ in ops:
+            t1 = time.perf_counter()
+            gmres.pgmres(A, prob.sys['rhs'][:, 0], 100, 1e-8, 10, P)
+            t += time.perf_counter() - t1
+        return len(ops) / t
+
+    one = run(1)
+    many = run(cores) if cores > 1 else one
+    omv.FFT_WORKERS = 1
+    return dict(value=many, unit='solves/s', cores=cores, kind='port', one_core=one, cpu=cpu_model(),
+                sample='%d bus solves (port 1 at f=%s Hz), oracle pgmres + SuperLU Schur solve, FFTs on %d '
+                       'threads (one-core figure in one_core); set-up %.1f s untimed'
+                       % (len(ops), ','.join('%.3g' % f for f in fs), cores, setup))
 
 
 def main():
@@ -302,6 +398,18 @@ def main():
                 bound='hbm', achieved=z['GBs'], peak=HBM_PEAK_GBS, unit='GB/s',
                 frac=(z['GBs'] / HBM_PEAK_GBS) if z['GBs'] else None, traffic=recorded_traffic())
 
+    configs = None
+    if not args.no_configs:
+        from voxhenry_amd import structures
+        configs = {}
+        log('[bench] bar 30x10x10, 10 frequencies')
+        configs['bar_30x10x10'] = sweep_config(ctx, structures.bar(30, 10, 10), np.logspace(4, 10, 10))
+        log('[bench] spiral, 32 frequencies')
+        configs['spiral_5turn'] = sweep_config(ctx, structures.spiral(), np.logspace(6, 11, 32))
+        if args.uneven_count > 0:
+            log('[bench] uneven batch, first %d geometries x %d frequencies' % (args.uneven_count, args.uneven_nfreq))
+            configs['uneven_batch'] = uneven_config(ctx, args.uneven_count, args.uneven_nfreq)
+
     cpu = None
     if not args.no_cpu:
         log('[bench] CPU baseline (oracle)')
@@ -315,7 +423,7 @@ def main():
                             parallelism='1 GPU'),
                 mixed=mixed, fp64=fp64,
                 fp64_speedup=(mixed['solves_per_s'] / fp64['solves_per_s']) if fp64 else None,
-                fft_matvec=mv, roofline=roof, cpu_baseline=cpu,
+                fft_matvec=mv, roofline=roof, cpu_baseline=cpu, configs=configs,
                 setup_s=t_setup)
     print(json.dumps(line))
 

@@ -131,6 +131,13 @@ int vh_precon_prepare(vh_precon* p, double jwmu_re, double jwmu_im, int64_t* sta
 int vh_precon_apply(vh_precon* p, int prec, int64_t nrhs, const void* x, int64_t ldx, void* y, int64_t ldy,
                     int on_device);
 int vh_precon_get_ainv(vh_precon* p, double* ainv);
+/* Prepare n preconditioners of one geometry, one per frequency: jwmu holds n
+ * interleaved complex values; z (optional) n impedance vectors of nv
+ * interleaved complex values each (superconductors: z depends on f), NULL
+ * for the geometry's impedance.  The host factorisations run on up to
+ * nthreads threads (0: one per core, at most 16); the symbolic analysis is
+ * done once.  The per-frequency loop of lse_sparse_precon_prepare.m calls. */
+int vh_precon_prepare_many(vh_precon* const* ps, int64_t n, const double* jwmu, const double* z, int nthreads);
 
 /* Host-only sparse LDL^T solve of a symmetric positive definite matrix given in
  * CSC with its full pattern (both triangles), the stand-in for CHOLMOD

@@ -10,6 +10,24 @@ Pinned by (tests/test_oracle_matvec.py, tests/test_oracle_solve.py):
 """
 import numpy as np
 
+# FFT threads of the oracle: 1 uses numpy.fft; more use scipy.fft's pocketfft
+# with that many workers (the CPU baseline's multi-core timing).
+FFT_WORKERS = 1
+
+
+def _fftn(a, s=None, axes=None):
+    if FFT_WORKERS > 1:
+        import scipy.fft
+        return scipy.fft.fftn(a, s=s, axes=axes, workers=FFT_WORKERS)
+    return np.fft.fftn(a, s=s, axes=axes)
+
+
+def _ifftn(a):
+    if FFT_WORKERS > 1:
+        import scipy.fft
+        return scipy.fft.ifftn(a, workers=FFT_WORKERS)
+    return np.fft.ifftn(a)
+
 # lse_matvect_mult.m:99-103 -- diagonal weights per component, times 1/dx
 DIAG_W = np.array([1.0, 1.0, 1.0, 1.0 / 6.0, 1.0 / 2.0])
 
@@ -30,9 +48,9 @@ def fft_block(Jc, fN_all, dims):
     """Convolution part only: crop(ifftn(fN * fftn(pad(J)))) on the (L,M,N,5) grid."""
     L, M, N = dims
     P = fN_all.shape[:3]
-    fJ = np.stack([np.fft.fftn(Jc[..., c], s=P, axes=(0, 1, 2)) for c in range(5)])
+    fJ = np.stack([_fftn(Jc[..., c], s=P, axes=(0, 1, 2)) for c in range(5)])
     out = spectral_apply(fJ, fN_all)
-    return np.stack([np.fft.ifftn(out[c])[:L, :M, :N] for c in range(5)], axis=-1)
+    return np.stack([_ifftn(out[c])[:L, :M, :N] for c in range(5)], axis=-1)
 
 
 def lse_matvect_mult(JIn0, fN_all, Ae, z_real, z_imag, dx, idx5):

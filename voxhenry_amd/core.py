@@ -325,6 +325,17 @@ class Precon:
         self.stats = dict(nnzL=int(st[0]), levels_fwd=int(st[1]), levels_bwd=int(st[2]), n_node=int(st[3]))
         return self.stats
 
+    @staticmethod
+    def prepare_many(precons, jwmus, z=None, nthreads=0):
+        """Prepare one preconditioner per frequency of one geometry on host
+        threads (vh_precon_prepare_many); z: optional (n, nv) complex
+        impedances (default: the geometry's)."""
+        n = len(precons)
+        hs = (C.c_void_p * n)(*[p.h.value for p in precons])
+        jw = np.ascontiguousarray(np.asarray(jwmus, dtype=np.complex128).ravel())
+        zz = None if z is None else np.ascontiguousarray(np.asarray(z, dtype=np.complex128))
+        check(lib().vh_precon_prepare_many(C.cast(hs, C.c_void_p), n, ptr(jw), ptr(zz), int(nthreads)))
+
     def ainv(self):
         out = np.zeros(self.geom.num_curr)
         check(lib().vh_precon_get_ainv(self.h, ptr(out)))

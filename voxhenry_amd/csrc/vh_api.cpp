@@ -385,6 +385,20 @@ int vh_precon_prepare(vh_precon* p, double jre, double jim, int64_t* stats) {
   });
 }
 
+int vh_precon_prepare_many(vh_precon* const* ps, int64_t n, const double* jwmu, const double* z, int nthreads) {
+  return guard([&] {
+    need(ps, "ps"); need(jwmu, "jwmu");
+    VH_REQUIRE(n >= 1 && n <= (1 << 20), "n out of range");
+    std::vector<vh::Precon*> P(n);
+    for (int64_t k = 0; k < n; ++k) {
+      VH_REQUIRE(ps[k] && ps[k]->P.G == ps[0]->P.G, "all preconditioners must belong to one geometry");
+      P[k] = &ps[k]->P;
+    }
+    vh::precon_prepare_many(P.data(), (int)n, (const std::complex<double>*)jwmu, (const std::complex<double>*)z,
+                            nthreads);
+  });
+}
+
 int vh_precon_get_ainv(vh_precon* p, double* ainv) {
   return guard([&] {
     need(p, "p"); need(ainv, "ainv");
@@ -591,7 +605,7 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
   return guard([&] {
     need(g, "g"); need(precons, "precons"); need(jwmu, "jwmu"); need(pidx, "pidx"); need(b, "b"); need(x, "x");
     auto& G = g->G;
-    VH_REQUIRE(G.has_tensor && G.has_inc && G.has_z, "geometry not complete (tensor, incidence, impedance)");
+    VH_REQUIRE(G.has_tensor && G.has_inc, "geometry not complete (tensor, incidence)");
     VH_REQUIRE(nprecon >= 1 && nsys >= 1 && nsys <= 13107, "nprecon / nsys out of range");
     VH_REQUIRE(method == 0 || method == 1, "method must be 0 (fp64) or 1 (mixed)");
     VH_REQUIRE(restart >= 1 && restart <= 127 && inner_restart >= 1 && inner_restart <= 127 && maxit >= 1,
@@ -620,6 +634,18 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
     vh::DevBuf<vh::PcTab<double>> tabs64(nprecon);
     vh::DevBuf<vh::PcTab<float>> tabs32(nprecon);
     vh::DevBuf<int> dpidx(nsys);
+    // impedance of every system: the one its preconditioner was prepared with
+    // (frequency-dependent for superconductors)
+    std::vector<const double2*> zp64(nsys);
+    std::vector<const float2*> zp32(nsys);
+    for (int64_t k = 0; k < nsys; ++k) {
+      zp64[k] = precons[pidx[k]]->P.z64.p;
+      zp32[k] = precons[pidx[k]]->P.z32.p;
+    }
+    vh::DevBuf<const double2*> zs64(nsys);
+    vh::DevBuf<const float2*> zs32(nsys);
+    VH_HIP(hipMemcpy(zs64.p, zp64.data(), nsys * sizeof(zp64[0]), hipMemcpyHostToDevice));
+    VH_HIP(hipMemcpy(zs32.p, zp32.data(), nsys * sizeof(zp32[0]), hipMemcpyHostToDevice));
     VH_HIP(hipMemcpy(tabs64.p, t64.data(), nprecon * sizeof(t64[0]), hipMemcpyHostToDevice));
     VH_HIP(hipMemcpy(tabs32.p, t32.data(), nprecon * sizeof(t32[0]), hipMemcpyHostToDevice));
     VH_HIP(hipMemcpy(dpidx.p, pidx, nsys * sizeof(int), hipMemcpyHostToDevice));
@@ -643,10 +669,10 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
     // out for the chunk); coff is the chunk's first system
     int64_t coff = 0;
     vh::OpFn<double> A64 = [&](const std::vector<int>& act, const int* sl, const double2* xx, double2* yy) {
-      vh::matvec<double>(G, (int)act.size(), xx, n, yy, n, 0.0, 0.0, 0, sl, alphas.p + coff);
+      vh::matvec<double>(G, (int)act.size(), xx, n, yy, n, 0.0, 0.0, 0, sl, alphas.p + coff, zs64.p + coff);
     };
     vh::OpFn<float> A32 = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
-      vh::matvec<float>(G, (int)act.size(), xx, n, yy, n, 0.0, 0.0, 0, sl, alphas.p + coff);
+      vh::matvec<float>(G, (int)act.size(), xx, n, yy, n, 0.0, 0.0, 0, sl, alphas.p + coff, zs32.p + coff);
     };
     // preconditioner of every active system of the chunk in one pass (the
     // system's preconditioner is dpidx[coff + chunk-relative slot])

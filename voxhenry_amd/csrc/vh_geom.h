@@ -6,6 +6,7 @@
 #include "vh_fft.h"
 #include "vh_fft2.h"
 #include <memory>
+#include <mutex>
 
 namespace vh {
 
@@ -79,6 +80,7 @@ struct Geometry {
   int zshape[2] = {0, 0};             // pass-Z shape index per precision [FP32, FP64] (autotuned)
   bool use_p2z = true;                // compile-time pass-Z kernels for power-of-two Pz
   std::shared_ptr<PreconSymbolic> precon_sym;   // shared by all preconditioners of this geometry
+  std::mutex precon_sym_mu;                     // guards its construction
   std::shared_ptr<KrylovCache> krylov;          // solver workspaces kept across calls
   // matvec workspaces, grown on demand.  A (pass X output) relies on its
   // never-written empty lines reading as zero, so each precision has its own.
@@ -94,10 +96,12 @@ struct Geometry {
 // resistive diagonal only (current rows); mode 2: convolution only.
 // slots (device, optional): vector k of the batch is x + slots[k]*ldx.
 // alphas (device, optional): j*omega*mu per vector slot (overrides alpha).
+// zslot (device, optional): impedance vector per vector slot (overrides the
+// geometry's impedance; frequency-dependent conductors in a batch).
 template <typename R>
 void matvec(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy,
             double alpha_re, double alpha_im, int mode = 0, const int* slots = nullptr,
-            const double2* alphas = nullptr);
+            const double2* alphas = nullptr, const cplx<R>* const* zslot = nullptr);
 
 // Build the spectral tensor on the device from a real-space normalised tensor
 // G (L*M*N*7 complex, device, column-major [x][y][z][c]).

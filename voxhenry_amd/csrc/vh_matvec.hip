@@ -448,7 +448,8 @@ __global__ __launch_bounds__(NT) void k_pass_x_inv(
     const cplx<R>* __restrict__ B, const cplx<R>* __restrict__ x, int64_t ldx, cplx<R>* __restrict__ y, int64_t ldy,
     const int* __restrict__ vmap, const int* __restrict__ lines, int nlines, int ltpp, GridDims g, Plan2 pl,
     const cplx<R>* __restrict__ z, R inv_dx, const int* __restrict__ atp, const int* __restrict__ atc,
-    const R* __restrict__ atv, int64_t num_curr, int mode, const int* __restrict__ slots, int dense) {
+    const R* __restrict__ atv, int64_t num_curr, int mode, const int* __restrict__ slots, int dense,
+    const cplx<R>* const* __restrict__ zslot) {
   // mode 0: conv + diag - Ar'phi ; mode 1: conv + diag ; mode 2: conv only
   using C = cplx<R>;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -470,6 +471,7 @@ __global__ __launch_bounds__(NT) void k_pass_x_inv(
   const R w = inv_dx * (comp < 3 ? R(1) : comp == 3 ? R(1) / R(6) : R(1) / R(2));
   const C* xs = x + (int64_t)slot * ldx;
   C* ys = y + (int64_t)slot * ldy;
+  const C* __restrict__ zv = zslot ? zslot[slot] : z;
   const int* vm = vmap + (int64_t)L * line;
   const int64_t lbase = (int64_t)L * line;
   auto gst = [&](int o, C acc) {
@@ -477,7 +479,7 @@ __global__ __launch_bounds__(NT) void k_pass_x_inv(
     const int vid = dense ? (int)(lbase + o) : vm[o];
     if (vid < 0) return;
     const int64_t row = (int64_t)comp * g.nv + vid;
-    if (mode <= 1) acc = cadd(acc, cscale(cmul(z[vid], xs[row]), w));
+    if (mode <= 1) acc = cadd(acc, cscale(cmul(zv[vid], xs[row]), w));
     if (mode == 0) {
       C s = czero<R>();
       for (int k = atp[row]; k < atp[row + 1]; ++k) {
@@ -644,7 +646,8 @@ static void launch_z(int shape, int rl, dim3 grid, size_t lds, hipStream_t st, c
 
 template <typename R>
 static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy,
-                       double are, double aim, int mode, const int* slots, const double2* alphas) {
+                       double are, double aim, int mode, const int* slots, const double2* alphas,
+                       const cplx<R>* const* zslot) {
   VH_REQUIRE(G.has_tensor, "matvec: Green's tensor not set");
   VH_REQUIRE(nrhs >= 1, "matvec: nrhs must be >= 1");
   VH_REQUIRE(nrhs * 5 <= 65535, "matvec: too many right-hand sides in one call");
@@ -725,11 +728,11 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
     if constexpr (sizeof(R) == 8) atv = G.ArT.v64.p; else atv = G.ArT.v32.p;
     const cplx<R>* zz;
     if constexpr (sizeof(R) == 8) zz = G.z64.p; else zz = G.z32.p;
-    if (mode <= 1) VH_REQUIRE(G.has_z, "matvec: impedance not set");
+    if (mode <= 1) VH_REQUIRE(G.has_z || zslot, "matvec: impedance not set");
     if (mode == 0) VH_REQUIRE(G.has_inc, "matvec: incidence not set");
     hipLaunchKernelGGL((k_pass_x_inv<R, EXY, NT_XY>), grid, dim3(NT_XY), lds_x, st, B, x, ldx, y, ldy, G.vmap.p,
                        G.lines.p, G.nlines, log2i(tx), g, px, zz, (R)(1.0 / G.dx), G.ArT.ptr.p, G.ArT.col.p, atv,
-                       G.num_curr, mode, slots, dense);
+                       G.num_curr, mode, slots, dense, zslot);
     VH_LAUNCHED();
   }
   mark(5);
@@ -767,13 +770,13 @@ void PassTimer::collect() {
 
 template <typename R>
 void matvec(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy, double are, double aim,
-            int mode, const int* slots, const double2* alphas) {
-  run_passes<R>(G, nrhs, x, ldx, y, ldy, are, aim, mode, slots, alphas);
+            int mode, const int* slots, const double2* alphas, const cplx<R>* const* zslot) {
+  run_passes<R>(G, nrhs, x, ldx, y, ldy, are, aim, mode, slots, alphas, zslot);
 }
 
 template void matvec<double>(Geometry&, int, const double2*, int64_t, double2*, int64_t, double, double, int,
-                             const int*, const double2*);
+                             const int*, const double2*, const double2* const*);
 template void matvec<float>(Geometry&, int, const float2*, int64_t, float2*, int64_t, double, double, int,
-                            const int*, const double2*);
+                            const int*, const double2*, const float2* const*);
 
 }  // namespace vh

@@ -12,7 +12,10 @@
 // Work vectors are [node][column], column fastest.
 #include "vh_solver.h"
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <mutex>
+#include <thread>
 
 namespace vh {
 
@@ -478,9 +481,9 @@ static void block_inverses(const PreconSymbolic& Y, const LdlFactor& F, std::vec
   }
 }
 
-void precon_prepare(Precon& P, double jre, double jim) {
+void precon_prepare(Precon& P, double jre, double jim, const std::complex<double>* zin) {
   Geometry& G = *P.G;
-  VH_REQUIRE(G.has_inc && G.has_z, "preconditioner: incidence and impedance must be set first");
+  VH_REQUIRE(G.has_inc && (G.has_z || zin), "preconditioner: incidence and impedance must be set first");
   const int nv = G.g.nv;
   const int64_t nc = G.num_curr;
   const int nn = (int)G.num_node;
@@ -488,9 +491,10 @@ void precon_prepare(Precon& P, double jre, double jim) {
   const std::complex<double> jw(jre, jim);
   std::complex<double> st[3];
   for (int k = 0; k < 3; ++k) st[k] = jw * std::complex<double>(G.st2[2 * k], G.st2[2 * k + 1]);
+  P.zh.assign(zin ? zin : G.zh.data(), (zin ? zin : G.zh.data()) + nv);
   std::vector<double> ainv(nc);
   for (int v = 0; v < nv; ++v) {
-    const std::complex<double> z = G.zh[v];
+    const std::complex<double> z = P.zh[v];
     const double dp = 1.0 / std::abs(z / G.dx + st[0]);
     const double d2 = 1.0 / std::abs(z / (6.0 * G.dx) + st[1]);
     const double d3 = 1.0 / std::abs(z / (2.0 * G.dx) + st[2]);
@@ -498,11 +502,17 @@ void precon_prepare(Precon& P, double jre, double jim) {
     ainv[3 * nv + v] = d2; ainv[4 * nv + v] = d3;
   }
   P.ainv_h = ainv;
+  // The first preparation of a geometry builds the shared symbolic data (and,
+  // after its numeric factorisation, the level schedules); concurrent
+  // preparations wait on the lock until it is complete, later ones skip it.
   bool fresh = false;
+  std::unique_lock<std::mutex> lk(G.precon_sym_mu);
   if (!G.precon_sym) {
     G.precon_sym = std::make_shared<PreconSymbolic>();
     build_symbolic(G, P.geometric_order, *G.precon_sym);
     fresh = true;
+  } else {
+    lk.unlock();
   }
   P.sym = G.precon_sym;
   PreconSymbolic& Y = *P.sym;
@@ -532,6 +542,7 @@ void precon_prepare(Precon& P, double jre, double jim) {
   if (fresh) {
     Y.F.Li = P.F.Li;
     build_levels(G, P.F, Y);
+    lk.unlock();
   }
   std::vector<double> Rx(Y.nnzL);
   for (int64_t p = 0; p < Y.nnzL; ++p) Rx[Y.csc2csr[p]] = P.F.Lx[p];
@@ -543,8 +554,16 @@ void precon_prepare(Precon& P, double jre, double jim) {
   upload(P.ainv64, ainv); upload_f32(P.ainv32, ainv);
   upload(P.Li64, Li); upload_f32(P.Li32, Li);
   upload(P.LiT64, LiT); upload_f32(P.LiT32, LiT);
-  P.tab64_h = PcTab<double>{P.Rx64.p, P.Cx64.p, P.D64.p, P.ainv64.p, P.Li64.p, P.LiT64.p};
-  P.tab32_h = PcTab<float>{P.Rx32.p, P.Cx32.p, P.D32.p, P.ainv32.p, P.Li32.p, P.LiT32.p};
+  {
+    std::vector<float2> z32(nv);
+    for (int v = 0; v < nv; ++v) { z32[v].x = (float)P.zh[v].real(); z32[v].y = (float)P.zh[v].imag(); }
+    P.z64.alloc(nv);
+    P.z32.alloc(nv);
+    VH_HIP(hipMemcpy(P.z64.p, P.zh.data(), nv * sizeof(double2), hipMemcpyHostToDevice));
+    VH_HIP(hipMemcpy(P.z32.p, z32.data(), nv * sizeof(float2), hipMemcpyHostToDevice));
+  }
+  P.tab64_h = PcTab<double>{P.Rx64.p, P.Cx64.p, P.D64.p, P.ainv64.p, P.Li64.p, P.LiT64.p, P.z64.p};
+  P.tab32_h = PcTab<float>{P.Rx32.p, P.Cx32.p, P.D32.p, P.ainv32.p, P.Li32.p, P.LiT32.p, P.z32.p};
   P.tab64.alloc(1);
   P.tab32.alloc(1);
   VH_HIP(hipMemcpy(P.tab64.p, &P.tab64_h, sizeof(PcTab<double>), hipMemcpyHostToDevice));
@@ -553,6 +572,40 @@ void precon_prepare(Precon& P, double jre, double jim) {
   P.F.Li.clear(); P.F.Li.shrink_to_fit();
   P.F.Lx.clear(); P.F.Lx.shrink_to_fit();
   P.ready = true;
+}
+
+void precon_prepare_many(Precon* const* P, int n, const std::complex<double>* jw, const std::complex<double>* z,
+                         int nthreads) {
+  if (n <= 0) return;
+  const int nv = P[0]->G->g.nv;
+  auto one = [&](int k) {
+    precon_prepare(*P[k], jw[k].real(), jw[k].imag(), z ? z + (size_t)k * nv : nullptr);
+  };
+  one(0);   // builds the symbolic data once
+  if (n == 1) return;
+  if (nthreads <= 0) nthreads = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+  nthreads = std::min(nthreads, n - 1);
+  const int dev = P[0]->G->ctx->device;
+  std::atomic<int> next(1);
+  std::exception_ptr err;
+  std::mutex em;
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nthreads; ++t)
+    pool.emplace_back([&] {
+      if (hipSetDevice(dev) != hipSuccess) return;
+      for (;;) {
+        const int k = next++;
+        if (k >= n) break;
+        try {
+          one(k);
+        } catch (...) {
+          std::lock_guard<std::mutex> g(em);
+          if (!err) err = std::current_exception();
+        }
+      }
+    });
+  for (auto& th : pool) th.join();
+  if (err) std::rethrow_exception(err);
 }
 
 #define VH_LPR_LAUNCH(lpr, KERNEL, grid, ...)                                                          \

@@ -14,6 +14,7 @@ template <typename R> struct PcTab {
   const R* ainv;
   const R* Li;     // inverses of the diagonal blocks, packed row-major lower
   const R* LiT;    // their transposes, packed row-major upper
+  const cplx<R>* z;  // impedance per voxel at this preconditioner's frequency
 };
 
 // Pattern data of the Schur factor, shared by every frequency of a geometry:
@@ -52,14 +53,24 @@ struct Precon {
   // (row-major lower: Linv; the transpose row-major: LinvT)
   DevBuf<double> Rx64, Cx64, D64, ainv64, Li64, LiT64;
   DevBuf<float> Rx32, Cx32, D32, ainv32, Li32, LiT32;
+  std::vector<std::complex<double>> zh;   // impedance per voxel it was prepared with
+  DevBuf<double2> z64;                // device copies of zh (the batched matvec reads them per system)
+  DevBuf<float2> z32;
   DevBuf<PcTab<double>> tab64;        // one-entry tables for the single apply
   DevBuf<PcTab<float>> tab32;
   PcTab<double> tab64_h{};
   PcTab<float> tab32_h{};
 };
 
-// A_inv, S, LDL^T for jwmu = j*omega*mu (lse_sparse_precon_prepare.m).
-void precon_prepare(Precon& P, double jwmu_re, double jwmu_im);
+// A_inv, S, LDL^T for jwmu = j*omega*mu (lse_sparse_precon_prepare.m) with
+// the impedance z (nv values; null: the geometry's current impedance).
+// Several preconditioners of one geometry may be prepared concurrently once
+// the geometry's symbolic analysis exists (precon_prepare_many does that).
+void precon_prepare(Precon& P, double jwmu_re, double jwmu_im, const std::complex<double>* z = nullptr);
+// Prepare n preconditioners (jwmu[k], impedance z + k*nv or the geometry's)
+// on up to nthreads host threads.
+void precon_prepare_many(Precon* const* P, int n, const std::complex<double>* jwmu, const std::complex<double>* z,
+                         int nthreads);
 // y = M^-1 x for nrhs vectors (device pointers), lse_sparse_precon_multiply.m:84-88.
 template <typename R>
 void precon_apply(Precon& P, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy,

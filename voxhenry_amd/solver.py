@@ -91,26 +91,34 @@ class Solver:
         st['solve_time'] = time.perf_counter() - t
         return X, st
 
-    def precons_for(self, freqs):
-        """One prepared preconditioner per frequency (symbolic analysis shared)."""
-        out = []
-        for f in freqs:
-            P = Precon(self.G)
-            if self.structure.get('lambdaL') is not None:
-                self.G.set_impedance(impedance(self.structure, self.idx, f))
-            P.prepare(1j * 2 * np.pi * f * MU0)
-            out.append(P)
+    def precons_for(self, freqs, nthreads=0):
+        """One prepared preconditioner per frequency (symbolic analysis shared,
+        host factorisations on parallel threads); each keeps the impedance of
+        its frequency, which the batched solve uses for its systems."""
+        freqs = np.asarray(freqs, float)
+        if self._z_f is None:
+            self.G.set_impedance(impedance(self.structure, self.idx, freqs[0]))
+            self._z_f = freqs[0]
+        out = [Precon(self.G) for _ in freqs]
+        z = None
+        if self.structure.get('lambdaL') is not None:
+            z = np.array([impedance(self.structure, self.idx, f) for f in freqs])
+        t = time.perf_counter()
+        Precon.prepare_many(out, 1j * 2 * np.pi * freqs * MU0, z, nthreads)
+        self.times['precon'] = self.times.get('precon', 0.0) + time.perf_counter() - t
         return out
 
-    def sweep_batched(self, freqs, method='mixed', tol=1e-8, precons=None, pidx_of_freq=None, **kw):
+    def sweep_batched(self, freqs, method='mixed', tol=1e-8, precons=None, pidx_of_freq=None, device=False, **kw):
         """Every (port, frequency) system in one batched solve (vh_solve_batch).
 
         precons[pidx_of_freq[k]] preconditions frequency k (default: one
-        preconditioner per frequency).  Only normal conductors: the impedance
-        diagonal is frequency independent."""
-        if self.structure.get('lambdaL') is not None:
-            raise ValueError('batched sweep needs frequency-independent impedance (normal conductors)')
+        preconditioner per frequency).  Every system uses the impedance its
+        preconditioner was prepared with, so two-fluid conductors need one
+        preconditioner per frequency (their impedance depends on f)."""
         freqs = np.asarray(freqs, float)
+        if self.structure.get('lambdaL') is not None and precons is not None and \
+                len(set(np.asarray(pidx_of_freq).tolist())) < len(freqs):
+            raise ValueError('two-fluid conductors need one preconditioner per frequency')
         if self._z_f is None:
             self.G.set_impedance(impedance(self.structure, self.idx, freqs[0]))
             self._z_f = freqs[0]
@@ -118,16 +126,34 @@ class Solver:
             precons = self.precons_for(freqs)
             pidx_of_freq = np.arange(len(freqs))
         npo = self.nports
-        B = np.repeat(self.rhs[None], len(freqs), axis=0).reshape(-1, self.G.n)
         jw = np.repeat(1j * 2 * np.pi * freqs * MU0, npo)
         pidx = np.repeat(np.asarray(pidx_of_freq), npo)
-        t = time.perf_counter()
-        X, st = self.G.solve_batch(B, jw, precons, pidx, method=method, tol=tol, **kw)
-        st['solve_time'] = time.perf_counter() - t
-        X = X.reshape(len(freqs), npo, -1)
+        if device:
+            # right-hand sides and solutions stay in device memory; only the
+            # port rows of the solutions (where the port vectors are nonzero)
+            # come back, for Y = P^T X
+            import torch
+            nz = np.flatnonzero(np.any(self.rhs != 0, axis=0))
+            rhs = torch.tensor(self.rhs, dtype=torch.complex128, device='cuda')
+            B = rhs.repeat(len(freqs), 1).contiguous()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            X, st = self.G.solve_batch(B, jw, precons, pidx, method=method, tol=tol, **kw)
+            Xp = X.index_select(1, torch.tensor(nz, device='cuda')).cpu().numpy()
+            st['solve_time'] = time.perf_counter() - t
+            Xp = Xp.reshape(len(freqs), npo, -1)
+            Pnz = self.rhs[:, nz]
+            Ys = [Pnz @ Xp[k].T for k in range(len(freqs))]
+        else:
+            B = np.repeat(self.rhs[None], len(freqs), axis=0).reshape(-1, self.G.n)
+            t = time.perf_counter()
+            X, st = self.G.solve_batch(B, jw, precons, pidx, method=method, tol=tol, **kw)
+            st['solve_time'] = time.perf_counter() - t
+            X = X.reshape(len(freqs), npo, -1)
+            Ys = [self.rhs @ X[k].T for k in range(len(freqs))]
         out = dict(freq=freqs, Y=[], Z=[], R=[], L=[], stats=st)
         for k, f in enumerate(freqs):
-            Y = self.rhs @ X[k].T
+            Y = Ys[k]
             Z = np.linalg.inv(Y)
             out['Y'].append(Y); out['Z'].append(Z)
             out['R'].append(np.abs(Z.real)); out['L'].append(np.abs(Z.imag) / (2 * np.pi * f))
