@@ -438,10 +438,12 @@ int vh_precon_apply(vh_precon* p, int prec, int64_t nrhs, const void* x, int64_t
     Io io;
     io.in(x, nrhs, ldx, es, on_device, st, ldy);
     void* dy = on_device ? y : io.y;
+    std::vector<int> slots(nrhs);
+    for (int64_t i = 0; i < nrhs; ++i) slots[i] = (int)i;
     if (prec == VH_C128)
-      vh::precon_apply<double>(p->P, (int)nrhs, (const double2*)io.x, ldx, (double2*)dy, ldy);
+      vh::precon_apply<double>(p->P, slots, (const double2*)io.x, ldx, (double2*)dy, ldy);
     else
-      vh::precon_apply<float>(p->P, (int)nrhs, (const float2*)io.x, ldx, (float2*)dy, ldy);
+      vh::precon_apply<float>(p->P, slots, (const float2*)io.x, ldx, (float2*)dy, ldy);
     if (!on_device) {
       d2h(y, io.y, (size_t)nrhs * ldy * es, st);
     }
@@ -480,7 +482,7 @@ int vh_gmres(vh_geom* g, vh_precon* p, double jre, double jim, int prec, int fle
         vh::matvec<double>(G, (int)act.size(), xx, n, yy, n, jre, jim, 0, sl);
       };
       vh::OpFn<double> M = [&](const std::vector<int>& act, const int* sl, const double2* xx, double2* yy) {
-        vh::precon_apply<double>(p->P, (int)act.size(), xx, n, yy, n, sl);
+        vh::precon_apply<double>(p->P, act, xx, n, yy, n);
       };
       vh::gmres_batch<double>(G.ctx, n, A, p ? &M : nullptr, (const double2*)io.x, (double2*)dx, sys, o, W, stats);
     } else {
@@ -490,7 +492,7 @@ int vh_gmres(vh_geom* g, vh_precon* p, double jre, double jim, int prec, int fle
         vh::matvec<float>(G, (int)act.size(), xx, n, yy, n, jre, jim, 0, sl);
       };
       vh::OpFn<float> M = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
-        vh::precon_apply<float>(p->P, (int)act.size(), xx, n, yy, n, sl);
+        vh::precon_apply<float>(p->P, act, xx, n, yy, n);
       };
       vh::gmres_batch<float>(G.ctx, n, A, p ? &M : nullptr, (const float2*)io.x, (float2*)dx, sys, o, W, stats);
     }
@@ -536,7 +538,7 @@ int vh_solve_mixed(vh_geom* g, vh_precon* p, double jre, double jim, int64_t nrh
       vh::matvec<float>(G, (int)act.size(), xx, n, yy, n, jre, jim, 0, sl);
     };
     vh::OpFn<float> M32 = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
-      vh::precon_apply<float>(p->P, (int)act.size(), xx, n, yy, n, sl);
+      vh::precon_apply<float>(p->P, act, xx, n, yy, n);
     };
     std::vector<vh::SysStats> stats;
     vh::KrylovCache& K = krylov_cache(G);
@@ -633,7 +635,6 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
     }
     vh::DevBuf<vh::PcTab<double>> tabs64(nprecon);
     vh::DevBuf<vh::PcTab<float>> tabs32(nprecon);
-    vh::DevBuf<int> dpidx(nsys);
     // impedance of every system: the one its preconditioner was prepared with
     // (frequency-dependent for superconductors)
     std::vector<const double2*> zp64(nsys);
@@ -648,7 +649,6 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
     VH_HIP(hipMemcpy(zs32.p, zp32.data(), nsys * sizeof(zp32[0]), hipMemcpyHostToDevice));
     VH_HIP(hipMemcpy(tabs64.p, t64.data(), nprecon * sizeof(t64[0]), hipMemcpyHostToDevice));
     VH_HIP(hipMemcpy(tabs32.p, t32.data(), nprecon * sizeof(t32[0]), hipMemcpyHostToDevice));
-    VH_HIP(hipMemcpy(dpidx.p, pidx, nsys * sizeof(int), hipMemcpyHostToDevice));
     // HBM budget -> chunk size.  The geometry's workspace arena counts as
     // available (it is reused), so the chunk size, and with it the arena,
     // is the same on every call with the same sizes.
@@ -675,12 +675,12 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
       vh::matvec<float>(G, (int)act.size(), xx, n, yy, n, 0.0, 0.0, 0, sl, alphas.p + coff, zs32.p + coff);
     };
     // preconditioner of every active system of the chunk in one pass (the
-    // system's preconditioner is dpidx[coff + chunk-relative slot])
+    // system's preconditioner is pidx[coff + chunk-relative slot])
     vh::OpFn<double> M64 = [&](const std::vector<int>& act, const int* sl, const double2* xx, double2* yy) {
-      vh::precon_apply_cols<double>(G, *sym, tabs64.p, dpidx.p + coff, (int)act.size(), xx, n, yy, n, sl);
+      vh::precon_apply_cols<double>(G, *sym, tabs64.p, pidx + coff, act, xx, n, yy, n);
     };
     vh::OpFn<float> M32 = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
-      vh::precon_apply_cols<float>(G, *sym, tabs32.p, dpidx.p + coff, (int)act.size(), xx, n, yy, n, sl);
+      vh::precon_apply_cols<float>(G, *sym, tabs32.p, pidx + coff, act, xx, n, yy, n);
     };
     for (int64_t s0 = 0; s0 < nsys; s0 += chunk) {
       coff = s0;

@@ -16,70 +16,78 @@
 #include <cmath>
 #include <mutex>
 #include <thread>
+#include <type_traits>
 
 namespace vh {
 
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
-// All kernels work on a batch of ncol columns; column c is the vector in slot
-// slots[c] (c when slots is null) and uses the factor values of table entry
-// grp_of_slot[slot] (entry 0 when grp_of_slot is null).  Work vectors are
-// [column][row] (one contiguous vector per column: the entries of a row of L
-// run through contiguous block-column ranges, so lanes over entries read
-// consecutive addresses), always FP64 with FP64 accumulation: with
-// FP32 factors the apply is then a fixed linear operator up to the rounding of
-// its output (FP32 work vectors would add rounding noise amplified by the
-// conditioning of L at every level, which stalls an FP32 Krylov solve).
-template <typename R>
-__device__ __forceinline__ const PcTab<R>& col_tab(const PcTab<R>* tab, const int* grp_of_slot, int slot) {
-  return tab[grp_of_slot ? grp_of_slot[slot] : 0];
-}
+// A batch is applied in groups: group g holds up to NPG columns (systems) that
+// share one preconditioner (one frequency), tab[gtab[g]]; its column j is the
+// vector in slot gslot[g*NPG + j] (-1: an unused position, computed as zero).
+// Work vectors are [group][node][port] (index (g*nn + i)*NPG + j), so the
+// ports of a group at one node are contiguous: one (row, group) item reads
+// each entry of L and its index once for all ports of the group, and one
+// line of the work vector per entry.  Work vectors are FP64 with FP64
+// accumulation in both precisions: with FP32 factors the apply is then a
+// fixed linear operator up to the rounding of its output (FP32 work vectors
+// add rounding noise, amplified by the conditioning of L at every level,
+// which stalls an FP32 Krylov solve).
+struct PcGroupDev {
+  int ng;
+  const int* gtab;    // ng
+  const int* gslot;   // ng * NPG
+};
 
-// w[i][c] = b_c[perm[i]] - sum_k Ar(perm[i],k) * Ainv[k] * a_c[k]
-template <typename R, typename WT>
+// w[g][i][j] = b[perm[i]] - sum_k Ar(perm[i],k) * Ainv[k] * a[k] of column (g, j)
+template <typename R, typename WT, int NPG>
 __global__ __launch_bounds__(256) void k_pc_gather(const cplx<R>* __restrict__ x, int64_t ldx, cplx<WT>* __restrict__ w,
-                                                   int ncol, int nn, const int* __restrict__ perm,
-                                                   const int* __restrict__ arp, const int* __restrict__ arc,
-                                                   const R* __restrict__ arv, const PcTab<R>* __restrict__ tab,
-                                                   const int* __restrict__ grp_of_slot, int64_t nc,
-                                                   const int* __restrict__ slots) {
+                                                   int nn, const int* __restrict__ perm, const int* __restrict__ arp,
+                                                   const int* __restrict__ arc, const R* __restrict__ arv,
+                                                   const PcTab<R>* __restrict__ tab, PcGroupDev gr, int64_t nc) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)nn * ncol) return;
-  const int c = (int)(t / nn), i = (int)(t - (int64_t)c * nn);
-  const int slot = slots ? slots[c] : c;
-  const R* ainv = col_tab(tab, grp_of_slot, slot).ainv;
-  const int node = perm[i];
-  const cplx<R>* xr = x + (int64_t)slot * ldx;
-  const cplx<R> b = xr[nc + node];
-  double2 s; s.x = b.x; s.y = b.y;
-  for (int p = arp[node]; p < arp[node + 1]; ++p) {
-    const int k = arc[p];
-    const double cf = (double)arv[p] * (double)ainv[k];
-    const cplx<R> a = xr[k];
-    s.x -= cf * a.x; s.y -= cf * a.y;
+  if (t >= (int64_t)gr.ng * nn * NPG) return;
+  const int j = (int)(t % NPG);
+  const int64_t gi = t / NPG;
+  const int g = (int)(gi / nn), i = (int)(gi - (int64_t)g * nn);
+  const int slot = gr.gslot[g * NPG + j];
+  cplx<WT> o; o.x = 0; o.y = 0;
+  if (slot >= 0) {
+    const R* ainv = tab[gr.gtab[g]].ainv;
+    const int node = perm[i];
+    const cplx<R>* xr = x + (int64_t)slot * ldx;
+    const cplx<R> b = xr[nc + node];
+    double sx = b.x, sy = b.y;
+    for (int p = arp[node]; p < arp[node + 1]; ++p) {
+      const int k = arc[p];
+      const double cf = (double)arv[p] * (double)ainv[k];
+      const cplx<R> a = xr[k];
+      sx -= cf * a.x; sy -= cf * a.y;
+    }
+    o.x = (WT)sx; o.y = (WT)sy;
   }
-  cplx<WT> o; o.x = (WT)s.x; o.y = (WT)s.y;
   w[t] = o;
 }
 
-// Lane layout of the level kernels: a wave serves 64/LPR work items (row of
-// the level, column) with LPR lanes each (item = lane mod NI, part = lane div
-// NI); the parts of an item stride over its row's entries, and the partial
-// sums meet with xor-shuffles over multiples of NI.  Items are numbered
-// column-major: item = column * nrows + row of the level.
-template <int LPR> struct Lanes {
-  static constexpr int NI = 64 / LPR;
+// Lanes of the level kernels: an item is (row q of the level, group g),
+// item = q * ng + g (the groups of one row are neighbours: they share the
+// index reads), served by NPG * LPR consecutive lanes; lane -> port
+// j = lane mod NPG and part (lane div NPG) mod LPR.  The parts stride over
+// the row's entries and are summed with butterflies inside the item's lanes.
+template <int NPG, int LPR> struct PLanes {
+  static_assert(NPG * LPR <= 64, "an item must fit in a wave");
   int64_t item;
-  int part;
-  __device__ __forceinline__ Lanes() {
+  int j, part;
+  __device__ __forceinline__ PLanes() {
     const int lane = threadIdx.x & 63;
-    item = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * NI + (lane % NI);
-    part = lane / NI;
+    item = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / (NPG * LPR);
+    j = lane % NPG;
+    part = (lane / NPG) % LPR;
   }
-  template <typename C> __device__ __forceinline__ C sum(C v) const {
+  __device__ __forceinline__ double2 sum(double2 v) const {
 #pragma unroll
-    for (int o = NI; o < 64; o <<= 1) { v.x += __shfl_xor(v.x, o, 64); v.y += __shfl_xor(v.y, o, 64); }
+    for (int o = NPG; o < NPG * LPR; o <<= 1) { v.x += __shfl_xor(v.x, o, 64); v.y += __shfl_xor(v.y, o, 64); }
     return v;
   }
 };
@@ -92,166 +100,153 @@ template <int LPR> struct Lanes {
 //   u_r = y_r / D_r - sum_{i >= c1} L(i,r) x_i   (k_sn_update_bwd)
 //   x_k = Linv_k' u_k                             (k_sn_gemv_bwd)
 // w holds b, t and finally x; w2 holds y then u.
-template <typename R, typename WT, int LPR>
+template <typename R, typename WT, int NPG, int LPR>
 __global__ __launch_bounds__(256) void k_sn_update_fwd(const int* __restrict__ rows, int nrows,
                                                        const int* __restrict__ rp, const int* __restrict__ split,
                                                        const int* __restrict__ ri, const PcTab<R>* __restrict__ tab,
-                                                       const int* __restrict__ grp_of_slot,
-                                                       const int* __restrict__ slots, int ncol, int nn,
-                                                       cplx<WT>* __restrict__ w, const cplx<WT>* __restrict__ w2) {
-  const Lanes<LPR> ln;
-  const bool valid = ln.item < (int64_t)nrows * ncol;
+                                                       PcGroupDev gr, int nn, cplx<WT>* __restrict__ w,
+                                                       const cplx<WT>* __restrict__ w2) {
+  const PLanes<NPG, LPR> ln;
+  if (ln.item >= (int64_t)nrows * gr.ng) return;   // all lanes of the item
+  const int q = (int)(ln.item / gr.ng), g = (int)(ln.item - (int64_t)q * gr.ng);
+  const int r = rows[q];
+  const R* rx = tab[gr.gtab[g]].Rx;
+  const cplx<WT>* wg = w2 + (int64_t)g * nn * NPG + ln.j;
   double2 s; s.x = 0; s.y = 0;
-  int r = 0, c = 0;
-  if (valid) {
-    c = (int)(ln.item / nrows);
-    r = rows[ln.item - (int64_t)c * nrows];
-    const R* rx = col_tab(tab, grp_of_slot, slots ? slots[c] : c).Rx;
-    const cplx<WT>* wc2 = w2 + (int64_t)c * nn;
-    for (int p = rp[r] + ln.part; p < split[r]; p += LPR) {
-      const double l = rx[p];
-      const cplx<WT> v = wc2[ri[p]];
-      s.x += l * v.x; s.y += l * v.y;
-    }
+  for (int p = rp[r] + ln.part; p < split[r]; p += LPR) {
+    const double l = rx[p];
+    const cplx<WT> v = wg[(int64_t)ri[p] * NPG];
+    s.x += l * v.x; s.y += l * v.y;
   }
   s = ln.sum(s);
-  if (valid && ln.part == 0) {
-    cplx<WT>& o = w[(int64_t)c * nn + r];
+  if (ln.part == 0) {
+    cplx<WT>& o = w[((int64_t)g * nn + r) * NPG + ln.j];
     o.x = (WT)((double)o.x - s.x); o.y = (WT)((double)o.y - s.y);
   }
 }
 
-template <typename R, typename WT, int LPR>
+template <typename R, typename WT, int NPG, int LPR>
 __global__ __launch_bounds__(256) void k_sn_gemv_fwd(const int* __restrict__ rows, int nrows,
                                                      const int* __restrict__ sn_of_row, const int* __restrict__ sn_ptr,
                                                      const int64_t* __restrict__ sn_off,
-                                                     const PcTab<R>* __restrict__ tab,
-                                                     const int* __restrict__ grp_of_slot,
-                                                     const int* __restrict__ slots, int ncol, int nn,
+                                                     const PcTab<R>* __restrict__ tab, PcGroupDev gr, int nn,
                                                      const cplx<WT>* __restrict__ w, cplx<WT>* __restrict__ w2) {
-  const Lanes<LPR> ln;
-  const bool valid = ln.item < (int64_t)nrows * ncol;
+  const PLanes<NPG, LPR> ln;
+  if (ln.item >= (int64_t)nrows * gr.ng) return;
+  const int q = (int)(ln.item / gr.ng), g = (int)(ln.item - (int64_t)q * gr.ng);
+  const int r = rows[q];
+  const int k = sn_of_row[r];
+  const int c0 = sn_ptr[k];
+  const int i = r - c0;
+  const R* row = tab[gr.gtab[g]].Li + sn_off[k] + (int64_t)i * (i + 1) / 2;
+  const cplx<WT>* tt = w + ((int64_t)g * nn + c0) * NPG + ln.j;
   double2 s; s.x = 0; s.y = 0;
-  int r = 0, c = 0;
-  if (valid) {
-    c = (int)(ln.item / nrows);
-    r = rows[ln.item - (int64_t)c * nrows];
-    const int k = sn_of_row[r];
-    const int c0 = sn_ptr[k];
-    const int i = r - c0;
-    const R* row = col_tab(tab, grp_of_slot, slots ? slots[c] : c).Li + sn_off[k] + (int64_t)i * (i + 1) / 2;
-    const cplx<WT>* tt = w + (int64_t)c * nn + c0;
-    for (int j = ln.part; j <= i; j += LPR) {
-      const double l = row[j];
-      const cplx<WT> v = tt[j];
-      s.x += l * v.x; s.y += l * v.y;
-    }
+  for (int jj = ln.part; jj <= i; jj += LPR) {
+    const double l = row[jj];
+    const cplx<WT> v = tt[(int64_t)jj * NPG];
+    s.x += l * v.x; s.y += l * v.y;
   }
   s = ln.sum(s);
-  if (valid && ln.part == 0) { cplx<WT> o; o.x = (WT)s.x; o.y = (WT)s.y; w2[(int64_t)c * nn + r] = o; }
+  if (ln.part == 0) {
+    cplx<WT> o; o.x = (WT)s.x; o.y = (WT)s.y;
+    w2[((int64_t)g * nn + r) * NPG + ln.j] = o;
+  }
 }
 
-template <typename R, typename WT, int LPR>
+template <typename R, typename WT, int NPG, int LPR>
 __global__ __launch_bounds__(256) void k_sn_update_bwd(const int* __restrict__ rows, int nrows,
                                                        const int* __restrict__ cp, const int* __restrict__ split,
                                                        const int* __restrict__ ci, const PcTab<R>* __restrict__ tab,
-                                                       const int* __restrict__ grp_of_slot,
-                                                       const int* __restrict__ slots, int ncol, int nn,
-                                                       const cplx<WT>* __restrict__ w, cplx<WT>* __restrict__ w2) {
-  const Lanes<LPR> ln;
-  const bool valid = ln.item < (int64_t)nrows * ncol;
+                                                       PcGroupDev gr, int nn, const cplx<WT>* __restrict__ w,
+                                                       cplx<WT>* __restrict__ w2) {
+  const PLanes<NPG, LPR> ln;
+  if (ln.item >= (int64_t)nrows * gr.ng) return;
+  const int q = (int)(ln.item / gr.ng), g = (int)(ln.item - (int64_t)q * gr.ng);
+  const int r = rows[q];
+  const PcTab<R>& T = tab[gr.gtab[g]];
+  const R* cx = T.Cx;
+  const cplx<WT>* wg = w + (int64_t)g * nn * NPG + ln.j;
   double2 s; s.x = 0; s.y = 0;
-  int r = 0, c = 0;
-  const R* D = nullptr;
-  if (valid) {
-    c = (int)(ln.item / nrows);
-    r = rows[ln.item - (int64_t)c * nrows];
-    const PcTab<R>& T = col_tab(tab, grp_of_slot, slots ? slots[c] : c);
-    const R* cx = T.Cx;
-    D = T.D;
-    const cplx<WT>* wc = w + (int64_t)c * nn;
-    for (int p = split[r] + ln.part; p < cp[r + 1]; p += LPR) {
-      const double l = cx[p];
-      const cplx<WT> v = wc[ci[p]];
-      s.x += l * v.x; s.y += l * v.y;
-    }
+  for (int p = split[r] + ln.part; p < cp[r + 1]; p += LPR) {
+    const double l = cx[p];
+    const cplx<WT> v = wg[(int64_t)ci[p] * NPG];
+    s.x += l * v.x; s.y += l * v.y;
   }
   s = ln.sum(s);
-  if (valid && ln.part == 0) {
-    cplx<WT>& o = w2[(int64_t)c * nn + r];
-    const double inv = 1.0 / (double)D[r];
+  if (ln.part == 0) {
+    cplx<WT>& o = w2[((int64_t)g * nn + r) * NPG + ln.j];
+    const double inv = 1.0 / (double)T.D[r];
     o.x = (WT)((double)o.x * inv - s.x);
     o.y = (WT)((double)o.y * inv - s.y);
   }
 }
 
-template <typename R, typename WT, int LPR>
+template <typename R, typename WT, int NPG, int LPR>
 __global__ __launch_bounds__(256) void k_sn_gemv_bwd(const int* __restrict__ rows, int nrows,
                                                      const int* __restrict__ sn_of_row, const int* __restrict__ sn_ptr,
                                                      const int64_t* __restrict__ sn_off,
-                                                     const PcTab<R>* __restrict__ tab,
-                                                     const int* __restrict__ grp_of_slot,
-                                                     const int* __restrict__ slots, int ncol, int nn,
+                                                     const PcTab<R>* __restrict__ tab, PcGroupDev gr, int nn,
                                                      const cplx<WT>* __restrict__ w2, cplx<WT>* __restrict__ w) {
-  const Lanes<LPR> ln;
-  const bool valid = ln.item < (int64_t)nrows * ncol;
+  const PLanes<NPG, LPR> ln;
+  if (ln.item >= (int64_t)nrows * gr.ng) return;
+  const int q = (int)(ln.item / gr.ng), g = (int)(ln.item - (int64_t)q * gr.ng);
+  const int r = rows[q];
+  const int k = sn_of_row[r];
+  const int c0 = sn_ptr[k], sz = sn_ptr[k + 1] - c0;
+  const int i = r - c0;
+  // row i of the transpose holds Linv(i..sz-1, i): offset i*sz - i(i-1)/2
+  const R* row = tab[gr.gtab[g]].LiT + sn_off[k] + (int64_t)i * sz - (int64_t)i * (i - 1) / 2;
+  const cplx<WT>* uu = w2 + ((int64_t)g * nn + r) * NPG + ln.j;
   double2 s; s.x = 0; s.y = 0;
-  int r = 0, c = 0;
-  if (valid) {
-    c = (int)(ln.item / nrows);
-    r = rows[ln.item - (int64_t)c * nrows];
-    const int k = sn_of_row[r];
-    const int c0 = sn_ptr[k], sz = sn_ptr[k + 1] - c0;
-    const int i = r - c0;
-    // row i of the transpose holds Linv(i..sz-1, i): offset i*sz - i(i-1)/2
-    const R* row = col_tab(tab, grp_of_slot, slots ? slots[c] : c).LiT + sn_off[k] + (int64_t)i * sz -
-                   (int64_t)i * (i - 1) / 2;
-    const cplx<WT>* uu = w2 + (int64_t)c * nn + r;
-    for (int j = ln.part; j < sz - i; j += LPR) {
-      const double l = row[j];
-      const cplx<WT> v = uu[j];
-      s.x += l * v.x; s.y += l * v.y;
-    }
+  for (int jj = ln.part; jj < sz - i; jj += LPR) {
+    const double l = row[jj];
+    const cplx<WT> v = uu[(int64_t)jj * NPG];
+    s.x += l * v.x; s.y += l * v.y;
   }
   s = ln.sum(s);
-  if (valid && ln.part == 0) { cplx<WT> o; o.x = (WT)s.x; o.y = (WT)s.y; w[(int64_t)c * nn + r] = o; }
+  if (ln.part == 0) {
+    cplx<WT> o; o.x = (WT)s.x; o.y = (WT)s.y;
+    w[((int64_t)g * nn + r) * NPG + ln.j] = o;
+  }
 }
 
-// y_c[nc + perm[i]] = w[i][c]
-template <typename R, typename WT>
+// y[nc + perm[i]] = w[g][i][j] of column (g, j)
+template <typename R, typename WT, int NPG>
 __global__ __launch_bounds__(256) void k_pc_scatter_nodes(const cplx<WT>* __restrict__ w, cplx<R>* __restrict__ y,
-                                                          int64_t ldy, int ncol, int nn, const int* __restrict__ perm,
-                                                          int64_t nc, const int* __restrict__ slots) {
+                                                          int64_t ldy, int nn, const int* __restrict__ perm,
+                                                          int64_t nc, PcGroupDev gr) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)nn * ncol) return;
-  const int c = (int)(t / nn), i = (int)(t - (int64_t)c * nn);
+  if (t >= (int64_t)gr.ng * nn * NPG) return;
+  const int j = (int)(t % NPG);
+  const int64_t gi = t / NPG;
+  const int g = (int)(gi / nn), i = (int)(gi - (int64_t)g * nn);
+  const int slot = gr.gslot[g * NPG + j];
+  if (slot < 0) return;
   cplx<R> o; o.x = (R)w[t].x; o.y = (R)w[t].y;
-  y[(int64_t)(slots ? slots[c] : c) * ldy + nc + perm[i]] = o;
+  y[(int64_t)slot * ldy + nc + perm[i]] = o;
 }
 
-// y_c[k] = (a_c[k] + sum_i ArTp(k,i) w[i][c]) * Ainv[k]   (ArTp columns are permuted node ids)
-template <typename R, typename WT>
+// y[k] = (a[k] + sum_i ArTp(k,i) w[g][i][j]) * Ainv[k]   (ArTp columns are permuted node ids)
+template <typename R, typename WT, int NPG>
 __global__ __launch_bounds__(256) void k_pc_currents(const cplx<R>* __restrict__ x, int64_t ldx,
                                                      const cplx<WT>* __restrict__ w, cplx<R>* __restrict__ y,
-                                                     int64_t ldy, int ncol, int nn, int64_t nc,
-                                                     const int* __restrict__ tp,
+                                                     int64_t ldy, int nn, int64_t nc, const int* __restrict__ tp,
                                                      const int* __restrict__ tc, const R* __restrict__ tv,
-                                                     const PcTab<R>* __restrict__ tab,
-                                                     const int* __restrict__ grp_of_slot,
-                                                     const int* __restrict__ slots) {
+                                                     const PcTab<R>* __restrict__ tab, PcGroupDev gr) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int c = blockIdx.y;
-  const int slot = slots ? slots[c] : c;
-  if (k >= nc) return;
+  const int gj = blockIdx.y, g = gj / NPG, j = gj - g * NPG;
+  const int slot = gr.gslot[gj];
+  if (k >= nc || slot < 0) return;
   const cplx<R> a = x[(int64_t)slot * ldx + k];
   double sx = a.x, sy = a.y;
   if (tc) {
+    const cplx<WT>* wg = w + (int64_t)g * nn * NPG + j;
     for (int p = tp[k]; p < tp[k + 1]; ++p) {
-      const cplx<WT> v = w[(int64_t)c * nn + tc[p]];
+      const cplx<WT> v = wg[(int64_t)tc[p] * NPG];
       sx += (double)tv[p] * v.x; sy += (double)tv[p] * v.y;
     }
   }
-  const double ai = col_tab(tab, grp_of_slot, slot).ainv[k];
+  const double ai = tab[gr.gtab[g]].ainv[k];
   cplx<R> o; o.x = (R)(sx * ai); o.y = (R)(sy * ai);
   y[(int64_t)slot * ldy + k] = o;
 }
@@ -393,6 +388,37 @@ static void build_levels(Geometry& G, const LdlFactor& F, PreconSymbolic& Y) {
   }
   level_rows(lf, Y.sn_ptr, Y.f_lptr, Y.f_rows);
   level_rows(lb, Y.sn_ptr, Y.b_lptr, Y.b_rows);
+  if (trace_on()) {
+    // block statistics: sizes, and the dense panels a block-wise update would use
+    int64_t off_nnz = 0, fpanel = 0, bpanel = 0;
+    int smax = 0, umax = 0;
+    std::vector<int> mark(n, -1);
+    for (int k = 0; k < nsn; ++k) {
+      const int c0 = Y.sn_ptr[k], c1 = Y.sn_ptr[k + 1];
+      smax = std::max(smax, c1 - c0);
+      int u = 0;
+      for (int r = c0; r < c1; ++r)
+        for (int p = Rp[r]; p < Y.row_split[r]; ++p) {
+          ++off_nnz;
+          if (mark[Ri[p]] != k) { mark[Ri[p]] = k; ++u; }
+        }
+      umax = std::max(umax, u);
+      fpanel += (int64_t)u * (c1 - c0);
+    }
+    std::fill(mark.begin(), mark.end(), -1);
+    for (int k = 0; k < nsn; ++k) {
+      const int c0 = Y.sn_ptr[k], c1 = Y.sn_ptr[k + 1];
+      int v = 0;
+      for (int r = c0; r < c1; ++r)
+        for (int q = Y.col_split[r]; q < F.Lp[r + 1]; ++q)
+          if (mark[F.Li[q]] != k) { mark[F.Li[q]] = k; ++v; }
+      bpanel += (int64_t)v * (c1 - c0);
+    }
+    fprintf(stderr, "[vh] schur: n %d nnzL %ld blocks %d (max %d, mean %.1f) levels f %zu b %zu; "
+                    "off-block nnz %ld, dense panels fwd %ld bwd %ld (max union %d), inverses %ld\n",
+            n, (long)F.Lp[n], nsn, smax, (double)n / nsn, Y.f_lptr.size() - 1, Y.b_lptr.size() - 1, (long)off_nnz,
+            (long)fpanel, (long)bpanel, umax, (long)Y.dense);
+  }
   // lanes per item: the power of two nearest above half the mean row length, in [1, 64]
   auto lpr = [](int64_t tot, int64_t cnt) {
     const double avg = cnt > 0 ? (double)tot / (double)cnt : 0.0;
@@ -608,97 +634,195 @@ void precon_prepare_many(Precon* const* P, int n, const std::complex<double>* jw
   if (err) std::rethrow_exception(err);
 }
 
-#define VH_LPR_LAUNCH(lpr, KERNEL, grid, ...)                                                          \
-  switch (lpr) {                                                                                      \
-    case 1: hipLaunchKernelGGL((KERNEL<R, WT, 1>), grid, dim3(256), 0, st, __VA_ARGS__); break;       \
-    case 2: hipLaunchKernelGGL((KERNEL<R, WT, 2>), grid, dim3(256), 0, st, __VA_ARGS__); break;       \
-    case 4: hipLaunchKernelGGL((KERNEL<R, WT, 4>), grid, dim3(256), 0, st, __VA_ARGS__); break;       \
-    case 8: hipLaunchKernelGGL((KERNEL<R, WT, 8>), grid, dim3(256), 0, st, __VA_ARGS__); break;       \
-    case 16: hipLaunchKernelGGL((KERNEL<R, WT, 16>), grid, dim3(256), 0, st, __VA_ARGS__); break;     \
-    case 32: hipLaunchKernelGGL((KERNEL<R, WT, 32>), grid, dim3(256), 0, st, __VA_ARGS__); break;     \
-    default: hipLaunchKernelGGL((KERNEL<R, WT, 64>), grid, dim3(256), 0, st, __VA_ARGS__); break;     \
+template <int V> using IC = std::integral_constant<int, V>;
+
+// f(IC<NPG>, IC<LPR>) for the runtime lane counts, LPR clamped to 64 / NPG
+template <int N, int L, typename F> static void lanes_if(F& f) {
+  if constexpr (N * L <= 64) f(IC<N>{}, IC<L>{});
+}
+template <int N, typename F> static void lanes_l(int lpr, F& f) {
+  switch (std::min(lpr, 64 / N)) {
+    case 1: lanes_if<N, 1>(f); break;
+    case 2: lanes_if<N, 2>(f); break;
+    case 4: lanes_if<N, 4>(f); break;
+    case 8: lanes_if<N, 8>(f); break;
+    case 16: lanes_if<N, 16>(f); break;
+    case 32: lanes_if<N, 32>(f); break;
+    default: lanes_if<N, 64>(f); break;
   }
+}
+template <typename F> static void with_lanes(int npg, int lpr, F&& f) {
+  switch (npg) {
+    case 1: lanes_l<1>(lpr, f); break;
+    case 2: lanes_l<2>(lpr, f); break;
+    case 4: lanes_l<4>(lpr, f); break;
+    default: lanes_l<8>(lpr, f); break;
+  }
+}
 
-// grid for nitems (row, column) items of lpr lanes each, 256-thread blocks
-static dim3 lane_grid(int64_t nitems, int lpr) { return dim3((unsigned)ceil_div(nitems * lpr, 256)); }
+// grid for nitems items of lanes threads each, 256-thread blocks
+static dim3 lane_grid(int64_t nitems, int lanes) { return dim3((unsigned)ceil_div(nitems * lanes, 256)); }
 
-template <typename R, typename WT>
-static void apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, const int* grp_of_slot, int ncol,
-                       const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy, const int* slots) {
-  VH_REQUIRE(ncol >= 1, "preconditioner: no columns");
+// Groups of the columns: the slots of each preconditioner in chunks of npg.
+// npg (1, 2, 4 or 8) minimises an estimate of the level kernels' traffic:
+// per group and entry of L, its value and index (12 bytes) and the line of
+// the work vector (at least 64 bytes, 16 per port).
+static int build_groups(const int* pidx, const std::vector<int>& slots, std::vector<int>& key) {
+  std::vector<int> order;            // preconditioners in order of first use
+  std::vector<std::vector<int>> cols;
+  {
+    std::vector<int> where;
+    for (int s : slots) {
+      const int t = pidx ? pidx[s] : 0;
+      if (t >= (int)where.size()) where.resize(t + 1, -1);
+      if (where[t] < 0) { where[t] = (int)order.size(); order.push_back(t); cols.emplace_back(); }
+      cols[where[t]].push_back(s);
+    }
+  }
+  int best = 1;
+  double best_cost = 0;
+  for (int npg = 1; npg <= 8; npg <<= 1) {
+    int64_t ng = 0;
+    for (auto& c : cols) ng += ceil_div((int64_t)c.size(), (int64_t)npg);
+    const double cost = (double)ng * (12.0 + std::max(64.0, 16.0 * npg));
+    if (npg == 1 || cost <= best_cost) { best = npg; best_cost = cost; }
+  }
+  // key: npg, ng, gtab[ng], gslot[ng * npg]
+  int ng = 0;
+  for (auto& c : cols) ng += (int)ceil_div((int64_t)c.size(), (int64_t)best);
+  key.assign(2 + (size_t)ng * (1 + best), -1);
+  key[0] = best; key[1] = ng;
+  int g = 0;
+  for (size_t t = 0; t < cols.size(); ++t)
+    for (size_t q = 0; q < cols[t].size(); q += best, ++g) {
+      key[2 + g] = order[t];
+      for (int j = 0; j < best && q + j < cols[t].size(); ++j) key[2 + ng + (size_t)g * best + j] = cols[t][q + j];
+    }
+  return ng;
+}
+
+// Device copy of the group table; uploaded only when it changes.  The host
+// staging is pinned and reused once its previous copy has completed.
+static PcGroupDev group_table(PreconSymbolic& Y, const std::vector<int>& key, hipStream_t st) {
+  PcGroups& C = Y.groups;
+  if (key != C.key) {
+    if (C.ev) VH_HIP(hipEventSynchronize(C.ev));
+    else VH_HIP(hipEventCreateWithFlags(&C.ev, hipEventDisableTiming));
+    if (C.d.n < key.size()) {
+      VH_HIP(hipStreamSynchronize(st));   // kernels in flight may still read the old table
+      C.d.alloc(key.size());
+    }
+    C.h.ensure(key.size());
+    std::copy(key.begin(), key.end(), C.h.p);
+    VH_HIP(hipMemcpyAsync(C.d.p, C.h.p, key.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    VH_HIP(hipEventRecord(C.ev, st));
+    C.key = key;
+  }
+  PcGroupDev gr;
+  gr.ng = key[1];
+  gr.gtab = C.d.p + 2;
+  gr.gslot = C.d.p + 2 + key[1];
+  return gr;
+}
+
+template <typename R>
+void precon_apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, const int* pidx,
+                       const std::vector<int>& slots, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy) {
+  using WT = double;
+  VH_REQUIRE(!slots.empty(), "preconditioner: no columns");
   hipStream_t st = G.ctx->stream;
   const int nn = (int)G.num_node;
   const int64_t nc = G.num_curr;
-  const size_t need = 2 * (size_t)nn * ncol * sizeof(cplx<WT>);
+  std::vector<int> key;
+  build_groups(pidx, slots, key);
+  const int npg = key[0];
+  const PcGroupDev gr = group_table(Y, key, st);
+  const int64_t cols = (int64_t)gr.ng * npg;
+  const size_t need = 2 * (size_t)nn * cols * sizeof(cplx<WT>);
   if (Y.ws.n < need) Y.ws.alloc(need);
   cplx<WT>* w = (cplx<WT>*)Y.ws.p;
-  cplx<WT>* w2 = w + (size_t)nn * ncol;
+  cplx<WT>* w2 = w + (size_t)nn * cols;
   const bool d = sizeof(R) == 8;
   const R* arv = d ? (const R*)G.Ar.v64.p : (const R*)G.Ar.v32.p;
+  auto by_npg = [&](auto&& f) {
+    switch (npg) {
+      case 1: f(IC<1>{}); break;
+      case 2: f(IC<2>{}); break;
+      case 4: f(IC<4>{}); break;
+      default: f(IC<8>{}); break;
+    }
+  };
   if (nn > 0) {
-    const int64_t items = (int64_t)nn * ncol;
-    hipLaunchKernelGGL((k_pc_gather<R, WT>), dim3(ceil_div(items, 256)), dim3(256), 0, st, x, ldx, w, ncol, nn,
-                       Y.d_perm.p, G.Ar.ptr.p, G.Ar.col.p, arv, tab, grp_of_slot, nc, slots);
+    const int64_t items = (int64_t)nn * cols;
+    by_npg([&](auto N) {
+      hipLaunchKernelGGL((k_pc_gather<R, WT, decltype(N)::value>), dim3(ceil_div(items, 256)), dim3(256), 0, st, x,
+                         ldx, w, nn, Y.d_perm.p, G.Ar.ptr.p, G.Ar.col.p, arv, tab, gr, nc);
+    });
     VH_LAUNCHED();
     const int nlf = (int)Y.f_lptr.size() - 1;
     for (int l = 0; l < nlf; ++l) {
       const int a = Y.f_lptr[l], nr = Y.f_lptr[l + 1] - a;
-      const int64_t it = (int64_t)nr * ncol;
+      const int64_t it = (int64_t)nr * gr.ng;
       if (Y.f_upd_nnz[l] > 0) {
-        VH_LPR_LAUNCH(Y.f_lpr_u[l], k_sn_update_fwd, lane_grid(it, Y.f_lpr_u[l]), Y.d_f_rows.p + a, nr, Y.Rp.p,
-                      Y.d_row_split.p, Y.Ri.p, tab, grp_of_slot, slots, ncol, nn, w, w2);
+        with_lanes(npg, Y.f_lpr_u[l], [&](auto N, auto L) {
+          constexpr int n = decltype(N)::value, lp = decltype(L)::value;
+          hipLaunchKernelGGL((k_sn_update_fwd<R, WT, n, lp>), lane_grid(it, n * lp), dim3(256), 0, st,
+                             Y.d_f_rows.p + a, nr, Y.Rp.p, Y.d_row_split.p, Y.Ri.p, tab, gr, nn, w, w2);
+        });
         VH_LAUNCHED();
       }
-      VH_LPR_LAUNCH(Y.f_lpr_g[l], k_sn_gemv_fwd, lane_grid(it, Y.f_lpr_g[l]), Y.d_f_rows.p + a, nr,
-                    Y.d_sn_of_row.p, Y.d_sn_ptr.p, Y.d_sn_off.p, tab, grp_of_slot, slots, ncol, nn, w, w2);
+      with_lanes(npg, Y.f_lpr_g[l], [&](auto N, auto L) {
+        constexpr int n = decltype(N)::value, lp = decltype(L)::value;
+        hipLaunchKernelGGL((k_sn_gemv_fwd<R, WT, n, lp>), lane_grid(it, n * lp), dim3(256), 0, st, Y.d_f_rows.p + a,
+                           nr, Y.d_sn_of_row.p, Y.d_sn_ptr.p, Y.d_sn_off.p, tab, gr, nn, w, w2);
+      });
       VH_LAUNCHED();
     }
     const int nlb = (int)Y.b_lptr.size() - 1;
     for (int l = 0; l < nlb; ++l) {
       const int a = Y.b_lptr[l], nr = Y.b_lptr[l + 1] - a;
-      const int64_t it = (int64_t)nr * ncol;
-      VH_LPR_LAUNCH(Y.b_lpr_u[l], k_sn_update_bwd, lane_grid(it, Y.b_lpr_u[l]), Y.d_b_rows.p + a, nr, Y.Cp.p,
-                    Y.d_col_split.p, Y.Ci.p, tab, grp_of_slot, slots, ncol, nn, w, w2);
+      const int64_t it = (int64_t)nr * gr.ng;
+      with_lanes(npg, Y.b_lpr_u[l], [&](auto N, auto L) {
+        constexpr int n = decltype(N)::value, lp = decltype(L)::value;
+        hipLaunchKernelGGL((k_sn_update_bwd<R, WT, n, lp>), lane_grid(it, n * lp), dim3(256), 0, st, Y.d_b_rows.p + a,
+                           nr, Y.Cp.p, Y.d_col_split.p, Y.Ci.p, tab, gr, nn, w, w2);
+      });
       VH_LAUNCHED();
-      VH_LPR_LAUNCH(Y.b_lpr_g[l], k_sn_gemv_bwd, lane_grid(it, Y.b_lpr_g[l]), Y.d_b_rows.p + a, nr,
-                    Y.d_sn_of_row.p, Y.d_sn_ptr.p, Y.d_sn_off.p, tab, grp_of_slot, slots, ncol, nn, w2, w);
+      with_lanes(npg, Y.b_lpr_g[l], [&](auto N, auto L) {
+        constexpr int n = decltype(N)::value, lp = decltype(L)::value;
+        hipLaunchKernelGGL((k_sn_gemv_bwd<R, WT, n, lp>), lane_grid(it, n * lp), dim3(256), 0, st, Y.d_b_rows.p + a,
+                           nr, Y.d_sn_of_row.p, Y.d_sn_ptr.p, Y.d_sn_off.p, tab, gr, nn, w2, w);
+      });
       VH_LAUNCHED();
     }
-    hipLaunchKernelGGL((k_pc_scatter_nodes<R, WT>), dim3(ceil_div(items, 256)), dim3(256), 0, st, w, y, ldy, ncol, nn,
-                       Y.d_perm.p, nc, slots);
+    by_npg([&](auto N) {
+      hipLaunchKernelGGL((k_pc_scatter_nodes<R, WT, decltype(N)::value>), dim3(ceil_div(items, 256)), dim3(256), 0, st,
+                         w, y, ldy, nn, Y.d_perm.p, nc, gr);
+    });
     VH_LAUNCHED();
   }
   const R* atv = d ? (const R*)G.ArT.v64.p : (const R*)G.ArT.v32.p;
-  hipLaunchKernelGGL((k_pc_currents<R, WT>), dim3(ceil_div(nc, 256), ncol), dim3(256), 0, st, x, ldx, w, y, ldy, ncol, nn, nc,
-                     G.ArT.ptr.p, nn > 0 ? Y.ArTp_col.p : nullptr, atv, tab, grp_of_slot, slots);
+  VH_REQUIRE(cols < 65536, "preconditioner: too many columns");
+  by_npg([&](auto N) {
+    hipLaunchKernelGGL((k_pc_currents<R, WT, decltype(N)::value>), dim3(ceil_div(nc, 256), (unsigned)cols), dim3(256), 0,
+                       st, x, ldx, w, y, ldy, nn, nc, G.ArT.ptr.p, nn > 0 ? Y.ArTp_col.p : nullptr, atv, tab, gr);
+  });
   VH_LAUNCHED();
 }
 
 template <typename R>
-void precon_apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, const int* grp_of_slot, int ncol,
-                       const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy, const int* slots) {
-  // FP32 work vectors only on request (VH_PC_WORK32=1): they halve the work
-  // vector traffic but make the FP32 apply noisier.
-  static const bool work32 = [] { const char* e = std::getenv("VH_PC_WORK32"); return e && e[0] == '1'; }();
-  if (sizeof(R) == 4 && work32)
-    apply_cols<R, float>(G, Y, tab, grp_of_slot, ncol, x, ldx, y, ldy, slots);
-  else
-    apply_cols<R, double>(G, Y, tab, grp_of_slot, ncol, x, ldx, y, ldy, slots);
-}
-
-template <typename R>
-void precon_apply(Precon& P, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy, const int* slots) {
+void precon_apply(Precon& P, const std::vector<int>& slots, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy) {
   VH_REQUIRE(P.ready, "preconditioner not prepared");
   const PcTab<R>* tab;
   if constexpr (sizeof(R) == 8) tab = P.tab64.p; else tab = P.tab32.p;
-  precon_apply_cols<R>(*P.G, *P.sym, tab, nullptr, nrhs, x, ldx, y, ldy, slots);
+  precon_apply_cols<R>(*P.G, *P.sym, tab, nullptr, slots, x, ldx, y, ldy);
 }
 
-template void precon_apply_cols<double>(Geometry&, PreconSymbolic&, const PcTab<double>*, const int*, int,
-                                        const double2*, int64_t, double2*, int64_t, const int*);
-template void precon_apply_cols<float>(Geometry&, PreconSymbolic&, const PcTab<float>*, const int*, int,
-                                       const float2*, int64_t, float2*, int64_t, const int*);
-template void precon_apply<double>(Precon&, int, const double2*, int64_t, double2*, int64_t, const int*);
-template void precon_apply<float>(Precon&, int, const float2*, int64_t, float2*, int64_t, const int*);
+template void precon_apply_cols<double>(Geometry&, PreconSymbolic&, const PcTab<double>*, const int*,
+                                        const std::vector<int>&, const double2*, int64_t, double2*, int64_t);
+template void precon_apply_cols<float>(Geometry&, PreconSymbolic&, const PcTab<float>*, const int*,
+                                       const std::vector<int>&, const float2*, int64_t, float2*, int64_t);
+template void precon_apply<double>(Precon&, const std::vector<int>&, const double2*, int64_t, double2*, int64_t);
+template void precon_apply<float>(Precon&, const std::vector<int>&, const float2*, int64_t, float2*, int64_t);
 
 }  // namespace vh

@@ -17,6 +17,19 @@ template <typename R> struct PcTab {
   const cplx<R>* z;  // impedance per voxel at this preconditioner's frequency
 };
 
+// Device table of the column groups of the last preconditioner apply
+// (npg, ng, gtab[ng], gslot[ng * npg]) with its pinned staging.
+struct PcGroups {
+  std::vector<int> key;
+  PinnedBuf<int> h;
+  DevBuf<int> d;
+  hipEvent_t ev = nullptr;   // completion of the last upload from h
+  PcGroups() = default;
+  PcGroups(const PcGroups&) = delete;
+  PcGroups& operator=(const PcGroups&) = delete;
+  ~PcGroups() { if (ev) (void)hipEventDestroy(ev); }
+};
+
 // Pattern data of the Schur factor, shared by every frequency of a geometry:
 // S pattern, ordering, elimination tree and L pattern, level schedules, and
 // their device copies.
@@ -36,7 +49,8 @@ struct PreconSymbolic {
   // length (update: entries outside the block; gemv: entries of the inverse)
   std::vector<int> f_lpr_u, f_lpr_g, b_lpr_u, b_lpr_g;
   std::vector<int64_t> f_upd_nnz;     // update entries of each forward level (0: no update launch)
-  DevBuf<char> ws;                    // apply workspace (two [node][column] vectors), shared
+  DevBuf<char> ws;                    // apply workspace (two [group][node][port] vectors), shared
+  PcGroups groups;
   DevBuf<int> d_perm, Rp, Ri, Cp, Ci, ArTp_col;
   DevBuf<int> d_sn_ptr, d_sn_of_row, d_row_split, d_col_split, d_f_rows, d_b_rows;
   DevBuf<int64_t> d_sn_off;
@@ -71,15 +85,15 @@ void precon_prepare(Precon& P, double jwmu_re, double jwmu_im, const std::comple
 // on up to nthreads host threads.
 void precon_prepare_many(Precon* const* P, int n, const std::complex<double>* jwmu, const std::complex<double>* z,
                          int nthreads);
-// y = M^-1 x for nrhs vectors (device pointers), lse_sparse_precon_multiply.m:84-88.
+// y = M^-1 x for the vectors in the given slots (vector s at x + s*ldx,
+// device memory), lse_sparse_precon_multiply.m:84-88.
 template <typename R>
-void precon_apply(Precon& P, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy,
-                  const int* slots = nullptr);
-// Several preconditioners of one geometry (sharing Y) applied in one pass to
-// ncol columns: column c is slot slots[c] and uses tab[grp_of_slot[slot]]
-// (tab[0] when grp_of_slot is null); tab is a device array.
+void precon_apply(Precon& P, const std::vector<int>& slots, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy);
+// Several preconditioners of one geometry (sharing Y) applied in one pass:
+// slot s uses tab[pidx[s]] (tab[0] when pidx is null); tab is a device
+// array, pidx a host array indexed by slot.
 template <typename R>
-void precon_apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, const int* grp_of_slot, int ncol,
-                       const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy, const int* slots);
+void precon_apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, const int* pidx,
+                       const std::vector<int>& slots, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy);
 
 }  // namespace vh
