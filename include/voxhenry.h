@@ -179,10 +179,14 @@ int vh_solve_mixed(vh_geom* g, vh_precon* p, double jwmu_re, double jwmu_im, int
  *           the FP32 inner GMRES (inner_restart, inner_tol).
  * All systems run in lock step and converged ones leave the batch; if the
  * Krylov workspaces exceed hbm_budget bytes (0: 80% of free memory) the batch
- * is processed in chunks.  Outputs per system: flag, relres, iters (outer
- * Arnoldi steps), inner_iters (FP32 steps, method 1). */
+ * is processed in chunks.  tols (nullable, host) gives system s its own
+ * tolerance tols[s] in place of tol: a sweep then follows the frequency rule
+ * of VoxHenry_executer.m:219-223 (1e-12 below 1 MHz, 1e-8 above) in one
+ * batch.  Outputs per system: flag, relres, iters (outer Arnoldi steps),
+ * inner_iters (FP32 steps, method 1). */
 int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64_t nsys, const double* jwmu,
-                   const int* pidx, const double* b, double* x, int on_device, int method, double tol, int restart,
+                   const int* pidx, const double* b, double* x, int on_device, int method, double tol,
+                   const double* tols, int restart,
                    int maxit, int inner_restart, double inner_tol, double hbm_budget, int* flag, double* relres,
                    int* iters, int* inner_iters);
 

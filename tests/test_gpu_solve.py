@@ -190,6 +190,33 @@ def test_batched_sweep_matches_per_frequency(ctx):
     assert eZ < 1e-6 and eR < 1e-6 and eL < 1e-6
 
 
+def test_batched_reference_tolerance(ctx):
+    """Per-system tolerances in one batch: the default sweep gives each system
+    the reference driver's tolerance for its frequency (1e-12 below 1 MHz,
+    1e-8 above); every system meets its own tolerance on the true residual,
+    and the low-frequency L (from the small imaginary part of Z) then agrees
+    with the per-frequency 1e-12 solves to 1e-6 in both precisions."""
+    from voxhenry_amd import structures, solver
+    say('reference tolerance rule')
+    geo = structures.bus(nbars=2, length=24, section=3, pitch=5, width_grid=10)
+    S = solver.Solver(ctx, geo)
+    freqs = np.logspace(5, 8, 6)
+    tols = np.repeat([solver.reference_tol(f) for f in freqs], S.nports)
+    assert np.sum(tols == 1e-12) == 2 * S.nports      # 1e5 and 4e5 Hz
+    ref = S.sweep(freqs, method='fp64', tol=1e-12)
+    for m in ('fp64', 'mixed'):
+        out = S.sweep_batched(freqs, method=m)
+        st = out['stats']
+        assert np.all(st['flag'] == 0)
+        assert np.all(st['relres'] <= tols), (m, st['relres'])
+        ii = np.arange(S.nports)
+        dL = np.max(np.abs(out['L'][:, ii, ii] - ref['L'][:, ii, ii]) / ref['L'][:, ii, ii])
+        dR = np.max(np.abs(out['R'][:, ii, ii] - ref['R'][:, ii, ii]) / ref['R'][:, ii, ii])
+        say('%s: relres max %.2e (low f %.2e), R err %.2e, L err %.2e'
+            % (m, st['relres'].max(), st['relres'][tols == 1e-12].max(), dR, dL))
+        assert dR < 1e-6 and dL < 1e-6
+
+
 def test_superconductor_batched_sweep(ctx):
     """Two-fluid strip: the impedance depends on f, so every system of the
     batched solve uses the impedance of its own frequency (the one its

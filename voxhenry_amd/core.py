@@ -248,7 +248,8 @@ class Geometry:
         the device (then X is a device tensor too); jwmu: (nsys,) complex;
         precons: list of Precon; pidx: (nsys,) index into precons.  method
         'fp64' (pgmres semantics, default restart 100 / maxit 10) or 'mixed'
-        (default outer restart 8 / maxit 4)."""
+        (default outer restart 8 / maxit 4).  tol: one tolerance, or one per
+        system (e.g. reference_tol of each system's frequency)."""
         m = 0 if method == 'fp64' else 1
         if restart is None:
             restart = 100 if m == 0 else 8
@@ -267,13 +268,19 @@ class Geometry:
         jw = np.ascontiguousarray(np.asarray(jwmu, dtype=np.complex128).ravel())
         pi = np.ascontiguousarray(np.asarray(pidx, dtype=np.int32).ravel())
         hs = (C.c_void_p * len(precons))(*[p.h.value for p in precons])
+        tols = None
+        if np.ndim(tol) > 0:
+            tols = np.ascontiguousarray(np.asarray(tol, dtype=np.float64).ravel())
+            assert tols.size == nsys, 'one tolerance per system'
+            tol = float(tols.max())
         X = _torch().zeros_like(BB) if on_dev else np.zeros_like(BB)
         flag = np.zeros(nsys, np.int32)
         relres = np.zeros(nsys)
         its = np.zeros(nsys, np.int32)
         iits = np.zeros(nsys, np.int32)
         check(lib().vh_solve_batch(self.h, len(precons), C.cast(hs, C.c_void_p), nsys, ptr(jw), ptr(pi), ptr(BB),
-                                   ptr(X), int(on_dev), m, tol, restart, maxit, inner_restart, inner_tol,
+                                   ptr(X), int(on_dev), m, float(tol), ptr(tols) if tols is not None else None,
+                                   restart, maxit, inner_restart, inner_tol,
                                    float(hbm_budget),
                                    ptr(flag), ptr(relres), ptr(its), ptr(iits)))
         return X, dict(flag=flag, relres=relres, iters=its, inner_iters=iits)

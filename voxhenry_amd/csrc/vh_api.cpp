@@ -601,7 +601,8 @@ int vh_ldl_solve_host(int64_t n, const int64_t* Ap, const int64_t* Ai, const dou
 }
 
 int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64_t nsys, const double* jwmu,
-                   const int* pidx, const double* b, double* x, int on_device, int method, double tol, int restart,
+                   const int* pidx, const double* b, double* x, int on_device, int method, double tol,
+                   const double* tols, int restart,
                    int maxit, int inner_restart, double inner_tol, double hbm_budget, int* flag, double* relres,
                    int* iters, int* inner_iters) {
   return guard([&] {
@@ -614,6 +615,9 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
                "bad restart / maxit");
     for (int64_t k = 0; k < nprecon; ++k) VH_REQUIRE(precons[k] && precons[k]->P.ready, "preconditioner not prepared");
     for (int64_t s = 0; s < nsys; ++s) VH_REQUIRE(pidx[s] >= 0 && pidx[s] < nprecon, "pidx out of range");
+    VH_REQUIRE(tol > 0, "tol must be positive");
+    if (tols)
+      for (int64_t s = 0; s < nsys; ++s) VH_REQUIRE(tols[s] > 0, "tols must be positive");
     const int64_t n = G.n();
     hipStream_t st = G.ctx->stream;
     Io io;
@@ -688,16 +692,20 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
       std::vector<int> sys(cnt);
       for (int i = 0; i < cnt; ++i) sys[i] = i;
       std::vector<vh::SysStats> cs;
+      std::vector<double> tolc;
+      if (tols) tolc.assign(tols + s0, tols + s0 + cnt);
       const double2* bc = db + s0 * n;
       double2* xc = dx + s0 * n;
       if (method == 0) {
         vh::SolveOpts o;
         o.restart = restart; o.maxit = maxit; o.tol = tol; o.final_residual = true; o.flexible = false;
+        if (tols) o.tol_slot = &tolc;
         vh::gmres_batch<double>(G.ctx, n, A64, &M64, bc, xc, sys, o, W64, cs);
       } else {
         vh::MixedOpts o;
         o.tol = tol; o.outer_restart = restart; o.outer_maxit = maxit;
         o.inner_restart = inner_restart; o.inner_tol = inner_tol;
+        if (tols) o.tol_slot = &tolc;
         vh::fgmres_mixed(G.ctx, n, A64, A32, M32, bc, xc, sys, o, K, cs);
       }
       for (int i = 0; i < cnt; ++i) stats[s0 + i] = cs[i];

@@ -101,6 +101,7 @@ struct MixedOpts {
   int inner_restart = 40, inner_maxit = 1;
   double inner_tol = 1e-5;          // tightest inner tolerance (the first inner solve)
   bool adaptive_inner = true;       // relax the inner tolerance as the outer residual falls
+  const std::vector<double>* tol_slot = nullptr;   // per-slot outer tolerance (overrides tol)
 };
 // Workspaces of the solvers, kept by a geometry across calls.  All large
 // vectors live in one device arena that only grows: releasing and allocating

@@ -283,7 +283,7 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
       const int s = sys[a];
       const double nb = std::sqrt(bn[a].x);
       st[s].relres = nb > 0 ? std::sqrt(nbuf[a].x) / nb : 0.0;
-      st[s].flag = (st[s].relres <= o.tol) ? 0 : 1;
+      st[s].flag = (st[s].relres <= tol_of(s)) ? 0 : 1;
       st[s].matvecs++;
     }
   } else {
@@ -331,7 +331,8 @@ void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<f
     const int na = (int)act.size();
     for (int s : act) {
       const double r = st[s].resvec.empty() ? 1.0 : st[s].resvec.back();
-      itol[s] = o.adaptive_inner && r > 0 ? std::min(0.1, std::max(o.inner_tol, 0.05 * o.tol / r)) : o.inner_tol;
+      const double t = o.tol_slot ? (*o.tol_slot)[s] : o.tol;
+      itol[s] = o.adaptive_inner && r > 0 ? std::min(0.1, std::max(o.inner_tol, 0.05 * t / r)) : o.inner_tol;
     }
     convert_vectors<float, double>(ctx, v, n, v32, n, n, na, slots, slots);
     gmres_batch<float>(ctx, n, A32, &M32, v32, z32, act, io, W32, inner_st);
@@ -342,6 +343,7 @@ void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<f
   oo.restart = o.outer_restart;
   oo.maxit = o.outer_maxit;
   oo.tol = o.tol;
+  oo.tol_slot = o.tol_slot;
   oo.final_residual = true;
   oo.flexible = true;
   gmres_batch<double>(ctx, n, A64, &inner, b, x, sys, oo, W64, st);

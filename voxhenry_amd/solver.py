@@ -108,8 +108,13 @@ class Solver:
         self.times['precon'] = self.times.get('precon', 0.0) + time.perf_counter() - t
         return out
 
-    def sweep_batched(self, freqs, method='mixed', tol=1e-8, precons=None, pidx_of_freq=None, device=False, **kw):
+    def sweep_batched(self, freqs, method='mixed', tol='reference', precons=None, pidx_of_freq=None, device=False,
+                      **kw):
         """Every (port, frequency) system in one batched solve (vh_solve_batch).
+
+        tol 'reference': each system gets the tolerance the reference's driver
+        uses at its frequency (reference_tol, VoxHenry_executer.m:219-223);
+        a number applies to every system.
 
         precons[pidx_of_freq[k]] preconditions frequency k (default: one
         preconditioner per frequency).  Every system uses the impedance its
@@ -128,6 +133,9 @@ class Solver:
         npo = self.nports
         jw = np.repeat(1j * 2 * np.pi * freqs * MU0, npo)
         pidx = np.repeat(np.asarray(pidx_of_freq), npo)
+        if isinstance(tol, str):
+            assert tol == 'reference', tol
+            tol = np.repeat([reference_tol(f) for f in freqs], npo)
         if device:
             # right-hand sides and solutions stay in device memory; only the
             # port rows of the solutions (where the port vectors are nonzero)
