@@ -50,6 +50,24 @@ __host__ __device__ constexpr double ks(int P, int m) {
           : m == 5 ? -0.97492791218182360701813168299393122 : -0.78183148246802980870844452667405775);
 }
 
+// cos and sin of 2 pi m / P for the one-radix lengths P in {2, 3, 4, 5, 7, 8}
+__host__ __device__ constexpr double root_cos(int P, int m) {
+  m %= P;
+  constexpr double h = 0.70710678118654752440084436210484904;
+  return P == 2 ? (m == 0 ? 1.0 : -1.0)
+       : P == 4 ? (m == 0 ? 1.0 : m == 2 ? -1.0 : 0.0)
+       : P == 8 ? (m == 0 ? 1.0 : m == 4 ? -1.0 : (m == 2 || m == 6) ? 0.0 : (m == 1 || m == 7) ? h : -h)
+       : kc(P, m);
+}
+__host__ __device__ constexpr double root_sin(int P, int m) {
+  m %= P;
+  constexpr double h = 0.70710678118654752440084436210484904;
+  return P == 2 ? 0.0
+       : P == 4 ? (m == 1 ? 1.0 : m == 3 ? -1.0 : 0.0)
+       : P == 8 ? (m == 0 || m == 4 ? 0.0 : m == 2 ? 1.0 : m == 6 ? -1.0 : (m == 1 || m == 3) ? h : -h)
+       : ks(P, m);
+}
+
 template <int RAD, int DIR, typename C> struct Dft;
 
 template <int DIR, typename C> struct Dft<2, DIR, C> {

@@ -281,11 +281,83 @@ __global__ __launch_bounds__(NT) void k_pass_z(cplx<R>* __restrict__ B, int ltpp
 }
 
 // ---------------------------------------------------------------------------
+// pass Z for a one-radix Pz (2, 3, 4, 5, 7, 8: thin grids such as 4 voxels in
+// z), one thread per (kx, ky) pencil: the thread loads the N <= (Pz+1)/2
+// voxels of all five components first (every load in flight at once; HBM
+// latency is hidden by memory-level parallelism, not by occupancy), takes the
+// five Pz-point DFTs in registers (the zero padding is compile-time zero),
+// applies the Green product at each kz and stores the first N points of the
+// inverse DFTs.  A workgroup covers NT consecutive kx of one ky row, so each
+// access is an NT-element contiguous run.  Same data flow and results as
+// k_pass_z with a one-stage plan.
+// ---------------------------------------------------------------------------
+template <typename R, int PZ, int NT>
+__global__ __launch_bounds__(NT, 1) void k_pass_z1(cplx<R>* __restrict__ B, GridDims g, TensorRef<R> tens,
+                                                cplx<R> alpha0, const double2* __restrict__ alphas,
+                                                const int* __restrict__ slots) {
+  using C = cplx<R>;
+  constexpr int NM = (PZ + 1) / 2;   // most voxels a circulant embedding of length PZ holds
+  const int Px = g.Px, Py = g.Py, N = g.N;
+  const int tiles_x = (Px + NT - 1) / NT;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = tile % tiles_x, u = tile / tiles_x;
+  const int v = u - 1;
+  const int ky = u == 0 ? 0 : ((v & 1) ? Py - 1 - (v >> 1) : 1 + (v >> 1));
+  const int kx = tx * NT + (int)threadIdx.x;
+  if (kx >= Px) return;   // no barriers below
+  const int ax = kx < g.Hx ? kx : Px - kx;
+  const int ay = ky < g.Hy ? ky : Py - ky;
+  TensorIdx ti;
+  ti.sx = kx < g.Hx ? 1.f : -1.f;
+  ti.sy = ky < g.Hy ? 1.f : -1.f;
+  const int axy = ax + g.Hx * ay;
+  const int HxHy = g.Hx * g.Hy;
+  const int rhs = blockIdx.y;
+  C alpha = alpha0;
+  if (alphas) {
+    const double2 a = alphas[slots ? slots[rhs] : rhs];
+    alpha.x = (R)a.x; alpha.y = (R)a.y;
+  }
+  const int64_t plane = (int64_t)Px * Py;
+  C* base = B + (int64_t)rhs * 5 * N * plane + (int64_t)ky * Px + kx;
+  auto point = [&](int k) {
+    const bool lo = k < g.Hz;
+    TensorIdx tk = ti;
+    tk.sz = lo ? 1.f : -1.f;
+    tk.idx = axy + HxHy * (lo ? k : g.Pz - k);
+    return tk;
+  };
+  C f[5][PZ];
+#pragma unroll
+  for (int c = 0; c < 5; ++c)
+#pragma unroll
+    for (int i = 0; i < PZ; ++i)
+      f[c][i] = (i < NM && i < N) ? base[(int64_t)(c * N + i) * plane] : czero<R>();
+#pragma unroll
+  for (int c = 0; c < 5; ++c) Dft<PZ, -1, C>::run(f[c]);
+#pragma unroll
+  for (int k = 0; k < PZ; ++k) {
+    C p[5] = {f[0][k], f[1][k], f[2][k], f[3][k], f[4][k]};
+    apply_point<R>(p, tens, point(k), alpha);
+#pragma unroll
+    for (int c = 0; c < 5; ++c) f[c][k] = p[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 5; ++c) {
+    Dft<PZ, +1, C>::run(f[c]);
+#pragma unroll
+    for (int o = 0; o < NM; ++o)
+      if (o < N) base[(int64_t)(c * N + o) * plane] = f[c][o];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // pass Z for power-of-two Pz (16..512), compile-time plan: radices, thread
 // mapping and twiddle offsets are constants, the middle stages ping-pong
 // between two LDS buffers (no barrier between a stage's reads and writes),
-// and the global loads of the next component are issued before the current
-// component is transformed.  HALF (N = Pz/2, the usual padding) makes the
+// and global loads run ahead of the transforms (FP32 with N = Pz/2: all five
+// components up front, at a register bound of three waves per SIMD;
+// otherwise one component ahead).  HALF (N = Pz/2, the usual padding) makes the
 // upper half of the first stage's inputs and of the last stage's outputs
 // compile-time zero / unused.  Same data flow and results as k_pass_z.
 // ---------------------------------------------------------------------------
@@ -306,7 +378,7 @@ template <typename R, int DIR, class P, int PZ, int TPP, int T, int S, int LAST>
 template <typename R> constexpr int e_p2() { return sizeof(R) == 8 ? 4 : 8; }
 
 template <typename R, int PZ, int NT, bool HALF>
-__global__ __launch_bounds__(NT) void k_pass_z_p2(cplx<R>* __restrict__ B, GridDims g, const cplx<R>* __restrict__ twf,
+__global__ __launch_bounds__(NT, (sizeof(R) == 4 && HALF) ? 3 : 1) void k_pass_z_p2(cplx<R>* __restrict__ B, GridDims g, const cplx<R>* __restrict__ twf,
                                                   const cplx<R>* __restrict__ twi, TensorRef<R> tens, cplx<R> alpha0,
                                                   const double2* __restrict__ alphas, const int* __restrict__ slots) {
   using C = cplx<R>;
@@ -361,16 +433,26 @@ __global__ __launch_bounds__(NT) void k_pass_z_p2(cplx<R>* __restrict__ B, GridD
       }
   };
   C held[5][BML][RL];
-  C pre[BM0][R0];
-  gload(0, pre);
+  // Forward transforms.  FP32 with N = Pz/2: all five components are loaded
+  // before the first transform, so a wave has its whole input in flight at
+  // once (memory-level parallelism hides the HBM latency) and component c's
+  // input registers die as its spectrum forms (the upper half of each input
+  // is the constant zero padding and takes no registers).  Otherwise (FP64,
+  // twice the bytes, or a full-length input) one component is loaded ahead.
+  constexpr bool ALL = sizeof(R) == 4 && HALF;
+  C in[ALL ? 5 : 2][BM0][R0];
+  if constexpr (ALL) {
+#pragma unroll
+    for (int c = 0; c < 5; ++c) gload(c, in[c]);
+  } else {
+    gload(0, in[0]);
+  }
 #pragma unroll
   for (int c = 0; c < 5; ++c) {
-    C cur[BM0][R0];
-#pragma unroll
-    for (int m = 0; m < BM0; ++m)
-#pragma unroll
-      for (int r = 0; r < R0; ++r) cur[m][r] = pre[m][r];
-    if (c < 4) gload(c + 1, pre);
+    C (&cur)[BM0][R0] = in[ALL ? c : (c & 1)];
+    if constexpr (!ALL) {
+      if (c < 4) gload(c + 1, in[(c + 1) & 1]);
+    }
     {
       auto st = [&](int o, C val) { bufs[0][o * T + t] = val; };
       sstage_regs<R, -1, PZ, R0, 1, TPP>(twf, jl, cur, st);
@@ -378,11 +460,11 @@ __global__ __launch_bounds__(NT) void k_pass_z_p2(cplx<R>* __restrict__ B, GridD
     __syncthreads();
     MidStages<R, -1, PF, PZ, TPP, T, 1, S - 1>::run(twf, jl, bufs, t);
     {
-      const C* in = bufs[(S - 2) & 1];
+      const C* lin = bufs[(S - 2) & 1];
 #pragma unroll
       for (int m = 0; m < BML; ++m)
 #pragma unroll
-        for (int r = 0; r < RL; ++r) held[c][m][r] = in[(jl + m * TPP + r * NBL) * T + t];
+        for (int r = 0; r < RL; ++r) held[c][m][r] = lin[(jl + m * TPP + r * NBL) * T + t];
       // last stage: Ns * RL = Pz, so j < Ns and j mod Ns = j
 #pragma unroll
       for (int m = 0; m < BML; ++m)
@@ -599,6 +681,30 @@ static void launch_z_rl(int rl, dim3 grid, size_t lds, hipStream_t st, cplx<R>* 
   }
 }
 
+// One-radix Pz: the one-thread-per-pencil kernel (returns false when it does
+// not apply).
+template <typename R>
+static bool launch_z1(int nrhs, hipStream_t st, cplx<R>* B, const GridDims& g, TensorRef<R> tr, cplx<R> alpha,
+                      const double2* alphas, const int* slots) {
+  constexpr int NT = 256;
+  auto go = [&](auto pzc) {
+    constexpr int PZ = decltype(pzc)::value;
+    if (g.N > (PZ + 1) / 2) return false;
+    const dim3 grid((unsigned)(ceil_div(g.Px, NT) * g.Py), nrhs);
+    hipLaunchKernelGGL((k_pass_z1<R, PZ, NT>), grid, dim3(NT), 0, st, B, g, tr, alpha, alphas, slots);
+    return true;
+  };
+  switch (g.Pz) {
+    case 2: return go(std::integral_constant<int, 2>());
+    case 3: return go(std::integral_constant<int, 3>());
+    case 4: return go(std::integral_constant<int, 4>());
+    case 5: return go(std::integral_constant<int, 5>());
+    case 7: return go(std::integral_constant<int, 7>());
+    case 8: return go(std::integral_constant<int, 8>());
+    default: return false;
+  }
+}
+
 // Power-of-two Pz in [16, 512] with the precision's default element bound:
 // the compile-time kernel (returns false when it does not apply).
 template <typename R, int NT>
@@ -703,8 +809,8 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
     if constexpr (sizeof(R) == 8) tr.t = G.t64.p; else tr.t = G.t32.p;
     const size_t lds = pzf.nstage > 1 ? (size_t)Tz * g.Pz * elem : 0;   // one stage: registers only
     const int ltz = log2i(tz);
-    bool done = false;
-    if (EZ == e_p2<R>() && G.use_p2z) {
+    bool done = launch_z1<R>(nrhs, st, B, g, tr, alpha, alphas, slots);
+    if (!done && EZ == e_p2<R>() && G.use_p2z) {
       if (NTZ == 512) done = launch_z_p2<R, 512>(g.Pz, nrhs, st, B, g, pzf, pzi, tr, alpha, alphas, slots);
       else done = launch_z_p2<R, 256>(g.Pz, nrhs, st, B, g, pzf, pzi, tr, alpha, alphas, slots);
     }
