@@ -40,34 +40,69 @@ struct PcGroupDev {
   const int* gslot;   // ng * NPG
 };
 
-// w[g][i][j] = b[perm[i]] - sum_k Ar(perm[i],k) * Ainv[k] * a[k] of column (g, j)
+// w[g][iperm[v]][j] = b[v] - sum_k Ar(v,k) * Ainv[k] * a[k] of column (g, j).
+// A workgroup covers NN = 256/NPG consecutive nodes v of group g.  The sums
+// are formed with the node fastest (each system's right-hand side and
+// currents are read in contiguous runs: consecutive nodes touch consecutive
+// currents) and transposed through shared memory, so that the ports of a
+// node write their work-vector line together.
 template <typename R, typename WT, int NPG>
 __global__ __launch_bounds__(256) void k_pc_gather(const cplx<R>* __restrict__ x, int64_t ldx, cplx<WT>* __restrict__ w,
-                                                   int nn, const int* __restrict__ perm, const int* __restrict__ arp,
+                                                   int nn, const int* __restrict__ iperm, const int* __restrict__ arp,
                                                    const int* __restrict__ arc, const R* __restrict__ arv,
                                                    const PcTab<R>* __restrict__ tab, PcGroupDev gr, int64_t nc) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)gr.ng * nn * NPG) return;
-  const int j = (int)(t % NPG);
-  const int64_t gi = t / NPG;
-  const int g = (int)(gi / nn), i = (int)(gi - (int64_t)g * nn);
-  const int slot = gr.gslot[g * NPG + j];
-  cplx<WT> o; o.x = 0; o.y = 0;
-  if (slot >= 0) {
-    const R* ainv = tab[gr.gtab[g]].ainv;
-    const int node = perm[i];
-    const cplx<R>* xr = x + (int64_t)slot * ldx;
-    const cplx<R> b = xr[nc + node];
-    double sx = b.x, sy = b.y;
-    for (int p = arp[node]; p < arp[node + 1]; ++p) {
-      const int k = arc[p];
-      const double cf = (double)arv[p] * (double)ainv[k];
-      const cplx<R> a = xr[k];
-      sx -= cf * a.x; sy -= cf * a.y;
+  constexpr int NN = 256 / NPG;
+  __shared__ cplx<WT> tile[NN * NPG];
+  const int tiles = (nn + NN - 1) / NN;
+  const int g = blockIdx.x / tiles, v0 = (blockIdx.x - g * tiles) * NN;
+  {
+    const int nl = threadIdx.x % NN, j = threadIdx.x / NN;
+    const int node = v0 + nl;
+    const int slot = gr.gslot[g * NPG + j];
+    cplx<WT> o; o.x = 0; o.y = 0;
+    if (node < nn && slot >= 0) {
+      const R* ainv = tab[gr.gtab[g]].ainv;
+      const cplx<R>* xr = x + (int64_t)slot * ldx;
+      const cplx<R> b = xr[nc + node];
+      double sx = b.x, sy = b.y;
+      // a node touches at most 6 currents: the entries of a short row are
+      // loaded all at once (independent loads in flight), a longer row loops
+      const int p0 = arp[node], p1 = arp[node + 1];
+      constexpr int D = 8;
+      if (p1 - p0 <= D) {
+        int kk[D];
+        R vv[D];
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+          kk[u] = p0 + u < p1 ? arc[p0 + u] : 0;
+          vv[u] = p0 + u < p1 ? arv[p0 + u] : R(0);
+        }
+#pragma unroll
+        for (int u = 0; u < D; ++u) {
+          if (p0 + u < p1) {
+            const double cf = (double)vv[u] * (double)ainv[kk[u]];
+            const cplx<R> a = xr[kk[u]];
+            sx -= cf * a.x; sy -= cf * a.y;
+          }
+        }
+      } else {
+        for (int p = p0; p < p1; ++p) {
+          const int k = arc[p];
+          const double cf = (double)arv[p] * (double)ainv[k];
+          const cplx<R> a = xr[k];
+          sx -= cf * a.x; sy -= cf * a.y;
+        }
+      }
+      o.x = (WT)sx; o.y = (WT)sy;
     }
-    o.x = (WT)sx; o.y = (WT)sy;
+    tile[nl * NPG + j] = o;
   }
-  w[t] = o;
+  __syncthreads();
+  {
+    const int j = threadIdx.x % NPG, nl = threadIdx.x / NPG;
+    const int node = v0 + nl;
+    if (node < nn) w[((int64_t)g * nn + iperm[node]) * NPG + j] = tile[nl * NPG + j];
+  }
 }
 
 // Lanes of the level kernels: an item is (row q of the level, group g),
@@ -210,40 +245,169 @@ __global__ __launch_bounds__(256) void k_sn_gemv_bwd(const int* __restrict__ row
   }
 }
 
-// y[nc + perm[i]] = w[g][i][j] of column (g, j)
-template <typename R, typename WT, int NPG>
-__global__ __launch_bounds__(256) void k_pc_scatter_nodes(const cplx<WT>* __restrict__ w, cplx<R>* __restrict__ y,
-                                                          int64_t ldy, int nn, const int* __restrict__ perm,
-                                                          int64_t nc, PcGroupDev gr) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)gr.ng * nn * NPG) return;
-  const int j = (int)(t % NPG);
-  const int64_t gi = t / NPG;
-  const int g = (int)(gi / nn), i = (int)(gi - (int64_t)g * nn);
-  const int slot = gr.gslot[g * NPG + j];
-  if (slot < 0) return;
-  cplx<R> o; o.x = (R)w[t].x; o.y = (R)w[t].y;
-  y[(int64_t)slot * ldy + nc + perm[i]] = o;
+// Block kernel: one workgroup per (block k of a level, group g) does the
+// off-block update of all the block's rows and its dense inverse product:
+//   forward   t_r = b_r - sum_j L(r,j) y_j   then  y_k = Linv_k t_k
+//   backward  u_r = y_r/D_r - sum_i L(i,r) x_i  then  x_k = Linv_k' u_k
+// The work-vector lines of the block's column union (ucol[up[k] ..
+// up[k+1]), sorted) are staged in shared memory once, so each line is read
+// from memory once per (block, group) instead of once per entry; an entry
+// addresses its line through loc[p], the position of its index in the union.
+// t (or u) stays in shared memory for the dense product.  Shared memory:
+// stg_n >= NPG * (largest union) elements for the lines, then NPG * s for t.
+// Lanes: NPG ports x 2^llu parts per row in the update, NPG x 2^llg in the
+// product (each at most 64).  (Serving several groups per workgroup was
+// measured slower: the larger shared-memory footprint cuts the number of
+// workgroups in flight, which is what hides the dependent loads here.)
+__device__ __forceinline__ double2 lane_sum(double2 v, int from, int to) {
+  for (int o = from; o < to; o <<= 1) { v.x += __shfl_xor(v.x, o, 64); v.y += __shfl_xor(v.y, o, 64); }
+  return v;
 }
 
-// y[k] = (a[k] + sum_i ArTp(k,i) w[g][i][j]) * Ainv[k]   (ArTp columns are permuted node ids)
+template <typename R, int NPG, bool FWD>
+__global__ __launch_bounds__(256) void k_blk_solve(const BlkInfo* __restrict__ info, const int* __restrict__ ep,
+                                                   const int* __restrict__ esplit, const int* __restrict__ loc,
+                                                   const int* __restrict__ ucol, const PcTab<R>* __restrict__ tab,
+                                                   PcGroupDev gr, int nn, double2* __restrict__ w,
+                                                   double2* __restrict__ w2, int stg_n, int llu, int llg) {
+  extern __shared__ double2 smem_pc[];
+  double2* stg = smem_pc;
+  double2* tt = smem_pc + stg_n;
+  const int q = blockIdx.x / gr.ng, g = blockIdx.x - q * gr.ng;
+  const BlkInfo bi = info[q];
+  const int c0 = bi.c0, s = bi.s, u0 = bi.u0, nu = bi.nu;
+  const PcTab<R>& T = tab[gr.gtab[g]];
+  const R* __restrict__ val = FWD ? T.Rx : T.Cx;
+  // forward: staged lines are y (w2), the block's own values b come from w;
+  // backward: staged lines are x (w), the own values y from w2
+  const double2* __restrict__ src = FWD ? w2 : w;
+  const double2* __restrict__ own = FWD ? w : w2;
+  double2* __restrict__ dst = FWD ? w2 : w;
+  const int64_t gb = (int64_t)g * nn * NPG;
+  const int tid = threadIdx.x;
+  const int j = tid % NPG;
+  // the block's own values (b, or y / D) go to t while the lines are staged
+  for (int e = tid; e < s * NPG; e += blockDim.x) {
+    const int i = e / NPG;
+    const double2 o = own[gb + (int64_t)(c0 + i) * NPG + (e - i * NPG)];
+    if (FWD) {
+      tt[e] = o;
+    } else {
+      const double inv = 1.0 / (double)T.D[c0 + i];
+      tt[e] = make_double2(o.x * inv, o.y * inv);
+    }
+  }
+  for (int e = tid; e < nu * NPG; e += blockDim.x) {
+    const int cq = e / NPG, jj = e - cq * NPG;
+    stg[e] = src[gb + (int64_t)ucol[u0 + cq] * NPG + jj];
+  }
+  __syncthreads();
+  {
+    const int lu = NPG << llu, lpr = 1 << llu;
+    const int part = (tid / NPG) & (lpr - 1), rs = tid / lu, rpp = blockDim.x / lu;
+    for (int i = rs; i < s; i += rpp) {
+      const int r = c0 + i;
+      const int lo = FWD ? ep[r] : esplit[r];
+      const int hi = FWD ? esplit[r] : ep[r + 1];
+      double2 a = make_double2(0.0, 0.0);
+      for (int p = lo + part; p < hi; p += lpr) {
+        const double l = (double)val[p];
+        const double2 v = stg[loc[p] * NPG + j];
+        a.x += l * v.x; a.y += l * v.y;
+      }
+      a = lane_sum(a, NPG, lu);
+      if (part == 0) {
+        tt[i * NPG + j].x -= a.x;
+        tt[i * NPG + j].y -= a.y;
+      }
+    }
+  }
+  __syncthreads();
+  {
+    const int lg = NPG << llg, lpr = 1 << llg;
+    const int part = (tid / NPG) & (lpr - 1), rs = tid / lg, rpp = blockDim.x / lg;
+    const R* __restrict__ Lb = (FWD ? T.Li : T.LiT) + bi.off;
+    for (int i = rs; i < s; i += rpp) {
+      // forward: row i of Linv holds Linv(i, 0..i) at i(i+1)/2; backward:
+      // row i of the transpose holds Linv(i..s-1, i) at i*s - i(i-1)/2
+      const R* __restrict__ row = FWD ? Lb + (int64_t)i * (i + 1) / 2 : Lb + (int64_t)i * s - (int64_t)i * (i - 1) / 2;
+      const int len = FWD ? i + 1 : s - i;
+      const double2* tv = FWD ? tt + j : tt + i * NPG + j;
+      double2 a = make_double2(0.0, 0.0);
+      for (int jj = part; jj < len; jj += lpr) {
+        const double l = (double)row[jj];
+        const double2 v = tv[jj * NPG];
+        a.x += l * v.x; a.y += l * v.y;
+      }
+      a = lane_sum(a, NPG, lg);
+      if (part == 0) dst[gb + (int64_t)(c0 + i) * NPG + j] = a;
+    }
+  }
+}
+
+// y[nc + v] = w[g][iperm[v]][j] of column (g, j): a workgroup reads the
+// lines of NN = 256/NPG consecutive nodes (ports fastest) and writes each
+// system's NN node entries contiguously (node fastest), through shared memory.
+template <typename R, typename WT, int NPG>
+__global__ __launch_bounds__(256) void k_pc_scatter_nodes(const cplx<WT>* __restrict__ w, cplx<R>* __restrict__ y,
+                                                          int64_t ldy, int nn, const int* __restrict__ iperm,
+                                                          int64_t nc, PcGroupDev gr) {
+  constexpr int NN = 256 / NPG;
+  __shared__ cplx<WT> tile[NN * NPG];
+  const int tiles = (nn + NN - 1) / NN;
+  const int g = blockIdx.x / tiles, v0 = (blockIdx.x - g * tiles) * NN;
+  {
+    const int j = threadIdx.x % NPG, nl = threadIdx.x / NPG;
+    const int node = v0 + nl;
+    cplx<WT> v; v.x = 0; v.y = 0;
+    if (node < nn) v = w[((int64_t)g * nn + iperm[node]) * NPG + j];
+    tile[nl * NPG + j] = v;
+  }
+  __syncthreads();
+  {
+    const int nl = threadIdx.x % NN, j = threadIdx.x / NN;
+    const int node = v0 + nl;
+    const int slot = gr.gslot[g * NPG + j];
+    if (node < nn && slot >= 0) {
+      const cplx<WT> v = tile[nl * NPG + j];
+      cplx<R> o; o.x = (R)v.x; o.y = (R)v.y;
+      y[(int64_t)slot * ldy + nc + node] = o;
+    }
+  }
+}
+
+// y[k] = (a[k] + sum_i ArTp(k,i) w[g][i][j]) * Ainv[k]   (ArTp columns are permuted node ids).
+// Thread -> (group g = blockIdx.y, current k, port j) with the port fastest:
+// a wave covers 64/NPG consecutive currents of NPG systems (one segment per
+// system for a and y) and reads each work-vector line whole.
 template <typename R, typename WT, int NPG>
 __global__ __launch_bounds__(256) void k_pc_currents(const cplx<R>* __restrict__ x, int64_t ldx,
                                                      const cplx<WT>* __restrict__ w, cplx<R>* __restrict__ y,
                                                      int64_t ldy, int nn, int64_t nc, const int* __restrict__ tp,
                                                      const int* __restrict__ tc, const R* __restrict__ tv,
                                                      const PcTab<R>* __restrict__ tab, PcGroupDev gr) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int gj = blockIdx.y, g = gj / NPG, j = gj - g * NPG;
-  const int slot = gr.gslot[gj];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = (int)(t % NPG);
+  const int64_t k = t / NPG;
+  const int g = blockIdx.y;
+  const int slot = gr.gslot[g * NPG + j];
   if (k >= nc || slot < 0) return;
   const cplx<R> a = x[(int64_t)slot * ldx + k];
   double sx = a.x, sy = a.y;
   if (tc) {
+    // a current touches at most 2 nodes: both loaded at once
     const cplx<WT>* wg = w + (int64_t)g * nn * NPG + j;
-    for (int p = tp[k]; p < tp[k + 1]; ++p) {
-      const cplx<WT> v = wg[(int64_t)tc[p] * NPG];
-      sx += (double)tv[p] * v.x; sy += (double)tv[p] * v.y;
+    const int p0 = tp[k], p1 = tp[k + 1];
+    if (p1 - p0 <= 2) {
+      const int n0 = p0 < p1 ? tc[p0] : 0, n1 = p0 + 1 < p1 ? tc[p0 + 1] : 0;
+      const double v0 = p0 < p1 ? (double)tv[p0] : 0.0, v1 = p0 + 1 < p1 ? (double)tv[p0 + 1] : 0.0;
+      if (p0 < p1) { const cplx<WT> v = wg[(int64_t)n0 * NPG]; sx += v0 * v.x; sy += v0 * v.y; }
+      if (p0 + 1 < p1) { const cplx<WT> v = wg[(int64_t)n1 * NPG]; sx += v1 * v.x; sy += v1 * v.y; }
+    } else {
+      for (int p = p0; p < p1; ++p) {
+        const cplx<WT> v = wg[(int64_t)tc[p] * NPG];
+        sx += (double)tv[p] * v.x; sy += (double)tv[p] * v.y;
+      }
     }
   }
   const double ai = tab[gr.gtab[g]].ainv[k];
@@ -311,19 +475,113 @@ static void build_symbolic(Geometry& G, bool geometric, PreconSymbolic& Y) {
   Y.nnzL = (int64_t)Y.F.Lp[nn];
 }
 
-// Rows of the blocks of each level, ordered by level.
-static void level_rows(const std::vector<int>& lev, const std::vector<int>& sn_ptr, std::vector<int>& lptr,
-                       std::vector<int>& rows) {
+// Blocks of each level (bptr/blk, ascending key within a level) and their
+// rows in the same order (lptr/rows), plus rowoff[i] = position in rows of
+// the first row of blk[i] (rowoff has one more entry, = rows.size()).
+static void level_blocks(const std::vector<int>& lev, const std::vector<int>& sn_ptr, const std::vector<int>& key,
+                         LevelBlocks& B) {
   const int nsn = (int)lev.size();
   int nl = 0;
   for (int l : lev) nl = std::max(nl, l + 1);
-  lptr.assign(nl + 1, 0);
-  for (int k = 0; k < nsn; ++k) lptr[lev[k] + 1] += sn_ptr[k + 1] - sn_ptr[k];
-  for (int l = 0; l < nl; ++l) lptr[l + 1] += lptr[l];
-  rows.assign(lptr[nl], 0);
-  std::vector<int> fill(lptr.begin(), lptr.end() - 1);
+  B.bptr.assign(nl + 1, 0);
+  for (int k = 0; k < nsn; ++k) B.bptr[lev[k] + 1]++;
+  for (int l = 0; l < nl; ++l) B.bptr[l + 1] += B.bptr[l];
+  B.blk.assign(nsn, 0);
+  {
+    std::vector<int> fill(B.bptr.begin(), B.bptr.end() - 1);
+    for (int k = 0; k < nsn; ++k) B.blk[fill[lev[k]]++] = k;
+  }
+  for (int l = 0; l < nl; ++l)
+    std::stable_sort(B.blk.begin() + B.bptr[l], B.blk.begin() + B.bptr[l + 1],
+                     [&](int a, int b) { return key[a] < key[b]; });
+  B.lptr.assign(nl + 1, 0);
+  B.rows.clear();
+  B.rowoff.assign(nsn + 1, 0);
+  for (int l = 0; l < nl; ++l) {
+    for (int i = B.bptr[l]; i < B.bptr[l + 1]; ++i) {
+      B.rowoff[i] = (int)B.rows.size();
+      for (int r = sn_ptr[B.blk[i]]; r < sn_ptr[B.blk[i] + 1]; ++r) B.rows.push_back(r);
+    }
+    B.lptr[l + 1] = (int)B.rows.size();
+  }
+  B.rowoff[nsn] = (int)B.rows.size();
+}
+
+// Column union of every block: up/ucol list, for each block k, the sorted
+// distinct indices idx[p] over the entries p in [beg(r), end(r)) of its rows
+// r; loc[p] is the position of idx[p] in that list.
+template <class BEG, class END>
+static void block_unions(const std::vector<int>& sn_ptr, const int* idx, int64_t nent, BEG beg, END end,
+                         std::vector<int>& up, std::vector<int>& ucol, std::vector<int>& loc) {
+  const int nsn = (int)sn_ptr.size() - 1;
+  up.assign(nsn + 1, 0);
+  ucol.clear();
+  loc.assign(nent, 0);
+  std::vector<int> tmp;
+  for (int k = 0; k < nsn; ++k) {
+    tmp.clear();
+    for (int r = sn_ptr[k]; r < sn_ptr[k + 1]; ++r)
+      for (int p = beg(r); p < end(r); ++p) tmp.push_back(idx[p]);
+    std::sort(tmp.begin(), tmp.end());
+    tmp.erase(std::unique(tmp.begin(), tmp.end()), tmp.end());
+    for (int r = sn_ptr[k]; r < sn_ptr[k + 1]; ++r)
+      for (int p = beg(r); p < end(r); ++p)
+        loc[p] = (int)(std::lower_bound(tmp.begin(), tmp.end(), idx[p]) - tmp.begin());
+    ucol.insert(ucol.end(), tmp.begin(), tmp.end());
+    up[k + 1] = (int)ucol.size();
+  }
+  // the block kernel relies on: loc addresses the entry's own index, and
+  // increases along each row (a chunk's entries are one run)
   for (int k = 0; k < nsn; ++k)
-    for (int r = sn_ptr[k]; r < sn_ptr[k + 1]; ++r) rows[fill[lev[k]]++] = r;
+    for (int r = sn_ptr[k]; r < sn_ptr[k + 1]; ++r)
+      for (int p = beg(r); p < end(r); ++p)
+        VH_REQUIRE(loc[p] >= 0 && loc[p] < up[k + 1] - up[k] && ucol[up[k] + loc[p]] == idx[p] &&
+                       (p == beg(r) || loc[p] > loc[p - 1]),
+                   "internal: block union index");
+}
+
+// Per level, along the sorted blocks: the sort key max(size, union) and the
+// prefix maxima of sizes and unions (the block kernel sizes its shared
+// memory from them).
+static void level_extents(const std::vector<int>& sn_ptr, const std::vector<int>& up, LevelBlocks& B) {
+  const int nb = (int)B.blk.size();
+  B.key.assign(nb, 0);
+  B.smax.assign(nb, 0);
+  B.umax.assign(nb, 0);
+  for (int l = 0; l + 1 < (int)B.bptr.size(); ++l) {
+    int ms = 0, mu = 0;
+    for (int i = B.bptr[l]; i < B.bptr[l + 1]; ++i) {
+      const int k = B.blk[i];
+      const int sz = sn_ptr[k + 1] - sn_ptr[k], un = up[k + 1] - up[k];
+      B.key[i] = std::max(sz, un);
+      ms = std::max(ms, sz);
+      mu = std::max(mu, un);
+      B.smax[i] = ms;
+      B.umax[i] = mu;
+    }
+  }
+}
+
+// Per sorted block position: rows, union range and inverse offset (what the
+// block kernel reads first, in one load).
+static std::vector<BlkInfo> block_info(const LevelBlocks& B, const std::vector<int>& sn_ptr,
+                                       const std::vector<int64_t>& sn_off, const std::vector<int>& up) {
+  std::vector<BlkInfo> v(B.blk.size());
+  for (size_t i = 0; i < B.blk.size(); ++i) {
+    const int k = B.blk[i];
+    v[i].c0 = sn_ptr[k];
+    v[i].s = sn_ptr[k + 1] - sn_ptr[k];
+    v[i].u0 = up[k];
+    v[i].nu = up[k + 1] - up[k];
+    v[i].off = sn_off[k];
+  }
+  return v;
+}
+
+static std::vector<int> block_keys(const std::vector<int>& sn_ptr, const std::vector<int>& up) {
+  std::vector<int> key(sn_ptr.size() - 1);
+  for (size_t k = 0; k + 1 < sn_ptr.size(); ++k) key[k] = std::max(sn_ptr[k + 1] - sn_ptr[k], up[k + 1] - up[k]);
+  return key;
 }
 
 // CSR copy of the L pattern, CSC->CSR map, blocks, their levels and the
@@ -386,34 +644,59 @@ static void build_levels(Geometry& G, const LdlFactor& F, PreconSymbolic& Y) {
       for (int q = Y.col_split[r]; q < F.Lp[r + 1]; ++q) l = std::max(l, lb[Y.sn_of_row[F.Li[q]]] + 1);
     lb[k] = l;
   }
-  level_rows(lf, Y.sn_ptr, Y.f_lptr, Y.f_rows);
-  level_rows(lb, Y.sn_ptr, Y.b_lptr, Y.b_rows);
+  // unions: forward over the CSR entries left of each row's block, backward
+  // over the CSC entries below each column's block
+  block_unions(Y.sn_ptr, Ri.data(), Rp[n], [&](int r) { return Rp[r]; }, [&](int r) { return Y.row_split[r]; },
+               Y.f_up, Y.f_ucol, Y.f_loc);
+  block_unions(Y.sn_ptr, F.Li.data(), F.Lp[n], [&](int r) { return Y.col_split[r]; },
+               [&](int r) { return F.Lp[r + 1]; }, Y.b_up, Y.b_ucol, Y.b_loc);
+  level_blocks(lf, Y.sn_ptr, block_keys(Y.sn_ptr, Y.f_up), Y.fl);
+  level_blocks(lb, Y.sn_ptr, block_keys(Y.sn_ptr, Y.b_up), Y.bl);
+  Y.f_lptr = Y.fl.lptr; Y.f_rows = Y.fl.rows;
+  Y.b_lptr = Y.bl.lptr; Y.b_rows = Y.bl.rows;
+  level_extents(Y.sn_ptr, Y.f_up, Y.fl);
+  level_extents(Y.sn_ptr, Y.b_up, Y.bl);
   if (trace_on()) {
     // block statistics: sizes, and the dense panels a block-wise update would use
-    int64_t off_nnz = 0, fpanel = 0, bpanel = 0;
+    int64_t off_nnz = 0, fpanel = 0, bpanel = 0, fu = 0, bu = 0;
     int smax = 0, umax = 0;
+    const int TH[5] = {128, 256, 512, 1024, 2048};
+    int64_t fin[5] = {0, 0, 0, 0, 0}, bin[5] = {0, 0, 0, 0, 0}, bnnz = 0;
     std::vector<int> mark(n, -1);
     for (int k = 0; k < nsn; ++k) {
       const int c0 = Y.sn_ptr[k], c1 = Y.sn_ptr[k + 1];
       smax = std::max(smax, c1 - c0);
       int u = 0;
+      int64_t e = 0;
       for (int r = c0; r < c1; ++r)
         for (int p = Rp[r]; p < Y.row_split[r]; ++p) {
-          ++off_nnz;
+          ++off_nnz; ++e;
           if (mark[Ri[p]] != k) { mark[Ri[p]] = k; ++u; }
         }
       umax = std::max(umax, u);
       fpanel += (int64_t)u * (c1 - c0);
+      fu += u;
+      for (int t = 0; t < 5; ++t) if (u <= TH[t]) fin[t] += e;
     }
     std::fill(mark.begin(), mark.end(), -1);
     for (int k = 0; k < nsn; ++k) {
       const int c0 = Y.sn_ptr[k], c1 = Y.sn_ptr[k + 1];
       int v = 0;
+      int64_t e = 0;
       for (int r = c0; r < c1; ++r)
-        for (int q = Y.col_split[r]; q < F.Lp[r + 1]; ++q)
+        for (int q = Y.col_split[r]; q < F.Lp[r + 1]; ++q) {
+          ++e;
           if (mark[F.Li[q]] != k) { mark[F.Li[q]] = k; ++v; }
+        }
       bpanel += (int64_t)v * (c1 - c0);
+      bu += v; bnnz += e;
+      for (int t = 0; t < 5; ++t) if (v <= TH[t]) bin[t] += e;
     }
+    fprintf(stderr, "[vh] schur unions: fwd sum %ld, bwd sum %ld (bwd nnz %ld); entries in blocks with union <= "
+                    "128/256/512/1024/2048: fwd %.3f %.3f %.3f %.3f %.3f bwd %.3f %.3f %.3f %.3f %.3f\n",
+            (long)fu, (long)bu, (long)bnnz, fin[0] / (double)off_nnz, fin[1] / (double)off_nnz,
+            fin[2] / (double)off_nnz, fin[3] / (double)off_nnz, fin[4] / (double)off_nnz, bin[0] / (double)bnnz,
+            bin[1] / (double)bnnz, bin[2] / (double)bnnz, bin[3] / (double)bnnz, bin[4] / (double)bnnz);
     fprintf(stderr, "[vh] schur: n %d nnzL %ld blocks %d (max %d, mean %.1f) levels f %zu b %zu; "
                     "off-block nnz %ld, dense panels fwd %ld bwd %ld (max union %d), inverses %ld\n",
             n, (long)F.Lp[n], nsn, smax, (double)n / nsn, Y.f_lptr.size() - 1, Y.b_lptr.size() - 1, (long)off_nnz,
@@ -454,6 +737,7 @@ static void build_levels(Geometry& G, const LdlFactor& F, PreconSymbolic& Y) {
     Y.b_lpr_g[l] = lpr(g, cnt);
   }
   upload(Y.d_perm, F.perm);
+  upload(Y.d_iperm, F.iperm);
   upload(Y.Rp, Rp);
   upload(Y.Ri, Ri);
   upload(Y.Cp, F.Lp);
@@ -464,6 +748,12 @@ static void build_levels(Geometry& G, const LdlFactor& F, PreconSymbolic& Y) {
   upload(Y.d_col_split, Y.col_split);
   upload(Y.d_f_rows, Y.f_rows);
   upload(Y.d_b_rows, Y.b_rows);
+  upload(Y.d_f_info, block_info(Y.fl, Y.sn_ptr, Y.sn_off, Y.f_up));
+  upload(Y.d_b_info, block_info(Y.bl, Y.sn_ptr, Y.sn_off, Y.b_up));
+  upload(Y.d_f_ucol, Y.f_ucol);
+  upload(Y.d_f_loc, Y.f_loc);
+  upload(Y.d_b_ucol, Y.b_ucol);
+  upload(Y.d_b_loc, Y.b_loc);
   upload(Y.d_sn_off, Y.sn_off);
   // Ar' with permuted node columns
   std::vector<int> tcp(G.atc_h.size());
@@ -725,6 +1015,11 @@ static PcGroupDev group_table(PreconSymbolic& Y, const std::vector<int>& key, hi
   return gr;
 }
 
+// shared-memory capacity of the block kernel, in complex FP64 elements, for
+// each of its two regions (staged lines, and t of the block's rows)
+constexpr int kBlkCap = 2048;
+static int ilog2_floor(int v) { int l = 0; while ((2 << l) <= v) ++l; return l; }
+
 template <typename R>
 void precon_apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, const int* pidx,
                        const std::vector<int>& slots, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy) {
@@ -755,13 +1050,53 @@ void precon_apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, cons
   if (nn > 0) {
     const int64_t items = (int64_t)nn * cols;
     by_npg([&](auto N) {
-      hipLaunchKernelGGL((k_pc_gather<R, WT, decltype(N)::value>), dim3(ceil_div(items, 256)), dim3(256), 0, st, x,
-                         ldx, w, nn, Y.d_perm.p, G.Ar.ptr.p, G.Ar.col.p, arv, tab, gr, nc);
+      constexpr int n = decltype(N)::value;
+      hipLaunchKernelGGL((k_pc_gather<R, WT, n>), dim3((unsigned)(gr.ng * ceil_div(nn, 256 / n))), dim3(256), 0, st, x,
+                         ldx, w, nn, Y.d_iperm.p, G.Ar.ptr.p, G.Ar.col.p, arv, tab, gr, nc);
     });
     VH_LAUNCHED();
+    // Blocks whose t fits the shared memory (s * npg <= kBlkCap) go through
+    // the block kernel; the rows of larger blocks through the row kernels.
+    // (VH_PC_BLOCK: bit 0 forward, bit 1 backward; default both)
+    static const int blk_on = [] { const char* e = std::getenv("VH_PC_BLOCK"); return e ? std::atoi(e) : 3; }();
+    auto blk_launch = [&](const LevelBlocks& B, int l, bool fwd, int lpr_u, int lpr_g) -> int {
+      const int b0 = B.bptr[l], b1 = B.bptr[l + 1];
+      if (!(blk_on & (fwd ? 1 : 2))) return B.lptr[l];
+      // half the row kernels' lanes per row: the block kernel keeps more rows
+      // in flight per wave (measured best on the bus; 64/128-thread
+      // workgroups and finer tiers were slower)
+      const int llu = std::max(0, std::min(ilog2_floor(lpr_u), ilog2_floor(64 / npg)) - 1);
+      const int llg = std::max(0, std::min(ilog2_floor(lpr_g), ilog2_floor(64 / npg)) - 1);
+      const BlkInfo* info = fwd ? Y.d_f_info.p : Y.d_b_info.p;
+      // eligible: the prefix of the blocks (sorted by max(size, union)) whose
+      // t and union fit the shared memory; launched in tiers of that bound
+      // so small blocks do not reserve the shared memory of large ones
+      int i0 = b0;
+      for (const int cap : {256, 1024, kBlkCap}) {
+        const int i1 = (int)(std::upper_bound(B.key.begin() + i0, B.key.begin() + b1, cap / npg) - B.key.begin());
+        if (i1 > i0) {
+          const int stg_n = B.umax[i1 - 1] * npg;
+          const size_t lds = (size_t)(stg_n + B.smax[i1 - 1] * npg) * sizeof(double2);
+          const dim3 grid((unsigned)((int64_t)(i1 - i0) * gr.ng));
+          by_npg([&](auto N) {
+            constexpr int n = decltype(N)::value;
+            if (fwd)
+              hipLaunchKernelGGL((k_blk_solve<R, n, true>), grid, dim3(256), lds, st, info + i0, Y.Rp.p,
+                                 Y.d_row_split.p, Y.d_f_loc.p, Y.d_f_ucol.p, tab, gr, nn, w, w2, stg_n, llu, llg);
+            else
+              hipLaunchKernelGGL((k_blk_solve<R, n, false>), grid, dim3(256), lds, st, info + i0, Y.Cp.p,
+                                 Y.d_col_split.p, Y.d_b_loc.p, Y.d_b_ucol.p, tab, gr, nn, w, w2, stg_n, llu, llg);
+          });
+          VH_LAUNCHED();
+        }
+        i0 = i1;
+      }
+      return B.rowoff[i0];
+    };
     const int nlf = (int)Y.f_lptr.size() - 1;
     for (int l = 0; l < nlf; ++l) {
-      const int a = Y.f_lptr[l], nr = Y.f_lptr[l + 1] - a;
+      const int a = blk_launch(Y.fl, l, true, Y.f_lpr_u[l], Y.f_lpr_g[l]), nr = Y.f_lptr[l + 1] - a;
+      if (nr == 0) continue;
       const int64_t it = (int64_t)nr * gr.ng;
       if (Y.f_upd_nnz[l] > 0) {
         with_lanes(npg, Y.f_lpr_u[l], [&](auto N, auto L) {
@@ -780,7 +1115,8 @@ void precon_apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, cons
     }
     const int nlb = (int)Y.b_lptr.size() - 1;
     for (int l = 0; l < nlb; ++l) {
-      const int a = Y.b_lptr[l], nr = Y.b_lptr[l + 1] - a;
+      const int a = blk_launch(Y.bl, l, false, Y.b_lpr_u[l], Y.b_lpr_g[l]), nr = Y.b_lptr[l + 1] - a;
+      if (nr == 0) continue;
       const int64_t it = (int64_t)nr * gr.ng;
       with_lanes(npg, Y.b_lpr_u[l], [&](auto N, auto L) {
         constexpr int n = decltype(N)::value, lp = decltype(L)::value;
@@ -796,15 +1132,17 @@ void precon_apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, cons
       VH_LAUNCHED();
     }
     by_npg([&](auto N) {
-      hipLaunchKernelGGL((k_pc_scatter_nodes<R, WT, decltype(N)::value>), dim3(ceil_div(items, 256)), dim3(256), 0, st,
-                         w, y, ldy, nn, Y.d_perm.p, nc, gr);
+      constexpr int n = decltype(N)::value;
+      hipLaunchKernelGGL((k_pc_scatter_nodes<R, WT, n>), dim3((unsigned)(gr.ng * ceil_div(nn, 256 / n))), dim3(256), 0, st,
+                         w, y, ldy, nn, Y.d_iperm.p, nc, gr);
     });
     VH_LAUNCHED();
   }
   const R* atv = d ? (const R*)G.ArT.v64.p : (const R*)G.ArT.v32.p;
   VH_REQUIRE(cols < 65536, "preconditioner: too many columns");
   by_npg([&](auto N) {
-    hipLaunchKernelGGL((k_pc_currents<R, WT, decltype(N)::value>), dim3(ceil_div(nc, 256), (unsigned)cols), dim3(256), 0,
+    constexpr int n = decltype(N)::value;
+    hipLaunchKernelGGL((k_pc_currents<R, WT, n>), dim3(ceil_div(nc * n, 256), (unsigned)gr.ng), dim3(256), 0,
                        st, x, ldx, w, y, ldy, nn, nc, G.ArT.ptr.p, nn > 0 ? Y.ArTp_col.p : nullptr, atv, tab, gr);
   });
   VH_LAUNCHED();

@@ -30,6 +30,22 @@ struct PcGroups {
   ~PcGroups() { if (ev) (void)hipEventDestroy(ev); }
 };
 
+// Blocks of the levels of one solve direction: blk[bptr[l] .. bptr[l+1]) are
+// the blocks of level l in ascending key = max(size, column union),
+// rows[lptr[l] .. lptr[l+1]) their rows in the same order (rowoff[i]: first
+// row of blk[i] in rows); smax[i] / umax[i] are the largest size / union
+// among the blocks of its level up to and including blk[i].
+// What the block kernel needs of one block: rows [c0, c0+s), column union
+// ucol[u0 .. u0+nu), packed inverse at offset off.
+struct BlkInfo {
+  int c0, s, u0, nu;
+  int64_t off;
+};
+
+struct LevelBlocks {
+  std::vector<int> bptr, blk, lptr, rows, rowoff, key, smax, umax;
+};
+
 // Pattern data of the Schur factor, shared by every frequency of a geometry:
 // S pattern, ordering, elimination tree and L pattern, level schedules, and
 // their device copies.
@@ -45,14 +61,21 @@ struct PreconSymbolic {
   std::vector<int> row_split;         // CSR: first entry of row r in its own block's columns
   std::vector<int> col_split;         // CSC: first entry of column r below its own block
   std::vector<int> f_lptr, f_rows, b_lptr, b_rows;   // rows of the blocks of each level
+  LevelBlocks fl, bl;                 // forward / backward levels by block
+  // column unions per block (forward: the columns left of the block its
+  // rows reach; backward: the rows below it its columns reach) and the
+  // position of every entry's index in its block's union
+  std::vector<int> f_up, f_ucol, f_loc, b_up, b_ucol, b_loc;
   // lanes per (row, column) item of each level's kernels, from the mean row
   // length (update: entries outside the block; gemv: entries of the inverse)
   std::vector<int> f_lpr_u, f_lpr_g, b_lpr_u, b_lpr_g;
   std::vector<int64_t> f_upd_nnz;     // update entries of each forward level (0: no update launch)
   DevBuf<char> ws;                    // apply workspace (two [group][node][port] vectors), shared
   PcGroups groups;
-  DevBuf<int> d_perm, Rp, Ri, Cp, Ci, ArTp_col;
+  DevBuf<int> d_perm, d_iperm, Rp, Ri, Cp, Ci, ArTp_col;
   DevBuf<int> d_sn_ptr, d_sn_of_row, d_row_split, d_col_split, d_f_rows, d_b_rows;
+  DevBuf<BlkInfo> d_f_info, d_b_info;  // per sorted block position (LevelBlocks order)
+  DevBuf<int> d_f_ucol, d_f_loc, d_b_ucol, d_b_loc;
   DevBuf<int64_t> d_sn_off;
 };
 
