@@ -128,7 +128,7 @@ struct TensorIdx {
 
 // 5x5 block product of lse_matvect_mult.m:57-86 with complex tensor entries.
 template <typename C>
-__device__ __forceinline__ void green_product(C* f, const C* G, C alpha) {
+__device__ __forceinline__ void green_product(C* f, const C* G) {
   const C a34 = cadd(f[3], f[4]), d34 = csub(f[3], f[4]);
   C o0 = cadd(cmul(G[0], f[0]), cmul(G[1], a34));
   C o1 = cadd(cmul(G[0], f[1]), cmul(G[2], d34));
@@ -136,14 +136,13 @@ __device__ __forceinline__ void green_product(C* f, const C* G, C alpha) {
   C o3 = csub(cadd(cmul(G[3], f[3]), cmul(G[5], f[4])), cadd(cmul(G[1], f[0]), cmul(G[2], f[1])));
   C o4 = cadd(csub(cmul(G[2], f[1]), cadd(cmul(G[1], f[0]), cmul(G[4], f[2]))),
               cadd(cmul(G[5], f[3]), cmul(G[6], f[4])));
-  f[0] = cmul(alpha, o0); f[1] = cmul(alpha, o1); f[2] = cmul(alpha, o2);
-  f[3] = cmul(alpha, o3); f[4] = cmul(alpha, o4);
+  f[0] = o0; f[1] = o1; f[2] = o2; f[3] = o3; f[4] = o4;
 }
 
 // Same product with the compressed real tensor: even components g0,g3,g5,g6 are
 // real, odd components G1 = i g1, G2 = i g2, G4 = i g4.
 template <typename C, typename R>
-__device__ __forceinline__ void green_product_real(C* f, const R* g, C alpha) {
+__device__ __forceinline__ void green_product_real(C* f, const R* g) {
   auto im = [](R s, C a) { C r; r.x = -s * a.y; r.y = s * a.x; return r; };   // i*s*a
   const C a34 = cadd(f[3], f[4]), d34 = csub(f[3], f[4]);
   C o0 = cadd(cscale(f[0], g[0]), im(g[1], a34));
@@ -152,8 +151,7 @@ __device__ __forceinline__ void green_product_real(C* f, const R* g, C alpha) {
   C o3 = csub(cadd(cscale(f[3], g[3]), cscale(f[4], g[5])), cadd(im(g[1], f[0]), im(g[2], f[1])));
   C o4 = cadd(csub(im(g[2], f[1]), cadd(im(g[1], f[0]), im(g[4], f[2]))),
               cadd(cscale(f[3], g[5]), cscale(f[4], g[6])));
-  f[0] = cmul(alpha, o0); f[1] = cmul(alpha, o1); f[2] = cmul(alpha, o2);
-  f[3] = cmul(alpha, o3); f[4] = cmul(alpha, o4);
+  f[0] = o0; f[1] = o1; f[2] = o2; f[3] = o3; f[4] = o4;
 }
 
 template <typename R> struct TensorRef;
@@ -161,10 +159,9 @@ template <> struct TensorRef<double> { const double2* t; };
 template <> struct TensorRef<float> { const float* t; };
 
 template <typename R>
-__device__ __forceinline__ void apply_point(cplx<R>* f, const TensorRef<R>& T, const TensorIdx& ti, cplx<R> alpha);
+__device__ __forceinline__ void apply_point(cplx<R>* f, const TensorRef<R>& T, const TensorIdx& ti);
 template <>
-__device__ __forceinline__ void apply_point<double>(double2* f, const TensorRef<double>& T, const TensorIdx& ti,
-                                                    double2 alpha) {
+__device__ __forceinline__ void apply_point<double>(double2* f, const TensorRef<double>& T, const TensorIdx& ti) {
   const double2* p = T.t + (size_t)ti.idx * TENSOR_STRIDE;
   double2 G[7];
 #pragma unroll
@@ -172,15 +169,14 @@ __device__ __forceinline__ void apply_point<double>(double2* f, const TensorRef<
   G[1] = cscale(G[1], (double)ti.sx);
   G[2] = cscale(G[2], (double)ti.sy);
   G[4] = cscale(G[4], (double)ti.sz);
-  green_product(f, G, alpha);
+  green_product(f, G);
 }
 template <>
-__device__ __forceinline__ void apply_point<float>(float2* f, const TensorRef<float>& T, const TensorIdx& ti,
-                                                   float2 alpha) {
+__device__ __forceinline__ void apply_point<float>(float2* f, const TensorRef<float>& T, const TensorIdx& ti) {
   const float4* p = (const float4*)(T.t + (size_t)ti.idx * TENSOR_STRIDE);
   const float4 a = p[0], b = p[1];
   float g[7] = {a.x, a.y * ti.sx, a.z * ti.sy, a.w, b.x * ti.sz, b.y, b.z};
-  green_product_real(f, g, alpha);
+  green_product_real(f, g);
 }
 
 template <typename C, int BL, int RL>
@@ -193,8 +189,7 @@ __device__ __forceinline__ void copy_regs(C (&d)[BL][RL], const C (&s)[BL][RL]) 
 
 template <typename R, int E, int NT, int RL>
 __global__ __launch_bounds__(NT) void k_pass_z(cplx<R>* __restrict__ B, int ltpp, GridDims g, Plan2 pf,
-                                               Plan2 pi, TensorRef<R> tens, cplx<R> alpha0,
-                                               const double2* __restrict__ alphas, const int* __restrict__ slots) {
+                                               Plan2 pi, TensorRef<R> tens) {
   using C = cplx<R>;
   constexpr int BL = (E + RL - 1) / RL;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -223,11 +218,6 @@ __global__ __launch_bounds__(NT) void k_pass_z(cplx<R>* __restrict__ B, int ltpp
   auto lx = [T](int i) { return i * T; };
   {
     const int rhs = blockIdx.y;
-    C alpha = alpha0;
-    if (alphas) {
-      const double2 a = alphas[slots ? slots[rhs] : rhs];
-      alpha.x = (R)a.x; alpha.y = (R)a.y;
-    }
     C* base = B + (int64_t)rhs * 5 * N * plane + (int64_t)ky * Px + kx;
     C h0[BL][RL], h1[BL][RL], h2[BL][RL], h3[BL][RL], h4[BL][RL];
 #pragma unroll 1
@@ -257,7 +247,7 @@ __global__ __launch_bounds__(NT) void k_pass_z(cplx<R>* __restrict__ B, int ltpp
           tk.sz = lo ? 1.f : -1.f;
           tk.idx = axy + HxHy * (lo ? kz : Pz - kz);
           C f[5] = {h0[m][r], h1[m][r], h2[m][r], h3[m][r], h4[m][r]};
-          apply_point<R>(f, tens, tk, alpha);
+          apply_point<R>(f, tens, tk);
           h0[m][r] = f[0]; h1[m][r] = f[1]; h2[m][r] = f[2]; h3[m][r] = f[3]; h4[m][r] = f[4];
         }
       }
@@ -292,9 +282,7 @@ __global__ __launch_bounds__(NT) void k_pass_z(cplx<R>* __restrict__ B, int ltpp
 // k_pass_z with a one-stage plan.
 // ---------------------------------------------------------------------------
 template <typename R, int PZ, int NT>
-__global__ __launch_bounds__(NT, 1) void k_pass_z1(cplx<R>* __restrict__ B, GridDims g, TensorRef<R> tens,
-                                                cplx<R> alpha0, const double2* __restrict__ alphas,
-                                                const int* __restrict__ slots) {
+__global__ __launch_bounds__(NT, 1) void k_pass_z1(cplx<R>* __restrict__ B, GridDims g, TensorRef<R> tens) {
   using C = cplx<R>;
   constexpr int NM = (PZ + 1) / 2;   // most voxels a circulant embedding of length PZ holds
   const int Px = g.Px, Py = g.Py, N = g.N;
@@ -313,11 +301,6 @@ __global__ __launch_bounds__(NT, 1) void k_pass_z1(cplx<R>* __restrict__ B, Grid
   const int axy = ax + g.Hx * ay;
   const int HxHy = g.Hx * g.Hy;
   const int rhs = blockIdx.y;
-  C alpha = alpha0;
-  if (alphas) {
-    const double2 a = alphas[slots ? slots[rhs] : rhs];
-    alpha.x = (R)a.x; alpha.y = (R)a.y;
-  }
   const int64_t plane = (int64_t)Px * Py;
   C* base = B + (int64_t)rhs * 5 * N * plane + (int64_t)ky * Px + kx;
   auto point = [&](int k) {
@@ -338,7 +321,7 @@ __global__ __launch_bounds__(NT, 1) void k_pass_z1(cplx<R>* __restrict__ B, Grid
 #pragma unroll
   for (int k = 0; k < PZ; ++k) {
     C p[5] = {f[0][k], f[1][k], f[2][k], f[3][k], f[4][k]};
-    apply_point<R>(p, tens, point(k), alpha);
+    apply_point<R>(p, tens, point(k));
 #pragma unroll
     for (int c = 0; c < 5; ++c) f[c][k] = p[c];
   }
@@ -379,8 +362,7 @@ template <typename R> constexpr int e_p2() { return sizeof(R) == 8 ? 4 : 8; }
 
 template <typename R, int PZ, int NT, bool HALF>
 __global__ __launch_bounds__(NT, (sizeof(R) == 4 && HALF) ? 3 : 1) void k_pass_z_p2(cplx<R>* __restrict__ B, GridDims g, const cplx<R>* __restrict__ twf,
-                                                  const cplx<R>* __restrict__ twi, TensorRef<R> tens, cplx<R> alpha0,
-                                                  const double2* __restrict__ alphas, const int* __restrict__ slots) {
+                                                  const cplx<R>* __restrict__ twi, TensorRef<R> tens) {
   using C = cplx<R>;
   constexpr int E = e_p2<R>();
   constexpr int TPP = PZ / E, T = NT / TPP;
@@ -414,11 +396,6 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && HALF) ? 3 : 1) void k_pass_z
   const int axy = valid ? ax + g.Hx * ay : 0;
   const int HxHy = g.Hx * g.Hy;
   const int rhs = blockIdx.y;
-  C alpha = alpha0;
-  if (alphas) {
-    const double2 a = alphas[slots ? slots[rhs] : rhs];
-    alpha.x = (R)a.x; alpha.y = (R)a.y;
-  }
   const int64_t plane = (int64_t)Px * Py;
   C* base = B + (int64_t)rhs * 5 * N * plane + (int64_t)ky * Px + kx;
   auto gload = [&](int c, C (&p)[BM0][R0]) {
@@ -483,7 +460,7 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && HALF) ? 3 : 1) void k_pass_z
         tk.sz = lo ? 1.f : -1.f;
         tk.idx = axy + HxHy * (lo ? kz : g.Pz - kz);
         C f[5] = {held[0][m][r], held[1][m][r], held[2][m][r], held[3][m][r], held[4][m][r]};
-        apply_point<R>(f, tens, tk, alpha);
+        apply_point<R>(f, tens, tk);
 #pragma unroll
         for (int c = 0; c < 5; ++c) held[c][m][r] = f[c];
       }
@@ -531,8 +508,11 @@ __global__ __launch_bounds__(NT) void k_pass_x_inv(
     const int* __restrict__ vmap, const int* __restrict__ lines, int nlines, int ltpp, GridDims g, Plan2 pl,
     const cplx<R>* __restrict__ z, R inv_dx, const int* __restrict__ atp, const int* __restrict__ atc,
     const R* __restrict__ atv, int64_t num_curr, int mode, const int* __restrict__ slots, int dense,
-    const cplx<R>* const* __restrict__ zslot) {
-  // mode 0: conv + diag - Ar'phi ; mode 1: conv + diag ; mode 2: conv only
+    const cplx<R>* const* __restrict__ zslot, cplx<R> alpha0, const double2* __restrict__ alphas) {
+  // mode 0: conv + diag - Ar'phi ; mode 1: conv + diag ; mode 2: conv only.
+  // conv = j*omega*mu (alpha, per system or one for all) times the cropped
+  // inverse transform: the scale is applied here, once per output element,
+  // instead of at every spectral point of pass Z.
   using C = cplx<R>;
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x;
@@ -554,6 +534,11 @@ __global__ __launch_bounds__(NT) void k_pass_x_inv(
   const C* xs = x + (int64_t)slot * ldx;
   C* ys = y + (int64_t)slot * ldy;
   const C* __restrict__ zv = zslot ? zslot[slot] : z;
+  C alpha = alpha0;
+  if (alphas) {
+    const double2 a = alphas[slot];
+    alpha.x = (R)a.x; alpha.y = (R)a.y;
+  }
   const int* vm = vmap + (int64_t)L * line;
   const int64_t lbase = (int64_t)L * line;
   auto gst = [&](int o, C acc) {
@@ -561,6 +546,7 @@ __global__ __launch_bounds__(NT) void k_pass_x_inv(
     const int vid = dense ? (int)(lbase + o) : vm[o];
     if (vid < 0) return;
     const int64_t row = (int64_t)comp * g.nv + vid;
+    acc = cmul(alpha, acc);
     if (mode <= 1) acc = cadd(acc, cscale(cmul(zv[vid], xs[row]), w));
     if (mode == 0) {
       C s = czero<R>();
@@ -651,29 +637,25 @@ template <typename R> static ZShape zshape(int i) { return sizeof(R) == 8 ? kZSh
 // reserved for butterflies that do not exist.
 template <typename R, int NT, int E, int RL>
 static void launch_z_k(dim3 grid, size_t lds, hipStream_t st, cplx<R>* B, int ltpp, const GridDims& g,
-                       const Plan2& pf, const Plan2& pi, TensorRef<R> tr, cplx<R> alpha, const double2* alphas,
-                       const int* slots) {
+                       const Plan2& pf, const Plan2& pi, TensorRef<R> tr) {
   if (pf.nstage == 1 && ltpp == 0)
-    hipLaunchKernelGGL((k_pass_z<R, RL, NT, RL>), grid, dim3(NT), lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas,
-                       slots);
+    hipLaunchKernelGGL((k_pass_z<R, RL, NT, RL>), grid, dim3(NT), lds, st, B, ltpp, g, pf, pi, tr);
   else
-    hipLaunchKernelGGL((k_pass_z<R, E, NT, RL>), grid, dim3(NT), lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas,
-                       slots);
+    hipLaunchKernelGGL((k_pass_z<R, E, NT, RL>), grid, dim3(NT), lds, st, B, ltpp, g, pf, pi, tr);
 }
 
 template <typename R, int NT, int E>
 static void launch_z_rl(int rl, dim3 grid, size_t lds, hipStream_t st, cplx<R>* B, int ltpp, const GridDims& g,
-                        const Plan2& pf, const Plan2& pi, TensorRef<R> tr, cplx<R> alpha, const double2* alphas,
-                        const int* slots) {
+                        const Plan2& pf, const Plan2& pi, TensorRef<R> tr) {
   switch (rl) {
-    case 2: launch_z_k<R, NT, E, 2>(grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
-    case 3: launch_z_k<R, NT, E, 3>(grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
-    case 4: launch_z_k<R, NT, E, 4>(grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
-    case 5: launch_z_k<R, NT, E, 5>(grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
-    case 7: launch_z_k<R, NT, E, 7>(grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots); break;
+    case 2: launch_z_k<R, NT, E, 2>(grid, lds, st, B, ltpp, g, pf, pi, tr); break;
+    case 3: launch_z_k<R, NT, E, 3>(grid, lds, st, B, ltpp, g, pf, pi, tr); break;
+    case 4: launch_z_k<R, NT, E, 4>(grid, lds, st, B, ltpp, g, pf, pi, tr); break;
+    case 5: launch_z_k<R, NT, E, 5>(grid, lds, st, B, ltpp, g, pf, pi, tr); break;
+    case 7: launch_z_k<R, NT, E, 7>(grid, lds, st, B, ltpp, g, pf, pi, tr); break;
     case 8:
       if constexpr (rmax_for(E) >= 8) {
-        launch_z_k<R, NT, E, 8>(grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots);
+        launch_z_k<R, NT, E, 8>(grid, lds, st, B, ltpp, g, pf, pi, tr);
         break;
       }
       [[fallthrough]];
@@ -684,14 +666,13 @@ static void launch_z_rl(int rl, dim3 grid, size_t lds, hipStream_t st, cplx<R>* 
 // One-radix Pz: the one-thread-per-pencil kernel (returns false when it does
 // not apply).
 template <typename R>
-static bool launch_z1(int nrhs, hipStream_t st, cplx<R>* B, const GridDims& g, TensorRef<R> tr, cplx<R> alpha,
-                      const double2* alphas, const int* slots) {
+static bool launch_z1(int nrhs, hipStream_t st, cplx<R>* B, const GridDims& g, TensorRef<R> tr) {
   constexpr int NT = 256;
   auto go = [&](auto pzc) {
     constexpr int PZ = decltype(pzc)::value;
     if (g.N > (PZ + 1) / 2) return false;
     const dim3 grid((unsigned)(ceil_div(g.Px, NT) * g.Py), nrhs);
-    hipLaunchKernelGGL((k_pass_z1<R, PZ, NT>), grid, dim3(NT), 0, st, B, g, tr, alpha, alphas, slots);
+    hipLaunchKernelGGL((k_pass_z1<R, PZ, NT>), grid, dim3(NT), 0, st, B, g, tr);
     return true;
   };
   switch (g.Pz) {
@@ -709,7 +690,7 @@ static bool launch_z1(int nrhs, hipStream_t st, cplx<R>* B, const GridDims& g, T
 // the compile-time kernel (returns false when it does not apply).
 template <typename R, int NT>
 static bool launch_z_p2(int pz, int nrhs, hipStream_t st, cplx<R>* B, const GridDims& g, const Plan2& pf,
-                        const Plan2& pi, TensorRef<R> tr, cplx<R> alpha, const double2* alphas, const int* slots) {
+                        const Plan2& pi, TensorRef<R> tr) {
   auto go = [&](auto pzc) {
     constexpr int PZ = decltype(pzc)::value;
     constexpr int TPP = PZ / e_p2<R>(), T = NT / TPP;
@@ -717,10 +698,10 @@ static bool launch_z_p2(int pz, int nrhs, hipStream_t st, cplx<R>* B, const Grid
     const size_t lds = 2 * (size_t)T * PZ * sizeof(cplx<R>);
     if (2 * g.N == PZ)
       hipLaunchKernelGGL((k_pass_z_p2<R, PZ, NT, true>), grid, dim3(NT), lds, st, B, g, (const cplx<R>*)pf.tw,
-                         (const cplx<R>*)pi.tw, tr, alpha, alphas, slots);
+                         (const cplx<R>*)pi.tw, tr);
     else
       hipLaunchKernelGGL((k_pass_z_p2<R, PZ, NT, false>), grid, dim3(NT), lds, st, B, g, (const cplx<R>*)pf.tw,
-                         (const cplx<R>*)pi.tw, tr, alpha, alphas, slots);
+                         (const cplx<R>*)pi.tw, tr);
     return true;
   };
   switch (pz) {
@@ -736,17 +717,16 @@ static bool launch_z_p2(int pz, int nrhs, hipStream_t st, cplx<R>* B, const Grid
 
 template <typename R>
 static void launch_z(int shape, int rl, dim3 grid, size_t lds, hipStream_t st, cplx<R>* B, int ltpp,
-                     const GridDims& g, const Plan2& pf, const Plan2& pi, TensorRef<R> tr, cplx<R> alpha,
-                     const double2* alphas, const int* slots) {
+                     const GridDims& g, const Plan2& pf, const Plan2& pi, TensorRef<R> tr) {
   const ZShape z = zshape<R>(shape);
   if constexpr (sizeof(R) == 8) {
-    if (z.nt == 512) launch_z_rl<R, 512, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots);
-    else launch_z_rl<R, 256, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots);
+    if (z.nt == 512) launch_z_rl<R, 512, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr);
+    else launch_z_rl<R, 256, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr);
   } else {
-    if (z.nt == 512 && z.e == 8) launch_z_rl<R, 512, 8>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots);
-    else if (z.e == 8) launch_z_rl<R, 256, 8>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots);
-    else if (z.nt == 512) launch_z_rl<R, 512, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots);
-    else launch_z_rl<R, 256, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr, alpha, alphas, slots);
+    if (z.nt == 512 && z.e == 8) launch_z_rl<R, 512, 8>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr);
+    else if (z.e == 8) launch_z_rl<R, 256, 8>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr);
+    else if (z.nt == 512) launch_z_rl<R, 512, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr);
+    else launch_z_rl<R, 256, 4>(rl, grid, lds, st, B, ltpp, g, pf, pi, tr);
   }
 }
 
@@ -802,20 +782,17 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
   // Z
   {
     dim3 grid(ceil_div(g.Px, Tz) * g.Py, nrhs);
-    cplx<R> alpha;
-    alpha.x = (R)are;
-    alpha.y = (R)aim;
     TensorRef<R> tr;
     if constexpr (sizeof(R) == 8) tr.t = G.t64.p; else tr.t = G.t32.p;
     const size_t lds = pzf.nstage > 1 ? (size_t)Tz * g.Pz * elem : 0;   // one stage: registers only
     const int ltz = log2i(tz);
-    bool done = launch_z1<R>(nrhs, st, B, g, tr, alpha, alphas, slots);
+    bool done = launch_z1<R>(nrhs, st, B, g, tr);
     if (!done && EZ == e_p2<R>() && G.use_p2z) {
-      if (NTZ == 512) done = launch_z_p2<R, 512>(g.Pz, nrhs, st, B, g, pzf, pzi, tr, alpha, alphas, slots);
-      else done = launch_z_p2<R, 256>(g.Pz, nrhs, st, B, g, pzf, pzi, tr, alpha, alphas, slots);
+      if (NTZ == 512) done = launch_z_p2<R, 512>(g.Pz, nrhs, st, B, g, pzf, pzi, tr);
+      else done = launch_z_p2<R, 256>(g.Pz, nrhs, st, B, g, pzf, pzi, tr);
     }
     if (!done)
-      launch_z<R>(zsh, G.fz.last_radix(rmax_for(EZ)), grid, lds, st, B, ltz, g, pzf, pzi, tr, alpha, alphas, slots);
+      launch_z<R>(zsh, G.fz.last_radix(rmax_for(EZ)), grid, lds, st, B, ltz, g, pzf, pzi, tr);
     VH_LAUNCHED();
   }
   mark(3);
@@ -836,9 +813,12 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
     if constexpr (sizeof(R) == 8) zz = G.z64.p; else zz = G.z32.p;
     if (mode <= 1) VH_REQUIRE(G.has_z || zslot, "matvec: impedance not set");
     if (mode == 0) VH_REQUIRE(G.has_inc, "matvec: incidence not set");
+    cplx<R> alpha;
+    alpha.x = (R)are;
+    alpha.y = (R)aim;
     hipLaunchKernelGGL((k_pass_x_inv<R, EXY, NT_XY>), grid, dim3(NT_XY), lds_x, st, B, x, ldx, y, ldy, G.vmap.p,
                        G.lines.p, G.nlines, log2i(tx), g, px, zz, (R)(1.0 / G.dx), G.ArT.ptr.p, G.ArT.col.p, atv,
-                       G.num_curr, mode, slots, dense, zslot);
+                       G.num_curr, mode, slots, dense, zslot, alpha, alphas);
     VH_LAUNCHED();
   }
   mark(5);
