@@ -131,6 +131,24 @@ int vh_precon_prepare(vh_precon* p, double jwmu_re, double jwmu_im, int64_t* sta
 int vh_precon_apply(vh_precon* p, int prec, int64_t nrhs, const void* x, int64_t ldx, void* y, int64_t ldy,
                     int on_device);
 int vh_precon_get_ainv(vh_precon* p, double* ainv);
+/* Preconditioner type, the fl_precon_type switch of lse_sparse_precon_prepare.m
+ * (:83-88) and lse_sparse_precon_multiply.m (:26-46):
+ *   VH_PC_SCHUR_INVERT (default): exact inverse through the LDL^T factor;
+ *   VH_PC_SCHUR_GMRES: the Schur system is solved by an inner GMRES on
+ *     S = Ar diag(A_inv) Ar' (restart min(100, num_curr-1), at most 10 cycles,
+ *     Jacobi preconditioner diag(S)) to the relative tolerance prectol
+ *     (VoxHenry_executer.m:38 uses 0.1); FP64 only, and a nonlinear operator, so
+ *     the outer solve is flexible (vh_gmres with flexible = 1, as the executer's
+ *     fpgmres at :297). */
+#define VH_PC_SCHUR_INVERT 0
+#define VH_PC_SCHUR_GMRES 1
+int vh_precon_set_type(vh_precon* p, int type, double prectol);
+/* Inner Arnoldi steps (summed over the systems) of the last schur_gmres apply. */
+int vh_precon_inner_steps(vh_precon* p, int64_t* steps);
+/* y = S x = Ar diag(A_inv) Ar' x for nrhs complex128 node vectors (num_node
+ * entries, strides ldx / ldy), lse_sparse_schur_compl_multiply.m:1-5. */
+int vh_precon_schur_multiply(vh_precon* p, int64_t nrhs, const void* x, int64_t ldx, void* y, int64_t ldy,
+                             int on_device);
 /* Prepare n preconditioners of one geometry, one per frequency: jwmu holds n
  * interleaved complex values; z (optional) n impedance vectors of nv
  * interleaved complex values each (superconductors: z depends on f), NULL

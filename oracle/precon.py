@@ -1,7 +1,8 @@
 """Schur-complement preconditioner, restated from
 src_lse_formation/lse_sparse_precon_prepare.m:28-104,149-159 (schur_invert,
 ldlt_decomp, CHOLMOD) and lse_sparse_precon_multiply.m:61-97 (apply),
-lse_sparse_schur_compl_multiply.m:1-5.
+lse_sparse_schur_compl_multiply.m:1-5, and the schur_gmres variant
+(lse_sparse_precon_prepare.m:83-88, lse_sparse_precon_multiply.m:26-46).
 
 CHOLMOD 3.0.12 ``ldlchol``/``ldlsolve`` (third party, not vendored) compute an
 exact sparse LDL^T solve of the SPD Schur matrix S = Ar*A_inv*Ar'; this oracle
@@ -59,3 +60,27 @@ def lse_sparse_precon_multiply(v, s):
 def lse_sparse_schur_compl_multiply(schur_in, Ar, Ainv):
     """S*v = Ae*A_inv*Ae'*v, lse_sparse_schur_compl_multiply.m:3."""
     return Ar @ (Ainv * (Ar.T @ schur_in))
+
+
+def schur_jacobi_diag(s):
+    """diag(S) of the schur_gmres Jacobi preconditioner, lse_sparse_precon_prepare.m:85-86."""
+    return np.asarray(s.S.diagonal()).ravel()
+
+
+def lse_sparse_precon_multiply_schur_gmres(v, s, prectol=0.1):
+    """schur_gmres branch, lse_sparse_precon_multiply.m:26-46: bb = b - Ae*A_inv*a;
+    d = gmres(S, bb, min(100, num_curr-1), prectol, 10, diag(S)) (MATLAB gmres:
+    left preconditioned, x0 = 0, stopping on the preconditioned residual -- the
+    semantics pgmres.m keeps); x = (a + Ae'*d) .* diag(A_inv).  Returns (x, inner
+    Arnoldi steps)."""
+    from . import gmres as og
+    nc = s.Ainv.shape[0]
+    a, b = v[:nc], v[nc:]
+    bb = b - s.Ar @ (s.Ainv * a)
+    dg = schur_jacobi_diag(s)
+    d, flag, relres, it, resvec, nmv = og.pgmres(lambda u: lse_sparse_schur_compl_multiply(u, s.Ar, s.Ainv), bb,
+                                                 min(100, nc - 1), prectol, 10, lambda u: u / dg)
+    out = np.empty(v.shape, dtype=complex)
+    out[nc:] = d
+    out[:nc] = (a + s.ArT @ d) * s.Ainv
+    return out, len(resvec) - 1

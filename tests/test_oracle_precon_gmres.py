@@ -40,6 +40,30 @@ def test_schur_matrix_spd():
     assert np.allclose(precon.lse_sparse_schur_compl_multiply(v, s['Ar'], P.Ainv), S @ v, rtol=1e-13)
 
 
+def test_schur_gmres_variant_pins():
+    """schur_gmres (lse_sparse_precon_multiply.m:26-46): with a tight inner
+    tolerance it is the exact inverse (the schur_invert apply); at the
+    reference's prectol = 0.1 the Jacobi-preconditioned inner residual meets
+    the tolerance, and diag(S) is the diagonal of Ar diag(A_inv) Ar'."""
+    geo = problems.bar(6, 3, 3)
+    s, P = _prep(geo, 1e9)
+    nc = s['num_curr']
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal(nc + s['num_node']) + 1j * rng.standard_normal(nc + s['num_node'])
+    exact = precon.lse_sparse_precon_multiply(v, P)
+    tight, it_tight = precon.lse_sparse_precon_multiply_schur_gmres(v, P, 1e-13)
+    assert np.linalg.norm(tight - exact) / np.linalg.norm(exact) < 1e-10
+    dg = precon.schur_jacobi_diag(P)
+    Ad = abs(s['Ar']).multiply(abs(s['Ar'])) @ P.Ainv          # sum_k Ar(v,k)^2 A_inv(k)
+    assert np.allclose(dg, Ad, rtol=1e-14)
+    loose, it_loose = precon.lse_sparse_precon_multiply_schur_gmres(v, P, 0.1)
+    bb = v[nc:] - P.Ar @ (P.Ainv * v[:nc])
+    d = loose[nc:]
+    r = (bb - P.S @ d) / dg
+    assert np.linalg.norm(r) / np.linalg.norm(bb / dg) <= 0.1
+    assert 0 < it_loose < it_tight
+
+
 def _dense_case(n=30, seed=0, diag=4.0):
     rng = np.random.default_rng(seed)
     A = np.eye(n) * diag + rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))

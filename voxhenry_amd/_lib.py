@@ -13,6 +13,8 @@ LIBPATH = os.path.join(HERE, 'libvoxhenry.so')
 
 VH_C128 = 0
 VH_C64 = 1
+VH_PC_SCHUR_INVERT = 0
+VH_PC_SCHUR_GMRES = 1
 
 _lib = None
 
@@ -61,6 +63,9 @@ SIGNATURES = {
     'vh_precon_prepare': [_P, _D, _D, _P],
     'vh_precon_apply': [_P, _I, _I64, _P, _I64, _P, _I64, _I],
     'vh_precon_get_ainv': [_P, _P],
+    'vh_precon_set_type': [_P, _I, _D],
+    'vh_precon_inner_steps': [_P, _P],
+    'vh_precon_schur_multiply': [_P, _I64, _P, _I64, _P, _I64, _I],
     'vh_gmres': [_P, _P, _D, _D, _I, _I, _I64, _P, _P, _I, _I, _D, _I, _P, _P, _P, _P, _P, _I64, _P],
     'vh_ldl_solve_host': [_I64, _P, _P, _P, _I64, _P, _P, _P, _P, _P],
     'vh_solve_mixed': [_P, _P, _D, _D, _I64, _P, _P, _I, _D, _I, _I, _I, _D, _P, _P, _P, _P],

@@ -8,7 +8,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import lib, check, ptr, VH_C128, VH_C64
+from ._lib import lib, check, ptr, VH_C128, VH_C64, VH_PC_SCHUR_INVERT, VH_PC_SCHUR_GMRES
 
 MU0 = 4 * np.pi * 1e-7        # EMconstants.m:13
 
@@ -347,6 +347,28 @@ class Precon:
         out = np.zeros(self.geom.num_curr)
         check(lib().vh_precon_get_ainv(self.h, ptr(out)))
         return out
+
+    def set_type(self, kind, prectol=0.1):
+        """'schur_invert' (exact LDL^T inverse, default) or 'schur_gmres'
+        (inner GMRES on the Schur complement to prectol, FP64;
+        lse_sparse_precon_multiply.m:26-46)."""
+        t = {'schur_invert': VH_PC_SCHUR_INVERT, 'schur_gmres': VH_PC_SCHUR_GMRES}[kind]
+        check(lib().vh_precon_set_type(self.h, t, float(prectol)))
+
+    @property
+    def inner_steps(self):
+        """Inner Arnoldi steps (all systems) of the last schur_gmres apply."""
+        v = C.c_int64(0)
+        check(lib().vh_precon_inner_steps(self.h, C.byref(v)))
+        return v.value
+
+    def schur_multiply(self, x):
+        """S x = Ar diag(A_inv) Ar' x for complex node vectors (rows of x)."""
+        xx = np.ascontiguousarray(np.atleast_2d(x), dtype=np.complex128)
+        nrhs, ld = xx.shape
+        y = np.zeros_like(xx)
+        check(lib().vh_precon_schur_multiply(self.h, nrhs, ptr(xx), ld, ptr(y), ld, 0))
+        return y.reshape(np.shape(x))
 
     def apply(self, x, prec=VH_C128):
         if isinstance(x, np.ndarray):

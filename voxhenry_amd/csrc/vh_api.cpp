@@ -452,6 +452,42 @@ int vh_precon_apply(vh_precon* p, int prec, int64_t nrhs, const void* x, int64_t
 }
 
 
+int vh_precon_set_type(vh_precon* p, int type, double prectol) {
+  return guard([&] {
+    need(p, "p");
+    VH_REQUIRE(type == VH_PC_SCHUR_INVERT || type == VH_PC_SCHUR_GMRES, "unknown preconditioner type");
+    VH_REQUIRE(prectol > 0 && prectol < 1, "prectol must be in (0, 1)");
+    p->P.type = type;
+    p->P.prectol = prectol;
+  });
+}
+
+int vh_precon_inner_steps(vh_precon* p, int64_t* steps) {
+  return guard([&] {
+    need(p, "p"); need(steps, "steps");
+    *steps = p->P.inner_steps;
+  });
+}
+
+int vh_precon_schur_multiply(vh_precon* p, int64_t nrhs, const void* x, int64_t ldx, void* y, int64_t ldy,
+                             int on_device) {
+  return guard([&] {
+    need(p, "p"); need(x, "x"); need(y, "y");
+    VH_REQUIRE(p->P.ready, "preconditioner not prepared");
+    auto& G = *p->P.G;
+    const int64_t nn = G.num_node;
+    VH_REQUIRE(ldx >= nn && ldy >= nn, "leading dimension smaller than the number of nodes");
+    VH_REQUIRE(nrhs >= 1 && nrhs < (1 << 16), "nrhs out of range");
+    hipStream_t st = G.ctx->stream;
+    Io io;
+    io.in(x, nrhs, ldx, 16, on_device, st, ldy);
+    void* dy = on_device ? y : io.y;
+    vh::schur_multiply(p->P, (int)nrhs, (const double2*)io.x, ldx, (double2*)dy, ldy);
+    if (!on_device) d2h(y, io.y, (size_t)nrhs * ldy * 16, st);
+    VH_HIP(hipStreamSynchronize(st));
+  });
+}
+
 int vh_gmres(vh_geom* g, vh_precon* p, double jre, double jim, int prec, int flexible, int64_t nrhs, const void* b,
              void* x, int on_device, int restart, double tol, int maxit, int* flag, double* relres, int* iter,
              int* niters, double* resvec, int64_t resvec_cap, int* nresvec) {
@@ -614,6 +650,8 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
     VH_REQUIRE(restart >= 1 && restart <= 127 && inner_restart >= 1 && inner_restart <= 127 && maxit >= 1,
                "bad restart / maxit");
     for (int64_t k = 0; k < nprecon; ++k) VH_REQUIRE(precons[k] && precons[k]->P.ready, "preconditioner not prepared");
+    for (int64_t k = 0; k < nprecon; ++k)
+      VH_REQUIRE(precons[k]->P.type == vh::PC_SCHUR_INVERT, "the batched solve takes schur_invert preconditioners");
     for (int64_t s = 0; s < nsys; ++s) VH_REQUIRE(pidx[s] >= 0 && pidx[s] < nprecon, "pidx out of range");
     VH_REQUIRE(tol > 0, "tol must be positive");
     if (tols)

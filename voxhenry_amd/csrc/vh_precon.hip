@@ -1151,6 +1151,14 @@ void precon_apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, cons
 template <typename R>
 void precon_apply(Precon& P, const std::vector<int>& slots, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy) {
   VH_REQUIRE(P.ready, "preconditioner not prepared");
+  if (P.type == PC_SCHUR_GMRES) {
+    if constexpr (sizeof(R) == 8) {
+      schur_gmres_apply(P, slots, x, ldx, y, ldy, &P.inner_steps);
+      return;
+    } else {
+      throw Error(ERR_ARG, "the schur_gmres preconditioner is applied in FP64 only");
+    }
+  }
   const PcTab<R>* tab;
   if constexpr (sizeof(R) == 8) tab = P.tab64.p; else tab = P.tab32.p;
   precon_apply_cols<R>(*P.G, *P.sym, tab, nullptr, slots, x, ldx, y, ldy);

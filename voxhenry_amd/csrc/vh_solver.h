@@ -79,9 +79,21 @@ struct PreconSymbolic {
   DevBuf<int64_t> d_sn_off;
 };
 
+struct SchurGmres;   // workspaces of the schur_gmres apply (vh_schur.hip)
+
+// Preconditioner types of lse_sparse_precon_prepare.m (fl_precon_type):
+// schur_invert (default) applies the exact inverse through the LDL^T factor;
+// schur_gmres solves the Schur system by an inner GMRES on S with a Jacobi
+// preconditioner to the relative tolerance prectol.
+enum PreconType { PC_SCHUR_INVERT = 0, PC_SCHUR_GMRES = 1 };
+
 struct Precon {
   Geometry* G = nullptr;
   bool ready = false;
+  int type = PC_SCHUR_INVERT;
+  double prectol = 0.1;               // VoxHenry_executer.m:38
+  std::shared_ptr<SchurGmres> schur;
+  int64_t inner_steps = 0;            // schur_gmres: inner Arnoldi steps of the last apply (all systems)
   bool geometric_order = true;        // nested dissection by voxel-face planes
   std::shared_ptr<PreconSymbolic> sym;
   LdlFactor F;                        // numeric values (Lx, D) of this frequency
@@ -112,6 +124,14 @@ void precon_prepare_many(Precon* const* P, int n, const std::complex<double>* jw
 // device memory), lse_sparse_precon_multiply.m:84-88.
 template <typename R>
 void precon_apply(Precon& P, const std::vector<int>& slots, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy);
+// schur_gmres apply (lse_sparse_precon_multiply.m:26-46) for the vectors in
+// the given slots, FP64; inner_steps (optional) receives the inner Arnoldi
+// steps summed over the systems.
+void schur_gmres_apply(Precon& P, const std::vector<int>& slots, const double2* x, int64_t ldx, double2* y,
+                       int64_t ldy, int64_t* inner_steps = nullptr);
+// y = S x = Ar diag(A_inv) Ar' x for nrhs node vectors (stride ldx / ldy,
+// device), lse_sparse_schur_compl_multiply.m:1-5.
+void schur_multiply(Precon& P, int nrhs, const double2* x, int64_t ldx, double2* y, int64_t ldy);
 // Several preconditioners of one geometry (sharing Y) applied in one pass:
 // slot s uses tab[pidx[s]] (tab[0] when pidx is null); tab is a device
 // array, pidx a host array indexed by slot.

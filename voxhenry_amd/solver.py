@@ -50,8 +50,14 @@ def impedance(structure, idx, f):
 
 
 class Solver:
-    def __init__(self, ctx, structure, pad_mode=1):
+    def __init__(self, ctx, structure, pad_mode=1, precon_type='schur_invert', prectol=0.1):
+        """precon_type: the executer's fl_precon_type (VoxHenry_executer.m:37-38):
+        'schur_invert' (exact Schur inverse, pgmres) or 'schur_gmres' (inner
+        GMRES on the Schur complement to prectol, with the flexible fpgmres
+        outer solver, :295-297)."""
         self.ctx = ctx
+        self.precon_type = precon_type
+        self.prectol = prectol
         self.structure = structure
         self.idx = nonempty(structure)
         dims = structure['sigma'].shape
@@ -62,6 +68,8 @@ class Solver:
         self.rhs = self.G.build_topology(structure['ports_P'], structure['ports_N'], structure.get('well'))
         t2 = time.perf_counter()
         self.P = Precon(self.G)
+        if precon_type != 'schur_invert':
+            self.P.set_type(precon_type, prectol)
         self.nports = len(structure['ports_P'])
         self.times = dict(tensor=t1 - t0, topology=t2 - t1)
         self._z_f = None
@@ -83,7 +91,8 @@ class Solver:
         jwmu = self.prepare(f)
         t = time.perf_counter()
         if method == 'fp64':
-            X, st = self.G.gmres(self.rhs, jwmu, self.P, restart=restart, tol=tol, maxit=maxit, prec=VH_C128)
+            X, st = self.G.gmres(self.rhs, jwmu, self.P, restart=restart, tol=tol, maxit=maxit, prec=VH_C128,
+                                 flexible=self.precon_type == 'schur_gmres')
         elif method == 'mixed':
             X, st = self.G.solve_mixed(self.rhs, jwmu, self.P, tol=tol, **kw)
         else:
