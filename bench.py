@@ -216,11 +216,13 @@ def bus_step(S, freqs, precs, method, data, **opts):
     return Z, st
 
 
-def sweep_config(ctx, st, freqs, tol='reference', methods=('mixed', 'fp64'), reps=1):
+def sweep_config(ctx, st, freqs, tol='reference', methods=('mixed', 'fp64'), reps=1, keep=None):
     """Batched (port x frequency) solve of one geometry, device-resident:
     set-up (tensor, topology, preconditioners) untimed, then one warm-up and
     `reps` timed sweeps per method; returns solves/s per method, R/L of the
-    FP64 sweep and the mixed-vs-FP64 errors (self terms, and normwise Z)."""
+    FP64 sweep and the mixed-vs-FP64 errors (self terms, and normwise Z).
+    keep (a list): the solver and its preconditioners are appended to it and
+    left open (for a later concurrent pass) instead of being released."""
     import torch
     from voxhenry_amd import solver as vsol
     t0 = time.perf_counter()
@@ -258,6 +260,9 @@ def sweep_config(ctx, st, freqs, tol='reference', methods=('mixed', 'fp64'), rep
     if 'fp64' in res:
         out['R_ohm'] = [float(v) for v in res['fp64']['R'][:, 0, 0]]
         out['L_H'] = [float(v) for v in res['fp64']['L'][:, 0, 0]]
+    if keep is not None:
+        keep.append((S, precs, pidx))
+        return out
     for P in precs:
         P.close()
     S.G.close()
@@ -269,16 +274,27 @@ def uneven_config(ctx, count, nfreq, seed=0):
     """A prefix of the seeded uneven batch (bars, round wires, loops, two-fluid
     strips), each geometry's port x frequency systems in one batched solve;
     aggregate solves/s over the geometries (set-up untimed)."""
+    import threading
+    import torch
+    import voxhenry_amd as vh
     from voxhenry_amd import structures
     freqs = np.logspace(6, 10, nfreq)
     tot = {'mixed': 0.0, 'fp64': 0.0}
     nsys = 0
     per = []
+    jobs = []      # (solver, preconditioners, pidx, context, stream) per geometry
     # R relative to itself over the normal conductors; R relative to |Z| over
     # all (a two-fluid strip's R is ~1e-9 of |Z|, below any solve's resolution)
     worst = dict(R_self_normal=0.0, R_over_absZ=0.0, L_self=0.0, Z_norm=0.0)
     for kind, st in structures.uneven_batch(seed, count):
-        r = sweep_config(ctx, st, freqs)
+        # each geometry has its own context and stream, so that the
+        # concurrent pass below can run all of them at once
+        gctx = vh.Context(0)
+        stream = torch.cuda.Stream()
+        keep = []
+        with torch.cuda.stream(stream):
+            r = sweep_config(gctx, st, freqs, keep=keep)
+        jobs.append(keep[0] + (gctx, stream))
         nsys += r['nfreq'] * r['nports']
         for m in tot:
             tot[m] += r[m]['s']
@@ -294,9 +310,49 @@ def uneven_config(ctx, count, nfreq, seed=0):
                         relres_max=dict(mixed=r['mixed']['relres_max'], fp64=r['fp64']['relres_max'])))
         log('[bench]   uneven %s %s: mixed %.1f fp64 %.1f solves/s' % (kind, r['grid'], r['mixed']['solves_per_s'],
                                                                      r['fp64']['solves_per_s']))
+    # Concurrent pass: every geometry's whole sweep (one batched solve each)
+    # on its own stream from its own host thread (the library calls release
+    # the interpreter lock), all started together; wall time of the batch.
+    conc = {}
+    for m in ('mixed', 'fp64'):
+        torch.cuda.synchronize()
+        go = threading.Barrier(len(jobs) + 1)
+        errs = []
+
+        def run(job):
+            S, precs, pidx, gctx, stream = job
+            try:
+                go.wait()
+                with torch.cuda.stream(stream):
+                    S.sweep_batched(freqs, method=m, precons=precs, pidx_of_freq=pidx, device=True)
+                    stream.synchronize()
+            except Exception as e:   # reported below
+                errs.append(repr(e))
+
+        th = [threading.Thread(target=run, args=(j,)) for j in jobs]
+        for t in th:
+            t.start()
+        go.wait()
+        t0 = time.perf_counter()
+        for t in th:
+            t.join()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        if errs:
+            raise RuntimeError('concurrent uneven batch failed: %s' % errs[0])
+        conc[m] = nsys / wall
+        log('[bench]   uneven concurrent %s: %.1f solves/s (%d geometries at once, %.3f s)' % (m, conc[m], len(jobs), wall))
+    for S, precs, pidx, gctx, stream in jobs:
+        for P in precs:
+            P.close()
+        S.G.close()
+    torch.cuda.empty_cache()
     return dict(seed=seed, geometries=count, nfreq=nfreq, systems=nsys,
                 mixed_solves_per_s=nsys / tot['mixed'], fp64_solves_per_s=nsys / tot['fp64'],
-                speedup_mixed_over_fp64=tot['fp64'] / tot['mixed'], mixed_vs_fp64_max=worst, per_geometry=per)
+                speedup_mixed_over_fp64=tot['fp64'] / tot['mixed'],
+                concurrent=dict(mixed_solves_per_s=conc['mixed'], fp64_solves_per_s=conc['fp64'],
+                                speedup_mixed_over_fp64=conc['mixed'] / conc['fp64']),
+                mixed_vs_fp64_max=worst, per_geometry=per)
 
 
 def timed_steps(fn, steps, warmup):

@@ -33,6 +33,11 @@ class Context:
         s = torch.cuda.current_stream(self.device)
         check(lib().vh_ctx_set_stream(self.h, C.c_void_p(s.cuda_stream)))
 
+    def use_stream(self, stream):
+        """Launch on the given torch stream (one stream per context lets the
+        geometries of several contexts run concurrently)."""
+        check(lib().vh_ctx_set_stream(self.h, C.c_void_p(stream.cuda_stream)))
+
     def synchronize(self):
         check(lib().vh_ctx_synchronize(self.h))
 

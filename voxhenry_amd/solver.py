@@ -153,7 +153,7 @@ class Solver:
             nz = np.flatnonzero(np.any(self.rhs != 0, axis=0))
             rhs = torch.tensor(self.rhs, dtype=torch.complex128, device='cuda')
             B = rhs.repeat(len(freqs), 1).contiguous()
-            torch.cuda.synchronize()
+            torch.cuda.current_stream().synchronize()   # this stream only: other geometries may run beside
             t = time.perf_counter()
             X, st = self.G.solve_batch(B, jw, precons, pidx, method=method, tol=tol, **kw)
             Xp = X.index_select(1, torch.tensor(nz, device='cuda')).cpu().numpy()
