@@ -274,10 +274,9 @@ def uneven_config(ctx, count, nfreq, seed=0):
     """A prefix of the seeded uneven batch (bars, round wires, loops, two-fluid
     strips), each geometry's port x frequency systems in one batched solve;
     aggregate solves/s over the geometries (set-up untimed)."""
-    import threading
     import torch
     import voxhenry_amd as vh
-    from voxhenry_amd import structures
+    from voxhenry_amd import structures, solver as vsol
     freqs = np.logspace(6, 10, nfreq)
     tot = {'mixed': 0.0, 'fp64': 0.0}
     nsys = 0
@@ -311,35 +310,14 @@ def uneven_config(ctx, count, nfreq, seed=0):
         log('[bench]   uneven %s %s: mixed %.1f fp64 %.1f solves/s' % (kind, r['grid'], r['mixed']['solves_per_s'],
                                                                      r['fp64']['solves_per_s']))
     # Concurrent pass: every geometry's whole sweep (one batched solve each)
-    # on its own stream from its own host thread (the library calls release
-    # the interpreter lock), all started together; wall time of the batch.
+    # on its own stream from its own host thread, all started together; wall
+    # time of the batch (solver.sweep_concurrent).
     conc = {}
     for m in ('mixed', 'fp64'):
         torch.cuda.synchronize()
-        go = threading.Barrier(len(jobs) + 1)
-        errs = []
-
-        def run(job):
-            S, precs, pidx, gctx, stream = job
-            try:
-                go.wait()
-                with torch.cuda.stream(stream):
-                    S.sweep_batched(freqs, method=m, precons=precs, pidx_of_freq=pidx, device=True)
-                    stream.synchronize()
-            except Exception as e:   # reported below
-                errs.append(repr(e))
-
-        th = [threading.Thread(target=run, args=(j,)) for j in jobs]
-        for t in th:
-            t.start()
-        go.wait()
-        t0 = time.perf_counter()
-        for t in th:
-            t.join()
+        _, wall = vsol.sweep_concurrent([(S, freqs, precs, pidx, stream) for S, precs, pidx, gctx, stream in jobs],
+                                        method=m)
         torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        if errs:
-            raise RuntimeError('concurrent uneven batch failed: %s' % errs[0])
         conc[m] = nsys / wall
         log('[bench]   uneven concurrent %s: %.1f solves/s (%d geometries at once, %.3f s)' % (m, conc[m], len(jobs), wall))
     for S, precs, pidx, gctx, stream in jobs:

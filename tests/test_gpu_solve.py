@@ -311,3 +311,32 @@ def test_prepare_many_matches_serial(ctx):
     x = rng.standard_normal((1, S.G.n)) + 1j * rng.standard_normal((1, S.G.n))
     for a, b in zip(par, ser):
         assert np.array_equal(a.apply(x), b.apply(x))
+
+
+def test_sweep_concurrent_matches_sequential(ctx):
+    """Several geometries solved at once (own context and stream each, host
+    threads) give the same Z as their sweeps one after another: the
+    computation of one geometry does not depend on what runs beside it."""
+    import torch
+    import voxhenry_amd as vh
+    from voxhenry_amd import structures, solver
+    say('concurrent geometries')
+    geos = [structures.bus(nbars=2, length=20, section=3, pitch=5, width_grid=10), structures.bar(24, 4, 3),
+            structures.strip(20, 4, thickness=2)]
+    freqs = np.logspace(6, 9, 4)
+    jobs, seq = [], []
+    for g in geos:
+        gctx = vh.Context(0)
+        stream = torch.cuda.Stream()
+        with torch.cuda.stream(stream):
+            S = solver.Solver(gctx, g)
+            P = S.precons_for(freqs)
+            pidx = np.arange(len(freqs))
+            seq.append(S.sweep_batched(freqs, method='mixed', precons=P, pidx_of_freq=pidx, device=True))
+        jobs.append((S, freqs, P, pidx, stream))
+    out, wall = solver.sweep_concurrent(jobs, method='mixed')
+    for a, b in zip(out, seq):
+        assert np.all(a['stats']['flag'] == 0)
+        d = max(np.linalg.norm(a['Z'][k] - b['Z'][k]) / np.linalg.norm(b['Z'][k]) for k in range(len(freqs)))
+        say('concurrent vs sequential Z', d)
+        assert d < 1e-12

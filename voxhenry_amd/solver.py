@@ -192,3 +192,40 @@ class Solver:
         for k in ('Y', 'Z', 'R', 'L'):
             out[k] = np.array(out[k])
         return out
+
+
+def sweep_concurrent(jobs, method='mixed', **kw):
+    """Batched sweeps of several geometries at once (the uneven batch): jobs
+    is a list of (solver, freqs, precons, pidx_of_freq, stream) where every
+    solver has a Context of its own and stream is a torch stream for it.
+    Each geometry's sweep (one vh_solve_batch over its port x frequency
+    systems) runs from its own host thread on its own stream; the library
+    calls release the interpreter lock, so the geometries overlap on the
+    GPU.  Returns the sweeps' outputs in job order and the wall time."""
+    import threading
+    import torch
+    out = [None] * len(jobs)
+    errs = []
+    go = threading.Barrier(len(jobs) + 1)
+
+    def run(i):
+        S, freqs, precons, pidx, stream = jobs[i]
+        try:
+            go.wait()
+            with torch.cuda.stream(stream):
+                out[i] = S.sweep_batched(freqs, method=method, precons=precons, pidx_of_freq=pidx, device=True, **kw)
+                stream.synchronize()
+        except Exception as e:
+            errs.append(e)
+
+    th = [threading.Thread(target=run, args=(i,)) for i in range(len(jobs))]
+    for t in th:
+        t.start()
+    go.wait()
+    t0 = time.perf_counter()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t0
+    if errs:
+        raise errs[0]
+    return out, wall
