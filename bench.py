@@ -55,7 +55,7 @@ def parse():
     ap.add_argument('--no-fp64', action='store_true')
     ap.add_argument('--cpu-solves', type=int, default=2)
     ap.add_argument('--no-configs', action='store_true', help='skip the bar / spiral / uneven-batch lines')
-    ap.add_argument('--uneven-count', type=int, default=8, help='geometries of the uneven batch to run')
+    ap.add_argument('--uneven-count', type=int, default=96, help='geometries of the uneven batch to run (all 96)')
     ap.add_argument('--uneven-nfreq', type=int, default=16)
     ap.add_argument('--probe-pass-z', action='store_true',
                     help='only the 128^3 complex64 matvec (3 calls) and a reference elementwise kernel of known '
@@ -270,65 +270,73 @@ def sweep_config(ctx, st, freqs, tol='reference', methods=('mixed', 'fp64'), rep
     return out
 
 
-def uneven_config(ctx, count, nfreq, seed=0):
+def uneven_config(ctx, count, nfreq, seed=0, wave=8):
     """A prefix of the seeded uneven batch (bars, round wires, loops, two-fluid
     strips), each geometry's port x frequency systems in one batched solve;
-    aggregate solves/s over the geometries (set-up untimed)."""
+    aggregate solves/s over the geometries (set-up untimed), one after
+    another and `wave` geometries at a time concurrently (a context and
+    stream each; geometries are released wave by wave, since every one holds
+    its Krylov workspace -- restart 100 in FP64)."""
     import torch
     import voxhenry_amd as vh
     from voxhenry_amd import structures, solver as vsol
     freqs = np.logspace(6, 10, nfreq)
     tot = {'mixed': 0.0, 'fp64': 0.0}
+    conc_wall = {'mixed': 0.0, 'fp64': 0.0}
     nsys = 0
     per = []
-    jobs = []      # (solver, preconditioners, pidx, context, stream) per geometry
     # R relative to itself over the normal conductors; R relative to |Z| over
     # all (a two-fluid strip's R is ~1e-9 of |Z|, below any solve's resolution)
-    worst = dict(R_self_normal=0.0, R_over_absZ=0.0, L_self=0.0, Z_norm=0.0)
-    for kind, st in structures.uneven_batch(seed, count):
-        # each geometry has its own context and stream, so that the
-        # concurrent pass below can run all of them at once
-        gctx = vh.Context(0)
-        stream = torch.cuda.Stream()
-        keep = []
-        with torch.cuda.stream(stream):
-            r = sweep_config(gctx, st, freqs, keep=keep)
-        jobs.append(keep[0] + (gctx, stream))
-        nsys += r['nfreq'] * r['nports']
-        for m in tot:
-            tot[m] += r[m]['s']
-        e = r['mixed_vs_fp64']
-        if kind != 'strip':
-            worst['R_self_normal'] = max(worst['R_self_normal'], e['R_self'])
-        for k in ('R_over_absZ', 'L_self', 'Z_norm'):
-            worst[k] = max(worst[k], e[k])
-        per.append(dict(kind=kind, grid=r['grid'], nv=r['nv'], setup_s=round(r['setup_s'], 2),
-                        mixed_solves_per_s=round(r['mixed']['solves_per_s'], 1),
-                        fp64_solves_per_s=round(r['fp64']['solves_per_s'], 1),
-                        mixed_converged=r['mixed']['all_converged'], fp64_converged=r['fp64']['all_converged'],
-                        relres_max=dict(mixed=r['mixed']['relres_max'], fp64=r['fp64']['relres_max'])))
-        log('[bench]   uneven %s %s: mixed %.1f fp64 %.1f solves/s' % (kind, r['grid'], r['mixed']['solves_per_s'],
-                                                                     r['fp64']['solves_per_s']))
-    # Concurrent pass: every geometry's whole sweep (one batched solve each)
-    # on its own stream from its own host thread, all started together; wall
-    # time of the batch (solver.sweep_concurrent).
-    conc = {}
-    for m in ('mixed', 'fp64'):
-        torch.cuda.synchronize()
-        _, wall = vsol.sweep_concurrent([(S, freqs, precs, pidx, stream) for S, precs, pidx, gctx, stream in jobs],
-                                        method=m)
-        torch.cuda.synchronize()
-        conc[m] = nsys / wall
-        log('[bench]   uneven concurrent %s: %.1f solves/s (%d geometries at once, %.3f s)' % (m, conc[m], len(jobs), wall))
-    for S, precs, pidx, gctx, stream in jobs:
-        for P in precs:
-            P.close()
-        S.G.close()
-    torch.cuda.empty_cache()
+    worst = dict(R_self_normal=0.0, L_self_normal=0.0, Z_norm_normal=0.0, R_over_absZ=0.0, L_self=0.0, Z_norm=0.0)
+    geos = list(structures.uneven_batch(seed, count))
+    for w0 in range(0, len(geos), wave):
+        jobs = []      # (solver, preconditioners, pidx, context, stream) per geometry
+        wsys = 0
+        for kind, st in geos[w0:w0 + wave]:
+            gctx = vh.Context(0)
+            stream = torch.cuda.Stream()
+            keep = []
+            with torch.cuda.stream(stream):
+                r = sweep_config(gctx, st, freqs, keep=keep)
+            jobs.append(keep[0] + (gctx, stream))
+            wsys += r['nfreq'] * r['nports']
+            for m in tot:
+                tot[m] += r[m]['s']
+            e = r['mixed_vs_fp64']
+            if kind != 'strip':
+                worst['R_self_normal'] = max(worst['R_self_normal'], e['R_self'])
+                worst['L_self_normal'] = max(worst['L_self_normal'], e['L_self'])
+                worst['Z_norm_normal'] = max(worst['Z_norm_normal'], e['Z_norm'])
+            for k in ('R_over_absZ', 'L_self', 'Z_norm'):
+                worst[k] = max(worst[k], e[k])
+            per.append(dict(kind=kind, grid=r['grid'], nv=r['nv'], setup_s=round(r['setup_s'], 2),
+                            mixed_solves_per_s=round(r['mixed']['solves_per_s'], 1),
+                            fp64_solves_per_s=round(r['fp64']['solves_per_s'], 1),
+                            mixed_converged=r['mixed']['all_converged'], fp64_converged=r['fp64']['all_converged'],
+                            relres_max=dict(mixed=r['mixed']['relres_max'], fp64=r['fp64']['relres_max'])))
+            log('[bench]   uneven %s %s: mixed %.1f fp64 %.1f solves/s' % (kind, r['grid'], r['mixed']['solves_per_s'],
+                                                                         r['fp64']['solves_per_s']))
+        nsys += wsys
+        # concurrent pass over the wave: every geometry's whole sweep on its
+        # own stream from its own host thread, started together
+        for m in ('mixed', 'fp64'):
+            torch.cuda.synchronize()
+            _, wall = vsol.sweep_concurrent([(S, freqs, precs, pidx, stream)
+                                             for S, precs, pidx, gctx, stream in jobs], method=m)
+            torch.cuda.synchronize()
+            conc_wall[m] += wall
+            log('[bench]   uneven wave of %d concurrent %s: %.1f solves/s (%.3f s)' % (len(jobs), m, wsys / wall, wall))
+        for S, precs, pidx, gctx, stream in jobs:
+            for P in precs:
+                P.close()
+            S.G.close()
+        del jobs
+        torch.cuda.empty_cache()
+    conc = {m: nsys / conc_wall[m] for m in conc_wall}
     return dict(seed=seed, geometries=count, nfreq=nfreq, systems=nsys,
                 mixed_solves_per_s=nsys / tot['mixed'], fp64_solves_per_s=nsys / tot['fp64'],
                 speedup_mixed_over_fp64=tot['fp64'] / tot['mixed'],
-                concurrent=dict(mixed_solves_per_s=conc['mixed'], fp64_solves_per_s=conc['fp64'],
+                concurrent=dict(wave=wave, mixed_solves_per_s=conc['mixed'], fp64_solves_per_s=conc['fp64'],
                                 speedup_mixed_over_fp64=conc['mixed'] / conc['fp64']),
                 mixed_vs_fp64_max=worst, per_geometry=per)
 
@@ -444,6 +452,15 @@ def main():
                 bound='hbm', achieved=z['GBs'], peak=HBM_PEAK_GBS, unit='GB/s',
                 frac=(z['GBs'] / HBM_PEAK_GBS) if z['GBs'] else None, traffic=recorded_traffic())
 
+    # the bus geometry holds a Krylov workspace sized from most of the free
+    # memory: release it before the configuration sweeps
+    import torch
+    del data
+    for P in precs:
+        P.close()
+    S.G.close()
+    torch.cuda.empty_cache()
+
     configs = None
     if not args.no_configs:
         from voxhenry_amd import structures
@@ -453,7 +470,7 @@ def main():
         log('[bench] spiral, 32 frequencies')
         configs['spiral_5turn'] = sweep_config(ctx, structures.spiral(), np.logspace(6, 11, 32))
         if args.uneven_count > 0:
-            log('[bench] uneven batch, first %d geometries x %d frequencies' % (args.uneven_count, args.uneven_nfreq))
+            log('[bench] uneven batch, %d geometries x %d frequencies' % (args.uneven_count, args.uneven_nfreq))
             configs['uneven_batch'] = uneven_config(ctx, args.uneven_count, args.uneven_nfreq)
 
     cpu = None
