@@ -59,7 +59,6 @@ struct SolveOpts {
   bool final_residual = true;     // compute relres with one more matvec
   bool flexible = false;          // fpgmres semantics (right preconditioner)
   bool reorth = false;            // classical Gram-Schmidt with a second pass (CGS2)
-  int reorth_from = 0;            // ... from the (reorth_from+1)-th basis vector of a cycle on
   const std::vector<double>* tol_slot = nullptr;   // per-slot tolerance (overrides tol)
 };
 

@@ -223,10 +223,9 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
       // classical Gram-Schmidt, one pass (pgmres_iter.m:107-111); with reorth a
       // second pass re-projects w (the FP32 inner solve: one pass loses
       // orthogonality once the residual nears sqrt(eps))
-      const bool two = o.reorth && k >= o.reorth_from;
       ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p);
-      ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, 1.0, -1.0, two ? nullptr : W.nrm.p);
-      if (two) {
+      ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, 1.0, -1.0, o.reorth ? nullptr : W.nrm.p);
+      if (o.reorth) {
         ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h2.p);
         ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h2.p, 1.0, -1.0, W.nrm.p);
         VH_HIP(hipMemcpyAsync(W.h_h2.p, W.h2.p, (size_t)nact * (m + 1) * sizeof(double2), hipMemcpyDeviceToHost,
@@ -242,7 +241,7 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
       for (int a = 0; a < nact; ++a) {
         const int s = act[a];
         for (int j = 0; j < k; ++j) hc[j] = cd(hbuf[(size_t)a * (m + 1) + j].x, hbuf[(size_t)a * (m + 1) + j].y);
-        if (two)
+        if (o.reorth)
           for (int j = 0; j < k; ++j) hc[j] += cd(W.h_h2[(size_t)a * (m + 1) + j].x, W.h_h2[(size_t)a * (m + 1) + j].y);
         const double hn = std::sqrt(nbuf[a].x);
         const double rres = gv[s].push(hc.data(), hn);
@@ -318,10 +317,6 @@ void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<f
   io.final_residual = false;
   io.flexible = false;
   io.reorth = true;
-  // the first basis vectors of a cycle stay orthogonal with one pass; the
-  // second pass starts at the fifth (bus: -1.7% time, spiral unchanged; from
-  // the ninth the spiral's inner solves lose orthogonality)
-  io.reorth_from = 4;
   // Inner tolerance per system: the outer residual only has to fall from its
   // current value r (relative) to tol, so an inner solve that reduces its
   // own residual by about tol / r suffices (inexact Krylov: the admissible
