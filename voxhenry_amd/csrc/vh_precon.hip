@@ -7,9 +7,11 @@
 // dissection ordering whose separator blocks become dense supernodes; L (CSR
 // and CSC copies), D and the inverses of the diagonal blocks go to the device.
 // The triangular solves run level by level over the block dependency levels,
-// one launch per level and kernel for every column of the batch: columns may
-// belong to different frequencies (different factor values, same pattern).
-// Work vectors are [node][column], column fastest.
+// every column of the batch in the same launches: columns may belong to
+// different frequencies (different factor values, same pattern) and are
+// grouped by frequency; work vectors are [group][node][port].  Blocks whose
+// column union fits the LDS go through the per-block kernel (update and
+// inverse product in one workgroup), the rest through the row kernels.
 #include "vh_solver.h"
 #include <algorithm>
 #include <atomic>
