@@ -549,10 +549,19 @@ __global__ __launch_bounds__(NT) void k_pass_x_inv(
     acc = cmul(alpha, acc);
     if (mode <= 1) acc = cadd(acc, cscale(cmul(zv[vid], xs[row]), w));
     if (mode == 0) {
+      // a current touches at most 2 nodes: both potentials loaded at once
       C s = czero<R>();
-      for (int k = atp[row]; k < atp[row + 1]; ++k) {
-        const C ph = xs[num_curr + atc[k]];
-        s.x += atv[k] * ph.x; s.y += atv[k] * ph.y;
+      const int k0 = atp[row], k1 = atp[row + 1];
+      if (k1 - k0 <= 2) {
+        const int n0 = k0 < k1 ? atc[k0] : 0, n1 = k0 + 1 < k1 ? atc[k0 + 1] : 0;
+        const R v0 = k0 < k1 ? atv[k0] : R(0), v1 = k0 + 1 < k1 ? atv[k0 + 1] : R(0);
+        if (k0 < k1) { const C ph = xs[num_curr + n0]; s.x += v0 * ph.x; s.y += v0 * ph.y; }
+        if (k0 + 1 < k1) { const C ph = xs[num_curr + n1]; s.x += v1 * ph.x; s.y += v1 * ph.y; }
+      } else {
+        for (int k = k0; k < k1; ++k) {
+          const C ph = xs[num_curr + atc[k]];
+          s.x += atv[k] * ph.x; s.y += atv[k] * ph.y;
+        }
       }
       acc = csub(acc, s);
     }
@@ -566,17 +575,48 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_node_rows(
     const cplx<R>* __restrict__ x, int64_t ldx, cplx<R>* __restrict__ y, int64_t ldy, const int* __restrict__ arp,
     const int* __restrict__ arc, const R* __restrict__ arv, int num_node, int64_t num_curr,
-    const int* __restrict__ slots) {
+    const int* __restrict__ slots, int nrhs) {
+  // One thread per node serves up to RB right-hand sides (blockIdx.y selects
+  // the group): the row's indices and values are loaded once for all of them
+  // instead of being streamed again for every vector.  A node touches at
+  // most 6 currents; a longer row takes the looping path.
+  constexpr int RB = 8, D = 8;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int rhs = slots ? slots[blockIdx.y] : blockIdx.y;
   if (i >= num_node) return;
-  const cplx<R>* xs = x + (int64_t)rhs * ldx;
-  cplx<R> s = czero<R>();
-  for (int k = arp[i]; k < arp[i + 1]; ++k) {
-    const cplx<R> v = xs[arc[k]];
-    s.x += arv[k] * v.x; s.y += arv[k] * v.y;
+  const int r0 = blockIdx.y * RB, r1 = min(nrhs, r0 + RB);
+  const int p0 = arp[i], p1 = arp[i + 1];
+  if (p1 - p0 <= D) {
+    int kk[D];
+    R vv[D];
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      kk[u] = p0 + u < p1 ? arc[p0 + u] : 0;
+      vv[u] = p0 + u < p1 ? arv[p0 + u] : R(0);
+    }
+    for (int q = r0; q < r1; ++q) {
+      const int rhs = slots ? slots[q] : q;
+      const cplx<R>* xs = x + (int64_t)rhs * ldx;
+      cplx<R> s = czero<R>();
+#pragma unroll
+      for (int u = 0; u < D; ++u)
+        if (p0 + u < p1) {
+          const cplx<R> v = xs[kk[u]];
+          s.x += vv[u] * v.x; s.y += vv[u] * v.y;
+        }
+      y[(int64_t)rhs * ldy + num_curr + i] = s;
+    }
+  } else {
+    for (int q = r0; q < r1; ++q) {
+      const int rhs = slots ? slots[q] : q;
+      const cplx<R>* xs = x + (int64_t)rhs * ldx;
+      cplx<R> s = czero<R>();
+      for (int k = p0; k < p1; ++k) {
+        const cplx<R> v = xs[arc[k]];
+        s.x += arv[k] * v.x; s.y += arv[k] * v.y;
+      }
+      y[(int64_t)rhs * ldy + num_curr + i] = s;
+    }
   }
-  y[(int64_t)rhs * ldy + num_curr + i] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -825,9 +865,9 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
   if (mode == 0 && G.num_node > 0) {
     const R* arv;
     if constexpr (sizeof(R) == 8) arv = G.Ar.v64.p; else arv = G.Ar.v32.p;
-    dim3 grid(ceil_div(G.num_node, 256), nrhs);
+    dim3 grid(ceil_div(G.num_node, 256), ceil_div(nrhs, 8));
     hipLaunchKernelGGL((k_node_rows<R>), grid, dim3(256), 0, st, x, ldx, y, ldy, G.Ar.ptr.p, G.Ar.col.p, arv,
-                       (int)G.num_node, G.num_curr, slots);
+                       (int)G.num_node, G.num_curr, slots, nrhs);
     VH_LAUNCHED();
   }
   mark(6);
