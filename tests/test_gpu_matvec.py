@@ -160,15 +160,17 @@ def test_golden_solution_residual(ctx):
 # Grids whose padded lengths exercise every radix of the pass kernels (2, 3,
 # 4, 5, 7, 8, 16), one- to four-stage plans, long x lines (Px = 800 needs 64
 # threads per line) and sparse voxel sets (the voxel-map gather and scatter).
-RADIX_SHAPES = [((400, 4, 4), False), ((9, 33, 17), True), ((64, 3, 40), False), ((5, 6, 2), True),
-                ((128, 20, 3), True), ((16, 16, 16), False), ((4, 4, 128), True), ((6, 5, 64), False),
-                ((3, 4, 100), False)]
+# Thin z (Pz = 3, 4, 5, 7, 8 from N = 2..4 under both paddings) takes the
+# one-radix pass-Z kernels, in FP64 the two-lanes-per-pencil one from Pz = 4.
+RADIX_SHAPES = [((400, 4, 4), False, 1), ((9, 33, 17), True, 1), ((64, 3, 40), False, 1), ((5, 6, 2), True, 1),
+                ((128, 20, 3), True, 1), ((16, 16, 16), False, 1), ((4, 4, 128), True, 1), ((6, 5, 64), False, 1),
+                ((3, 4, 100), False, 1), ((9, 5, 2), False, 0), ((12, 6, 4), True, 0), ((70, 3, 3), True, 0)]
 
 
-@pytest.mark.parametrize('dims,hollow', RADIX_SHAPES)
-def test_matvec_radix_shapes(ctx, dims, hollow):
+@pytest.mark.parametrize('dims,hollow,pad_mode', RADIX_SHAPES)
+def test_matvec_radix_shapes(ctx, dims, hollow, pad_mode):
     import voxhenry_amd as vh
-    say('matvec shapes', dims, hollow)
+    say('matvec shapes', dims, hollow, pad_mode)
     L, M, N = dims
     geo = problems.bar(L, M, N)
     if hollow:
@@ -181,7 +183,7 @@ def test_matvec_radix_shapes(ctx, dims, hollow):
     f = 1e9
     jwmu = 1j * 2 * np.pi * f * MU0
     fN2, _ = green.circulant_tensor(L, M, N, geo['dx'])
-    G = problems.device_geometry(ctx, geo, s, pad_mode=1, f=f)
+    G = problems.device_geometry(ctx, geo, s, pad_mode=pad_mode, f=f)
     zr, zi = topology.impedance(geo, s['idxS'], f)
     rng = np.random.default_rng(5)
     n = s['num_curr'] + s['num_node']

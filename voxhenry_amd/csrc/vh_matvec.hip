@@ -334,6 +334,94 @@ __global__ __launch_bounds__(NT, 1) void k_pass_z1(cplx<R>* __restrict__ B, Grid
   }
 }
 
+// FP64 one-radix pass Z with two lanes per pencil (adjacent lanes 2p, 2p+1):
+// one thread holding five Pz-point FP64 spectra plus the tensor needs more
+// registers than two waves per SIMD allow.  Lane h owns components 3h + i
+// (lane 0: 0,1,2; lane 1: 3,4) for the loads, the forward DFTs and the
+// inverse DFTs; the Green product is split by kz (lane 0: kz < KA, lane 1:
+// kz >= KA) and the lanes exchange the spectra they need by shuffles.  Same
+// sums as k_pass_z1.
+__device__ __forceinline__ double2 lane_xor1(double2 v) {
+  return make_double2(__shfl_xor(v.x, 1, 64), __shfl_xor(v.y, 1, 64));
+}
+
+template <int PZ, int NT>
+__global__ __launch_bounds__(NT, 2) void k_pass_z1_pair(double2* __restrict__ B, GridDims g, TensorRef<double> tens) {
+  using C = double2;
+  constexpr int NM = (PZ + 1) / 2;   // most voxels an embedding of length PZ holds
+  constexpr int KA = (PZ + 1) / 2;   // kz points of lane 0
+  constexpr int TP = NT / 2;         // pencils per workgroup
+  const int h = threadIdx.x & 1, pl = threadIdx.x >> 1;
+  const int Px = g.Px, Py = g.Py, N = g.N;
+  const int tiles_x = (Px + TP - 1) / TP;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = tile % tiles_x, u = tile / tiles_x;
+  const int v = u - 1;
+  const int ky = u == 0 ? 0 : ((v & 1) ? Py - 1 - (v >> 1) : 1 + (v >> 1));
+  const int kx = tx * TP + pl;
+  if (kx >= Px) return;   // both lanes of a pencil leave together; no barriers below
+  const int ax = kx < g.Hx ? kx : Px - kx;
+  const int ay = ky < g.Hy ? ky : Py - ky;
+  TensorIdx ti;
+  ti.sx = kx < g.Hx ? 1.f : -1.f;
+  ti.sy = ky < g.Hy ? 1.f : -1.f;
+  const int axy = ax + g.Hx * ay;
+  const int HxHy = g.Hx * g.Hy;
+  const int rhs = blockIdx.y;
+  const int64_t plane = (int64_t)Px * Py;
+  C* base = B + (int64_t)rhs * 5 * N * plane + (int64_t)ky * Px + kx;
+  C S[3][PZ];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int c = 3 * h + i;
+#pragma unroll
+    for (int n = 0; n < PZ; ++n)
+      S[i][n] = (n < NM && n < N && c < 5) ? base[(int64_t)(c * N + n) * plane] : czero<double>();
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) Dft<PZ, -1, C>::run(S[i]);
+  C Rs[3][PZ];   // product spectra of the lane's components, all kz
+#pragma unroll
+  for (int j = 0; j < KA; ++j) {
+    const int j2c = KA + j < PZ ? KA + j : PZ - 1;   // lane 1's kz (clamped; unused past PZ - 1)
+    const int k = h ? KA + j : j;
+    // each lane sends what its partner needs at the partner's kz
+    C rv[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) rv[i] = lane_xor1(h ? S[i][j] : S[i][j2c]);
+    C f[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c)
+      f[c] = h ? (c < 3 ? rv[c] : S[c - 3][j2c]) : (c < 3 ? S[c][j] : rv[c - 3]);
+    if (k < PZ) {
+      const bool lo = k < g.Hz;
+      TensorIdx tk = ti;
+      tk.sz = lo ? 1.f : -1.f;
+      tk.idx = axy + HxHy * (lo ? k : g.Pz - k);
+      apply_point<double>(f, tens, tk);
+    }
+    // back: lane 0 keeps components 0..2 at kz j and receives them at kz
+    // KA + j; lane 1 keeps 3,4 at KA + j and receives them at j
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const C mine_other = i < 2 ? f[i < 2 ? 3 + i : 0] : czero<double>();   // components 3, 4
+      const C back = lane_xor1(h ? f[i] : mine_other);
+      Rs[i][j] = h ? back : f[i];
+      if (KA + j < PZ) Rs[i][KA + j < PZ ? KA + j : 0] = h ? mine_other : back;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const int c = 3 * h + i;
+    Dft<PZ, +1, C>::run(Rs[i]);
+    if (c < 5) {
+#pragma unroll
+      for (int o = 0; o < NM; ++o)
+        if (o < N) base[(int64_t)(c * N + o) * plane] = Rs[i][o];
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // pass Z for power-of-two Pz (16..512), compile-time plan: radices, thread
 // mapping and twiddle offsets are constants, the middle stages ping-pong
@@ -712,7 +800,13 @@ static bool launch_z1(int nrhs, hipStream_t st, cplx<R>* B, const GridDims& g, T
     constexpr int PZ = decltype(pzc)::value;
     if (g.N > (PZ + 1) / 2) return false;
     const dim3 grid((unsigned)(ceil_div(g.Px, NT) * g.Py), nrhs);
-    hipLaunchKernelGGL((k_pass_z1<R, PZ, NT>), grid, dim3(NT), 0, st, B, g, tr);
+    if constexpr (sizeof(R) == 8 && PZ >= 4) {
+      // FP64 at four or more kz points: two lanes per pencil
+      const dim3 grid2((unsigned)(ceil_div(g.Px, NT / 2) * g.Py), nrhs);
+      hipLaunchKernelGGL((k_pass_z1_pair<PZ, NT>), grid2, dim3(NT), 0, st, B, g, tr);
+    } else {
+      hipLaunchKernelGGL((k_pass_z1<R, PZ, NT>), grid, dim3(NT), 0, st, B, g, tr);
+    }
     return true;
   };
   switch (g.Pz) {
