@@ -334,20 +334,24 @@ __global__ __launch_bounds__(NT, 1) void k_pass_z1(cplx<R>* __restrict__ B, Grid
   }
 }
 
-// FP64 one-radix pass Z with two lanes per pencil (adjacent lanes 2p, 2p+1):
-// one thread holding five Pz-point FP64 spectra plus the tensor needs more
+// One-radix pass Z with two lanes per pencil (adjacent lanes 2p, 2p+1): one
+// thread holding five Pz-point FP64 spectra plus the tensor needs more
 // registers than two waves per SIMD allow.  Lane h owns components 3h + i
 // (lane 0: 0,1,2; lane 1: 3,4) for the loads, the forward DFTs and the
 // inverse DFTs; the Green product is split by kz (lane 0: kz < KA, lane 1:
 // kz >= KA) and the lanes exchange the spectra they need by shuffles.  Same
 // sums as k_pass_z1.
-__device__ __forceinline__ double2 lane_xor1(double2 v) {
-  return make_double2(__shfl_xor(v.x, 1, 64), __shfl_xor(v.y, 1, 64));
+template <typename C>
+__device__ __forceinline__ C lane_xor1(C v) {
+  C r;
+  r.x = __shfl_xor(v.x, 1, 64);
+  r.y = __shfl_xor(v.y, 1, 64);
+  return r;
 }
 
-template <int PZ, int NT>
-__global__ __launch_bounds__(NT, 2) void k_pass_z1_pair(double2* __restrict__ B, GridDims g, TensorRef<double> tens) {
-  using C = double2;
+template <typename R, int PZ, int NT, int OCC>
+__global__ __launch_bounds__(NT, OCC) void k_pass_z1_pair(cplx<R>* __restrict__ B, GridDims g, TensorRef<R> tens) {
+  using C = cplx<R>;
   constexpr int NM = (PZ + 1) / 2;   // most voxels an embedding of length PZ holds
   constexpr int KA = (PZ + 1) / 2;   // kz points of lane 0
   constexpr int TP = NT / 2;         // pencils per workgroup
@@ -376,7 +380,7 @@ __global__ __launch_bounds__(NT, 2) void k_pass_z1_pair(double2* __restrict__ B,
     const int c = 3 * h + i;
 #pragma unroll
     for (int n = 0; n < PZ; ++n)
-      S[i][n] = (n < NM && n < N && c < 5) ? base[(int64_t)(c * N + n) * plane] : czero<double>();
+      S[i][n] = (n < NM && n < N && c < 5) ? base[(int64_t)(c * N + n) * plane] : czero<R>();
   }
 #pragma unroll
   for (int i = 0; i < 3; ++i) Dft<PZ, -1, C>::run(S[i]);
@@ -398,13 +402,13 @@ __global__ __launch_bounds__(NT, 2) void k_pass_z1_pair(double2* __restrict__ B,
       TensorIdx tk = ti;
       tk.sz = lo ? 1.f : -1.f;
       tk.idx = axy + HxHy * (lo ? k : g.Pz - k);
-      apply_point<double>(f, tens, tk);
+      apply_point<R>(f, tens, tk);
     }
     // back: lane 0 keeps components 0..2 at kz j and receives them at kz
     // KA + j; lane 1 keeps 3,4 at KA + j and receives them at j
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      const C mine_other = i < 2 ? f[i < 2 ? 3 + i : 0] : czero<double>();   // components 3, 4
+      const C mine_other = i < 2 ? f[i < 2 ? 3 + i : 0] : czero<R>();   // components 3, 4
       const C back = lane_xor1(h ? f[i] : mine_other);
       Rs[i][j] = h ? back : f[i];
       if (KA + j < PZ) Rs[i][KA + j < PZ ? KA + j : 0] = h ? mine_other : back;
@@ -801,9 +805,10 @@ static bool launch_z1(int nrhs, hipStream_t st, cplx<R>* B, const GridDims& g, T
     if (g.N > (PZ + 1) / 2) return false;
     const dim3 grid((unsigned)(ceil_div(g.Px, NT) * g.Py), nrhs);
     if constexpr (sizeof(R) == 8 && PZ >= 4) {
-      // FP64 at four or more kz points: two lanes per pencil
+      // FP64 at four or more kz points: two lanes per pencil (in FP32 one
+      // thread per pencil at three waves per SIMD measured faster)
       const dim3 grid2((unsigned)(ceil_div(g.Px, NT / 2) * g.Py), nrhs);
-      hipLaunchKernelGGL((k_pass_z1_pair<PZ, NT>), grid2, dim3(NT), 0, st, B, g, tr);
+      hipLaunchKernelGGL((k_pass_z1_pair<R, PZ, NT, 2>), grid2, dim3(NT), 0, st, B, g, tr);
     } else {
       hipLaunchKernelGGL((k_pass_z1<R, PZ, NT>), grid, dim3(NT), 0, st, B, g, tr);
     }
