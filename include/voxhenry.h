@@ -90,6 +90,19 @@ int vh_geom_get_rhs(vh_geom* g, double* rhs);
 /* z = z_real + z_imag per non-empty voxel (nvox complex), VoxHenry_executer.m:231-259. */
 int vh_geom_set_impedance(vh_geom* g, const double* z);
 
+/* Current coefficients on the voxel grid, replaces
+ * post_obtain_curr_coefs_on_grid(x, Mc) (post_obtain_curr_coefs_on_grid.m:1-45)
+ * for nrhs solutions x (complex128, stride ldx >= 5*nv; entries 0..5nv-1 are
+ * Jx, Jy, Jz, J2d, J3d per non-empty voxel): grid receives, per solution at
+ * stride ldg >= 5*L*M*N, the five L x M x N column-major grids Jx_currs_grid ..
+ * J3d_currs_grid (zero at empty voxels); minmax (optional, host, 2*nrhs)
+ * receives [cmin, cmax] = min / max of |J| over the Jx, Jy and Jz entries
+ * (:24-25).  The voxel set is the geometry's idx (the reference's |Mc| >=
+ * 1e-12 test selects the same voxels for conductors).  on_device: x and
+ * grid are device pointers. */
+int vh_geom_currents_on_grid(vh_geom* g, int64_t nrhs, const void* x, int64_t ldx, void* grid, int64_t ldg,
+                             double* minmax, int on_device);
+
 /* ---- matvec ---------------------------------------------------------------
  * y = A x, replaces lse_matvect_mult(J, fN_all, Ae, z_real, z_imag, dx, idx, ...)
  * (lse_matvect_mult.m:1-153) for nrhs vectors with leading dimensions ldx, ldy

@@ -340,3 +340,43 @@ def test_sweep_concurrent_matches_sequential(ctx):
         d = max(np.linalg.norm(a['Z'][k] - b['Z'][k]) / np.linalg.norm(b['Z'][k]) for k in range(len(freqs)))
         say('concurrent vs sequential Z', d)
         assert d < 1e-12
+
+
+def test_currents_on_grid_matches_oracle(ctx, tmp_path):
+    """post_obtain_curr_coefs_on_grid on the GPU: the reference's own numex1
+    solution scattered to the 60x20x20 grid is bit-identical to the oracle's
+    (a pure copy), for one and for two stacked solutions; the currents file
+    holds the same grids."""
+    import scipy.io as sio
+    import voxhenry_amd as vh
+    from oracle import post
+    from voxhenry_amd import results, solver, structures
+    say('currents on grid')
+    geo = problems.numex1()
+    s = topology.build_system(geo)
+    d = np.load(os.path.join(problems.GOLDEN, 'numex1_x.npz'))
+    x, f = d['x'], float(d['freq'])
+    G = vh.Geometry(ctx, s['dims'], geo['dx'], s['idxS'])
+    grids, cmin, cmax = G.currents_on_grid(x)
+    ref, rmin, rmax = post.curr_coefs_on_grid(x, post.susceptibility(geo['sigma'], f))
+    for c in range(5):
+        assert np.array_equal(grids[c], ref[c]), c
+    # |J| by the device hypot: within 2 ulp of numpy's
+    ulp2 = lambda a, b: abs(a - b) <= 2 * np.spacing(abs(b))
+    assert ulp2(cmin, rmin) and ulp2(cmax, rmax), (cmin - rmin, cmax - rmax)
+    X = np.stack([x, 2 * x])
+    g2, mn, mx = G.currents_on_grid(X)
+    assert g2.shape == (2,) + grids.shape
+    assert np.array_equal(g2[1], 2 * grids) and mx[1] == 2 * cmax and mn[0] == cmin
+    # hollow geometry through the solver and the currents file
+    sgeo = structures.bar(9, 6, 5)
+    sgeo['sigma'][3:6, 2:4, 1:4] = 0.0
+    S = solver.Solver(ctx, sgeo)
+    xs = np.random.default_rng(4).standard_normal(S.G.n) + 1j * np.random.default_rng(5).standard_normal(S.G.n)
+    path = results.save_currents(str(tmp_path / 'hollow'), S, xs, 1e9)
+    m = sio.loadmat(path)
+    refh, hmin, hmax = post.curr_coefs_on_grid(xs, post.susceptibility(sgeo['sigma'], 1e9))
+    for name, r in zip(results.GRID_NAMES, refh):
+        assert m[name].shape == (9, 6, 5) and np.array_equal(m[name], r), name
+    assert m['x'].shape == (S.G.n, 1) and m['Mc'].shape == (9, 6, 5) and m['r'].shape == (9, 6, 5, 3)
+    assert ulp2(float(m['cmin'][0, 0]), hmin) and ulp2(float(m['cmax'][0, 0]), hmax)

@@ -156,6 +156,29 @@ class Geometry:
         assert z.shape == (self.nv,)
         check(lib().vh_geom_set_impedance(self.h, ptr(z)))
 
+    # -- post-processing --------------------------------------------------------
+    def currents_on_grid(self, x):
+        """Current coefficients on the voxel grid, post_obtain_curr_coefs_on_grid.m:1-45,
+        computed on the GPU.
+
+        x: solution(s) of shape (n,) or (nrhs, n) (only the first 5*nv entries,
+        Jx, Jy, Jz, J2d, J3d per voxel, are read), numpy complex128.  Returns
+        (grids, cmin, cmax): grids has shape (5, L, M, N) (or (nrhs, 5, L, M, N)),
+        component c of voxel (i, j, k) at grids[c, i, j, k], zero at empty voxels;
+        cmin / cmax = min / max of |J| over Jx, Jy, Jz (:24-25).
+        """
+        xx = np.ascontiguousarray(np.atleast_2d(x), dtype=np.complex128)
+        nrhs, ld = xx.shape
+        L, M, N = self.dims
+        ncell = L * M * N
+        g = np.zeros((nrhs, 5 * ncell), dtype=np.complex128)
+        mm = np.zeros(2 * nrhs)
+        check(lib().vh_geom_currents_on_grid(self.h, nrhs, ptr(xx), ld, ptr(g), 5 * ncell, ptr(mm), 0))
+        grids = g.reshape(nrhs, 5, N, M, L).transpose(0, 1, 4, 3, 2)     # column-major cells -> [x][y][z]
+        if np.ndim(x) == 1:
+            return grids[0], mm[0], mm[1]
+        return grids, mm[0::2], mm[1::2]
+
     # -- matvec ---------------------------------------------------------------
     def matvec(self, x, jwmu, prec=VH_C128, mode=0):
         """y = A x for a host (numpy) or device (torch) vector/stack of vectors.

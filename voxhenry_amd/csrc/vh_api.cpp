@@ -488,6 +488,21 @@ int vh_precon_schur_multiply(vh_precon* p, int64_t nrhs, const void* x, int64_t 
   });
 }
 
+int vh_geom_currents_on_grid(vh_geom* g, int64_t nrhs, const void* x, int64_t ldx, void* grid, int64_t ldg,
+                             double* minmax, int on_device) {
+  return guard([&] {
+    need(g, "g"); need(x, "x"); need(grid, "grid");
+    auto& G = g->G;
+    hipStream_t st = G.ctx->stream;
+    Io io;
+    io.in(x, nrhs, ldx, 16, on_device, st, ldg);
+    void* dg = on_device ? grid : io.y;
+    vh::currents_on_grid(G, (int)nrhs, (const double2*)io.x, ldx, (double2*)dg, ldg, minmax);
+    if (!on_device) d2h(grid, io.y, (size_t)nrhs * ldg * 16, st);
+    VH_HIP(hipStreamSynchronize(st));
+  });
+}
+
 int vh_gmres(vh_geom* g, vh_precon* p, double jre, double jim, int prec, int flexible, int64_t nrhs, const void* b,
              void* x, int on_device, int restart, double tol, int maxit, int* flag, double* relres, int* iter,
              int* niters, double* resvec, int64_t resvec_cap, int* nresvec) {

@@ -126,6 +126,12 @@ void geometry_set_impedance(Geometry& G, const double* z_interleaved);
 void geometry_build_topology(Geometry& G, const std::vector<std::vector<int64_t>>& Pfaces,
                              const std::vector<std::vector<int64_t>>& Nfaces, const std::vector<int64_t>& well,
                              std::vector<double>& rhs_out);
+// Current coefficients of nrhs solutions x (device, stride ldx) on the voxel
+// grid: grid (device, stride ldg >= 5*L*M*N) receives Jx, Jy, Jz, J2d, J3d as
+// five column-major L x M x N blocks, zero at empty voxels; minmax (host,
+// optional) the min and max of |J| over the Jx, Jy, Jz entries per solution.
+void currents_on_grid(Geometry& G, int nrhs, const double2* x, int64_t ldx, double2* grid, int64_t ldg,
+                      double* minmax);
 // Number of compiled pass-Z shapes of a precision (prec 0 = FP64, 1 = FP32).
 int matvec_num_zshapes(int prec);
 void geometry_set_spectrum_full(Geometry& G, const double* fN, int64_t P0, int64_t P1, int64_t P2,
