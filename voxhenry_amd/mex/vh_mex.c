@@ -188,13 +188,29 @@ static void make_geom(int64_t L, int64_t M, int64_t N, double dx, int64_t nv, co
   free(S.idx);
   S.idx = NULL;
   S.has_inc = S.has_z = S.prepared = S.has_jw = 0;
+  /* the geometry is published only once it is complete: a failure leaves no geometry */
+  int64_t* keep = (int64_t*)malloc((size_t)nv * sizeof(int64_t));
+  if (!keep) fail("voxhenry:lib", "out of host memory");
+  memcpy(keep, idx, (size_t)nv * sizeof(int64_t));
   vh_geom* g = NULL;
-  lib(vh_geom_create(S.ctx, L, M, N, dx, nv, idx, 1, &g));
+  int rc = vh_geom_create(S.ctx, L, M, N, dx, nv, idx, 1, &g);
+  if (rc == 0) {
+    rc = vh_geom_generate_green(g, S.st2);
+    if (rc != 0) {
+      snprintf(msgbuf, sizeof msgbuf, "%s", vh_last_error());
+      vh_geom_destroy(g);
+    }
+  } else {
+    snprintf(msgbuf, sizeof msgbuf, "%s", vh_last_error());
+  }
+  if (rc != 0) {
+    free(keep);
+    char m[sizeof msgbuf];
+    memcpy(m, msgbuf, sizeof m);
+    fail("voxhenry:lib", "libvoxhenry error %d: %s", rc, m);
+  }
   S.geom = g;
-  lib(vh_geom_generate_green(g, S.st2));
-  S.idx = (int64_t*)malloc((size_t)nv * sizeof(int64_t));
-  if (!S.idx) fail("voxhenry:lib", "out of host memory");
-  memcpy(S.idx, idx, (size_t)nv * sizeof(int64_t));
+  S.idx = keep;
   S.dims[0] = L;
   S.dims[1] = M;
   S.dims[2] = N;
