@@ -225,3 +225,42 @@ def test_matvec_every_zshape(ctx, dims):
         best, ms = G.autotune(prec, nrhs=2, reps=1)
         assert 0 <= best < nsh and len(ms) == nsh and np.all(ms > 0)
     G.close()
+
+
+@pytest.mark.parametrize('prec', ['c128', 'c64'])
+def test_matvec_line_runs(ctx, prec):
+    """x lines of every kind the X passes distinguish: one run starting past
+    x = 0, one run ending before x = L, a full line, two runs, a single voxel
+    at the far end, empty lines (run descriptors and the voxel-map path)."""
+    import voxhenry_amd as vh
+    say('line runs', prec)
+    L, M, N = 20, 6, 3
+    geo = problems.bar(L, M, N)
+    s0 = geo['sigma'].copy()
+    geo['sigma'][:] = 0.0
+    geo['sigma'][4:15, 0, 0] = s0[4:15, 0, 0]          # one run inside the line
+    geo['sigma'][:, 1, 0] = s0[:, 1, 0]                # full line
+    geo['sigma'][0:5, 2, 0] = s0[0:5, 2, 0]            # two runs
+    geo['sigma'][8:, 2, 0] = s0[8:, 2, 0]
+    geo['sigma'][19, 3, 1] = s0[19, 3, 1]              # one voxel at x = L
+    geo['sigma'][:12, 4, 2] = s0[:12, 4, 2]            # run from x = 0
+    geo['sigma'][3:, 5, 2] = s0[3:, 5, 2]              # run to x = L
+    geo['ports_P'] = [np.array([[20, 2, 1, 2]])]
+    geo['ports_N'] = [np.array([[1, 2, 1, 1]])]
+    s = topology.build_system(geo)
+    f = 1e9
+    jwmu = 1j * 2 * np.pi * f * MU0
+    fN2, _ = green.circulant_tensor(L, M, N, geo['dx'])
+    G = problems.device_geometry(ctx, geo, s, pad_mode=1, f=f)
+    zr, zi = topology.impedance(geo, s['idxS'], f)
+    rng = np.random.default_rng(21)
+    n = s['num_curr'] + s['num_node']
+    X = rng.standard_normal((2, n)) + 1j * rng.standard_normal((2, n))
+    if prec == 'c128':
+        Y, tol = G.matvec(X, jwmu, vh.VH_C128, 0), TOL64
+    else:
+        Y, tol = G.matvec(X.astype(np.complex64), jwmu, vh.VH_C64, 0), TOL32
+    for k in range(2):
+        ref = omv.lse_matvect_mult(X[k], jwmu * fN2, s['Ar'], zr, zi, geo['dx'], s['idxS5'])
+        assert rel(Y[k].astype(complex), ref) < tol, (k, rel(Y[k].astype(complex), ref))
+    G.close()

@@ -52,6 +52,9 @@ struct Geometry {
   std::vector<int64_t> vox;           // host copy: 0-based column-major linear index per voxel
   DevBuf<int> vmap;                   // L*M*N -> voxel id or -1
   DevBuf<int> lines;                  // non-empty (y + M z) lines
+  // per non-empty line: {x0, x1, vid(x0) - x0, 0} when its voxels are the one
+  // run x0 <= x < x1 (voxel id = x + vid(x0) - x0), else {-1, 0, 0, 0}
+  DevBuf<int4> runs;
   int nlines = 0;
   FftAxis ax, ay, az;                 // Green's tensor transforms
   FftAxis2 fx, fy, fz;                // matvec passes

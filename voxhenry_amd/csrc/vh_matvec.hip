@@ -49,7 +49,8 @@ static inline int lpad_host(int i) { return i + (i >> 4); }
 template <typename R, int E, int NT>
 __global__ __launch_bounds__(NT) void k_pass_x_fwd(
     const cplx<R>* __restrict__ x, int64_t ldx, const int* __restrict__ vmap, const int* __restrict__ lines,
-    int nlines, int ltpp, cplx<R>* __restrict__ A, GridDims g, Plan2 pl, const int* __restrict__ slots, int dense) {
+    const int4* __restrict__ runs, int nlines, int ltpp, cplx<R>* __restrict__ A, GridDims g, Plan2 pl,
+    const int* __restrict__ slots, int dense) {
   using C = cplx<R>;
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x;
@@ -65,12 +66,15 @@ __global__ __launch_bounds__(NT) void k_pass_x_fwd(
   const C* xs = x + (int64_t)slot * ldx + (int64_t)comp * g.nv;
   const int* vm = vmap + (int64_t)L * line;
   const int64_t lbase = (int64_t)L * line;
+  // a line whose voxels are one run needs no voxel-map lookup (no dependent load)
+  const int4 run = valid && !dense ? runs[l] : make_int4(-1, 0, 0, 0);
   C* lds = (C*)smem + t * (lpad(Px - 1) + 1);
   auto lx = [](int i) { return lpad(i); };
   auto gld = [&](int i) -> C {
     C v = czero<R>();
     if (valid && i < L) {
       if (dense) v = xs[lbase + i];
+      else if (run.x >= 0) { if (i >= run.x && i < run.y) v = xs[run.z + i]; }
       else { const int vid = vm[i]; if (vid >= 0) v = xs[vid]; }
     }
     return v;
@@ -597,8 +601,8 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && HALF) ? 3 : 1) void k_pass_z
 template <typename R, int E, int NT>
 __global__ __launch_bounds__(NT) void k_pass_x_inv(
     const cplx<R>* __restrict__ B, const cplx<R>* __restrict__ x, int64_t ldx, cplx<R>* __restrict__ y, int64_t ldy,
-    const int* __restrict__ vmap, const int* __restrict__ lines, int nlines, int ltpp, GridDims g, Plan2 pl,
-    const cplx<R>* __restrict__ z, R inv_dx, const int* __restrict__ atp, const int* __restrict__ atc,
+    const int* __restrict__ vmap, const int* __restrict__ lines, const int4* __restrict__ runs, int nlines, int ltpp,
+    GridDims g, Plan2 pl, const cplx<R>* __restrict__ z, R inv_dx, const int* __restrict__ atp, const int* __restrict__ atc,
     const R* __restrict__ atv, int64_t num_curr, int mode, const int* __restrict__ slots, int dense,
     const cplx<R>* const* __restrict__ zslot, cplx<R> alpha0, const double2* __restrict__ alphas) {
   // mode 0: conv + diag - Ar'phi ; mode 1: conv + diag ; mode 2: conv only.
@@ -633,9 +637,13 @@ __global__ __launch_bounds__(NT) void k_pass_x_inv(
   }
   const int* vm = vmap + (int64_t)L * line;
   const int64_t lbase = (int64_t)L * line;
+  const int4 run = valid && !dense ? runs[l] : make_int4(-1, 0, 0, 0);
   auto gst = [&](int o, C acc) {
     if (!valid || o >= L) return;
-    const int vid = dense ? (int)(lbase + o) : vm[o];
+    int vid;
+    if (dense) vid = (int)(lbase + o);
+    else if (run.x >= 0) vid = (o >= run.x && o < run.y) ? run.z + o : -1;
+    else vid = vm[o];
     if (vid < 0) return;
     const int64_t row = (int64_t)comp * g.nv + vid;
     acc = cmul(alpha, acc);
@@ -906,7 +914,7 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
   if (G.nlines > 0) {
     dim3 grid(ceil_div(G.nlines, Tx), nrhs * 5);
     hipLaunchKernelGGL((k_pass_x_fwd<R, EXY, NT_XY>), grid, dim3(NT_XY), lds_x, st, x, ldx, G.vmap.p, G.lines.p,
-                       G.nlines, log2i(tx), A, g, px, slots, dense);
+                       G.runs.p, G.nlines, log2i(tx), A, g, px, slots, dense);
     VH_LAUNCHED();
   }
   mark(1);
@@ -956,7 +964,7 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
     alpha.x = (R)are;
     alpha.y = (R)aim;
     hipLaunchKernelGGL((k_pass_x_inv<R, EXY, NT_XY>), grid, dim3(NT_XY), lds_x, st, B, x, ldx, y, ldy, G.vmap.p,
-                       G.lines.p, G.nlines, log2i(tx), g, px, zz, (R)(1.0 / G.dx), G.ArT.ptr.p, G.ArT.col.p, atv,
+                       G.lines.p, G.runs.p, G.nlines, log2i(tx), g, px, zz, (R)(1.0 / G.dx), G.ArT.ptr.p, G.ArT.col.p, atv,
                        G.num_curr, mode, slots, dense, zslot, alpha, alphas);
     VH_LAUNCHED();
   }

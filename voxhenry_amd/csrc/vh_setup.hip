@@ -173,6 +173,19 @@ void geometry_init(Geometry& G, Context* ctx, int L, int M, int N, double dx, in
     if (any) lines.push_back((int)line);
   }
   G.nlines = (int)lines.size();
+  std::vector<int4> runs(lines.size());
+  for (size_t l = 0; l < lines.size(); ++l) {
+    const int* vm = vmap.data() + (int64_t)L * lines[l];
+    int x0 = 0;
+    while (vm[x0] < 0) ++x0;
+    int x1 = x0;
+    while (x1 < L && vm[x1] >= 0) ++x1;
+    bool one = true;
+    for (int i = x1; i < L && one; ++i) one = vm[i] < 0;
+    runs[l] = one ? make_int4(x0, x1, vm[x0] - x0, 0) : make_int4(-1, 0, 0, 0);
+  }
+  G.runs.alloc(std::max<size_t>(runs.size(), 1));
+  if (!runs.empty()) VH_HIP(hipMemcpy(G.runs.p, runs.data(), runs.size() * sizeof(int4), hipMemcpyHostToDevice));
   G.vmap.alloc(V);
   VH_HIP(hipMemcpy(G.vmap.p, vmap.data(), V * sizeof(int), hipMemcpyHostToDevice));
   G.lines.alloc(std::max<size_t>(lines.size(), 1));
