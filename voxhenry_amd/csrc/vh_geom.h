@@ -55,6 +55,10 @@ struct Geometry {
   // per non-empty line: {x0, x1, vid(x0) - x0, 0} when its voxels are the one
   // run x0 <= x < x1 (voxel id = x + vid(x0) - x0), else {-1, 0, 0, 0}
   DevBuf<int4> runs;
+  // per z plane, 4 words: bit y set when the x line (y, z) holds voxels; used
+  // when M <= 256 and some line is empty (row_bits_on)
+  DevBuf<unsigned long long> rowbits;
+  bool row_bits_on = false;
   int nlines = 0;
   FftAxis ax, ay, az;                 // Green's tensor transforms
   FftAxis2 fx, fy, fz;                // matvec passes

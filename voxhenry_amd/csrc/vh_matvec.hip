@@ -89,11 +89,17 @@ __global__ __launch_bounds__(NT) void k_pass_x_fwd(
 // consecutive kx of one z plane of one (rhs, comp); consecutive threads take
 // consecutive kx, so every load and store moves T contiguous elements.  in and
 // out may alias (the inverse works in place): all loads of a workgroup precede
-// its first store.
+// its first store.  rowbits (optional, M <= 256: 4 words per z plane) marks
+// the y rows whose x line holds voxels: the inverse writes only those (pass Xi
+// reads no other row).  The plane is uniform across the workgroup, so its
+// words are loaded once and the per-row test is arithmetic only.  (Skipping
+// the forward pass's reads of the empty, zero rows the same way measured
+// slower: that pass is bound by instruction issue, not by its loads.)
 // ---------------------------------------------------------------------------
 template <typename R, int DIR, int E, int NT>
 __global__ __launch_bounds__(NT) void k_pass_y(const cplx<R>* in, int in_rows, int nin, cplx<R>* out, int out_rows,
-                                               int nout, int ltpp, GridDims g, Plan2 pl) {
+                                               int nout, int ltpp, GridDims g, Plan2 pl,
+                                               const unsigned long long* __restrict__ rowbits) {
   using C = cplx<R>;
   extern __shared__ __align__(16) unsigned char smem[];
   const int tid = threadIdx.x;
@@ -110,9 +116,18 @@ __global__ __launch_bounds__(NT) void k_pass_y(const cplx<R>* in, int in_rows, i
   const C* is = in + ((int64_t)rc * g.N + z) * in_rows * Px + kx;
   C* os = out + ((int64_t)rc * g.N + z) * out_rows * Px + kx;
   C* lds = (C*)smem + t;
+  unsigned long long w0 = ~0ull, w1 = ~0ull, w2 = ~0ull, w3 = ~0ull;
+  if (rowbits) {
+    const unsigned long long* rb = rowbits + (int64_t)z * 4;
+    w0 = rb[0]; w1 = rb[1]; w2 = rb[2]; w3 = rb[3];
+  }
+  auto live = [&](int i) -> bool {
+    const unsigned long long w = i < 64 ? w0 : i < 128 ? w1 : i < 192 ? w2 : w3;
+    return (w >> (i & 63)) & 1ull;
+  };
   auto lx = [T](int i) { return i * T; };
   auto gld = [&](int i) -> C { return (valid && i < nin) ? is[i * Px] : czero<R>(); };
-  auto gst = [&](int o, C v) { if (valid && o < nout) os[o * Px] = v; };
+  auto gst = [&](int o, C v) { if (valid && o < nout && (DIR < 0 || live(o))) os[o * Px] = v; };
   fft_pencil<R, DIR, E>(pl, lds, lx, jl, tpp, gld, gst);
 }
 
@@ -922,7 +937,7 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
   {
     dim3 grid(ceil_div(g.Px, Ty) * g.N, nrhs * 5);
     hipLaunchKernelGGL((k_pass_y<R, -1, EXY, NT_XY>), grid, dim3(NT_XY), lds_y, st, A, g.M, g.M,
-                       B, g.Py, g.Py, log2i(ty), g, py);
+                       B, g.Py, g.Py, log2i(ty), g, py, nullptr);
     VH_LAUNCHED();
   }
   mark(2);
@@ -947,7 +962,7 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
   {
     dim3 grid(ceil_div(g.Px, Ty) * g.N, nrhs * 5);
     hipLaunchKernelGGL((k_pass_y<R, +1, EXY, NT_XY>), grid, dim3(NT_XY), lds_y, st, B, g.Py,
-                       g.Py, B, g.Py, g.M, log2i(ty), g, py);
+                       g.Py, B, g.Py, g.M, log2i(ty), g, py, G.row_bits_on ? G.rowbits.p : nullptr);
     VH_LAUNCHED();
   }
   mark(4);

@@ -184,6 +184,13 @@ void geometry_init(Geometry& G, Context* ctx, int L, int M, int N, double dx, in
     for (int i = x1; i < L && one; ++i) one = vm[i] < 0;
     runs[l] = one ? make_int4(x0, x1, vm[x0] - x0, 0) : make_int4(-1, 0, 0, 0);
   }
+  G.row_bits_on = M <= 256 && lines.size() < (size_t)M * N;
+  if (G.row_bits_on) {
+    std::vector<unsigned long long> bits((size_t)4 * N, 0ull);
+    for (int line : lines) bits[(size_t)4 * (line / M) + (line % M) / 64] |= 1ull << ((line % M) % 64);
+    G.rowbits.alloc(bits.size());
+    VH_HIP(hipMemcpy(G.rowbits.p, bits.data(), bits.size() * sizeof(unsigned long long), hipMemcpyHostToDevice));
+  }
   G.runs.alloc(std::max<size_t>(runs.size(), 1));
   if (!runs.empty()) VH_HIP(hipMemcpy(G.runs.p, runs.data(), runs.size() * sizeof(int4), hipMemcpyHostToDevice));
   G.vmap.alloc(V);
