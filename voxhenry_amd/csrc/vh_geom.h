@@ -29,6 +29,12 @@ struct DevCsr {
 
 // Per-pass timing: when Geometry::timer is set, matvec records an event before
 // and after every pass (X, Y, Z, Yi, Xi, node rows) and accumulates the times.
+// Layout of the packed Ar' rows (per current component: offset, entries per current).
+struct PackedInc {
+  int off[5];
+  int k[5];
+};
+
 struct PassTimer {
   static constexpr int NP = 6;
   hipEvent_t ev[NP + 1] = {};
@@ -75,6 +81,11 @@ struct Geometry {
   bool has_z = false;
   DevCsr Ar;    // num_node x num_curr
   DevCsr ArT;   // num_curr x num_node
+  // Ar' rows packed for pass Xi: component c holds pinc.k[c] entries per
+  // current from pinc.off[c], entry = node << 3 | code (code 0 none, 1 +1,
+  // 2 -1, 3 +1/2, 4 -1/2); empty when Ar does not fit that form
+  DevBuf<int> atpk;
+  PackedInc pinc{};
   // host copies used by the preconditioner set-up
   std::vector<int> arp_h, arc_h, atp_h, atc_h;
   std::vector<double> arv_h, atv_h;

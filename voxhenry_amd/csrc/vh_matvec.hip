@@ -613,12 +613,13 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && HALF) ? 3 : 1) void k_pass_z
 // ---------------------------------------------------------------------------
 // pass Xi: inverse FFT along x, crop, diagonal, -Ar' phi, scatter.
 // ---------------------------------------------------------------------------
-template <typename R, int E, int NT>
+template <typename R, int E, int NT, int mode>
 __global__ __launch_bounds__(NT) void k_pass_x_inv(
     const cplx<R>* __restrict__ B, const cplx<R>* __restrict__ x, int64_t ldx, cplx<R>* __restrict__ y, int64_t ldy,
     const int* __restrict__ vmap, const int* __restrict__ lines, const int4* __restrict__ runs, int nlines, int ltpp,
     GridDims g, Plan2 pl, const cplx<R>* __restrict__ z, R inv_dx, const int* __restrict__ atp, const int* __restrict__ atc,
-    const R* __restrict__ atv, int64_t num_curr, int mode, const int* __restrict__ slots, int dense,
+    const R* __restrict__ atv, const int* __restrict__ atpk, PackedInc pinc, int64_t num_curr,
+    const int* __restrict__ slots, int dense,
     const cplx<R>* const* __restrict__ zslot, cplx<R> alpha0, const double2* __restrict__ alphas) {
   // mode 0: conv + diag - Ar'phi ; mode 1: conv + diag ; mode 2: conv only.
   // conv = j*omega*mu (alpha, per system or one for all) times the cropped
@@ -662,10 +663,28 @@ __global__ __launch_bounds__(NT) void k_pass_x_inv(
     if (vid < 0) return;
     const int64_t row = (int64_t)comp * g.nv + vid;
     acc = cmul(alpha, acc);
-    if (mode <= 1) acc = cadd(acc, cscale(cmul(zv[vid], xs[row]), w));
-    if (mode == 0) {
+    if constexpr (mode <= 1) acc = cadd(acc, cscale(cmul(zv[vid], xs[row]), w));
+    if constexpr (mode == 0) {
       // a current touches at most 2 nodes: both potentials loaded at once
       C s = czero<R>();
+      if (atpk) {
+        // packed row: nodes and coefficients without the row-pointer chain;
+        // the component (so the entry count) is uniform across the workgroup
+        const int K = pinc.k[comp];
+        const int2* e2 = (const int2*)(atpk + pinc.off[comp] + (int64_t)vid * K);
+        auto coef = [](int c) { return c == 1 ? R(1) : c == 2 ? R(-1) : c == 3 ? R(0.5) : R(-0.5); };
+        int2 e[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) e[q] = 2 * q < K ? e2[q] : make_int2(0, 0);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const int ca = e[q].x & 7, cb = e[q].y & 7;
+          if (ca) { const C ph = xs[num_curr + (e[q].x >> 3)]; const R v = coef(ca); s.x += v * ph.x; s.y += v * ph.y; }
+          if (cb) { const C ph = xs[num_curr + (e[q].y >> 3)]; const R v = coef(cb); s.x += v * ph.x; s.y += v * ph.y; }
+        }
+        ys[row] = csub(acc, s);
+        return;
+      }
       const int k0 = atp[row], k1 = atp[row + 1];
       if (k1 - k0 <= 2) {
         const int n0 = k0 < k1 ? atc[k0] : 0, n1 = k0 + 1 < k1 ? atc[k0 + 1] : 0;
@@ -978,9 +997,16 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
     cplx<R> alpha;
     alpha.x = (R)are;
     alpha.y = (R)aim;
-    hipLaunchKernelGGL((k_pass_x_inv<R, EXY, NT_XY>), grid, dim3(NT_XY), lds_x, st, B, x, ldx, y, ldy, G.vmap.p,
-                       G.lines.p, G.runs.p, G.nlines, log2i(tx), g, px, zz, (R)(1.0 / G.dx), G.ArT.ptr.p, G.ArT.col.p, atv,
-                       G.num_curr, mode, slots, dense, zslot, alpha, alphas);
+    // the output mode is a template parameter: each variant is compiled for its own work
+    auto xi = [&](auto mc) {
+      constexpr int MODE = decltype(mc)::value;
+      hipLaunchKernelGGL((k_pass_x_inv<R, EXY, NT_XY, MODE>), grid, dim3(NT_XY), lds_x, st, B, x, ldx, y, ldy,
+                         G.vmap.p, G.lines.p, G.runs.p, G.nlines, log2i(tx), g, px, zz, (R)(1.0 / G.dx), G.ArT.ptr.p,
+                         G.ArT.col.p, atv, G.atpk.p, G.pinc, G.num_curr, slots, dense, zslot, alpha, alphas);
+    };
+    if (mode == 0) xi(std::integral_constant<int, 0>{});
+    else if (mode == 1) xi(std::integral_constant<int, 1>{});
+    else xi(std::integral_constant<int, 2>{});
     VH_LAUNCHED();
   }
   mark(5);

@@ -244,6 +244,47 @@ void geometry_set_incidence_csc(Geometry& G, int64_t num_node, int64_t nnz, cons
   G.num_node = num_node;
   upload_csr(G.Ar, (int)num_node, (int)nc, rp, rc, rv);
   upload_csr(G.ArT, (int)nc, (int)num_node, tp, tc, tv);
+  // Packed rows of Ar' (pass Xi): the currents of one component touch a fixed
+  // small number of nodes (2 for Jx, Jy, Jz, up to 4 for J2d, 6 for J3d) with
+  // coefficients +-1 or +-1/2 (lse_compute_Ae_matrix.m), so component c keeps
+  // k[c] (even) entries per current, entry = node << 3 | code (code 0 none,
+  // 1 +1, 2 -1, 3 +1/2, 4 -1/2), and a row is read without the row-pointer
+  // chain.  Any other matrix keeps the CSR path.
+  auto code_of = [](double v) { return v == 1.0 ? 1 : v == -1.0 ? 2 : v == 0.5 ? 3 : v == -0.5 ? 4 : 0; };
+  const int64_t nvx = G.g.nv;
+  bool packable = num_node < (1LL << 28) && nc == 5 * nvx;
+  PackedInc pinc{};
+  int64_t total = 0;
+  for (int c = 0; c < 5 && packable; ++c) {
+    int kmax = 0;
+    for (int64_t v = 0; v < nvx; ++v) {
+      const int64_t j = c * nvx + v;
+      kmax = std::max<int>(kmax, (int)(colptr[j + 1] - colptr[j]));
+    }
+    kmax = std::max(2, (kmax + 1) & ~1);
+    if (kmax > 6) packable = false;
+    pinc.k[c] = kmax;
+    pinc.off[c] = (int)total;
+    total += (int64_t)kmax * nvx;
+  }
+  packable = packable && total < (1LL << 31);
+  std::vector<int> pk(packable ? std::max<int64_t>(total, 2) : 0, 0);
+  for (int c = 0; c < 5 && packable; ++c)
+    for (int64_t v = 0; v < nvx && packable; ++v) {
+      const int64_t j = c * nvx + v;
+      int* e = pk.data() + pinc.off[c] + v * pinc.k[c];
+      for (int64_t k = colptr[j]; k < colptr[j + 1]; ++k) {
+        const int code = code_of(val[k]);
+        if (code == 0) { packable = false; break; }
+        e[k - colptr[j]] = (int)(rowind[k] << 3) | code;
+      }
+    }
+  G.atpk.release();
+  G.pinc = pinc;
+  if (packable) {
+    G.atpk.alloc(pk.size());
+    VH_HIP(hipMemcpy(G.atpk.p, pk.data(), pk.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
   G.arp_h = std::move(rp); G.arc_h = std::move(rc); G.arv_h = std::move(rv);
   G.atp_h = std::move(tp); G.atc_h = std::move(tc); G.atv_h = std::move(tv);
   G.has_inc = true;
