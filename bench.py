@@ -440,9 +440,13 @@ def main():
                                                           for k in range(len(freqs)))),
                     L_self_nH_at_fmin=float(d64[0].imag.mean() / w[0, 0] * 1e9),
                     iters_mean=float(st64['iters'].mean()), iters_max=int(st64['iters'].max()),
-                    relres_max=float(st64['relres'].max()), all_converged=bool(np.all(st64['flag'] == 0)))
+                    relres_max=float(st64['relres'].max()),
+                    # the reference-style stopping residual ||M r|| / ||M b|| (pgmres.m:126)
+                    precond_resid_max=float(st64['resid_est'].max()),
+                    all_converged=bool(np.all(st64['flag'] == 0)))
     mixed = dict(solves_per_s=nsol / ms_mixed, s_per_step=ms_mixed, outer_iters_mean=float(stm['iters'].mean()),
                  inner_iters_mean=float(stm['inner_iters'].mean()), relres_max=float(stm['relres'].max()),
+                 outer_resid_est_max=float(stm['resid_est'].max()),
                  all_converged=bool(np.all(stm['flag'] == 0)))
 
     z = mv['c64']['passes']['Z']

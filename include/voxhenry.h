@@ -213,13 +213,17 @@ int vh_solve_mixed(vh_geom* g, vh_precon* p, double jwmu_re, double jwmu_im, int
  * is processed in chunks.  tols (nullable, host) gives system s its own
  * tolerance tols[s] in place of tol: a sweep then follows the frequency rule
  * of VoxHenry_executer.m:219-223 (1e-12 below 1 MHz, 1e-8 above) in one
- * batch.  Outputs per system: flag, relres, iters (outer Arnoldi steps),
- * inner_iters (FP32 steps, method 1). */
+ * batch.  Outputs per system (each nullable): flag, relres (true residual
+ * ||b - A x|| / ||b|| of the returned x), iters (outer Arnoldi steps),
+ * inner_iters (FP32 steps, method 1), resid_est (the solver's own residual
+ * estimate at exit: method 0 the reference-style preconditioned
+ * ||M r|| / ||M b|| that pgmres.m:126 stops on, method 1 the outer FGMRES
+ * estimate of ||r|| / ||b||). */
 int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64_t nsys, const double* jwmu,
                    const int* pidx, const double* b, double* x, int on_device, int method, double tol,
                    const double* tols, int restart,
                    int maxit, int inner_restart, double inner_tol, double hbm_budget, int* flag, double* relres,
-                   int* iters, int* inner_iters);
+                   int* iters, int* inner_iters, double* resid_est);
 
 /* Test hook: in-place 3-D FFT (dir -1 forward, +1 inverse unnormalised) of ncomp
  * device grids of the geometry's padded size. */

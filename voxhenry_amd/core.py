@@ -306,12 +306,14 @@ class Geometry:
         relres = np.zeros(nsys)
         its = np.zeros(nsys, np.int32)
         iits = np.zeros(nsys, np.int32)
+        rest = np.zeros(nsys)
         check(lib().vh_solve_batch(self.h, len(precons), C.cast(hs, C.c_void_p), nsys, ptr(jw), ptr(pi), ptr(BB),
                                    ptr(X), int(on_dev), m, float(tol), ptr(tols) if tols is not None else None,
                                    restart, maxit, inner_restart, inner_tol,
                                    float(hbm_budget),
-                                   ptr(flag), ptr(relres), ptr(its), ptr(iits)))
-        return X, dict(flag=flag, relres=relres, iters=its, inner_iters=iits)
+                                   ptr(flag), ptr(relres), ptr(its), ptr(iits), ptr(rest)))
+        # resid_est: fp64 the reference-style preconditioned residual (pgmres.m:126), mixed the outer estimate
+        return X, dict(flag=flag, relres=relres, iters=its, inner_iters=iits, resid_est=rest)
 
     def autotune(self, prec, nrhs=1, reps=3):
         """Time every compiled pass-Z shape on nrhs vectors of this geometry

@@ -655,7 +655,7 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
                    const int* pidx, const double* b, double* x, int on_device, int method, double tol,
                    const double* tols, int restart,
                    int maxit, int inner_restart, double inner_tol, double hbm_budget, int* flag, double* relres,
-                   int* iters, int* inner_iters) {
+                   int* iters, int* inner_iters, double* resid_est) {
   return guard([&] {
     need(g, "g"); need(precons, "precons"); need(jwmu, "jwmu"); need(pidx, "pidx"); need(b, "b"); need(x, "x");
     auto& G = g->G;
@@ -770,6 +770,7 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
       if (relres) relres[s] = stats[s].relres;
       if (iters) iters[s] = stats[s].iters;
       if (inner_iters) inner_iters[s] = stats[s].inner_iters;
+      if (resid_est) resid_est[s] = stats[s].resvec.empty() ? 1.0 : stats[s].resvec.back();
     }
   });
 }
