@@ -23,7 +23,8 @@ launch stream.  `roofline` reports its dominant kernel (pass Z: z-FFT, Green's
 product, inverse z-FFT).
 
 The CPU baseline is the FP64 numpy/scipy oracle (oracle/) timed on a bounded
-sample of the same bus workload on this host.
+sample of the same bus workload on this host; cpu_baseline_matvec times the
+oracle's FFT-block matvec on the 128^3 cube next to the GPU figure.
 """
 import argparse
 import json
@@ -364,6 +365,37 @@ def cpu_model():
     return 'unknown'
 
 
+def cpu_matvec_baseline(n=128, reps=2):
+    """Oracle FFT block (lse_matvect_mult.m:57-93: 5 padded forward FFTs, the
+    5x5 spectral product, 5 inverse FFTs, crop) on the n^3 cube, timed on the
+    host's cores (at most 16).  The tensor and currents are random of the
+    right shapes (the timing does not depend on their values; generating the
+    real tensor on the CPU takes minutes).  Reported as matvecs/s and as GB/s
+    under the same byte model as the GPU figure."""
+    from oracle import matvec as omv
+    rng = np.random.default_rng(0)
+    P = (2 * n, 2 * n, 2 * n)
+    fN = np.empty(P + (7,), np.complex128)
+    fN.real = rng.standard_normal(fN.shape)
+    fN.imag = rng.standard_normal(fN.shape)
+    J = rng.standard_normal((n, n, n, 5)) + 1j * rng.standard_normal((n, n, n, 5))
+    cores = max(1, min(16, os.cpu_count() or 1))
+    omv.FFT_WORKERS = cores
+    try:
+        omv.fft_block(J, fN, (n, n, n))                      # warm-up (FFT plans)
+        t = time.perf_counter()
+        for _ in range(reps):
+            omv.fft_block(J, fN, (n, n, n))
+        dt = (time.perf_counter() - t) / reps
+    finally:
+        omv.FFT_WORKERS = 1
+    V = n ** 3
+    gb = b_fft(V, V, 16, 7 * (2 * n) ** 3 * 16) / 1e9
+    return dict(value=1.0 / dt, unit='matvecs/s', cores=cores, kind='port', GBs=gb / dt, cpu=cpu_model(),
+                sample='%d FFT-block matvecs (complex128) of the %d^3 cube, FFTs on %d threads, spectral '
+                       'product in numpy; random tensor and currents of the real shapes' % (reps, n, cores))
+
+
 def cpu_baseline(nsolves):
     """Oracle (FP64 numpy/scipy restatement of the reference's algorithm) on a
     bounded sample of the bus workload: set-up untimed, then `nsolves`
@@ -477,10 +509,12 @@ def main():
             log('[bench] uneven batch, %d geometries x %d frequencies' % (args.uneven_count, args.uneven_nfreq))
             configs['uneven_batch'] = uneven_config(ctx, args.uneven_count, args.uneven_nfreq)
 
-    cpu = None
+    cpu = cpu_mv = None
     if not args.no_cpu:
         log('[bench] CPU baseline (oracle)')
         cpu = cpu_baseline(args.cpu_solves)
+        log('[bench] CPU baseline of the 128^3 FFT-block matvec (oracle)')
+        cpu_mv = cpu_matvec_baseline(128, 2)
 
     line = dict(metric=METRIC, value=mixed['solves_per_s'], unit='solves/s', n_gpus=1, steps=args.steps,
                 warmup=args.warmup, ms_per_step=ms_mixed * 1e3, higher_is_better=True, scaling='weak',
@@ -491,7 +525,7 @@ def main():
                             parallelism='1 GPU'),
                 mixed=mixed, fp64=fp64,
                 fp64_speedup=(mixed['solves_per_s'] / fp64['solves_per_s']) if fp64 else None,
-                fft_matvec=mv, roofline=roof, cpu_baseline=cpu, configs=configs,
+                fft_matvec=mv, roofline=roof, cpu_baseline=cpu, cpu_baseline_matvec=cpu_mv, configs=configs,
                 setup_s=t_setup)
     print(json.dumps(line))
 
