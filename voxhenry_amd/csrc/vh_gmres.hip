@@ -168,6 +168,17 @@ __global__ void k_convert(const cplx<RS>* __restrict__ src, int64_t lds, cplx<RD
   dst[(int64_t)sd * ldd + i] = o;
 }
 
+template <typename R>
+__global__ void k_zero_slots(cplx<R>* __restrict__ x, int64_t ld, int64_t n, const int* __restrict__ slots) {
+  const int s = slots[blockIdx.y];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  cplx<R> z;
+  z.x = 0;
+  z.y = 0;
+  x[(int64_t)s * ld + i] = z;
+}
+
 // y[s] = x[s] - y[s]   (residual r = b - A x given y = A x)
 template <typename R>
 __global__ void k_sub_from(const cplx<R>* __restrict__ b, int64_t ldb, cplx<R>* __restrict__ y, int64_t ldy, int64_t n,
@@ -247,6 +258,13 @@ void VecOps<R>::scale(const cplx<R>* src, cplx<R>* dst, int nact, const int* slo
   if (nact <= 0) return;
   hipLaunchKernelGGL((k_scale<R>), dim3(ceil_div(n, 256), nact), dim3(256), 0, ctx->stream, src, dst, ld, n, slots,
                      nrm2);
+  VH_LAUNCHED();
+}
+
+template <typename R>
+void VecOps<R>::zero(cplx<R>* x, int nact, const int* slots) {
+  if (nact <= 0) return;
+  hipLaunchKernelGGL((k_zero_slots<R>), dim3(ceil_div(n, 256), nact), dim3(256), 0, ctx->stream, x, ld, n, slots);
   VH_LAUNCHED();
 }
 

@@ -20,6 +20,8 @@ template <typename R> struct VecOps {
   void norm2(const cplx<R>* x, int nact, const int* slots, double2* nrm2);
   void scale(const cplx<R>* src, cplx<R>* dst, int nact, const int* slots, const double2* nrm2);
   void sub_from(const cplx<R>* b, int64_t ldb, cplx<R>* y, int nact, const int* slots);
+  // x[slot] = 0 for the nact slots (one launch; no memset node in a captured graph)
+  void zero(cplx<R>* x, int nact, const int* slots);
 };
 
 template <typename RD, typename RS>

@@ -134,10 +134,8 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
 
   std::vector<int> act = sys;
   upload(act, W.slots_dev.p);
-  for (int s : sys) {
-    VH_HIP(hipMemsetAsync(x + (int64_t)s * n, 0, n * sizeof(C), strm));
-    st[s] = SysStats();
-  }
+  ops.zero(x, (int)act.size(), W.slots_dev.p);   // x0 = 0 for every system
+  for (int s : sys) st[s] = SysStats();
   // r = M b (pgmres.m:120-139) or b (fpgmres.m:91-95)
   if (left) {
     (*M)(act, W.slots_dev.p, b, W.r);
