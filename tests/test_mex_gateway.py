@@ -160,6 +160,7 @@ def test_usage_and_state_errors():
     expect('voxhenry:state', m, 'incidence', sp.eye(3, format='csc'))
     expect('voxhenry:usage', m, 'pgmres', np.ones(10), 100, -1.0, 10)  # tol <= 0 is a usage error first
     expect('voxhenry:state', m, 'pgmres', np.ones(10), 100, 1e-8, 10)
+    expect('voxhenry:state', m, 'solve_batch', np.ones((10, 1)), np.array([1j]), [1.0], 'fp64', 1e-8)
     # prepare: Ae must have 5*numel(idxS) columns; the type must be supported
     zr = np.ones((2, 2, 1))
     idx = np.array([1.0, 2.0, 3.0, 4.0])
@@ -284,3 +285,42 @@ def test_gateway_exit_handler_releases():
     assert m.exit() == 1            # MATLAB clearing the MEX: the handler releases the device objects
     expect('voxhenry:state', m, 'incidence', s['Ar'])
     assert m.leaked() == 0
+
+
+@pytest.mark.gpu
+def test_gateway_solve_batch():
+    """The batched tier from MATLAB: every (port, frequency) column in one
+    device solve, a preconditioner per frequency, the reference's tolerance
+    per column; Y of each column against the oracle's pgmres."""
+    m = matlab()
+    m('reset')
+    geo = problems.bar(12, 5, 4)
+    prob = driver.Problem(geo)
+    s = prob.sys
+    L, M, N = s['dims']
+    m('geom', L, M, N, geo['dx'], s['idxS'] + 1.0)
+    m('incidence', s['Ar'])
+    zr, zi = topology.impedance(geo, s['idxS'], 1e9)
+    m('impedance', zr, zi)
+    freqs = np.array([1e3, 1e8, 2.5e9])
+    jw = 1j * 2 * np.pi * freqs * MU0
+    b = s['rhs'][:, 0]
+    B = np.repeat(b[:, None], 3, axis=1)                   # real-typed, as the executer builds it
+    pidx = np.array([1.0, 2.0, 3.0])
+    tols = np.array([driver.reference_tol(f) for f in freqs])
+    expect('voxhenry:usage', m, 'solve_batch', B, jw, np.array([1.0, 2.0, 4.0]), 'fp64', tols)
+    expect('voxhenry:usage', m, 'solve_batch', B, jw, pidx, 'fp16', tols)
+    X, flag, relres, its, inner = m('solve_batch', B, jw, pidx, 'fp64', tols, nout=5)
+    assert X.shape == (len(b), 3) and flag.shape == (1, 3)
+    assert np.all(flag == 0) and np.all(relres[0] <= tols)
+    for k, f in enumerate(freqs):
+        A, Pm, *_ = prob.operators(f)
+        xo = ogm.pgmres(A, b, 100, tols[k], 10, Pm)[0]
+        Yg, Yo = b @ X[:, k], b @ xo
+        assert abs(Yg - Yo) / abs(Yo) < 1e-9, (f, abs(Yg - Yo) / abs(Yo))
+    Xm, flagm, relm = m('solve_batch', B, jw, pidx, 'mixed', 1e-8, nout=3)
+    assert np.all(flagm == 0) and np.all(relm[0] <= 1e-8)
+    for k in range(1, 3):
+        assert abs(b @ Xm[:, k] - b @ X[:, k]) / abs(b @ X[:, k]) < 1e-6
+    m('reset')
+    assert m.locked() == 0 and m.leaked() == 0

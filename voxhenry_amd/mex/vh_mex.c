@@ -12,6 +12,10 @@
  *   lse_sparse_schur_compl_multiply -> y = vh_mex('schur_multiply', v)
  *   pgmres / fpgmres            -> [x,flag,relres,iter,resvec] = vh_mex('pgmres' | 'fpgmres', b, restart, tol, maxit)
  *
+ * the batched tier (every port x frequency system in one device solve):
+ *
+ *   [X,flag,relres,iters,inner] = vh_mex('solve_batch', B, jwmu, pidx, 'fp64' | 'mixed', tol)
+ *
  * and the lower-level commands the prepare step is made of:
  *
  *   st2 = vh_mex('geom', L, M, N, dx, idxS)   grid, 1-based voxel indices; the
@@ -463,6 +467,100 @@ static void cmd_gmres(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]
   if (tmp) mxFree(tmp);
 }
 
+/* [X, flag, relres, iters, inner_iters] = vh_mex('solve_batch', B, jwmu, pidx, method, tol):
+ * the batched tier (vh_solve_batch): column s of B (n x nsys, real or complex)
+ * is solved at frequency jwmu(pidx(s)) (jwmu: the j*omega*mu of nf
+ * frequencies, pidx 1-based) with that frequency's preconditioner, prepared
+ * here for the geometry's current impedance; method 'fp64' (pgmres
+ * semantics, restart 100, 10 cycles) or 'mixed' (FP64 flexible outer, FP32
+ * inner); tol a scalar or one per column.  Outputs are 1 x nsys rows. */
+static void cmd_solve_batch(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
+  nargs(nrhs, 6, 6, nlhs, 5, "solve_batch");
+  need_operator();
+  if (!mxIsDouble(prhs[2]) || mxIsSparse(prhs[2]) || mxGetNumberOfElements(prhs[2]) < 1)
+    fail("voxhenry:usage", "jwmu must be a double array of one value per frequency");
+  const int64_t nf = (int64_t)mxGetNumberOfElements(prhs[2]);
+  if (!is_real_double(prhs[3])) fail("voxhenry:usage", "pidx must be a real double vector");
+  char mth[16];
+  if (!mxIsChar(prhs[4]) || mxGetString(prhs[4], mth, sizeof mth) != 0 || (strcmp(mth, "fp64") && strcmp(mth, "mixed")))
+    fail("voxhenry:usage", "method must be 'fp64' or 'mixed'");
+  const int method = strcmp(mth, "mixed") == 0;
+  int64_t ns = 0;
+  double* tmp = NULL;
+  const double* b = columns(prhs[1], S.n, "B", &ns, &tmp);
+  if ((int64_t)mxGetNumberOfElements(prhs[3]) != ns) fail("voxhenry:usage", "pidx must have one entry per column of B");
+  if (!is_real_double(prhs[5]) || (mxGetNumberOfElements(prhs[5]) != 1 && (int64_t)mxGetNumberOfElements(prhs[5]) != ns))
+    fail("voxhenry:usage", "tol must be a scalar or one value per column of B");
+  const double* tv = mxGetDoubles(prhs[5]);
+  const int per_col = mxGetNumberOfElements(prhs[5]) != 1;
+  for (int64_t s = 0; s < (per_col ? ns : 1); ++s)
+    if (!(tv[s] > 0)) fail("voxhenry:usage", "tol must be positive");
+  double* jw = (double*)mxMalloc((size_t)(2 * nf) * sizeof(double));
+  for (int64_t f = 0; f < nf; ++f) {
+    if (mxIsComplex(prhs[2])) {
+      jw[2 * f] = mxGetComplexDoubles(prhs[2])[f].real;
+      jw[2 * f + 1] = mxGetComplexDoubles(prhs[2])[f].imag;
+    } else {
+      jw[2 * f] = mxGetDoubles(prhs[2])[f];
+      jw[2 * f + 1] = 0.0;
+    }
+  }
+  int* pidx = (int*)mxMalloc((size_t)ns * sizeof(int));
+  double* jws = (double*)mxMalloc((size_t)(2 * ns) * sizeof(double));
+  const double* pd = mxGetDoubles(prhs[3]);
+  for (int64_t s = 0; s < ns; ++s) {
+    if (!(pd[s] >= 1 && pd[s] <= (double)nf && pd[s] == floor(pd[s])))
+      fail("voxhenry:usage", "pidx(%lld) = %g is not a frequency index in 1..%lld", (long long)(s + 1), pd[s], (long long)nf);
+    pidx[s] = (int)pd[s] - 1;
+    jws[2 * s] = jw[2 * pidx[s]];
+    jws[2 * s + 1] = jw[2 * pidx[s] + 1];
+  }
+  double* tols = NULL;
+  if (per_col) tols = (double*)mxMalloc((size_t)ns * sizeof(double));
+  if (per_col) memcpy(tols, tv, (size_t)ns * sizeof(double));
+  double tmax = tv[0];
+  for (int64_t s = 1; s < (per_col ? ns : 1); ++s) tmax = tv[s] > tmax ? tv[s] : tmax;
+  /* outputs first, then the preconditioners: from here on every failure
+   * releases them before raising the error */
+  plhs[0] = mxCreateDoubleMatrix((mwSize)S.n, (mwSize)ns, mxCOMPLEX);
+  int* flag = (int*)mxMalloc((size_t)ns * sizeof(int));
+  int* its = (int*)mxMalloc((size_t)ns * sizeof(int));
+  int* iits = (int*)mxMalloc((size_t)ns * sizeof(int));
+  double* rr = (double*)mxMalloc((size_t)ns * sizeof(double));
+  vh_precon** ps = (vh_precon**)mxCalloc((size_t)nf, sizeof(vh_precon*));
+  int rc = 0;
+  for (int64_t f = 0; f < nf && rc == 0; ++f) rc = vh_precon_create(S.geom, &ps[f]);
+  if (rc == 0) rc = vh_precon_prepare_many(ps, nf, jw, NULL, 0);
+  if (rc == 0)
+    rc = vh_solve_batch(S.geom, nf, ps, ns, jws, pidx, b, (double*)mxGetComplexDoubles(plhs[0]), 0, method, tmax,
+                        tols, 100, 10, 40, 1e-5, 0.0, flag, rr, its, iits, NULL);
+  if (rc != 0) snprintf(msgbuf, sizeof msgbuf, "%s", vh_last_error());
+  for (int64_t f = 0; f < nf; ++f)
+    if (ps[f]) vh_precon_destroy(ps[f]);
+  if (rc != 0) {
+    char m[sizeof msgbuf];
+    memcpy(m, msgbuf, sizeof m);
+    fail("voxhenry:lib", "libvoxhenry error %d: %s", rc, m);
+  }
+  mxArray* outs[4] = {NULL, NULL, NULL, NULL};
+  for (int q = 0; q < 4 && q + 1 < (nlhs > 1 ? nlhs : 1); ++q) {
+    outs[q] = mxCreateDoubleMatrix(1, (mwSize)ns, mxREAL);
+    double* o = mxGetDoubles(outs[q]);
+    for (int64_t s = 0; s < ns; ++s) o[s] = q == 0 ? flag[s] : q == 1 ? rr[s] : q == 2 ? its[s] : iits[s];
+    plhs[q + 1] = outs[q];
+  }
+  mxFree(ps);
+  mxFree(rr);
+  mxFree(iits);
+  mxFree(its);
+  mxFree(flag);
+  if (tols) mxFree(tols);
+  mxFree(jws);
+  mxFree(pidx);
+  mxFree(jw);
+  if (tmp) mxFree(tmp);
+}
+
 void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   char cmd[32];
   if (nrhs < 1 || !mxIsChar(prhs[0]) || mxGetString(prhs[0], cmd, sizeof cmd) != 0)
@@ -476,6 +574,7 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   else if (!strcmp(cmd, "schur_multiply")) cmd_schur_multiply(nlhs, plhs, nrhs, prhs);
   else if (!strcmp(cmd, "pgmres")) cmd_gmres(nlhs, plhs, nrhs, prhs, 0);
   else if (!strcmp(cmd, "fpgmres")) cmd_gmres(nlhs, plhs, nrhs, prhs, 1);
+  else if (!strcmp(cmd, "solve_batch")) cmd_solve_batch(nlhs, plhs, nrhs, prhs);
   else if (!strcmp(cmd, "reset")) {
     nargs(nrhs, 1, 1, nlhs, 0, "reset");
     const int locked = S.ctx != NULL;
