@@ -433,8 +433,9 @@ def cpu_baseline(nsolves):
 def main():
     args = parse()
     if args.gpus != 1:
-        print('bench.py runs on one GPU only: this build has no multi-GPU path '
-              '(each (port, frequency) system and each geometry is independent; see DESIGN.md)', file=sys.stderr)
+        print('bench.py runs on one GPU only: there is no multi-GPU path.  Every (port, frequency) system and '
+              'every geometry is independent, so several GPUs would each run their own share of the batch.',
+              file=sys.stderr)
         sys.exit(2)
     import torch
     import voxhenry_amd as vh
