@@ -196,7 +196,7 @@ __global__ void k_sub_from(const cplx<R>* __restrict__ b, int64_t ldb, cplx<R>* 
 // ---------------------------------------------------------------------------
 // host helpers
 // ---------------------------------------------------------------------------
-static constexpr int BLOCK_BUDGET = 2048;   // blocks per reduction launch (all systems together)
+static constexpr int BLOCK_BUDGET = 32768;   // blocks per reduction launch (all systems together)
 
 static int blocks_for(int nchunk, int nact) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(nchunk, std::max(8, BLOCK_BUDGET / std::max(1, nact))));
