@@ -841,18 +841,38 @@ static void launch_z_rl(int rl, dim3 grid, size_t lds, hipStream_t st, cplx<R>* 
 // not apply).
 template <typename R>
 static bool launch_z1(int nrhs, hipStream_t st, cplx<R>* B, const GridDims& g, TensorRef<R> tr) {
-  constexpr int NT = 256;
+  // pencils per workgroup: the width (64, 128 or 256 kx) that leaves the
+  // fewest idle pencils in the last kx tile (Px = 800: 64 wide, 4% idle,
+  // instead of 256 wide, 22% idle); the wider one on a tie
+  auto width = [&](int wmax) {
+    int best = wmax;
+    for (int w = wmax; w >= 64; w >>= 1)
+      if (ceil_div(g.Px, w) * w < ceil_div(g.Px, best) * best) best = w;
+    return best;
+  };
   auto go = [&](auto pzc) {
     constexpr int PZ = decltype(pzc)::value;
     if (g.N > (PZ + 1) / 2) return false;
-    const dim3 grid((unsigned)(ceil_div(g.Px, NT) * g.Py), nrhs);
     if constexpr (sizeof(R) == 8 && PZ >= 4) {
       // FP64 at four or more kz points: two lanes per pencil (in FP32 one
       // thread per pencil at three waves per SIMD measured faster)
-      const dim3 grid2((unsigned)(ceil_div(g.Px, NT / 2) * g.Py), nrhs);
-      hipLaunchKernelGGL((k_pass_z1_pair<R, PZ, NT, 2>), grid2, dim3(NT), 0, st, B, g, tr);
+      auto pair = [&](auto ntc) {
+        constexpr int NT = decltype(ntc)::value;
+        const dim3 grid((unsigned)(ceil_div(g.Px, NT / 2) * g.Py), nrhs);
+        hipLaunchKernelGGL((k_pass_z1_pair<R, PZ, NT, 2>), grid, dim3(NT), 0, st, B, g, tr);
+      };
+      if (width(128) == 64) pair(std::integral_constant<int, 128>());
+      else pair(std::integral_constant<int, 256>());
     } else {
-      hipLaunchKernelGGL((k_pass_z1<R, PZ, NT>), grid, dim3(NT), 0, st, B, g, tr);
+      auto one = [&](auto ntc) {
+        constexpr int NT = decltype(ntc)::value;
+        const dim3 grid((unsigned)(ceil_div(g.Px, NT) * g.Py), nrhs);
+        hipLaunchKernelGGL((k_pass_z1<R, PZ, NT>), grid, dim3(NT), 0, st, B, g, tr);
+      };
+      const int w = width(256);
+      if (w == 64) one(std::integral_constant<int, 64>());
+      else if (w == 128) one(std::integral_constant<int, 128>());
+      else one(std::integral_constant<int, 256>());
     }
     return true;
   };
