@@ -17,7 +17,7 @@ import time
 import numpy as np
 
 from .core import Geometry, Precon, MU0
-from ._lib import VH_C128
+from ._lib import VH_C128, VH_C64
 
 
 def reference_tol(f):
@@ -50,11 +50,13 @@ def impedance(structure, idx, f):
 
 
 class Solver:
-    def __init__(self, ctx, structure, pad_mode=1, precon_type='schur_invert', prectol=0.1):
+    def __init__(self, ctx, structure, pad_mode=1, precon_type='schur_invert', prectol=0.1, autotune=True):
         """precon_type: the executer's fl_precon_type (VoxHenry_executer.m:37-38):
         'schur_invert' (exact Schur inverse, pgmres) or 'schur_gmres' (inner
         GMRES on the Schur complement to prectol, with the flexible fpgmres
-        outer solver, :295-297)."""
+        outer solver, :295-297).  autotune: time every compiled pass-Z shape on
+        this grid in both precisions and keep the fastest (the best shape
+        depends on the padded grid: 1.6-2x on a 200x200x9 or 60x20x20 grid)."""
         self.ctx = ctx
         self.precon_type = precon_type
         self.prectol = prectol
@@ -64,6 +66,9 @@ class Solver:
         t0 = time.perf_counter()
         self.G = Geometry(ctx, dims, structure['dx'], self.idx, pad_mode=pad_mode)
         self.G.generate_green()
+        if autotune:
+            for p in (VH_C64, VH_C128):
+                self.G.autotune(p, nrhs=8, reps=2)
         t1 = time.perf_counter()
         self.rhs = self.G.build_topology(structure['ports_P'], structure['ports_N'], structure.get('well'))
         t2 = time.perf_counter()
