@@ -1,30 +1,37 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X VoxHenry solve path (one GPU).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--detail FILE]
 
 Workload (a "step"): the batched multi-port bus -- 8 parallel copper bars of
 400x4x4 um at 8 um pitch, 1 um voxels in a 400x64x4 grid, 8 ports x 64
 log-spaced frequencies 1e5..1e10 Hz = 512 right-hand sides, every (port,
 frequency) system solved to a true FP64 relative residual <= 1e-8, or 1e-12
-below 1 MHz (the reference driver's rule, VoxHenry_executer.m:219-223: at
-low frequency L comes from the small imaginary part of Z, which a 1e-8
-residual does not fix to 1e-6).  The mixed
-path (complex64 inner FFT matvec + preconditioner under FP64 flexible GMRES) is
-the headline `value` (solves/s); the plain FP64 path (pgmres semantics in
-complex128) is timed on the same step for the ratio.  Inputs are synthetic
+below 1 MHz (the reference driver's rule, VoxHenry_executer.m:219-223).  The
+mixed path (complex64 inner FFT matvec + preconditioner under FP64 flexible
+GMRES) is the headline `value`; the plain FP64 path (pgmres semantics in
+complex128) is timed on the same step for the ratio.  A solve counts only
+when its true FP64 residual ||b - A x|| / ||b|| meets its tolerance: solves/s
+= converged systems / seconds.  R and L of both methods are compared with an
+FP64 solve of the same systems to 1e-12 (untimed).  Inputs are synthetic
 (generated geometry), resident in HBM before timing; per-frequency
 preconditioner factors are prepared before the timed region.
 
 Also measured: the bare FFT-matvec microbench -- a fully conductive 128^3
 copper grid (padded to 256^3), 1 um voxels at 1 GHz, currents N(0,1) -- in
-complex128 and complex64, with the per-pass kernel times from HIP events on the
-launch stream.  `roofline` reports its dominant kernel (pass Z: z-FFT, Green's
-product, inverse z-FFT).
+complex128 and complex64, with per-pass kernel times from HIP events on the
+launch stream; `roofline` reports its dominant kernel (pass Z: z-FFT, Green's
+product, inverse z-FFT).  Further lines: the 30x10x10 bar (10 f), the 5-turn
+spiral (32 f), the numex1 bar (60x20x20 at 0.5 um, the reference's 41
+frequencies) and the 96-geometry uneven batch (1-4 ports each, 16 f),
+admitted into concurrent groups by HBM budget.
 
 The CPU baseline is the FP64 numpy/scipy oracle (oracle/) timed on a bounded
-sample of the same bus workload on this host; cpu_baseline_matvec times the
-oracle's FFT-block matvec on the 128^3 cube next to the GPU figure.
+sample of the same bus workload on this host.
+
+stdout receives exactly one compact JSON line (the last line); progress
+goes to stderr and the full record (per-geometry figures, per-pass times)
+to the --detail file.
 """
 import argparse
 import json
@@ -42,6 +49,9 @@ METRIC = "batched port x freq solves/s at 1e-8 resid, mixed vs FP64; FFT-matvec 
 PASS_NAMES = ['X', 'Y', 'Z', 'Yi', 'Xi', 'node']
 # pass-Z shapes (threads, elements per thread) in the library's FP32 table order
 ZSHAPES32 = [(512, 8), (256, 8), (512, 4), (256, 4)]
+TIGHT = 1e-12          # tolerance of the FP64 reference solves the R/L errors are measured against
+# the reference's numex1 frequencies (1, 2.5, 5, 7.5 per decade, 1 Hz .. 10 GHz; results_numex1_*)
+NUMEX1_FREQS = np.array([m * 10.0 ** e for e in range(10) for m in (1, 2.5, 5, 7.5)] + [1e10])
 
 
 def parse():
@@ -55,9 +65,12 @@ def parse():
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--no-fp64', action='store_true')
     ap.add_argument('--cpu-solves', type=int, default=2)
-    ap.add_argument('--no-configs', action='store_true', help='skip the bar / spiral / uneven-batch lines')
+    ap.add_argument('--no-configs', action='store_true', help='skip the bar / spiral / numex1 / uneven-batch lines')
     ap.add_argument('--uneven-count', type=int, default=96, help='geometries of the uneven batch to run (all 96)')
     ap.add_argument('--uneven-nfreq', type=int, default=16)
+    ap.add_argument('--uneven-workers', type=int, default=8)
+    ap.add_argument('--detail', default=os.path.join(ROOT, 'gpurun_out', 'bench_detail.json'),
+                    help='file that receives the full record (JSON)')
     ap.add_argument('--probe-pass-z', action='store_true',
                     help='only the 128^3 complex64 matvec (3 calls) and a reference elementwise kernel of known '
                          'traffic (1 GiB read + 1 GiB written), for hardware-counter runs')
@@ -79,6 +92,17 @@ def recorded_traffic():
     return None
 
 
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def r3(v):
+    """Round for the compact line (3 significant digits)."""
+    if v is None:
+        return None
+    return float('%.3g' % v)
+
+
 def probe_pass_z(ctx, n=128):
     import torch
     import voxhenry_amd as vh
@@ -97,10 +121,6 @@ def probe_pass_z(ctx, n=128):
         torch.mul(a, 2.0, out=b)
     torch.cuda.synchronize()
     log('[probe] done')
-
-
-def log(*a):
-    print(*a, file=sys.stderr, flush=True)
 
 
 def pass_bytes(g, nv, c, g_t, octant, nrhs=1):
@@ -152,6 +172,7 @@ def matvec_bench(ctx, n, reps):
         pb = pass_bytes((L, M, N, Px, Py, Pz), L * M * N, c, gt, octant)
         bf = b_fft(L * M * N, L * M * N, c, 7 * octant * gt)
         out[name] = dict(ms=ms, matvecs_per_s=1e3 / ms, B_fft_GB=bf / 1e9, GBs_B_fft=bf / ms / 1e6,
+                         frac_B_fft=bf / ms / 1e6 / HBM_PEAK_GBS,
                          passes={k: dict(ms=ms_pass[i], GB=pb.get(k, 0) / 1e9,
                                          GBs=(pb[k] / ms_pass[i] / 1e6) if k in pb and ms_pass[i] > 0 else None)
                                  for i, k in enumerate(PASS_NAMES)})
@@ -179,10 +200,11 @@ def bus_setup(ctx, nfreq):
     return S, freqs, precs, time.perf_counter() - t0
 
 
-def bus_rhs(S, freqs):
+def bus_rhs(S, freqs, tol=None):
     """Device-resident right-hand sides of one step: the port vectors repeated
     for every frequency (system s = frequency s // nports, port s % nports),
-    with each system's j*omega*mu, preconditioner index and tolerance."""
+    with each system's j*omega*mu, preconditioner index and tolerance (the
+    reference driver's rule, or `tol` for every system)."""
     import torch
     import voxhenry_amd as vh
     from voxhenry_amd import solver as vsol
@@ -191,8 +213,10 @@ def bus_rhs(S, freqs):
     B = rhs.repeat(len(freqs), 1).contiguous()
     jw = np.repeat(1j * 2 * np.pi * np.asarray(freqs) * vh.MU0, npo)
     pidx = np.repeat(np.arange(len(freqs)), npo)
-    # the reference driver's tolerance at each system's frequency
-    tols = np.repeat([vsol.reference_tol(f) for f in freqs], npo)
+    if tol is None:
+        tols = np.repeat([vsol.reference_tol(f) for f in freqs], npo)
+    else:
+        tols = np.full(len(freqs) * npo, float(tol))
     # the port vectors are nonzero on the port node rows only
     nz = torch.tensor(np.flatnonzero(np.any(S.rhs != 0, axis=0)), device='cuda')
     Pnz = rhs.index_select(1, nz).transpose(0, 1).contiguous()     # (nnz, nports)
@@ -211,19 +235,40 @@ def bus_step(S, freqs, precs, method, data, **opts):
     X, st = S.G.solve_batch(B, jw, precs, pidx, method=method, tol=tols, **opts)
     Xp = X.index_select(1, nz).reshape(len(freqs), npo, -1)      # port rows of every solution
     Yh = torch.matmul(Xp, Pnz).cpu().numpy()                    # Y[k] = X_k P^T (nports x nports)
-    log('[bench]   %s step solve %.3f s, iters mean %.1f, inner mean %.1f, max relres %.2e'
-        % (method, time.perf_counter() - t, st['iters'].mean(), st['inner_iters'].mean(), st['relres'].max()))
+    log('[bench]   %s step solve %.3f s, iters mean %.1f, inner mean %.1f, max relres %.2e, converged %d/%d'
+        % (method, time.perf_counter() - t, st['iters'].mean(), st['inner_iters'].mean(), st['relres'].max(),
+           int(np.sum(st['flag'] == 0)), len(st['flag'])))
     Z = np.stack([np.linalg.inv(Yh[k].T) for k in range(len(freqs))])
     return Z, st
 
 
-def sweep_config(ctx, st, freqs, tol='reference', methods=('mixed', 'fp64'), reps=1, keep=None):
+def rl_errors(Z, Zref, freqs):
+    """Largest relative error over frequencies and ports of the self
+    resistance R = |Re Z_pp| and inductance L = |Im Z_pp| / (2 pi f), of R
+    relative to |Z_pp|, and of Z normwise per frequency, against Zref."""
+    d, dr = np.diagonal(Z, axis1=-2, axis2=-1), np.diagonal(Zref, axis1=-2, axis2=-1)
+    R, Rr = np.abs(d.real), np.abs(dr.real)
+    L, Lr = np.abs(d.imag), np.abs(dr.imag)
+    return dict(R=float(np.max(np.abs(R - Rr) / Rr)), L=float(np.max(np.abs(L - Lr) / Lr)),
+                R_over_absZ=float(np.max(np.abs(R - Rr) / np.abs(dr))),
+                Z=float(max(np.linalg.norm(Z[k] - Zref[k]) / np.linalg.norm(Zref[k]) for k in range(len(Z)))))
+
+
+def method_summary(nsys, dt, st):
+    conv = int(np.sum(st['flag'] == 0))
+    return dict(solves_per_s=conv / dt, s=dt, converged=conv, systems=nsys,
+                relres_max=float(np.max(st['relres'])), iters_mean=float(np.mean(st['iters'])),
+                iters_max=int(np.max(st['iters'])), inner_iters_mean=float(np.mean(st['inner_iters'])))
+
+
+def sweep_config(ctx, st, freqs, methods=('mixed', 'fp64'), reps=1, keep=None, tight=True, **solve_kw):
     """Batched (port x frequency) solve of one geometry, device-resident:
     set-up (tensor, topology, preconditioners) untimed, then one warm-up and
-    `reps` timed sweeps per method; returns solves/s per method, R/L of the
-    FP64 sweep and the mixed-vs-FP64 errors (self terms, and normwise Z).
-    keep (a list): the solver and its preconditioners are appended to it and
-    left open (for a later concurrent pass) instead of being released."""
+    `reps` timed sweeps per method (the reference driver's tolerance per
+    system); the R/L of each method are compared with an untimed FP64 sweep
+    to 1e-12.  keep (a list): the solver and its preconditioners are
+    appended to it and left open (for a later concurrent pass) instead of
+    being released."""
     import torch
     from voxhenry_amd import solver as vsol
     t0 = time.perf_counter()
@@ -233,34 +278,26 @@ def sweep_config(ctx, st, freqs, tol='reference', methods=('mixed', 'fp64'), rep
     setup = time.perf_counter() - t0
     out = dict(nv=int(len(S.idx)), n=int(S.G.n), nports=S.nports, nfreq=len(freqs), setup_s=setup,
                grid=list(st['sigma'].shape))
+    nsys = len(freqs) * S.nports
     res = {}
     for m in methods:
-        S.sweep_batched(freqs, method=m, tol=tol, precons=precs, pidx_of_freq=pidx, device=True)
+        S.sweep_batched(freqs, method=m, precons=precs, pidx_of_freq=pidx, device=True, **solve_kw)
         t = time.perf_counter()
         for _ in range(reps):
-            r = S.sweep_batched(freqs, method=m, tol=tol, precons=precs, pidx_of_freq=pidx, device=True)
+            r = S.sweep_batched(freqs, method=m, precons=precs, pidx_of_freq=pidx, device=True, **solve_kw)
         torch.cuda.synchronize()
-        dt = (time.perf_counter() - t) / reps
         res[m] = r
-        nsys = len(freqs) * S.nports
-        out[m] = dict(solves_per_s=nsys / dt, s=dt, all_converged=bool(np.all(r['stats']['flag'] == 0)),
-                      relres_max=float(r['stats']['relres'].max()), iters_max=int(r['stats']['iters'].max()))
-    if 'mixed' in res and 'fp64' in res:
-        a, b = res['mixed'], res['fp64']
-        ii = np.arange(S.nports)
-        out['mixed_vs_fp64'] = dict(
-            R_self=float(np.max(np.abs(a['R'][:, ii, ii] - b['R'][:, ii, ii]) / np.abs(b['R'][:, ii, ii]))),
-            # R relative to |Z|: where R << |Z| (skin-effect limit, or the
-            # tiny loss of a superconductor) a solve to a 1e-8 residual
-            # fixes R only to about 1e-8 |Z| / R
-            R_over_absZ=float(np.max(np.abs(a['R'][:, ii, ii] - b['R'][:, ii, ii]) / np.abs(b['Z'][:, ii, ii]))),
-            L_self=float(np.max(np.abs(a['L'][:, ii, ii] - b['L'][:, ii, ii]) / np.abs(b['L'][:, ii, ii]))),
-            Z_norm=float(max(np.linalg.norm(a['Z'][k] - b['Z'][k]) / np.linalg.norm(b['Z'][k])
-                             for k in range(len(freqs)))))
-        out['speedup_mixed_over_fp64'] = out['mixed']['solves_per_s'] / out['fp64']['solves_per_s']
-    if 'fp64' in res:
-        out['R_ohm'] = [float(v) for v in res['fp64']['R'][:, 0, 0]]
-        out['L_H'] = [float(v) for v in res['fp64']['L'][:, 0, 0]]
+        out[m] = method_summary(nsys, (time.perf_counter() - t) / reps, r['stats'])
+    if tight:
+        ref = S.sweep_batched(freqs, method='fp64', tol=TIGHT, precons=precs, pidx_of_freq=pidx, device=True,
+                              maxit=20, **solve_kw)
+        out['ref_relres_max'] = float(ref['stats']['relres'].max())
+        for m in methods:
+            out[m]['err_vs_tight_fp64'] = rl_errors(res[m]['Z'], ref['Z'], freqs)
+        out['R_ohm'] = [float(v) for v in ref['R'][:, 0, 0]]
+        out['L_H'] = [float(v) for v in ref['L'][:, 0, 0]]
+    if 'mixed' in out and 'fp64' in out:
+        out['speedup_mixed_over_fp64'] = out['mixed']['solves_per_s'] / max(out['fp64']['solves_per_s'], 1e-300)
     if keep is not None:
         keep.append((S, precs, pidx))
         return out
@@ -271,75 +308,132 @@ def sweep_config(ctx, st, freqs, tol='reference', methods=('mixed', 'fp64'), rep
     return out
 
 
-def uneven_config(ctx, count, nfreq, seed=0, wave=8):
-    """A prefix of the seeded uneven batch (bars, round wires, loops, two-fluid
-    strips), each geometry's port x frequency systems in one batched solve;
-    aggregate solves/s over the geometries (set-up untimed), one after
-    another and `wave` geometries at a time concurrently (a context and
-    stream each; geometries are released wave by wave, since every one holds
-    its Krylov workspace -- restart 100 in FP64)."""
+def compact_config(c):
+    """The few figures of a configuration line that go into the stdout line."""
+    o = dict(mixed=r3(c['mixed']['solves_per_s']), fp64=r3(c['fp64']['solves_per_s']),
+             ratio=r3(c.get('speedup_mixed_over_fp64')),
+             converged='%d/%d mixed, %d/%d fp64' % (c['mixed']['converged'], c['mixed']['systems'],
+                                                    c['fp64']['converged'], c['fp64']['systems']))
+    if 'err_vs_tight_fp64' in c['mixed']:
+        em, ef = c['mixed']['err_vs_tight_fp64'], c['fp64']['err_vs_tight_fp64']
+        o['RL_err_vs_fp64_1e-12'] = dict(mixed=[r3(em['R']), r3(em['L'])], fp64=[r3(ef['R']), r3(ef['L'])])
+    return o
+
+
+def uneven_config(count, nfreq, seed=0, workers=8, budget_frac=0.7):
+    """The seeded uneven batch (bars, round wires, loops, two-fluid strips,
+    1-4 ports each): every geometry's port x frequency systems in one
+    batched solve, the geometries run as one job queue served by `workers`
+    host threads (a stream each), admitted in groups whose device memory
+    (set-up plus the solves' workspaces, vh_solve_batch_bytes) fits
+    budget_frac of the free HBM.  Per group: set-up and one warm-up sweep per
+    method (untimed; they allocate the workspaces), then one timed
+    concurrent pass per method, then an untimed FP64 sweep to 1e-12 as the
+    R/L reference.  solves/s = converged systems / summed timed wall."""
     import torch
     import voxhenry_amd as vh
     from voxhenry_amd import structures, solver as vsol
     freqs = np.logspace(6, 10, nfreq)
-    tot = {'mixed': 0.0, 'fp64': 0.0}
-    conc_wall = {'mixed': 0.0, 'fp64': 0.0}
-    nsys = 0
-    per = []
-    # R relative to itself over the normal conductors; R relative to |Z| over
-    # all (a two-fluid strip's R is ~1e-9 of |Z|, below any solve's resolution)
-    worst = dict(R_self_normal=0.0, L_self_normal=0.0, Z_norm_normal=0.0, R_over_absZ=0.0, L_self=0.0, Z_norm=0.0)
+    pidx = np.arange(nfreq)
     geos = list(structures.uneven_batch(seed, count))
-    for w0 in range(0, len(geos), wave):
-        jobs = []      # (solver, preconditioners, pidx, context, stream) per geometry
-        wsys = 0
-        for kind, st in geos[w0:w0 + wave]:
+    # largest first: the big geometries set the group sizes
+    order = sorted(range(len(geos)), key=lambda i: -np.count_nonzero(geos[i][1]['sigma']) * len(geos[i][1]['ports_P']))
+    free0 = torch.cuda.mem_get_info()[0]
+    budget = budget_frac * free0
+    cap = 0.5 * budget                 # one geometry's workspaces: at most half the budget (it chunks beyond)
+    tot = dict(mixed=dict(wall=0.0, conv=0), fp64=dict(wall=0.0, conv=0))
+    nsys_all = 0
+    worst = {}
+    kinds = {}
+    groups = []
+    per = []
+    pos = 0
+    setup_s = 0.0
+    while pos < len(order):
+        jobs = []
+        need = 0.0
+        while pos < len(order):
+            i = order[pos]
+            kind, st = geos[i]
+            t0 = time.perf_counter()
             gctx = vh.Context(0)
+            S = vsol.Solver(gctx, st)
+            precs = S.precons_for(freqs)
+            setup_s += time.perf_counter() - t0
+            nsys = nfreq * S.nports
+            ws = min(vsol.workspace_bytes(S, nsys), cap)
+            free = torch.cuda.mem_get_info()[0]
+            if jobs and need + ws > free - (1 - budget_frac) * free0:
+                # does not fit beside the group: it opens the next group
+                for P in precs:
+                    P.close()
+                S.G.close()
+                gctx.close()
+                torch.cuda.empty_cache()
+                break
             stream = torch.cuda.Stream()
-            keep = []
-            with torch.cuda.stream(stream):
-                r = sweep_config(gctx, st, freqs, keep=keep)
-            jobs.append(keep[0] + (gctx, stream))
-            wsys += r['nfreq'] * r['nports']
-            for m in tot:
-                tot[m] += r[m]['s']
-            e = r['mixed_vs_fp64']
-            if kind != 'strip':
-                worst['R_self_normal'] = max(worst['R_self_normal'], e['R_self'])
-                worst['L_self_normal'] = max(worst['L_self_normal'], e['L_self'])
-                worst['Z_norm_normal'] = max(worst['Z_norm_normal'], e['Z_norm'])
-            for k in ('R_over_absZ', 'L_self', 'Z_norm'):
-                worst[k] = max(worst[k], e[k])
-            per.append(dict(kind=kind, grid=r['grid'], nv=r['nv'], setup_s=round(r['setup_s'], 2),
-                            mixed_solves_per_s=round(r['mixed']['solves_per_s'], 1),
-                            fp64_solves_per_s=round(r['fp64']['solves_per_s'], 1),
-                            mixed_converged=r['mixed']['all_converged'], fp64_converged=r['fp64']['all_converged'],
-                            relres_max=dict(mixed=r['mixed']['relres_max'], fp64=r['fp64']['relres_max'])))
-            log('[bench]   uneven %s %s: mixed %.1f fp64 %.1f solves/s' % (kind, r['grid'], r['mixed']['solves_per_s'],
-                                                                         r['fp64']['solves_per_s']))
-        nsys += wsys
-        # concurrent pass over the wave: every geometry's whole sweep on its
-        # own stream from its own host thread, started together
+            jobs.append(dict(i=i, kind=kind, S=S, precs=precs, ctx=gctx, stream=stream, ws=ws, nsys=nsys))
+            need += ws
+            pos += 1
+        groups.append(len(jobs))
+        # warm-up (allocates every geometry's workspaces), one geometry after another
+        for j in jobs:
+            with torch.cuda.stream(j['stream']):
+                for m in ('mixed', 'fp64'):
+                    j['S'].sweep_batched(freqs, method=m, precons=j['precs'], pidx_of_freq=pidx, device=True,
+                                         hbm_budget=j['ws'])
+            j['stream'].synchronize()
+        res = {}
         for m in ('mixed', 'fp64'):
             torch.cuda.synchronize()
-            _, wall = vsol.sweep_concurrent([(S, freqs, precs, pidx, stream)
-                                             for S, precs, pidx, gctx, stream in jobs], method=m)
+            outs, wall = vsol.sweep_concurrent([(j['S'], freqs, j['precs'], pidx, j['stream'], j['ws']) for j in jobs],
+                                               method=m, workers=workers)
             torch.cuda.synchronize()
-            conc_wall[m] += wall
-            log('[bench]   uneven wave of %d concurrent %s: %.1f solves/s (%.3f s)' % (len(jobs), m, wsys / wall, wall))
-        for S, precs, pidx, gctx, stream in jobs:
-            for P in precs:
+            res[m] = outs
+            conv = sum(int(np.sum(o['stats']['flag'] == 0)) for o in outs)
+            tot[m]['wall'] += wall
+            tot[m]['conv'] += conv
+            log('[bench]   uneven group of %d, %s: %d/%d converged in %.3f s'
+                % (len(jobs), m, conv, sum(j['nsys'] for j in jobs), wall))
+        for k, j in enumerate(jobs):
+            with torch.cuda.stream(j['stream']):
+                ref = j['S'].sweep_batched(freqs, method='fp64', tol=TIGHT, precons=j['precs'], pidx_of_freq=pidx,
+                                           device=True, maxit=20, hbm_budget=j['ws'])
+            j['stream'].synchronize()
+            nsys_all += j['nsys']
+            kd = kinds.setdefault(j['kind'], dict(geometries=0, systems=0, mixed_conv=0, fp64_conv=0))
+            kd['geometries'] += 1
+            kd['systems'] += j['nsys']
+            rec = dict(kind=j['kind'], grid=list(geos[j['i']][1]['sigma'].shape), ports=j['S'].nports,
+                       nv=int(len(j['S'].idx)))
+            for m in ('mixed', 'fp64'):
+                o = res[m][k]
+                c = int(np.sum(o['stats']['flag'] == 0))
+                kd[m + '_conv'] += c
+                e = rl_errors(o['Z'], ref['Z'], freqs)
+                rec[m] = dict(converged=c, relres_max=float(o['stats']['relres'].max()), err=e)
+                w = worst.setdefault(m, {})
+                for q in ('R', 'L', 'R_over_absZ', 'Z'):
+                    w[q] = max(w.get(q, 0.0), e[q])
+                    if j['kind'] != 'strip':
+                        w[q + '_normal'] = max(w.get(q + '_normal', 0.0), e[q])
+            per.append(rec)
+        for j in jobs:
+            for P in j['precs']:
                 P.close()
-            S.G.close()
+            j['S'].G.close()
+            j['ctx'].close()
         del jobs
         torch.cuda.empty_cache()
-    conc = {m: nsys / conc_wall[m] for m in conc_wall}
-    return dict(seed=seed, geometries=count, nfreq=nfreq, systems=nsys,
-                mixed_solves_per_s=nsys / tot['mixed'], fp64_solves_per_s=nsys / tot['fp64'],
-                speedup_mixed_over_fp64=tot['fp64'] / tot['mixed'],
-                concurrent=dict(wave=wave, mixed_solves_per_s=conc['mixed'], fp64_solves_per_s=conc['fp64'],
-                                speedup_mixed_over_fp64=conc['mixed'] / conc['fp64']),
-                mixed_vs_fp64_max=worst, per_geometry=per)
+    sps = {m: tot[m]['conv'] / tot[m]['wall'] for m in tot}
+    return dict(seed=seed, geometries=count, nfreq=nfreq, systems=nsys_all, workers=workers, groups=groups,
+                budget_GB=budget / 1e9, setup_s=setup_s,
+                mixed=dict(solves_per_s=sps['mixed'], s=tot['mixed']['wall'], converged=tot['mixed']['conv'],
+                           systems=nsys_all),
+                fp64=dict(solves_per_s=sps['fp64'], s=tot['fp64']['wall'], converged=tot['fp64']['conv'],
+                          systems=nsys_all),
+                speedup_mixed_over_fp64=sps['mixed'] / max(sps['fp64'], 1e-300),
+                by_kind=kinds, err_vs_tight_fp64_max=worst, per_geometry=per)
 
 
 def timed_steps(fn, steps, warmup):
@@ -430,6 +524,57 @@ def cpu_baseline(nsolves):
                        % (len(ops), ','.join('%.3g' % f for f in fs), cores, setup))
 
 
+def bus_kernels(S, precs, freqs):
+    """Dominant kernels of the bus step at its real batch width (64 systems per
+    launch, the widest active set of one preconditioner group): per-pass
+    matvec times from HIP events, and one preconditioner apply (one
+    frequency's factors, 64 vectors) timed with events on the launch
+    stream.  Bytes per launch from the pass model and B_pc = b*c*(4*n_node +
+    10*nv) + factor bytes (L in CSR and CSC, D, block inverses)."""
+    import torch
+    import voxhenry_amd as vh
+    G = S.G
+    L, M, N = G.dims
+    Px, Py, Pz = G.pad
+    octant = (Px // 2 + 1) * (Py // 2 + 1) * (Pz // 2 + 1)
+    nb = 64
+    jw = 1j * 2 * np.pi * 1e9 * vh.MU0
+    out = dict(batch=nb)
+    rng = np.random.default_rng(0)
+    for prec, name, c, gt, dt in ((vh.VH_C128, 'c128', 16, 16, torch.complex128), (vh.VH_C64, 'c64', 8, 4, torch.complex64)):
+        xh = rng.standard_normal((nb, G.n)) + 1j * rng.standard_normal((nb, G.n))
+        x = torch.tensor(xh, dtype=dt, device='cuda')
+        y = torch.zeros_like(x)
+        G.matvec_timed(x, y, jw, prec, 0, 2)
+        ms, ms_pass = G.matvec_timed_passes(x, y, jw, prec, 0, 5)
+        pb = pass_bytes((L, M, N, Px, Py, Pz), G.nv, c, gt, octant, nrhs=nb)
+        zi = PASS_NAMES.index('Z')
+        out[name] = dict(matvec_ms=ms, pass_z_ms=ms_pass[zi], pass_z_GB=pb['Z'] / 1e9,
+                         pass_z_GBs=pb['Z'] / ms_pass[zi] / 1e6)
+        # preconditioner apply of one frequency on the same 64 vectors
+        k = len(precs) // 2
+        P = precs[k]
+        if P.stats is None:
+            P.prepare(1j * 2 * np.pi * freqs[k] * vh.MU0)     # same factors again, with their sizes
+        P.apply(x, prec)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(5):
+            P.apply(x, prec)
+        e1.record()
+        e1.synchronize()
+        pms = e0.elapsed_time(e1) / 5
+        st = P.stats
+        nnzL = st['nnzL']
+        fb = 8 if prec == vh.VH_C128 else 4
+        bpc = nb * c * (4 * st['n_node'] + 10 * G.nv) + 2 * nnzL * (4 + fb) + st['n_node'] * fb
+        out[name].update(precon_ms=pms, precon_GB=bpc / 1e9, precon_GBs=bpc / pms / 1e6)
+        del x, y
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -444,9 +589,11 @@ def main():
     if args.probe_pass_z:
         probe_pass_z(ctx)
         return
+    detail = {}
 
     log('[bench] FFT-matvec microbench %d^3' % args.cube)
     mv = matvec_bench(ctx, args.cube, args.matvec_reps)
+    detail['fft_matvec'] = mv
     log('[bench] matvec c128 %.3f ms  c64 %.3f ms' % (mv['c128']['ms'], mv['c64']['ms']))
 
     log('[bench] bus set-up (%d frequencies)' % args.nfreq)
@@ -455,60 +602,58 @@ def main():
     log('[bench] set-up %.1f s, %d systems per step' % (t_setup, nsol))
 
     data = bus_rhs(S, freqs)
-    ms_mixed, (Zm, stm) = timed_steps(lambda: bus_step(S, freqs, precs, 'mixed', data), args.steps, args.warmup)
-    log('[bench] mixed step %.3f s' % ms_mixed)
+    s_mixed, (Zm, stm) = timed_steps(lambda: bus_step(S, freqs, precs, 'mixed', data), args.steps, args.warmup)
+    log('[bench] mixed step %.3f s' % s_mixed)
+    mixed = method_summary(nsol, s_mixed, stm)
     fp64 = None
     if not args.no_fp64:
         s64, (Z64, st64) = timed_steps(lambda: bus_step(S, freqs, precs, 'fp64', data), max(1, args.steps // 2),
                                        max(1, args.warmup))
         log('[bench] fp64 step %.3f s' % s64)
-        w = 2 * np.pi * np.asarray(freqs)[:, None]
-        dm, d64 = np.diagonal(Zm, axis1=1, axis2=2), np.diagonal(Z64, axis1=1, axis2=2)
-        fp64 = dict(solves_per_s=nsol / s64, s_per_step=s64,
-                    # self resistance / inductance of every port, mixed vs FP64, relative
-                    max_rel_err_R_mixed_vs_fp64=float(np.max(np.abs(dm.real - d64.real) / np.abs(d64.real))),
-                    max_rel_err_L_mixed_vs_fp64=float(np.max(np.abs(dm.imag - d64.imag) / np.abs(d64.imag))),
-                    # whole port impedance matrix, normwise per frequency
-                    max_rel_err_Z_mixed_vs_fp64=float(max(np.linalg.norm(Zm[k] - Z64[k]) / np.linalg.norm(Z64[k])
-                                                          for k in range(len(freqs)))),
-                    L_self_nH_at_fmin=float(d64[0].imag.mean() / w[0, 0] * 1e9),
-                    iters_mean=float(st64['iters'].mean()), iters_max=int(st64['iters'].max()),
-                    relres_max=float(st64['relres'].max()),
-                    # the reference-style stopping residual ||M r|| / ||M b|| (pgmres.m:126)
-                    precond_resid_max=float(st64['resid_est'].max()),
-                    all_converged=bool(np.all(st64['flag'] == 0)))
-    mixed = dict(solves_per_s=nsol / ms_mixed, s_per_step=ms_mixed, outer_iters_mean=float(stm['iters'].mean()),
-                 inner_iters_mean=float(stm['inner_iters'].mean()), relres_max=float(stm['relres'].max()),
-                 outer_resid_est_max=float(stm['resid_est'].max()),
-                 all_converged=bool(np.all(stm['flag'] == 0)))
+        fp64 = method_summary(nsol, s64, st64)
+        # the reference-style stopping residual ||M r|| / ||M b|| (pgmres.m:126)
+        fp64['precond_resid_max'] = float(st64['resid_est'].max())
+    log('[bench] bus FP64 reference solve to %g' % TIGHT)
+    Zt, stt = bus_step(S, freqs, precs, 'fp64', bus_rhs(S, freqs, TIGHT), maxit=20)
+    ref_relres = float(stt['relres'].max())
+    mixed['err_vs_tight_fp64'] = rl_errors(Zm, Zt, freqs)
+    if fp64:
+        fp64['err_vs_tight_fp64'] = rl_errors(Z64, Zt, freqs)
+    log('[bench] bus kernels at the solve batch width')
+    bk = bus_kernels(S, precs, freqs)
+    detail['bus'] = dict(mixed=mixed, fp64=fp64, setup_s=t_setup, ref_relres_max=ref_relres, kernels=bk,
+                         L_self_nH_at_fmin=float(np.abs(np.diagonal(Zt[0]).imag).mean() / (2 * np.pi * freqs[0]) * 1e9))
 
     z = mv['c64']['passes']['Z']
     nt, e = ZSHAPES32[mv['zshape']['c64']]
     kname = ('k_pass_z_p2<float, 256, %d, true>' % nt) if e == 8 else ('k_pass_z<float, %d, %d, 4>' % (e, nt))
-    roof = dict(kernel=kname + ' (z-FFT + Green product + inverse z-FFT), 128^3 grid, complex64',
+    roof = dict(kernel=kname + ' (z-FFT + Green product + inverse z-FFT), 128^3 grid, complex64, 1 vector per launch',
                 bound='hbm', achieved=z['GBs'], peak=HBM_PEAK_GBS, unit='GB/s',
-                frac=(z['GBs'] / HBM_PEAK_GBS) if z['GBs'] else None, traffic=recorded_traffic())
+                frac=(z['GBs'] / HBM_PEAK_GBS) if z['GBs'] else None, traffic=recorded_traffic(),
+                bytes_per_launch=z['GB'] * 1e9, launch_ms=z['ms'])
 
     # the bus geometry holds a Krylov workspace sized from most of the free
     # memory: release it before the configuration sweeps
-    import torch
     del data
     for P in precs:
         P.close()
     S.G.close()
     torch.cuda.empty_cache()
 
-    configs = None
+    configs = {}
     if not args.no_configs:
         from voxhenry_amd import structures
-        configs = {}
         log('[bench] bar 30x10x10, 10 frequencies')
         configs['bar_30x10x10'] = sweep_config(ctx, structures.bar(30, 10, 10), np.logspace(4, 10, 10))
         log('[bench] spiral, 32 frequencies')
         configs['spiral_5turn'] = sweep_config(ctx, structures.spiral(), np.logspace(6, 11, 32))
+        log('[bench] numex1 bar 60x20x20 at 0.5 um, 41 frequencies')
+        configs['numex1_bar'] = sweep_config(ctx, structures.bar(60, 20, 20, dx=0.5e-6), NUMEX1_FREQS)
         if args.uneven_count > 0:
             log('[bench] uneven batch, %d geometries x %d frequencies' % (args.uneven_count, args.uneven_nfreq))
-            configs['uneven_batch'] = uneven_config(ctx, args.uneven_count, args.uneven_nfreq)
+            configs['uneven_batch'] = uneven_config(args.uneven_count, args.uneven_nfreq,
+                                                    workers=args.uneven_workers)
+    detail['configs'] = configs
 
     cpu = cpu_mv = None
     if not args.no_cpu:
@@ -516,19 +661,62 @@ def main():
         cpu = cpu_baseline(args.cpu_solves)
         log('[bench] CPU baseline of the 128^3 FFT-block matvec (oracle)')
         cpu_mv = cpu_matvec_baseline(128, 2)
+    detail['cpu_baseline'] = cpu
+    detail['cpu_baseline_matvec'] = cpu_mv
+    detail['roofline'] = roof
 
-    line = dict(metric=METRIC, value=mixed['solves_per_s'], unit='solves/s', n_gpus=1, steps=args.steps,
-                warmup=args.warmup, ms_per_step=ms_mixed * 1e3, higher_is_better=True, scaling='weak',
+    # ---- the compact stdout line -------------------------------------------------
+    def msum(m):
+        if m is None:
+            return None
+        o = dict(solves_per_s=r3(m['solves_per_s']), converged='%d/%d' % (m['converged'], m['systems']),
+                 s_per_step=r3(m['s']), iters_mean=r3(m['iters_mean']), relres_max=r3(m['relres_max']))
+        if m.get('inner_iters_mean'):
+            o['inner_iters_mean'] = r3(m['inner_iters_mean'])
+        if 'err_vs_tight_fp64' in m:
+            e = m['err_vs_tight_fp64']
+            o['err_vs_fp64_1e-12'] = dict(R=r3(e['R']), L=r3(e['L']), Z=r3(e['Z']))
+        return o
+
+    cfg_line = {}
+    for k, c in configs.items():
+        cfg_line[k] = compact_config(c)
+    if 'uneven_batch' in configs:
+        u = configs['uneven_batch']
+        cfg_line['uneven_batch'].update(geometries=u['geometries'], systems=u['systems'], groups=len(u['groups']))
+    line = dict(metric=METRIC, value=r3(mixed['solves_per_s']), unit='solves/s', n_gpus=1, steps=args.steps,
+                warmup=args.warmup, ms_per_step=r3(s_mixed * 1e3), higher_is_better=True, scaling='weak',
                 vs_baseline=None, dtype='c64 inner / f64 outer', data='synthetic',
-                config=dict(workload='bus: 8 bars 400x4x4 um, 400x64x4 grid, 8 ports x %d freqs 1e5-1e10 Hz '
-                                     '= %d systems per step, tol 1e-8 (1e-12 below 1 MHz: %d freqs), true FP64 residual'
-                                     % (len(freqs), nsol, int(np.sum(np.asarray(freqs) < 1e6))),
+                config=dict(workload='bus: 8 copper bars 400x4x4 um, 400x64x4 grid, 8 ports x %d freqs 1e5-1e10 Hz '
+                                     '= %d systems per step, tol 1e-8 (1e-12 below 1 MHz), true FP64 residual; '
+                                     'value counts converged systems only' % (len(freqs), nsol),
                             parallelism='1 GPU'),
-                mixed=mixed, fp64=fp64,
-                fp64_speedup=(mixed['solves_per_s'] / fp64['solves_per_s']) if fp64 else None,
-                fft_matvec=mv, roofline=roof, cpu_baseline=cpu, cpu_baseline_matvec=cpu_mv, configs=configs,
-                setup_s=t_setup)
-    print(json.dumps(line))
+                mixed=msum(mixed), fp64=msum(fp64),
+                fp64_speedup=r3(mixed['solves_per_s'] / fp64['solves_per_s']) if fp64 else None,
+                matvec_128cube=dict(c128_ms=r3(mv['c128']['ms']), c64_ms=r3(mv['c64']['ms']),
+                                    c128_GBs_B_fft=r3(mv['c128']['GBs_B_fft']), c64_GBs_B_fft=r3(mv['c64']['GBs_B_fft']),
+                                    c128_pass_z_GBs=r3(mv['c128']['passes']['Z']['GBs'])),
+                roofline=dict(bound='hbm', achieved=r3(roof['achieved']), peak=HBM_PEAK_GBS, unit='GB/s',
+                              frac=r3(roof['frac']), traffic=roof['traffic'], kernel=roof['kernel']),
+                roofline_bus=dict(batch=bk['batch'],
+                                  pass_z_c128=dict(GBs=r3(bk['c128']['pass_z_GBs']), ms=r3(bk['c128']['pass_z_ms'])),
+                                  pass_z_c64=dict(GBs=r3(bk['c64']['pass_z_GBs']), ms=r3(bk['c64']['pass_z_ms'])),
+                                  precon_c128=dict(GBs=r3(bk['c128']['precon_GBs']), ms=r3(bk['c128']['precon_ms'])),
+                                  precon_c64=dict(GBs=r3(bk['c64']['precon_GBs']), ms=r3(bk['c64']['precon_ms']))),
+                cpu_baseline=None if cpu is None else dict(value=r3(cpu['value']), unit=cpu['unit'], cores=cpu['cores'],
+                                                           kind=cpu['kind'], sample=cpu['sample'],
+                                                           one_core=r3(cpu['one_core']), cpu=cpu['cpu']),
+                cpu_baseline_matvec=None if cpu_mv is None else dict(value=r3(cpu_mv['value']), unit=cpu_mv['unit'],
+                                                                     cores=cpu_mv['cores'], GBs=r3(cpu_mv['GBs'])),
+                configs=cfg_line, setup_s=r3(t_setup))
+    if args.detail:
+        os.makedirs(os.path.dirname(os.path.abspath(args.detail)), exist_ok=True)
+        with open(args.detail, 'w') as fh:
+            json.dump(dict(line=line, detail=detail), fh, default=float)
+            fh.write('\n')
+        log('[bench] full record in %s' % args.detail)
+    sys.stderr.flush()
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == '__main__':

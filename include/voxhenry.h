@@ -225,6 +225,14 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
                    int maxit, int inner_restart, double inner_tol, double hbm_budget, int* flag, double* relres,
                    int* iters, int* inner_iters, double* resid_est);
 
+/* Device bytes per system that vh_solve_batch budgets for this geometry
+ * (Krylov workspace of the method plus the system's share of the matvec
+ * workspaces); a caller running several geometries at once passes each its
+ * own hbm_budget from these figures instead of letting every call take 80%
+ * of the free memory.  No counterpart in the reference (MATLAB allocates per
+ * call). */
+int vh_solve_batch_bytes(vh_geom* g, int method, int restart, int inner_restart, double* bytes_per_system);
+
 /* Test hook: in-place 3-D FFT (dir -1 forward, +1 inverse unnormalised) of ncomp
  * device grids of the geometry's padded size. */
 int vh_fft3d(vh_geom* g, void* data, int ncomp, int prec, int dir);

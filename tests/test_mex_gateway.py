@@ -226,6 +226,10 @@ def test_gateway_matvec(kind):
     for k in range(2):
         assert rel(Y[:, k], A(X[:, k])) < 1e-12
     expect('voxhenry:usage', m, 'matvec', X[:-1], jwmu)      # wrong row count
+    if zi is not None:
+        # the batched tier uses one impedance for every frequency: a two-fluid
+        # (frequency-dependent) impedance is refused, not silently reused
+        expect('voxhenry:usage', m, 'solve_batch', X, np.array([jwmu]), np.ones(2), 'fp64', 1e-8)
     m('reset')
     assert m.locked() == 0 and m.leaked() == 0
     expect('voxhenry:state', m, 'matvec', X, jwmu)

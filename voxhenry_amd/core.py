@@ -315,6 +315,16 @@ class Geometry:
         # resid_est: fp64 the reference-style preconditioned residual (pgmres.m:126), mixed the outer estimate
         return X, dict(flag=flag, relres=relres, iters=its, inner_iters=iits, resid_est=rest)
 
+    def batch_bytes(self, method='mixed', restart=None, inner_restart=40):
+        """Device bytes per system that solve_batch budgets on this geometry
+        (Krylov workspace + the system's share of the matvec workspaces)."""
+        m = 0 if method == 'fp64' else 1
+        if restart is None:
+            restart = 100 if m == 0 else 8
+        b = np.zeros(1)
+        check(lib().vh_solve_batch_bytes(self.h, m, int(restart), int(inner_restart), ptr(b)))
+        return float(b[0])
+
     def autotune(self, prec, nrhs=1, reps=3):
         """Time every compiled pass-Z shape on nrhs vectors of this geometry
         and keep the fastest (vh_geom_autotune); returns (best index, ms per

@@ -6,6 +6,7 @@
 #include "../../include/voxhenry.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 
@@ -23,6 +24,17 @@ namespace {
 vh::KrylovCache& krylov_cache(vh::Geometry& G) {
   if (!G.krylov) G.krylov = std::make_shared<vh::KrylovCache>();
   return *G.krylov;
+}
+// Device bytes one system of vh_solve_batch needs: its Krylov workspace and
+// its share of the matvec workspaces on the padded grid (the pass-X output A
+// of each precision used, Px*M*N per component, and the shared B grid,
+// Px*Py*N per component, sized for complex128).
+double batch_bytes_per_system(const vh::Geometry& G, int method, int restart, int inner_restart) {
+  const bool mixed = method == 1;
+  double b = (double)vh::KrylovCache::bytes(G.n(), 1, restart, mixed, mixed ? inner_restart : 0, false, mixed);
+  const vh::GridDims& gd = G.g;
+  const double a = 5.0 * gd.Px * (double)gd.M * gd.N, bgrid = 5.0 * gd.Px * (double)gd.Py * gd.N;
+  return b + bgrid * 16 + a * (16 + (mixed ? 8 : 0));
 }
 void h2d(void* dst, const void* src, size_t bytes) { VH_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); }
 void d2h(void* dst, const void* src, size_t bytes, hipStream_t st) {
@@ -271,6 +283,10 @@ int vh_geom_autotune(vh_geom* g, int prec, int64_t nrhs, int reps, int* best, do
     double tbest = 1e300;
     try {
       for (int i = 0; i < ns; ++i) {
+        if (ms_shape && nshape && i < *nshape) ms_shape[i] = HUGE_VAL;
+        // shapes that cannot hold a z pencil, or that the grid's kernel
+        // ignores, are not timed (a shape-less grid keeps shape 0)
+        if (!vh::matvec_zshape_applies(G, p, i)) continue;
         G.zshape[prec == VH_C128 ? 1 : 0] = i;
         run();   // warm-up
         VH_HIP(hipEventRecord(e0, st));
@@ -283,6 +299,7 @@ int vh_geom_autotune(vh_geom* g, int prec, int64_t nrhs, int reps, int* best, do
         if (ms_shape && nshape && i < *nshape) ms_shape[i] = t;
         if (t < tbest) { tbest = t; arg = i; }
       }
+      if (tbest == 1e300) arg = 0;
     } catch (...) {
       G.zshape[prec == VH_C128 ? 1 : 0] = keep;
       (void)hipEventDestroy(e0);
@@ -712,10 +729,10 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
     vh::KrylovCache& K = krylov_cache(G);
     size_t freeb = 0, totalb = 0;
     VH_HIP(hipMemGetInfo(&freeb, &totalb));
-    const double budget = hbm_budget > 0 ? hbm_budget : 0.8 * ((double)freeb + (double)K.arena.n);
+    const double reusable = (double)K.arena.n + (double)G.wsA64.n + (double)G.wsA32.n + (double)G.wsB.n;
+    const double budget = hbm_budget > 0 ? hbm_budget : 0.8 * ((double)freeb + reusable);
     const bool mixed = method == 1;
-    double per_sys = (double)vh::KrylovCache::bytes(n, 1, restart, mixed, mixed ? inner_restart : 0, false, mixed);
-    per_sys += 5.0 * n * 16;   // matvec workspaces (A + B grids are ~4-5 vectors per system)
+    const double per_sys = batch_bytes_per_system(G, method, restart, inner_restart);
     int chunk = (int)std::max(1.0, std::min((double)nsys, std::floor(budget / per_sys)));
     std::vector<vh::SysStats> stats(nsys);
     if (vh::trace_on()) fprintf(stderr, "[vh] solve_batch nsys %ld chunk %d budget %.1f GB per_sys %.3f GB\n",
@@ -772,6 +789,15 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
       if (inner_iters) inner_iters[s] = stats[s].inner_iters;
       if (resid_est) resid_est[s] = stats[s].resvec.empty() ? 1.0 : stats[s].resvec.back();
     }
+  });
+}
+
+int vh_solve_batch_bytes(vh_geom* g, int method, int restart, int inner_restart, double* bytes_per_system) {
+  return guard([&] {
+    need(g, "g"); need(bytes_per_system, "bytes_per_system");
+    VH_REQUIRE(method == 0 || method == 1, "method must be 0 (fp64) or 1 (mixed)");
+    VH_REQUIRE(restart >= 1 && restart <= 127 && inner_restart >= 1 && inner_restart <= 127, "bad restart");
+    *bytes_per_system = batch_bytes_per_system(g->G, method, restart, inner_restart);
   });
 }
 

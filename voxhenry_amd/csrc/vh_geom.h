@@ -152,6 +152,10 @@ void currents_on_grid(Geometry& G, int nrhs, const double2* x, int64_t ldx, doub
                       double* minmax);
 // Number of compiled pass-Z shapes of a precision (prec 0 = FP64, 1 = FP32).
 int matvec_num_zshapes(int prec);
+// Whether pass-Z shape `shape` changes anything on this grid: false when its
+// threads x elements cannot hold one z pencil, or when the grid's z length is
+// served by the one-radix kernel, which has no shape.
+bool matvec_zshape_applies(const Geometry& G, int prec, int shape);
 void geometry_set_spectrum_full(Geometry& G, const double* fN, int64_t P0, int64_t P1, int64_t P2,
                                 double scale_re, double scale_im);
 

@@ -202,12 +202,19 @@ static int blocks_for(int nchunk, int nact) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(nchunk, std::max(8, BLOCK_BUDGET / std::max(1, nact))));
 }
 
+// nact * nb <= max(BLOCK_BUDGET, 8 * nact) partials per basis vector
 template <typename R>
-VecOps<R>::VecOps(Context* c, int64_t n_, int64_t ld_, int kmax_, int nslot_) : ctx(c), n(n_), ld(ld_), kmax(kmax_) {
+size_t VecOps<R>::part_size(int kmax, int nslot) {
+  return (size_t)kmax * std::max<int64_t>(BLOCK_BUDGET, 8 * (int64_t)nslot) + 64;
+}
+
+template <typename R>
+VecOps<R>::VecOps(Context* c, int64_t n_, int64_t ld_, int kmax_, int nslot_, double2* part_)
+    : ctx(c), n(n_), ld(ld_), kmax(kmax_), part(part_) {
   VH_REQUIRE(kmax <= KMAX, "Krylov basis larger than 128 vectors");
+  VH_REQUIRE(part != nullptr, "internal: no reduction partials buffer");
+  (void)nslot_;
   nchunk = (int)ceil_div(n, CHUNK);
-  // nact * nb <= max(BLOCK_BUDGET, 8 * nact) partials per basis vector
-  part.alloc((size_t)kmax * std::max<int64_t>(BLOCK_BUDGET, 8 * (int64_t)nslot_) + 64);
 }
 
 // h[a][j] = V_j^H w for j < k (device, double2 [a][kmax])
@@ -217,11 +224,11 @@ void VecOps<R>::mdot(const cplx<R>* V, int64_t vstride, const cplx<R>* w, int k,
   if (k <= 0 || nact <= 0) return;
   hipStream_t st = ctx->stream;
   const int nb = blocks_for(nchunk, nact);
-  hipLaunchKernelGGL((k_mdot<R>), dim3(nb, nact), dim3(KT), 0, st, V, vstride, w, ld, n, k, kmax, slots, part.p,
+  hipLaunchKernelGGL((k_mdot<R>), dim3(nb, nact), dim3(KT), 0, st, V, vstride, w, ld, n, k, kmax, slots, part,
                      nchunk);
   VH_LAUNCHED();
   // reduce each (a, j < k) over nb partials into h[a][kmax]
-  hipLaunchKernelGGL(k_reduce2, dim3(k, nact), dim3(256), 0, st, part.p, nb, h, kmax);
+  hipLaunchKernelGGL(k_reduce2, dim3(k, nact), dim3(256), 0, st, part, nb, h, kmax);
   VH_LAUNCHED();
 }
 
@@ -234,10 +241,10 @@ void VecOps<R>::combine(const cplx<R>* V, int64_t vstride, cplx<R>* out, int k, 
   hipStream_t st = ctx->stream;
   const int nb = blocks_for(nchunk, nact);
   hipLaunchKernelGGL((k_combine<R>), dim3(nb, nact), dim3(KT), 0, st, V, vstride, out, ld, n, k, kmax, slots, coef,
-                     beta, sign, nrm2 ? part.p : nullptr, nchunk);
+                     beta, sign, nrm2 ? part : nullptr, nchunk);
   VH_LAUNCHED();
   if (nrm2) {
-    hipLaunchKernelGGL(k_reduce2, dim3(nact), dim3(256), 0, st, part.p, nb, nrm2, 0);
+    hipLaunchKernelGGL(k_reduce2, dim3(nact), dim3(256), 0, st, part, nb, nrm2, 0);
     VH_LAUNCHED();
   }
 }
@@ -246,10 +253,10 @@ template <typename R>
 void VecOps<R>::norm2(const cplx<R>* x, int nact, const int* slots, double2* nrm2) {
   hipStream_t st = ctx->stream;
   const int nb = blocks_for(nchunk, nact);
-  hipLaunchKernelGGL((k_mdot<R>), dim3(nb, nact), dim3(KT), 0, st, x, (int64_t)0, x, ld, n, 1, 1, slots, part.p,
+  hipLaunchKernelGGL((k_mdot<R>), dim3(nb, nact), dim3(KT), 0, st, x, (int64_t)0, x, ld, n, 1, 1, slots, part,
                      nchunk);
   VH_LAUNCHED();
-  hipLaunchKernelGGL(k_reduce2, dim3(nact), dim3(256), 0, st, part.p, nb, nrm2, 0);
+  hipLaunchKernelGGL(k_reduce2, dim3(nact), dim3(256), 0, st, part, nb, nrm2, 0);
   VH_LAUNCHED();
 }
 

@@ -12,8 +12,9 @@ template <typename R> struct VecOps {
   int64_t n, ld;
   int kmax;
   int nchunk;
-  DevBuf<double2> part;
-  VecOps(Context* c, int64_t n, int64_t ld, int kmax, int nslot);
+  double2* part;   // reduction partials, part_size(kmax, nslot) entries (owned by the caller's workspace)
+  VecOps(Context* c, int64_t n, int64_t ld, int kmax, int nslot, double2* part);
+  static size_t part_size(int kmax, int nslot);
   void mdot(const cplx<R>* V, int64_t vstride, const cplx<R>* w, int k, int nact, const int* slots, double2* h);
   void combine(const cplx<R>* V, int64_t vstride, cplx<R>* out, int k, int nact, const int* slots,
                const double2* coef, double beta, double sign, double2* nrm2);
@@ -77,6 +78,7 @@ template <typename R> struct KrylovWork {
   cplx<R>* t = nullptr;
   cplx<R>* r = nullptr;
   DevBuf<double2> h, h2, nrm, coef;
+  DevBuf<double2> part;   // partials of the reductions (VecOps), sized for m + 1 vectors and nslot systems
   DevBuf<int> slots_dev, aux_slots;
   // pinned host staging: slot lists, LS coefficients, Hessenberg columns, norms
   PinnedBuf<int> h_slots, h_aux;

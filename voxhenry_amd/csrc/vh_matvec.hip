@@ -1044,6 +1044,23 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
 
 int matvec_num_zshapes(int prec) { return prec == 0 ? num_zshapes<double>() : num_zshapes<float>(); }
 
+template <typename R> static bool zshape_applies(const Geometry& G, int shape) {
+  const GridDims& g = G.g;
+  switch (g.Pz) {   // the one-radix kernel (launch_z1) has no shape
+    case 2: case 3: case 4: case 5: case 7: case 8:
+      if (g.N <= (g.Pz + 1) / 2) return false;
+      break;
+    default: break;
+  }
+  const ZShape z = zshape<R>(shape);
+  return fft_tpp(G.fz.n, z.e, z.nt) >= 0;
+}
+
+bool matvec_zshape_applies(const Geometry& G, int prec, int shape) {
+  if (shape < 0 || shape >= matvec_num_zshapes(prec)) return false;
+  return prec == 0 ? zshape_applies<double>(G, shape) : zshape_applies<float>(G, shape);
+}
+
 void PassTimer::create() {
   for (auto& e : ev) VH_HIP(hipEventCreate(&e));
 }

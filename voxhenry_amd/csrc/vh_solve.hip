@@ -38,6 +38,7 @@ void KrylovWork<R>::bind(char* base, int64_t n_, int64_t ld_, int m_, int nslot_
   h.alloc((size_t)nslot * (m + 1));
   h2.alloc((size_t)nslot * (m + 1));
   coef.alloc((size_t)nslot * (m + 1));
+  part.alloc(VecOps<R>::part_size(m + 1, nslot));
   nrm.alloc(nslot);
   slots_dev.alloc(nslot);
   aux_slots.alloc(nslot);
@@ -109,7 +110,7 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
   VH_REQUIRE(W.V && W.ld == n && W.m == m && W.nslot >= nslot && (!o.flexible || W.Z),
              "internal: Krylov workspace not laid out for this solve");
   const double t_alloc = std::chrono::duration<double>(clk::now() - t_start).count();
-  VecOps<R> ops(ctx, n, n, m + 1, W.nslot);
+  VecOps<R> ops(ctx, n, n, m + 1, W.nslot, W.part.p);
   const int64_t vstr = (int64_t)W.nslot * n;
   if ((int)st.size() < nslot) st.resize(nslot);
   const bool left = (M != nullptr) && !o.flexible;

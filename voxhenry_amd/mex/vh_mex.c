@@ -54,6 +54,7 @@ static struct {
   int64_t num_curr, num_node, n;
   double st2[6];
   int has_inc, has_z, prepared, has_jw;
+  int z_has_imag; /* the impedance was set with a nonzero z_imag (two-fluid: depends on f) */
   double jw[2]; /* j*omega*mu of the last prepare */
   int pc_type;
   double prectol;
@@ -191,7 +192,7 @@ static void make_geom(int64_t L, int64_t M, int64_t N, double dx, int64_t nv, co
   if (S.geom) { vh_geom_destroy(S.geom); S.geom = NULL; }
   free(S.idx);
   S.idx = NULL;
-  S.has_inc = S.has_z = S.prepared = S.has_jw = 0;
+  S.has_inc = S.has_z = S.prepared = S.has_jw = S.z_has_imag = 0;
   /* the geometry is published only once it is complete: a failure leaves no geometry */
   int64_t* keep = (int64_t*)malloc((size_t)nv * sizeof(int64_t));
   if (!keep) fail("voxhenry:lib", "out of host memory");
@@ -265,7 +266,12 @@ static void set_impedance(const mxArray* zr, const mxArray* zi) {
     z[2 * v] = r[S.idx[v]];
     z[2 * v + 1] = 0.0;
   }
+  S.z_has_imag = 0;
   if (zi && !mxIsEmpty(zi)) {
+    const double* zd = mxIsComplex(zi) ? NULL : mxGetDoubles(zi);
+    const mxComplexDouble* zc = mxIsComplex(zi) ? mxGetComplexDoubles(zi) : NULL;
+    for (int64_t v = 0; v < S.nv && !S.z_has_imag; ++v)
+      S.z_has_imag = zc ? (zc[S.idx[v]].real != 0.0 || zc[S.idx[v]].imag != 0.0) : zd[S.idx[v]] != 0.0;
     if (mxIsComplex(zi)) {
       const mxComplexDouble* c = mxGetComplexDoubles(zi);
       for (int64_t v = 0; v < S.nv; ++v) {
@@ -477,6 +483,12 @@ static void cmd_gmres(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]
 static void cmd_solve_batch(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   nargs(nrhs, 6, 6, nlhs, 5, "solve_batch");
   need_operator();
+  /* every frequency's preconditioner and matvec use the geometry's one
+   * impedance; a two-fluid impedance (VoxHenry_executer.m:243-254) changes
+   * with f, so it would be wrong at every frequency but the one it was set for */
+  if (S.z_has_imag)
+    fail("voxhenry:usage", "solve_batch uses one impedance for all frequencies; the impedance has a nonzero z_imag "
+                           "(two-fluid conductors, frequency dependent): solve per frequency instead");
   if (!mxIsDouble(prhs[2]) || mxIsSparse(prhs[2]) || mxGetNumberOfElements(prhs[2]) < 1)
     fail("voxhenry:usage", "jwmu must be a double array of one value per frequency");
   const int64_t nf = (int64_t)mxGetNumberOfElements(prhs[2]);

@@ -199,31 +199,62 @@ class Solver:
         return out
 
 
-def sweep_concurrent(jobs, method='mixed', **kw):
-    """Batched sweeps of several geometries at once (the uneven batch): jobs
-    is a list of (solver, freqs, precons, pidx_of_freq, stream) where every
-    solver has a Context of its own and stream is a torch stream for it.
-    Each geometry's sweep (one vh_solve_batch over its port x frequency
-    systems) runs from its own host thread on its own stream; the library
-    calls release the interpreter lock, so the geometries overlap on the
-    GPU.  Returns the sweeps' outputs in job order and the wall time."""
+def workspace_bytes(S, nsys, methods=('mixed', 'fp64'), inner_restart=40):
+    """Device bytes the batched solves of `nsys` systems of solver S allocate
+    when every listed method runs on it: the Krylov arena grows to the
+    largest method's layout, the matvec keeps a pass-X grid per precision
+    and one shared B grid (vh_solve_batch_bytes per method)."""
+    per = max(S.G.batch_bytes(m, inner_restart=inner_restart) for m in methods)
+    if 'mixed' in methods and 'fp64' in methods:
+        Px = S.G.pad[0]
+        L, M, N = S.G.dims
+        per += 5.0 * Px * M * N * 8     # the FP32 pass-X grid beside the FP64 one
+    return nsys * per
+
+
+def sweep_concurrent(jobs, method='mixed', workers=None, **kw):
+    """Batched sweeps of several geometries as one run (the uneven batch).
+
+    jobs: list of (solver, freqs, precons, pidx_of_freq, stream) or of
+    (solver, freqs, precons, pidx_of_freq, stream, hbm_budget), where every
+    solver has a Context of its own.  workers (default: one per job) host
+    threads take the jobs from one queue in list order, each running its
+    geometry's whole sweep (one vh_solve_batch over its port x frequency
+    systems) on the job's stream; the library calls release the interpreter
+    lock, so the geometries overlap on the GPU.  A job's hbm_budget (bytes)
+    caps its solve's workspaces (vh_solve_batch chunks the systems to fit),
+    so that geometries that run together cannot oversubscribe HBM.  Returns
+    the sweeps' outputs in job order and the wall time."""
     import threading
     import torch
     out = [None] * len(jobs)
     errs = []
-    go = threading.Barrier(len(jobs) + 1)
+    nw = len(jobs) if workers is None else max(1, min(int(workers), len(jobs)))
+    go = threading.Barrier(nw + 1)
+    lock = threading.Lock()
+    nxt = [0]
 
-    def run(i):
-        S, freqs, precons, pidx, stream = jobs[i]
+    def run():
         try:
             go.wait()
-            with torch.cuda.stream(stream):
-                out[i] = S.sweep_batched(freqs, method=method, precons=precons, pidx_of_freq=pidx, device=True, **kw)
-                stream.synchronize()
+            while not errs:
+                with lock:
+                    i = nxt[0]
+                    nxt[0] += 1
+                if i >= len(jobs):
+                    return
+                S, freqs, precons, pidx, stream = jobs[i][:5]
+                opts = dict(kw)
+                if len(jobs[i]) > 5 and jobs[i][5]:
+                    opts['hbm_budget'] = float(jobs[i][5])
+                with torch.cuda.stream(stream):
+                    out[i] = S.sweep_batched(freqs, method=method, precons=precons, pidx_of_freq=pidx, device=True,
+                                             **opts)
+                    stream.synchronize()
         except Exception as e:
             errs.append(e)
 
-    th = [threading.Thread(target=run, args=(i,)) for i in range(len(jobs))]
+    th = [threading.Thread(target=run) for _ in range(nw)]
     for t in th:
         t.start()
     go.wait()

@@ -145,23 +145,67 @@ def loop(radius, section=5, gap=2, dx=1e-6, sigma=5.8e7, freq=(1e9,)):
     """Circular loop in the xy plane (square section `section`), cut by a gap of
     `gap` voxels on the +x axis; the port spans the gap (torus test of
     VoxHenry_generator_numex3_circular_coil.m:97-124 on a square section)."""
-    R = float(radius)
+    return loops([radius], section, gap, dx, sigma, freq)
+
+
+def loops(radii, section=5, gap=2, dx=1e-6, sigma=5.8e7, freq=(1e9,)):
+    """Concentric circular loops in the xy plane (square section `section`,
+    radii must differ by more than `section`), all cut by one gap of `gap`
+    voxels on the +x axis; loop i has its own port across the gap (P on the
+    +y side of the cut, N on the -y side), so k loops give k ports."""
+    radii = [float(r) for r in radii]
+    R = max(radii)
     n = int(2 * R + section + 2)
     c = (n - 1) / 2.0
     xx, yy = np.meshgrid(np.arange(n), np.arange(n), indexing='ij')
     r = np.hypot(xx - c, yy - c)
-    ring = np.abs(r - R) < section / 2.0
     cut = (np.abs(yy - c) < gap / 2.0 + 0.5) & (xx > c)
-    ring &= ~cut
-    s = np.zeros((n, n, section))
-    s[ring, :] = sigma
-    # the two ends of the cut: voxels adjacent to the gap on either side (faces normal to y)
     ys = np.flatnonzero(cut.any(axis=0))
     y_lo, y_hi = ys.min() - 1, ys.max() + 1
-    P = [[x + 1, y_hi + 1, z + 1, 3] for x in range(n) for z in range(section) if s[x, y_hi, z] > 0 and x > c]
-    Nn = [[x + 1, y_lo + 1, z + 1, 4] for x in range(n) for z in range(section) if s[x, y_lo, z] > 0 and x > c]
+    s = np.zeros((n, n, section))
+    P, Nn = [], []
+    for Ri in radii:
+        ring = (np.abs(r - Ri) < section / 2.0) & ~cut
+        s[ring, :] = sigma
+        # the two ends of this ring at the cut: its voxels adjacent to the gap (faces normal to y)
+        P.append(np.array([[x + 1, y_hi + 1, z + 1, 3] for x in range(n) for z in range(section)
+                           if ring[x, y_hi] and x > c], np.int64).reshape(-1, 4))
+        Nn.append(np.array([[x + 1, y_lo + 1, z + 1, 4] for x in range(n) for z in range(section)
+                            if ring[x, y_lo] and x > c], np.int64).reshape(-1, 4))
     return dict(sigma=s, lambdaL=None, freq=np.asarray(freq, float), dx=dx,
-                ports_P=[np.array(P, np.int64)], ports_N=[np.array(Nn, np.int64)], well=np.zeros((0, 4), np.int64))
+                ports_P=P, ports_N=Nn, well=np.zeros((0, 4), np.int64))
+
+
+def parallel(kind, count, length, section, dx=1e-6, freq=(1e9,), **kw):
+    """`count` parallel conductors along x, one port each (P on the +x end
+    faces, N on the -x end faces of that conductor): kind 'bar' (square
+    section x section, y pitch 2*section), 'wire' (round, diameter section,
+    pitch section + 2) or 'strip' (two-fluid, width section, thickness
+    kw.get('thickness', 2), pitch section + 4)."""
+    if kind == 'bar':
+        one, pitch = bar(length, section, section, dx=dx, freq=freq), 2 * section
+    elif kind == 'wire':
+        one, pitch = wire(length, section, dx=dx, freq=freq), section + 2
+    elif kind == 'strip':
+        one, pitch = strip(length, section, kw.get('thickness', 2), dx=dx, freq=freq), section + 4
+    else:
+        raise ValueError(kind)
+    s1 = one['sigma']
+    L, M1, N = s1.shape
+    M = pitch * (count - 1) + M1
+    s = np.zeros((L, M, N))
+    lam = None if one['lambdaL'] is None else np.zeros((L, M, N))
+    P, Nn = [], []
+    for i in range(count):
+        y0 = pitch * i
+        s[:, y0:y0 + M1, :] = s1
+        if lam is not None:
+            lam[:, y0:y0 + M1, :] = one['lambdaL']
+        box = (slice(None), slice(y0, y0 + M1), slice(None))
+        P.append(end_faces(s, 0, True, box))
+        Nn.append(end_faces(s, 0, False, box))
+    return dict(sigma=s, lambdaL=lam, freq=np.asarray(freq, float), dx=dx, ports_P=P, ports_N=Nn,
+                well=np.zeros((0, 4), np.int64))
 
 
 def strip(length, width, thickness=2, dx=1e-6, sigma=1e7, lam=90e-9, freq=(1e9,)):
@@ -180,19 +224,35 @@ def cube(n, dx=1e-6, sigma=5.8e7):
 
 
 def uneven_batch(seed, count=96):
-    """Seeded mixture of bars, round wires, loops and two-fluid strips (the
-    uneven-batch config); returns a list of (kind, structure)."""
+    """Seeded mixture of the four conductor families of the uneven-batch
+    workload, cycled bar, wire, loop, strip; every geometry has 1-4 ports
+    (one per conductor of a parallel bundle, or per concentric loop):
+
+    * copper bars, length 16-512 voxels, square section 2-16;
+    * round copper wires, length 16-256, diameter 4-20;
+    * circular copper loops, outer radius 20-150, 5 x 5 section, concentric
+      loops 8 voxels apart (as many of the drawn port count as fit inside);
+    * two-fluid strips (sigma 1e7 S/m, London depth 90 nm), length 16-512,
+      width 4-32, thickness 2.
+
+    Returns a list of (kind, structure)."""
     rng = np.random.default_rng(seed)
     out = []
     kinds = ['bar', 'wire', 'loop', 'strip']
     for i in range(count):
         k = kinds[i % 4]
+        nports = int(rng.integers(1, 5))
         if k == 'bar':
-            out.append((k, bar(int(rng.integers(16, 513)), int(rng.integers(2, 17)), int(rng.integers(2, 17)))))
+            L, sec = int(rng.integers(16, 513)), int(rng.integers(2, 17))
+            out.append((k, parallel('bar', nports, L, sec)))
         elif k == 'wire':
-            out.append((k, wire(int(rng.integers(16, 257)), int(rng.integers(4, 21)))))
+            L, d = int(rng.integers(16, 257)), int(rng.integers(4, 21))
+            out.append((k, parallel('wire', nports, L, d)))
         elif k == 'loop':
-            out.append((k, loop(int(rng.integers(20, 151)))))
+            R = int(rng.integers(20, 151))
+            cnt = max(1, min(nports, 1 + (R - 12) // 8))
+            out.append((k, loops([R - 8 * j for j in range(cnt)])))
         else:
-            out.append((k, strip(int(rng.integers(16, 513)), int(rng.integers(4, 33)))))
+            L, w = int(rng.integers(16, 513)), int(rng.integers(4, 33))
+            out.append((k, parallel('strip', nports, L, w)))
     return out
