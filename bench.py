@@ -326,10 +326,12 @@ def uneven_config(count, nfreq, seed=0, workers=8, budget_frac=0.7):
     batched solve, the geometries run as one job queue served by `workers`
     host threads (a stream each), admitted in groups whose device memory
     (set-up plus the solves' workspaces, vh_solve_batch_bytes) fits
-    budget_frac of the free HBM.  Per group: set-up and one warm-up sweep per
-    method (untimed; they allocate the workspaces), then one timed
-    concurrent pass per method, then an untimed FP64 sweep to 1e-12 as the
-    R/L reference.  solves/s = converged systems / summed timed wall."""
+    budget_frac of the free HBM; a geometry that does not fit beside its
+    group opens the next one (its set-up is kept).  Per group, through the
+    job queue: a mixed warm-up sweep and the FP64 sweep to 1e-12 that serves
+    as the R/L reference (untimed; together they allocate the workspaces of
+    both methods), then one timed pass per method.  solves/s = converged
+    systems / summed timed wall."""
     import torch
     import voxhenry_amd as vh
     from voxhenry_amd import structures, solver as vsol
@@ -349,46 +351,49 @@ def uneven_config(count, nfreq, seed=0, workers=8, budget_frac=0.7):
     per = []
     pos = 0
     setup_s = 0.0
-    while pos < len(order):
+    t_all = time.perf_counter()
+    pending = None                     # a set-up geometry that opens the next group
+
+    def setup(i):
+        kind, st = geos[i]
+        gctx = vh.Context(0)
+        S = vsol.Solver(gctx, st)
+        precs = S.precons_for(freqs)
+        nsys = nfreq * S.nports
+        return dict(i=i, kind=kind, S=S, precs=precs, ctx=gctx, stream=torch.cuda.Stream(), nsys=nsys,
+                    ws=min(vsol.workspace_bytes(S, nsys), cap))
+
+    while pos < len(order) or pending is not None:
         jobs = []
         need = 0.0
-        while pos < len(order):
-            i = order[pos]
-            kind, st = geos[i]
-            t0 = time.perf_counter()
-            gctx = vh.Context(0)
-            S = vsol.Solver(gctx, st)
-            precs = S.precons_for(freqs)
-            setup_s += time.perf_counter() - t0
-            nsys = nfreq * S.nports
-            ws = min(vsol.workspace_bytes(S, nsys), cap)
+        while pending is not None or pos < len(order):
+            if pending is None:
+                t0 = time.perf_counter()
+                j = setup(order[pos])
+                pos += 1
+                setup_s += time.perf_counter() - t0
+            else:
+                j, pending = pending, None
             free = torch.cuda.mem_get_info()[0]
-            if jobs and need + ws > free - (1 - budget_frac) * free0:
-                # does not fit beside the group: it opens the next group
-                for P in precs:
-                    P.close()
-                S.G.close()
-                gctx.close()
-                torch.cuda.empty_cache()
+            if jobs and need + j['ws'] > free - (1 - budget_frac) * free0:
+                pending = j            # does not fit beside the group: it opens the next group
                 break
-            stream = torch.cuda.Stream()
-            jobs.append(dict(i=i, kind=kind, S=S, precs=precs, ctx=gctx, stream=stream, ws=ws, nsys=nsys))
-            need += ws
-            pos += 1
+            jobs.append(j)
+            need += j['ws']
         groups.append(len(jobs))
-        # warm-up (allocates every geometry's workspaces), one geometry after another
-        for j in jobs:
-            with torch.cuda.stream(j['stream']):
-                for m in ('mixed', 'fp64'):
-                    j['S'].sweep_batched(freqs, method=m, precons=j['precs'], pidx_of_freq=pidx, device=True,
-                                         hbm_budget=j['ws'])
-            j['stream'].synchronize()
-        res = {}
-        for m in ('mixed', 'fp64'):
+
+        def pool(method, **kw):
             torch.cuda.synchronize()
             outs, wall = vsol.sweep_concurrent([(j['S'], freqs, j['precs'], pidx, j['stream'], j['ws']) for j in jobs],
-                                               method=m, workers=workers)
+                                               method=method, workers=workers, **kw)
             torch.cuda.synchronize()
+            return outs, wall
+
+        pool('mixed')                                   # warm-up
+        refs, _ = pool('fp64', tol=TIGHT, maxit=20)     # the R/L reference (and the FP64 warm-up)
+        res = {}
+        for m in ('mixed', 'fp64'):
+            outs, wall = pool(m)
             res[m] = outs
             conv = sum(int(np.sum(o['stats']['flag'] == 0)) for o in outs)
             tot[m]['wall'] += wall
@@ -396,16 +401,13 @@ def uneven_config(count, nfreq, seed=0, workers=8, budget_frac=0.7):
             log('[bench]   uneven group of %d, %s: %d/%d converged in %.3f s'
                 % (len(jobs), m, conv, sum(j['nsys'] for j in jobs), wall))
         for k, j in enumerate(jobs):
-            with torch.cuda.stream(j['stream']):
-                ref = j['S'].sweep_batched(freqs, method='fp64', tol=TIGHT, precons=j['precs'], pidx_of_freq=pidx,
-                                           device=True, maxit=20, hbm_budget=j['ws'])
-            j['stream'].synchronize()
+            ref = refs[k]
             nsys_all += j['nsys']
             kd = kinds.setdefault(j['kind'], dict(geometries=0, systems=0, mixed_conv=0, fp64_conv=0))
             kd['geometries'] += 1
             kd['systems'] += j['nsys']
             rec = dict(kind=j['kind'], grid=list(geos[j['i']][1]['sigma'].shape), ports=j['S'].nports,
-                       nv=int(len(j['S'].idx)))
+                       nv=int(len(j['S'].idx)), ref_relres_max=float(ref['stats']['relres'].max()))
             for m in ('mixed', 'fp64'):
                 o = res[m][k]
                 c = int(np.sum(o['stats']['flag'] == 0))
@@ -427,7 +429,7 @@ def uneven_config(count, nfreq, seed=0, workers=8, budget_frac=0.7):
         torch.cuda.empty_cache()
     sps = {m: tot[m]['conv'] / tot[m]['wall'] for m in tot}
     return dict(seed=seed, geometries=count, nfreq=nfreq, systems=nsys_all, workers=workers, groups=groups,
-                budget_GB=budget / 1e9, setup_s=setup_s,
+                budget_GB=budget / 1e9, setup_s=setup_s, wall_s=time.perf_counter() - t_all,
                 mixed=dict(solves_per_s=sps['mixed'], s=tot['mixed']['wall'], converged=tot['mixed']['conv'],
                            systems=nsys_all),
                 fp64=dict(solves_per_s=sps['fp64'], s=tot['fp64']['wall'], converged=tot['fp64']['conv'],

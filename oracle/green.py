@@ -152,14 +152,29 @@ def generate_G(L, M, N, dx, k0=K0_ONE_OVER_R, ktable=None):
     Kt = Kt * np.array([s ** 5, s ** 6, s ** 7])
     G[:2, :2, :2, :] = 0
     _scatter_sym(G, rows, Kt)
-    # normalisation, :761-768
+    G = _normalise(G, dx)
+    st = np.array([G[0, 0, 0, 0], G[0, 0, 0, 3], G[0, 0, 0, 6]])      # :772
+    return G, st
+
+
+def _normalise(G, dx):
+    """Basis normalisation of the 7 components, :761-768."""
     G = G / dx ** 4
     G[..., 1:3] /= dx
     G[..., 3] /= dx ** 2
     G[..., 4] /= dx
     G[..., 5:7] /= dx ** 2
-    st = np.array([G[0, 0, 0, 0], G[0, 0, 0, 3], G[0, 0, 0, 6]])      # :772
-    return G, st
+    return G
+
+
+def far_entries(m, dx, k0=K0_ONE_OVER_R):
+    """Normalised tensor entries G_mn (C,7) at integer offsets m (C,3) from the
+    far-field rule alone (:56-74, then :761-768): the values generate_G holds
+    at every offset outside the medium zone [0, 5)^3, on any grid.  Lets a
+    test check single entries of a large grid's tensor without computing the
+    whole grid."""
+    m = np.asarray(m, dtype=float).reshape(-1, 3)
+    return _normalise(_assemble7(_vv_sums(m * dx, dx, NP_FAR, k0)), dx)
 
 
 def embed(G, pad=None):

@@ -10,7 +10,10 @@ Restated from:
 All indices here are 0-based; the numbering ORDER is the reference's.
 Pinned by: exact structural checks (Ae column sums / node degrees, the numex1
 golden solution's node layout) and the closed-form DC resistance of a bar,
-tests/test_oracle_topology.py and tests/test_oracle_solve.py.
+tests/test_oracle_topology.py and tests/test_oracle_solve.py.  The two-fluid
+impedance (VoxHenry_executer.m:243-254) is pinned by the parallel-channel
+identity z = 1/(sigma + 1/(j w mu lam^2)) and its sigma = 0 limit
+j w mu lam^2 (tests/test_oracle_matvec.py::test_two_fluid_impedance_identity).
 """
 import numpy as np
 import scipy.sparse as sp

@@ -75,32 +75,3 @@ def read_vhr(path):
                 freq=np.array(freq if freq else [1.0]), dx=dx,
                 ports_P=[as_arr(p) for p in P], ports_N=[as_arr(n) for n in N],
                 well_cond=as_arr(well))
-
-
-def write_vhr(path, sigma, dx, freq, ports_P, ports_N, lambdaL=None):
-    """Writer for synthetic geometries, in the format the reader above accepts."""
-    with open(path, 'w') as fh:
-        fh.write('* VoxHenry input file (synthetic)\n')
-        fr = list(freq)
-        for k in range(0, len(fr), 10):
-            fh.write('freq= ' + ' '.join('%.17g' % f for f in fr[k:k + 10]) + '\n')
-        fh.write('dx=%.17g\n' % dx)
-        L, M, N = sigma.shape
-        fh.write('LMN=%d,%d,%d\n' % (L, M, N))
-        if lambdaL is not None:
-            fh.write('superconductor\n')
-        fh.write('StartVoxelList\n')
-        nz = np.argwhere(np.abs(sigma.transpose(2, 1, 0)) > 0)[:, ::-1] if lambdaL is None else \
-            np.argwhere((np.abs(sigma) + np.abs(lambdaL)).transpose(2, 1, 0) > 0)[:, ::-1]
-        for x, y, z in nz:
-            if lambdaL is None:
-                fh.write('V %d %d %d %.17g\n' % (x + 1, y + 1, z + 1, sigma[x, y, z]))
-            else:
-                fh.write('V %d %d %d %.17g %.17g\n' % (x + 1, y + 1, z + 1, sigma[x, y, z], lambdaL[x, y, z]))
-        fh.write('EndVoxelList\n')
-        inv = {v: k for k, v in SIDES.items()}
-        for p, (PP, NN) in enumerate(zip(ports_P, ports_N)):
-            for r in PP:
-                fh.write('N port%d P %d %d %d %s\n' % (p + 1, r[0], r[1], r[2], inv[int(r[3])]))
-            for r in NN:
-                fh.write('N port%d N %d %d %d %s\n' % (p + 1, r[0], r[1], r[2], inv[int(r[3])]))

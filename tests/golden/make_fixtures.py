@@ -11,6 +11,10 @@ Outputs:
                          data_curr_plot.mat (60x20x20 bar @0.5um, 2.5 GHz, port 1)
   numex1_geometry.npz -- the numex1 .vhr reduced to (dims, dx, sigma, freq, ports);
                          the grid is full, so the voxel list is stored as a check sum
+  numex3_geometry.npz -- the numex3 circular coil .vhr (circular_coil_loop150.0u_rad5.0u,
+                         310x310x10 @1um) reduced to (dims, dx, sigma, the 0-based
+                         column-major indices of its voxels, ports); its 49-value
+                         frequency list is the one of VoxHenry_generator_numex3_circular_coil.m:39-44
 K_near_interactions_1mm.txt and wire_len2.0u_dia4.0u.vhr are plain copies of
 reference data files (the near-field table and the smallest input).
 """
@@ -45,6 +49,14 @@ def main(ref):
     assert (s == 5.8e7).all(), 'numex1 grid expected full'
     np.savez_compressed(os.path.join(HERE, 'numex1_geometry.npz'), dims=np.array(s.shape), dx=g['dx'],
                         sigma=5.8e7, freq=g['freq'], P=g['ports_P'][0], N=g['ports_N'][0])
+    g = vhr.read_vhr(os.path.join(ref, 'Input_files', 'circular_coil_loop150.0u_rad5.0u.vhr'))
+    s = g['sigma']
+    vals = np.unique(s[s != 0])
+    assert len(vals) == 1 and g['lambdaL'] is None and len(g['ports_P']) == 1
+    idx = np.flatnonzero(s.ravel(order='F') != 0).astype(np.int32)
+    np.savez_compressed(os.path.join(HERE, 'numex3_geometry.npz'), dims=np.array(s.shape), dx=g['dx'],
+                        sigma=float(vals[0]), idx=idx, freq=g['freq'], P=g['ports_P'][0], N=g['ports_N'][0],
+                        well=g['well_cond'])
 
 
 if __name__ == '__main__':

@@ -49,3 +49,30 @@ def device_geometry(ctx, geo, sysd, pad_mode=1, f=1e9, tensor='gpu'):
     G.set_incidence(sysd['Ar'])
     G.set_impedance(voxel_z(geo, sysd, f))
     return G
+
+
+def numex3():
+    """numex3 circular coil (circular_coil_loop150.0u_rad5.0u.vhr, 310x310x10
+    @1um, 74 680 voxels, one port), reduced by make_fixtures.py; freq holds the
+    49 frequencies of its committed R/L table."""
+    d = np.load(os.path.join(GOLDEN, 'numex3_geometry.npz'))
+    dims = tuple(int(v) for v in d['dims'])
+    s = np.zeros(int(np.prod(dims)))
+    s[d['idx']] = float(d['sigma'])
+    rl = np.load(os.path.join(GOLDEN, 'numex_RL.npz'))
+    return dict(sigma=s.reshape(dims, order='F'), lambdaL=None, freq=rl['freq3'], dx=float(d['dx']),
+                ports_P=[d['P']], ports_N=[d['N']], well_cond=d['well'])
+
+
+def structure(geo):
+    """Oracle geometry dict (well_cond) -> product structure dict (well)."""
+    out = dict(geo)
+    out['well'] = out.pop('well_cond')
+    return out
+
+
+def oracle_geometry(st):
+    """Product structure dict (well) -> oracle geometry dict (well_cond)."""
+    out = dict(st)
+    out['well_cond'] = out.pop('well')
+    return out

@@ -1,0 +1,365 @@
+"""GPU parity at the benchmark configurations' real sizes, against the FP64
+oracle and against what the mathematics fixes:
+
+* the 128^3 FFT-matvec microbench (padded 256^3) in both precisions against
+  the oracle's FFT block on the GPU-generated tensor (whose entries are
+  checked against the oracle's quadrature), plus the complex symmetry
+  y^T A x = x^T A y of the full-size operator;
+* the 30x10x10 bar over its 10 frequencies and the 5-turn spiral at 4
+  frequencies, both methods against the oracle's per-frequency sweep;
+* the 400x64x4 eight-bar bus with all 512 (port x frequency) systems in one
+  batched solve: true residuals recomputed with the oracle's operator,
+  reciprocity Z = Z^T, the DC resistance l/(sigma A) at 100 kHz;
+* the uneven batch: every conductor family as a small multi-port geometry
+  against the oracle, and one full-size geometry of each family from the
+  seeded batch under the same property checks;
+* the reference's numex3 circular coil (310x310x10 @1um) against its
+  committed R/L table (results_numex3_circular_coil/data_R_jL_mat.mat);
+* fpgmres iteration parity (fpgmres.m:58-152) against the oracle.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import topology, green, matvec as omv, driver, gmres as ogm
+from tests import problems
+
+pytestmark = pytest.mark.gpu
+MU0 = 4 * np.pi * 1e-7
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def workers():
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def oracle_operator(prob, f):
+    """A(x) of the oracle (lse_matvect_mult.m) at frequency f, without the
+    preconditioner set-up of Problem.operators."""
+    s = prob.sys
+    zr, zi = topology.impedance(prob.geo, s['idxS'], f)
+    fN = 1j * 2 * np.pi * f * MU0 * prob.fN2
+    return lambda J: omv.lse_matvect_mult(J, fN, s['Ar'], zr, zi, s['dx'], s['idxS5'])
+
+
+def port_admittance(X, rhs, nfreq):
+    """Y[f][q, p] = rhs_q . x_(f, p) (VoxHenry_executer.m:311-316) for the
+    solutions X of a batch ordered frequency-major, port-minor."""
+    npo = rhs.shape[0]
+    return np.swapaxes(X.reshape(nfreq, npo, -1) @ rhs.T, 1, 2)
+
+
+def self_rl_err(out, ref):
+    """Largest relative error of the self R and L, of R relative to |Z| and
+    of Z normwise, of a sweep against a reference sweep (same frequencies)."""
+    d = np.diagonal(out['Z'], axis1=1, axis2=2)
+    dr = np.diagonal(ref['Z'], axis1=1, axis2=2)
+    eR = np.max(np.abs(np.abs(d.real) - np.abs(dr.real)) / np.abs(dr.real))
+    eL = np.max(np.abs(np.abs(d.imag) - np.abs(dr.imag)) / np.abs(dr.imag))
+    eRZ = np.max(np.abs(np.abs(d.real) - np.abs(dr.real)) / np.abs(dr))
+    eZ = max(rel(out['Z'][k], ref['Z'][k]) for k in range(len(d)))
+    return eR, eL, eRZ, eZ
+
+
+# ---------------------------------------------------------------------------
+# cfg 2: 128^3 FFT-matvec microbench
+# ---------------------------------------------------------------------------
+def test_cube128_matvec_vs_oracle(ctx):
+    import voxhenry_amd as vh
+    import scipy.fft
+    n = 128
+    say('cube128: tensor')
+    G = vh.Geometry(ctx, (n, n, n), 1e-6, np.arange(n ** 3), pad_mode=1)
+    G.generate_green()
+    G.set_impedance(np.full(n ** 3, 1 / 5.8e7, dtype=complex))
+    Gr = G.get_green_realspace()                       # (n, n, n, 7) from the GPU generator
+    # the near and medium zones (offsets < 6) against the oracle's own grid,
+    # far offsets across the whole grid against the oracle's far-field rule
+    Go, _ = green.generate_G(6, 6, 6, 1e-6)
+    assert np.abs(Gr[:6, :6, :6] - Go).max() <= 1e-13 * np.abs(Go).max()
+    rng = np.random.default_rng(4)
+    m = rng.integers(0, n, size=(400, 3))
+    m = m[np.max(m, axis=1) >= 5]
+    Gf = green.far_entries(m, 1e-6)
+    got = Gr[m[:, 0], m[:, 1], m[:, 2]]
+    for c in range(7):
+        scale = np.abs(Gf[:, c]).max()
+        if scale > 0:
+            assert np.abs(got[:, c] - Gf[:, c]).max() <= 1e-13 * scale, c
+    say('cube128: oracle spectrum and FFT block')
+    P = G.pad
+    fN = scipy.fft.fftn(green.embed(Gr, P), axes=(0, 1, 2), workers=workers())
+    del Gr
+    f = 1e9
+    jwmu = 1j * 2 * np.pi * f * MU0
+    V = n ** 3
+    x = rng.standard_normal(5 * V) + 1j * rng.standard_normal(5 * V)
+    omv.FFT_WORKERS = workers()
+    try:
+        ref = omv.fft_block(x.reshape((n, n, n, 5), order='F'), jwmu * fN, (n, n, n))
+    finally:
+        omv.FFT_WORKERS = 1
+    del fN
+    ref = np.concatenate([ref[..., c].ravel(order='F') for c in range(5)])
+    say('cube128: GPU')
+    y64 = G.matvec(x, jwmu, vh.VH_C128, 2)
+    e64 = rel(y64, ref)
+    y32 = G.matvec(x.astype(np.complex64), jwmu, vh.VH_C64, 2).astype(complex)
+    e32 = rel(y32, ref)
+    say('cube128 convolution vs oracle: c128 %.2e  c64 %.2e' % (e64, e32))
+    assert e64 < 1e-12 and e32 < 2e-5
+    # complex symmetry of the FFT block with its diagonal (mode 1) at full size
+    xx = rng.standard_normal(5 * V) + 1j * rng.standard_normal(5 * V)
+    for prec, tol in ((vh.VH_C128, 1e-12), (vh.VH_C64, 1e-5)):
+        dt = np.complex128 if prec == vh.VH_C128 else np.complex64
+        Ax, Axx = G.matvec(np.stack([x, xx]).astype(dt), jwmu, prec, 1).astype(complex)
+        a, b = xx @ Ax, x @ Axx
+        say('symmetry', prec, abs(a - b) / abs(a))
+        assert abs(a - b) <= tol * abs(a)
+    G.close()
+
+
+# ---------------------------------------------------------------------------
+# cfg 1 and cfg 3: sweeps against the oracle
+# ---------------------------------------------------------------------------
+def _sweep_vs_oracle(ctx, st, freqs, tol64, tolmixed_L, tolmixed_RZ):
+    from voxhenry_amd import solver
+    prob = driver.Problem(problems.oracle_geometry(st))
+    ref = driver.solve_sweep(prob, freqs)               # reference tolerance rule
+    tight = driver.solve_sweep(prob, freqs, tol=1e-12, maxit=20)
+    S = solver.Solver(ctx, st)
+    out = {}
+    for m in ('fp64', 'mixed'):
+        out[m] = S.sweep_batched(freqs, method=m)
+        st_ = out[m]['stats']
+        say(m, 'flags', st_['flag'], 'relres max %.2e' % st_['relres'].max())
+    # FP64 (pgmres semantics, same tolerance rule) against the oracle's pgmres
+    eR, eL, eRZ, eZ = self_rl_err(out['fp64'], ref)
+    say('fp64 vs oracle: R %.2e L %.2e R/|Z| %.2e Z %.2e' % (eR, eL, eRZ, eZ))
+    assert eR < tol64 and eL < tol64
+    # mixed against the oracle solved to 1e-12
+    eR, eL, eRZ, eZ = self_rl_err(out['mixed'], tight)
+    say('mixed vs oracle(1e-12): R %.2e L %.2e R/|Z| %.2e Z %.2e' % (eR, eL, eRZ, eZ))
+    assert np.all(out['mixed']['stats']['flag'] == 0)
+    assert eL < tolmixed_L and eRZ < tolmixed_RZ
+    S.G.close()
+    return out, ref, tight
+
+
+def test_bar30_sweep_vs_oracle(ctx):
+    from voxhenry_amd import structures
+    out, ref, tight = _sweep_vs_oracle(ctx, structures.bar(30, 10, 10), np.logspace(4, 10, 10), 1e-9, 1e-6, 1e-6)
+    # the bar's R is resolved at every frequency: R itself within 1e-6 for the mixed path
+    eR = self_rl_err(out['mixed'], tight)[0]
+    assert eR < 1e-6
+    # DC limit: R(10 kHz) = l / (sigma A) = 30e-6 / (5.8e7 * 1e-10)
+    R0 = 30e-6 / (5.8e7 * 1e-10)
+    assert abs(out['fp64']['R'][0, 0, 0] - R0) / R0 < 1e-6
+
+
+def test_spiral_sweep_vs_oracle(ctx):
+    from voxhenry_amd import structures
+    _sweep_vs_oracle(ctx, structures.spiral(), np.array([1e6, 1e8, 1e10, 1e11]), 1e-9, 1e-6, 1e-6)
+
+
+# ---------------------------------------------------------------------------
+# cfg 4: the full bus, 512 systems in one batched solve
+# ---------------------------------------------------------------------------
+def test_bus_full_batch(ctx):
+    import torch
+    import voxhenry_amd as vh
+    from voxhenry_amd import structures, solver
+    st = structures.bus()
+    freqs = np.logspace(5, 10, 64)
+    S = solver.Solver(ctx, st)
+    precs = S.precons_for(freqs)
+    npo = S.nports
+    assert npo == 8
+    B = torch.tensor(np.repeat(S.rhs[None], len(freqs), axis=0).reshape(-1, S.G.n), dtype=torch.complex128,
+                     device='cuda')
+    jw = np.repeat(1j * 2 * np.pi * freqs * MU0, npo)
+    pidx = np.repeat(np.arange(len(freqs)), npo)
+    tols = np.repeat([solver.reference_tol(f) for f in freqs], npo)
+    prob = driver.Problem(problems.oracle_geometry(st))
+    A = {}
+    Zs = {}
+    for m in ('mixed', 'fp64'):
+        say('bus', m)
+        X, stt = S.G.solve_batch(B, jw, precs, pidx, method=m, tol=tols)
+        assert np.all(stt['flag'] == 0) and np.all(stt['relres'] <= tols)
+        Xh = X.cpu().numpy()
+        # true residuals of sampled systems with the oracle's operator
+        for k, p in ((0, 0), (7, 3), (40, 5), (63, 7)):
+            s_ = k * npo + p
+            f = freqs[k]
+            if f not in A:
+                A[f] = oracle_operator(prob, f)
+            b = S.rhs[p]
+            r = np.linalg.norm(b - A[f](Xh[s_])) / np.linalg.norm(b)
+            say('  f %.3g port %d: oracle relres %.2e (tol %.0e, library %.2e)' % (f, p, r, tols[s_], stt['relres'][s_]))
+            assert r <= 2 * tols[s_] + 1e-12
+        Z = np.linalg.inv(port_admittance(Xh, S.rhs, len(freqs)))
+        Zs[m] = Z
+        # reciprocity of the port impedance matrix (complex-symmetric operator)
+        asym = max(rel(Z[k], Z[k].T) for k in range(len(freqs)))
+        say('  reciprocity ||Z - Z^T|| / ||Z|| max %.2e' % asym)
+        assert asym < 1e-6
+        # DC limit of every bar at 100 kHz (skin depth ~209 um >> 4 um section)
+        R0 = 400e-6 / (5.8e7 * 16e-12)
+        eR0 = np.max(np.abs(np.diagonal(Z[0]).real - R0)) / R0
+        say('  R(100 kHz) vs l/(sigma A): %.2e' % eR0)
+        assert eR0 < 1e-6
+        del X
+    d = max(rel(Zs['mixed'][k], Zs['fp64'][k]) for k in range(len(freqs)))
+    say('bus mixed vs fp64 Z %.2e' % d)
+    assert d < 1e-6
+    for P in precs:
+        P.close()
+    S.G.close()
+    torch.cuda.empty_cache()
+
+
+# ---------------------------------------------------------------------------
+# cfg 5: the uneven batch's conductor families
+# ---------------------------------------------------------------------------
+SMALL_FAMILIES = {
+    'bar': lambda s: s.parallel('bar', 2, 24, 3),
+    'wire': lambda s: s.parallel('wire', 2, 20, 5),
+    'loop': lambda s: s.loops([9, 15], section=3),
+    'strip': lambda s: s.parallel('strip', 2, 20, 4),
+}
+
+
+@pytest.mark.parametrize('kind', list(SMALL_FAMILIES))
+def test_uneven_family_small_vs_oracle(ctx, kind):
+    """Two-port geometry of each family (the two-fluid strip included): the
+    batched sweep of both methods against the oracle's per-frequency solves
+    to 1e-12."""
+    from voxhenry_amd import structures, solver
+    st = SMALL_FAMILIES[kind](structures)
+    freqs = np.array([1e6, 1e8, 1e10])
+    prob = driver.Problem(problems.oracle_geometry(st))
+    tight = driver.solve_sweep(prob, freqs, tol=1e-12, maxit=20)
+    S = solver.Solver(ctx, st)
+    assert S.nports == 2
+    for m in ('mixed', 'fp64'):
+        out = S.sweep_batched(freqs, method=m)
+        eR, eL, eRZ, eZ = self_rl_err(out, tight)
+        say('%s %s: flags %s, R %.2e L %.2e R/|Z| %.2e Z %.2e' % (kind, m, out['stats']['flag'], eR, eL, eRZ, eZ))
+        asym = max(rel(out['Z'][k], out['Z'][k].T) for k in range(len(freqs)))
+        assert asym < 1e-6
+        if m == 'mixed' or kind != 'strip':
+            # the FP64 pgmres semantics may stop a two-fluid system on its
+            # preconditioned residual above tol (flag 1); the mixed path's
+            # outer FGMRES tests the true residual
+            assert np.all(out['stats']['flag'] == 0)
+            assert eL < 1e-6 and eRZ < 1e-6 and eZ < 1e-6
+    S.G.close()
+
+
+@pytest.mark.parametrize('kind', ['bar', 'wire', 'loop', 'strip'])
+def test_uneven_family_full_size(ctx, kind):
+    """The first geometry of a family in the seeded 96-geometry batch at full
+    size, 16 frequencies, mixed path: every system converges, the oracle's
+    operator confirms one true residual, Z is reciprocal, and the FP64 path
+    solved to 1e-10 gives the same Z."""
+    import torch
+    from voxhenry_amd import structures, solver
+    freqs = np.logspace(6, 10, 16)
+    st = next(g for k, g in structures.uneven_batch(0, 8) if k == kind)
+    say('family', kind, st['sigma'].shape, 'ports', len(st['ports_P']))
+    S = solver.Solver(ctx, st)
+    precs = S.precons_for(freqs)
+    npo = S.nports
+    B = torch.tensor(np.repeat(S.rhs[None], len(freqs), axis=0).reshape(-1, S.G.n), dtype=torch.complex128,
+                     device='cuda')
+    jw = np.repeat(1j * 2 * np.pi * freqs * MU0, npo)
+    pidx = np.repeat(np.arange(len(freqs)), npo)
+    tols = np.repeat([solver.reference_tol(f) for f in freqs], npo)
+    X, stt = S.G.solve_batch(B, jw, precs, pidx, method='mixed', tol=tols)
+    assert np.all(stt['flag'] == 0) and np.all(stt['relres'] <= tols)
+    Xh = X.cpu().numpy()
+    del X
+    prob = driver.Problem(problems.oracle_geometry(st))
+    k, p = 11, npo - 1
+    s_ = k * npo + p
+    r = np.linalg.norm(S.rhs[p] - oracle_operator(prob, freqs[k])(Xh[s_])) / np.linalg.norm(S.rhs[p])
+    say('  oracle relres %.2e (library %.2e)' % (r, stt['relres'][s_]))
+    assert r <= 2 * tols[s_]
+    Z = np.linalg.inv(port_admittance(Xh, S.rhs, len(freqs)))
+    asym = max(rel(Z[q], Z[q].T) for q in range(len(freqs)))
+    X64, st64 = S.G.solve_batch(B, jw, precs, pidx, method='fp64', tol=1e-10)
+    Z64 = np.linalg.inv(port_admittance(X64.cpu().numpy(), S.rhs, len(freqs)))
+    d = max(rel(Z[q], Z64[q]) for q in range(len(freqs)))
+    say('  reciprocity %.2e, mixed vs fp64(1e-10) Z %.2e' % (asym, d))
+    assert asym < 1e-6 and d < 1e-6
+    del X64, B
+    for P in precs:
+        P.close()
+    S.G.close()
+    torch.cuda.empty_cache()
+
+
+# ---------------------------------------------------------------------------
+# numex3 golden R/L
+# ---------------------------------------------------------------------------
+def test_numex3_RL_table_gpu(ctx):
+    """The reference's committed R/L of its circular coil example (49
+    frequencies, 1 Hz - 1 THz) through the GPU FP64 path with the driver's
+    tolerance rule (VoxHenry_executer.m:219-223), and the mixed path."""
+    from voxhenry_amd import solver
+    geo = problems.numex3()
+    st = problems.structure(geo)
+    freqs = geo['freq']
+    rl = np.load(os.path.join(problems.GOLDEN, 'numex_RL.npz'))['RL3']
+    S = solver.Solver(ctx, st)
+    for m in ('fp64', 'mixed'):
+        out = S.sweep_batched(freqs, method=m)
+        R, L = out['R'][:, 0, 0], out['L'][:, 0, 0]
+        eR = np.abs(R - rl.real) / rl.real
+        eL = np.abs(L - rl.imag) / rl.imag
+        say('numex3 %s: converged %d/%d, R err max %.2e (median %.2e), L err max %.2e (median %.2e)'
+            % (m, np.sum(out['stats']['flag'] == 0), len(freqs), eR.max(), np.median(eR), eL.max(), np.median(eL)))
+        for k in np.argsort(-eR)[:3]:
+            say('   f %.3g  R err %.2e  L err %.2e' % (freqs[k], eR[k], eL[k]))
+        assert eR.max() < 1e-5 and eL.max() < 1e-5
+    S.G.close()
+
+
+# ---------------------------------------------------------------------------
+# fpgmres iteration parity
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize('f', [1e3, 1e8])
+def test_fpgmres_matches_oracle(ctx, f):
+    """Flexible GMRES (fpgmres.m:58-152, right preconditioner, bnorm = ||b||):
+    same iteration count and residual history as the oracle's fpgmres."""
+    import voxhenry_amd as vh
+    geo = problems.bar(12, 5, 4)
+    s = topology.build_system(geo)
+    G = problems.device_geometry(ctx, geo, s, pad_mode=1, f=f)
+    P = vh.Precon(G)
+    jwmu = 1j * 2 * np.pi * f * MU0
+    P.prepare(jwmu)
+    prob = driver.Problem(geo)
+    A, Pm, *_ = prob.operators(f)
+    b = s['rhs'][:, 0]
+    tol = driver.reference_tol(f)
+    xo, flo, rro, ito, rvo = ogm.fpgmres(A, b, 100, tol, 10, Pm)
+    x, st = G.gmres(b, jwmu, P, restart=100, tol=tol, maxit=10, flexible=True, resvec_cap=64)
+    say('fpgmres f %.3g: iters gpu %d oracle %d, relres %.2e / %.2e' % (f, st['niters'][0], len(rvo) - 1,
+                                                                         st['relres'][0], rro))
+    assert st['flag'][0] == 0 and flo == 0
+    assert st['niters'][0] == len(rvo) - 1
+    assert np.allclose(st['resvec'][0], rvo, rtol=1e-6, atol=1e-14)
+    assert abs(b @ x - b @ xo) / abs(b @ xo) < 1e-9
+    G.close()

@@ -80,3 +80,25 @@ def test_superconductor_diagonal():
     geo2 = problems.bar(3, 2, 2, sigma=1e7, lam=np.zeros((3, 2, 2)))
     zr2, zi2 = topology.impedance(geo2, s['idxS'], f)
     assert np.allclose(zr2, 1e-7) and np.all(zi2 == 0)
+
+
+@pytest.mark.parametrize('sigma,lam,f', [(1e7, 90e-9, 1e6), (1e7, 90e-9, 1e10), (5.8e7, 50e-9, 3e8),
+                                         (1e6, 200e-9, 1e11), (0.0, 90e-9, 1e9)])
+def test_two_fluid_impedance_identity(sigma, lam, f):
+    """Two-fluid conductor (VoxHenry_executer.m:243-254): the normal channel
+    sigma and the superfluid channel 1/(j w mu lam^2) conduct in parallel, so
+    z_real + z_imag = 1 / (sigma + 1/(j w mu lam^2)); at sigma = 0 the
+    impedance is the kinetic inductance j w mu lam^2 alone.  A dropped square
+    or sign in the reference's den = (sigma w mu lam^2)^2 + 1 fails this.
+    The product's per-voxel impedance (solver.impedance) obeys it too."""
+    from voxhenry_amd import solver as vsol
+    mu = 4 * np.pi * 1e-7
+    jwml = 1j * 2 * np.pi * f * mu * lam ** 2
+    expect = jwml if sigma == 0 else 1.0 / (sigma + 1.0 / jwml)
+    geo = problems.bar(3, 2, 2, sigma=sigma, lam=np.full((3, 2, 2), lam))
+    zr, zi = topology.impedance(geo, np.arange(12), f)
+    z = zr + zi
+    assert np.all(np.abs(z - expect) <= 1e-14 * abs(expect)), (z.ravel()[0], expect)
+    st = dict(sigma=np.full((3, 2, 2), float(sigma)), lambdaL=np.full((3, 2, 2), lam))
+    zp = vsol.impedance(st, np.arange(12), f)
+    assert np.all(np.abs(zp - expect) <= 1e-14 * abs(expect))

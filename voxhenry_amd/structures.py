@@ -64,6 +64,55 @@ def read_vhr(path):
                 ports_P=[arr(p) for p in P], ports_N=[arr(n) for n in N], well=arr(well))
 
 
+def write_vhr(path, structure, title='synthetic geometry'):
+    """Write a structure as a `.vhr` input deck in the layout of
+    src_pre_process/pre_output_file.m:44-130: header comment, freq lines of at
+    most ten values, dx, LMN, the Superconductor keyword when lambdaL is
+    given, the V voxel lines between StartVoxelList / EndVoxelList in
+    column-major voxel order, one N line per excitation (P) and ground (N)
+    face of every port (portK names), and the grounded faces (N portGND G).
+    Numbers are written with 17 significant digits (the reference uses %g),
+    so read_vhr gives back the same structure."""
+    inv = {v: k for k, v in SIDES.items()}
+    sig = np.asarray(structure['sigma'], float)
+    lam = structure.get('lambdaL')
+    L, M, N = sig.shape
+    occ = np.abs(sig) > 0
+    if lam is not None:
+        lam = np.asarray(lam, float)
+        occ |= np.abs(lam) > 0
+    g = lambda v: '%.17g' % v
+    with open(path, 'w') as fh:
+        fh.write('* VoxHenry input file\n* %s\n\n' % title)
+        fh.write('* Frequency points (Hz)\n')
+        fr = [float(f) for f in np.atleast_1d(structure['freq'])]
+        for k in range(0, len(fr), 10):
+            fh.write('freq= ' + ' '.join(g(f) for f in fr[k:k + 10]) + '\n')
+        fh.write('\n* Voxel size (m)\ndx=%s\n\n' % g(structure['dx']))
+        fh.write('* Voxel grid dimension in voxel units; x, y, z\nLMN=%d,%d,%d\n\n' % (L, M, N))
+        if lam is not None:
+            fh.write('* Specify there are superconductors\nSuperconductor\n\n')
+        fh.write('* Voxel list\n* Format is:\n* V <index_x> <index_y> <index_z> <conductivity S/m>\n*\n')
+        fh.write('StartVoxelList\n')
+        for li in np.flatnonzero(occ.ravel(order='F')):
+            x, y, z = np.unravel_index(li, (L, M, N), order='F')
+            if lam is None:
+                fh.write('V %d %d %d %s\n' % (x + 1, y + 1, z + 1, g(sig[x, y, z])))
+            else:
+                fh.write('V %d %d %d %s %s\n' % (x + 1, y + 1, z + 1, g(sig[x, y, z]), g(lam[x, y, z])))
+        fh.write('EndVoxelList\n\n* Port nodes list\n')
+        for p, (PP, NN) in enumerate(zip(structure['ports_P'], structure['ports_N'])):
+            fh.write('*\n')
+            for kind, rows in (('P', PP), ('N', NN)):
+                for r in np.asarray(rows, np.int64).reshape(-1, 4):
+                    fh.write('N port%d %s %d %d %d %s\n' % (p + 1, kind, r[0], r[1], r[2], inv[int(r[3])]))
+        well = np.asarray(structure.get('well', np.zeros((0, 4))), np.int64).reshape(-1, 4)
+        if len(well):
+            fh.write('\n* Grounded nodes list\n*\n')
+            for r in well:
+                fh.write('N portGND G %d %d %d %s\n' % (r[0], r[1], r[2], inv[int(r[3])]))
+
+
 def _faces(mask_slice_coords, side):
     return np.array([[x + 1, y + 1, z + 1, side] for (x, y, z) in mask_slice_coords], dtype=np.int64)
 
