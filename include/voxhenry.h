@@ -113,6 +113,21 @@ int vh_geom_currents_on_grid(vh_geom* g, int64_t nrhs, const void* x, int64_t ld
 int vh_matvec(vh_geom* g, double jwmu_re, double jwmu_im, int prec, int mode, int64_t nrhs, const void* x,
               int64_t ldx, void* y, int64_t ldy, int on_device);
 
+/* Allocation-free variant of vh_matvec.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  x and y
+ * are device pointers (nrhs vectors, leading dimensions ldx, ldy); the pass
+ * intermediates live in a fixed device array of VH_MATVEC_NOALLOC_BYTES per
+ * GPU, held from library load until the process ends.  A call whose
+ * intermediates need more (matvec_ws: 5*nrhs*(Px*M*N + Px*Py*N) complex
+ * elements on the padded grid) fails with VH_ERR_ARG.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (several streams or
+ * threads) race and must be serialised by the caller.  Same operator,
+ * arguments and results as vh_matvec with on_device = 1. */
+#define VH_MATVEC_NOALLOC_BYTES (64u << 20)
+int vh_matvec_noalloc(vh_geom* g, double jwmu_re, double jwmu_im, int prec, int mode, int64_t nrhs, const void* x,
+                      int64_t ldx, void* y, int64_t ldy);
+
 /* Timing helper: run the matvec `reps` times on device-resident data and
  * report the average time of one call (ms); if ms_pass is given, a second
  * sweep records events around each fused pass and reports the average time of

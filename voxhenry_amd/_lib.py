@@ -55,6 +55,7 @@ SIGNATURES = {
     'vh_geom_build_topology': [_P, _I64, _P, _P, _P, _P, _I64, _P, _P],
     'vh_geom_get_rhs': [_P, _P],
     'vh_matvec': [_P, _D, _D, _I, _I, _I64, _P, _I64, _P, _I64, _I],
+    'vh_matvec_noalloc': [_P, _D, _D, _I, _I, _I64, _P, _I64, _P, _I64],
     'vh_matvec_timed': [_P, _D, _D, _I, _I, _I64, _P, _I64, _P, _I64, _I, _P, _P, _P],
     'vh_fft3d': [_P, _P, _I, _I, _I],
     'vh_solve_batch': [_P, _I64, _P, _I64, _P, _P, _P, _P, _I, _I, _D, _P, _I, _I, _I, _D, _D, _P, _P, _P, _P, _P],

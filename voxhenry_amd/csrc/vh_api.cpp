@@ -91,6 +91,7 @@ int vh_ctx_create(int device, vh_ctx** out) {
     VH_HIP(hipGetDeviceProperties(&prop, device));
     c->c.num_cus = prop.multiProcessorCount;
     c->c.lds_per_block = prop.sharedMemPerBlock;
+    vh::scratch_init(device);   // device address of the allocation-free entry point's scratch (device current)
     *out = c.release();
   });
 }
@@ -321,6 +322,30 @@ int vh_geom_set_zshape(vh_geom* g, int prec, int shape) {
     const int p = prec == VH_C128 ? 0 : 1;
     VH_REQUIRE(shape >= 0 && shape < vh::matvec_num_zshapes(p), "shape index out of range");
     g->G.zshape[prec == VH_C128 ? 1 : 0] = shape;
+  });
+}
+
+int vh_matvec_noalloc(vh_geom* g, double jre, double jim, int prec, int mode, int64_t nrhs, const void* x,
+                      int64_t ldx, void* y, int64_t ldy) {
+  return guard([&] {
+    need(g, "g"); need(x, "x"); need(y, "y");
+    auto& G = g->G;
+    VH_REQUIRE(prec == VH_C128 || prec == VH_C64, "prec must be VH_C128 or VH_C64");
+    VH_REQUIRE(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
+    const int64_t n = mode == 0 ? G.n() : G.num_curr;
+    VH_REQUIRE(ldx >= n && ldy >= n, "leading dimension smaller than the vector length");
+    VH_REQUIRE(nrhs >= 1 && nrhs <= 13107, "nrhs out of range");
+    const size_t need_b = vh::matvec_ws_bytes(G, prec == VH_C128 ? 0 : 1, (int)nrhs);
+    if (need_b > vh::scratch_bytes())
+      throw vh::Error(vh::ERR_ARG, "vh_matvec_noalloc: " + std::to_string(nrhs) + " vectors of this geometry need " +
+                                       std::to_string(need_b) + " bytes of scratch, more than the fixed " +
+                                       std::to_string(vh::scratch_bytes()) +
+                                       " (VH_MATVEC_NOALLOC_BYTES): use fewer vectors per call or vh_matvec");
+    char* ws = vh::scratch_ptr(G.ctx->device);
+    if (prec == VH_C128)
+      vh::matvec_ws<double>(G, (int)nrhs, (const double2*)x, ldx, (double2*)y, ldy, jre, jim, mode, ws);
+    else
+      vh::matvec_ws<float>(G, (int)nrhs, (const float2*)x, ldx, (float2*)y, ldy, jre, jim, mode, ws);
   });
 }
 

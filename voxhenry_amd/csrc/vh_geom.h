@@ -121,6 +121,20 @@ void matvec(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, in
             double alpha_re, double alpha_im, int mode = 0, const int* slots = nullptr,
             const double2* alphas = nullptr, const cplx<R>* const* zslot = nullptr);
 
+// The same operator on a caller-owned workspace of matvec_ws_bytes(G, prec,
+// nrhs) bytes (256-byte aligned device memory; prec 0 = FP64, 1 = FP32): no
+// allocation, unit slots, the geometry's j*omega*mu scaling by (are, aim) and
+// impedance.  The workspace is cleared where the passes rely on zeros.
+size_t matvec_ws_bytes(const Geometry& G, int prec, int nrhs);
+template <typename R>
+void matvec_ws(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy, double are,
+               double aim, int mode, char* ws);
+// Device address of this library's fixed scratch array on `device` (looked
+// up once per device by scratch_init, with that device current) and its size.
+void scratch_init(int device);
+char* scratch_ptr(int device);
+size_t scratch_bytes();
+
 // Build the spectral tensor on the device from a real-space normalised tensor
 // G (L*M*N*7 complex, device, column-major [x][y][z][c]).
 void tensor_from_realspace(Geometry& G, const double2* Greal_dev);

@@ -931,17 +931,13 @@ static void launch_z(int shape, int rl, dim3 grid, size_t lds, hipStream_t st, c
   }
 }
 
+// The passes on the pass-X output grid A (whose rows of empty x lines read
+// as zero) and the B grid of the caller.
 template <typename R>
 static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy,
                        double are, double aim, int mode, const int* slots, const double2* alphas,
-                       const cplx<R>* const* zslot) {
-  VH_REQUIRE(G.has_tensor, "matvec: Green's tensor not set");
-  VH_REQUIRE(nrhs >= 1, "matvec: nrhs must be >= 1");
-  VH_REQUIRE(nrhs * 5 <= 65535, "matvec: too many right-hand sides in one call");
+                       const cplx<R>* const* zslot, cplx<R>* A, cplx<R>* B) {
   const GridDims& g = G.g;
-  ensure_ws<R>(G, nrhs);
-  cplx<R>* A = (cplx<R>*)(sizeof(R) == 8 ? G.wsA64.p : G.wsA32.p);
-  cplx<R>* B = (cplx<R>*)G.wsB.p;
   hipStream_t st = G.ctx->stream;
   constexpr int EXY = e_xy<R>();
   const int zsh = std::min(std::max(G.zshape[sizeof(R) == 8 ? 1 : 0], 0), num_zshapes<R>() - 1);
@@ -1079,11 +1075,56 @@ void PassTimer::collect() {
   pending = false;
 }
 
+static void check_matvec(const Geometry& G, int nrhs) {
+  VH_REQUIRE(G.has_tensor, "matvec: Green's tensor not set");
+  VH_REQUIRE(nrhs >= 1, "matvec: nrhs must be >= 1");
+  VH_REQUIRE(nrhs * 5 <= 65535, "matvec: too many right-hand sides in one call");
+}
+
 template <typename R>
 void matvec(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy, double are, double aim,
             int mode, const int* slots, const double2* alphas, const cplx<R>* const* zslot) {
-  run_passes<R>(G, nrhs, x, ldx, y, ldy, are, aim, mode, slots, alphas, zslot);
+  check_matvec(G, nrhs);
+  ensure_ws<R>(G, nrhs);
+  run_passes<R>(G, nrhs, x, ldx, y, ldy, are, aim, mode, slots, alphas, zslot,
+                (cplx<R>*)(sizeof(R) == 8 ? G.wsA64.p : G.wsA32.p), (cplx<R>*)G.wsB.p);
 }
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t matvec_ws_bytes(const Geometry& G, int prec, int nrhs) {
+  const GridDims& g = G.g;
+  const size_t c = prec == 0 ? 16 : 8;
+  return align256((size_t)nrhs * 5 * g.Px * g.M * g.N * c) + (size_t)nrhs * 5 * g.Px * g.Py * g.N * c;
+}
+
+__global__ void k_zero16(uint4* __restrict__ p, int64_t n) {
+  const uint4 z = make_uint4(0, 0, 0, 0);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = z;
+}
+
+template <typename R>
+void matvec_ws(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy, double are,
+               double aim, int mode, char* ws) {
+  check_matvec(G, nrhs);
+  const GridDims& g = G.g;
+  const size_t a = (size_t)nrhs * 5 * g.Px * g.M * g.N * sizeof(cplx<R>);
+  // the rows of x lines without voxels are never written by pass X: clear
+  // them (the whole grid, with a fill kernel: no memset node if captured)
+  if ((int64_t)G.nlines < (int64_t)g.M * g.N) {
+    const int64_t n16 = (int64_t)(a / 16);
+    hipLaunchKernelGGL(k_zero16, dim3((unsigned)std::min<int64_t>(ceil_div(n16, 256), 8192)), dim3(256), 0,
+                       G.ctx->stream, (uint4*)ws, n16);
+    VH_LAUNCHED();
+  }
+  run_passes<R>(G, nrhs, x, ldx, y, ldy, are, aim, mode, nullptr, nullptr, nullptr, (cplx<R>*)ws,
+                (cplx<R>*)(ws + align256(a)));
+}
+
+template void matvec_ws<double>(Geometry&, int, const double2*, int64_t, double2*, int64_t, double, double, int,
+                                char*);
+template void matvec_ws<float>(Geometry&, int, const float2*, int64_t, float2*, int64_t, double, double, int, char*);
 
 template void matvec<double>(Geometry&, int, const double2*, int64_t, double2*, int64_t, double, double, int,
                              const int*, const double2*, const double2* const*);
