@@ -72,8 +72,8 @@ def parse():
     ap.add_argument('--detail', default=os.path.join(ROOT, 'gpurun_out', 'bench_detail.json'),
                     help='file that receives the full record (JSON)')
     ap.add_argument('--probe-pass-z', action='store_true',
-                    help='only the 128^3 complex64 matvec (3 calls) and a reference elementwise kernel of known '
-                         'traffic (1 GiB read + 1 GiB written), for hardware-counter runs')
+                    help='only the 128^3 matvec (3 calls in complex64, then 3 in complex128) and a reference '
+                         'elementwise kernel of known traffic (1 GiB read + 1 GiB written), for hardware-counter runs')
     return ap.parse_args()
 
 
@@ -108,12 +108,14 @@ def probe_pass_z(ctx, n=128):
     import voxhenry_amd as vh
     G = vh.Geometry(ctx, (n, n, n), 1e-6, np.arange(n ** 3), pad_mode=1)
     G.generate_green()
-    best, ms = G.autotune(vh.VH_C64, nrhs=1, reps=3)
-    log('[probe] pass-Z shape %d (ms per shape %s)' % (best, np.round(ms, 3)))
-    x = torch.zeros((1, G.num_curr), dtype=torch.complex64, device='cuda')
-    y = torch.zeros_like(x)
-    for _ in range(3):
-        G.matvec_timed(x, y, 1j * 1e4, vh.VH_C64, 2, 1)
+    for prec, dt in ((vh.VH_C64, torch.complex64), (vh.VH_C128, torch.complex128)):
+        best, ms = G.autotune(prec, nrhs=1, reps=3)
+        log('[probe] prec %d pass-Z shape %d (ms per shape %s)' % (prec, best, np.round(ms, 3)))
+        x = torch.zeros((1, G.num_curr), dtype=dt, device='cuda')
+        y = torch.zeros_like(x)
+        for _ in range(3):
+            G.matvec_timed(x, y, 1j * 1e4, prec, 2, 1)
+        del x, y
     a = torch.empty(1 << 28, dtype=torch.float32, device='cuda')
     a.fill_(1.0)
     b = torch.empty_like(a)
