@@ -263,18 +263,26 @@ def test_bus_full_batch(ctx):
 # cfg 5: the uneven batch's conductor families
 # ---------------------------------------------------------------------------
 SMALL_FAMILIES = {
-    'bar': lambda s: s.parallel('bar', 2, 24, 3),
-    'wire': lambda s: s.parallel('wire', 2, 20, 5),
-    'loop': lambda s: s.loops([9, 15], section=3),
-    'strip': lambda s: s.parallel('strip', 2, 20, 4),
+    # one conductor with its end (gap) faces split into two terminal pairs,
+    # as the uneven batch builds its multi-port geometries
+    'bar': lambda s: s.split_ports(s.bar(24, 3, 3), 2),
+    'wire': lambda s: s.split_ports(s.wire(20, 5), 2),
+    'loop': lambda s: s.split_ports(s.loop(9, section=3), 2),
+    'strip': lambda s: s.split_ports(s.strip(20, 4), 2),
+    # two conductors, one port each
+    'bar-bundle': lambda s: s.parallel('bar', 2, 24, 3),
+    'wire-bundle': lambda s: s.parallel('wire', 2, 20, 5),
+    'loop-concentric': lambda s: s.loops([9, 15], section=3),
+    'strip-bundle': lambda s: s.parallel('strip', 2, 20, 4),
 }
 
 
 @pytest.mark.parametrize('kind', list(SMALL_FAMILIES))
 def test_uneven_family_small_vs_oracle(ctx, kind):
-    """Two-port geometry of each family (the two-fluid strip included): the
-    batched sweep of both methods against the oracle's per-frequency solves
-    to 1e-12."""
+    """Two-port geometries of each family (the two-fluid strip included),
+    as split terminals of one conductor and as two conductors: the batched
+    sweep of both methods against the oracle's per-frequency solves to
+    1e-12, and reciprocity of Z."""
     from voxhenry_amd import structures, solver
     st = SMALL_FAMILIES[kind](structures)
     freqs = np.array([1e6, 1e8, 1e10])
@@ -288,7 +296,7 @@ def test_uneven_family_small_vs_oracle(ctx, kind):
         say('%s %s: flags %s, R %.2e L %.2e R/|Z| %.2e Z %.2e' % (kind, m, out['stats']['flag'], eR, eL, eRZ, eZ))
         asym = max(rel(out['Z'][k], out['Z'][k].T) for k in range(len(freqs)))
         assert asym < 1e-6
-        if m == 'mixed' or kind != 'strip':
+        if m == 'mixed' or not kind.startswith('strip'):
             # the FP64 pgmres semantics may stop a two-fluid system on its
             # preconditioned residual above tol (flag 1); the mixed path's
             # outer FGMRES tests the true residual

@@ -272,15 +272,31 @@ def cube(n, dx=1e-6, sigma=5.8e7):
                 well=np.zeros((0, 4), np.int64))
 
 
+def split_ports(structure, k):
+    """The structure's one port split into k terminal pairs: its excitation
+    faces and its ground faces, each ordered by (z, y, x), are cut into k
+    contiguous bands, and band i of both forms port i (a multi-terminal
+    conductor; every other terminal is held at zero potential while one is
+    driven, as for any port's nodes, VoxHenry_executer.m:165-193).  k is
+    limited by the number of faces at either end."""
+    P, N = structure['ports_P'][0], structure['ports_N'][0]
+    k = max(1, min(int(k), len(P), len(N)))
+    order = lambda a: a[np.lexsort((a[:, 0], a[:, 1], a[:, 2]))]
+    out = dict(structure)
+    out['ports_P'] = [np.ascontiguousarray(b) for b in np.array_split(order(P), k)]
+    out['ports_N'] = [np.ascontiguousarray(b) for b in np.array_split(order(N), k)]
+    return out
+
+
 def uneven_batch(seed, count=96):
     """Seeded mixture of the four conductor families of the uneven-batch
-    workload, cycled bar, wire, loop, strip; every geometry has 1-4 ports
-    (one per conductor of a parallel bundle, or per concentric loop):
+    workload, cycled bar, wire, loop, strip; every geometry has 1-4 ports,
+    its end (or gap) faces split into that many terminal pairs
+    (split_ports):
 
     * copper bars, length 16-512 voxels, square section 2-16;
     * round copper wires, length 16-256, diameter 4-20;
-    * circular copper loops, outer radius 20-150, 5 x 5 section, concentric
-      loops 8 voxels apart (as many of the drawn port count as fit inside);
+    * circular copper loops, radius 20-150, 5 x 5 section;
     * two-fluid strips (sigma 1e7 S/m, London depth 90 nm), length 16-512,
       width 4-32, thickness 2.
 
@@ -293,15 +309,12 @@ def uneven_batch(seed, count=96):
         nports = int(rng.integers(1, 5))
         if k == 'bar':
             L, sec = int(rng.integers(16, 513)), int(rng.integers(2, 17))
-            out.append((k, parallel('bar', nports, L, sec)))
+            st = bar(L, sec, sec)
         elif k == 'wire':
-            L, d = int(rng.integers(16, 257)), int(rng.integers(4, 21))
-            out.append((k, parallel('wire', nports, L, d)))
+            st = wire(int(rng.integers(16, 257)), int(rng.integers(4, 21)))
         elif k == 'loop':
-            R = int(rng.integers(20, 151))
-            cnt = max(1, min(nports, 1 + (R - 12) // 8))
-            out.append((k, loops([R - 8 * j for j in range(cnt)])))
+            st = loop(int(rng.integers(20, 151)))
         else:
-            L, w = int(rng.integers(16, 513)), int(rng.integers(4, 33))
-            out.append((k, parallel('strip', nports, L, w)))
+            st = strip(int(rng.integers(16, 513)), int(rng.integers(4, 33)))
+        out.append((k, split_ports(st, nports)))
     return out
