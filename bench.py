@@ -24,7 +24,10 @@ launch stream; `roofline` reports its dominant kernel (pass Z: z-FFT, Green's
 product, inverse z-FFT).  Further lines: the 30x10x10 bar (10 f), the 5-turn
 spiral (32 f), the numex1 bar (60x20x20 at 0.5 um, the reference's 41
 frequencies) and the 96-geometry uneven batch (1-4 ports each, 16 f),
-admitted into concurrent groups by HBM budget.
+admitted into concurrent groups by HBM budget.  In these lines the mixed
+sweeps also continue the systems whose R is a small part of |Z| to the
+tolerance that fixes R and L to 1e-6 (sweep_batched rl_rtol; counted in the
+timed sweep).
 
 The CPU baseline is the FP64 numpy/scipy oracle (oracle/) timed on a bounded
 sample of the same bus workload on this host.
@@ -263,6 +266,9 @@ def method_summary(nsys, dt, st):
                 iters_max=int(np.max(st['iters'])), inner_iters_mean=float(np.mean(st['inner_iters'])))
 
 
+RL_RTOL = 1e-6     # accuracy of R and L the mixed sweeps of the configuration lines refine to (sweep_batched)
+
+
 def sweep_config(ctx, st, freqs, methods=('mixed', 'fp64'), reps=1, keep=None, tight=True, **solve_kw):
     """Batched (port x frequency) solve of one geometry, device-resident:
     set-up (tensor, topology, preconditioners) untimed, then one warm-up and
@@ -283,13 +289,15 @@ def sweep_config(ctx, st, freqs, methods=('mixed', 'fp64'), reps=1, keep=None, t
     nsys = len(freqs) * S.nports
     res = {}
     for m in methods:
-        S.sweep_batched(freqs, method=m, precons=precs, pidx_of_freq=pidx, device=True, **solve_kw)
+        kw = dict(solve_kw, rl_rtol=RL_RTOL) if m == 'mixed' else solve_kw
+        S.sweep_batched(freqs, method=m, precons=precs, pidx_of_freq=pidx, device=True, **kw)
         t = time.perf_counter()
         for _ in range(reps):
-            r = S.sweep_batched(freqs, method=m, precons=precs, pidx_of_freq=pidx, device=True, **solve_kw)
+            r = S.sweep_batched(freqs, method=m, precons=precs, pidx_of_freq=pidx, device=True, **kw)
         torch.cuda.synchronize()
         res[m] = r
         out[m] = method_summary(nsys, (time.perf_counter() - t) / reps, r['stats'])
+        out[m]['refined'] = int(r['stats'].get('refined', 0))
     if tight:
         ref = S.sweep_batched(freqs, method='fp64', tol=TIGHT, precons=precs, pidx_of_freq=pidx, device=True,
                               maxit=20, **solve_kw)
@@ -313,7 +321,7 @@ def sweep_config(ctx, st, freqs, methods=('mixed', 'fp64'), reps=1, keep=None, t
 def compact_config(c):
     """The few figures of a configuration line that go into the stdout line."""
     o = dict(mixed=r3(c['mixed']['solves_per_s']), fp64=r3(c['fp64']['solves_per_s']),
-             ratio=r3(c.get('speedup_mixed_over_fp64')),
+             ratio=r3(c.get('speedup_mixed_over_fp64')), mixed_refined=c['mixed'].get('refined', 0),
              converged='%d/%d mixed, %d/%d fp64' % (c['mixed']['converged'], c['mixed']['systems'],
                                                     c['fp64']['converged'], c['fp64']['systems']))
     if 'err_vs_tight_fp64' in c['mixed']:
@@ -385,6 +393,8 @@ def uneven_config(count, nfreq, seed=0, workers=8, budget_frac=0.7):
         groups.append(len(jobs))
 
         def pool(method, **kw):
+            if method == 'mixed':
+                kw.setdefault('rl_rtol', RL_RTOL)
             torch.cuda.synchronize()
             outs, wall = vsol.sweep_concurrent([(j['S'], freqs, j['precs'], pidx, j['stream'], j['ws']) for j in jobs],
                                                method=method, workers=workers, **kw)
@@ -400,6 +410,9 @@ def uneven_config(count, nfreq, seed=0, workers=8, budget_frac=0.7):
             conv = sum(int(np.sum(o['stats']['flag'] == 0)) for o in outs)
             tot[m]['wall'] += wall
             tot[m]['conv'] += conv
+            tot[m]['refined'] = tot[m].get('refined', 0) + sum(int(o['stats'].get('refined', 0)) for o in outs)
+            tot[m]['rl_unresolved'] = tot[m].get('rl_unresolved', 0) + sum(int(o['stats'].get('rl_unresolved', 0))
+                                                                            for o in outs)
             log('[bench]   uneven group of %d, %s: %d/%d converged in %.3f s'
                 % (len(jobs), m, conv, sum(j['nsys'] for j in jobs), wall))
         for k, j in enumerate(jobs):
@@ -433,7 +446,8 @@ def uneven_config(count, nfreq, seed=0, workers=8, budget_frac=0.7):
     return dict(seed=seed, geometries=count, nfreq=nfreq, systems=nsys_all, workers=workers, groups=groups,
                 budget_GB=budget / 1e9, setup_s=setup_s, wall_s=time.perf_counter() - t_all,
                 mixed=dict(solves_per_s=sps['mixed'], s=tot['mixed']['wall'], converged=tot['mixed']['conv'],
-                           systems=nsys_all),
+                           systems=nsys_all, refined=tot['mixed'].get('refined', 0),
+                           rl_unresolved=tot['mixed'].get('rl_unresolved', 0)),
                 fp64=dict(solves_per_s=sps['fp64'], s=tot['fp64']['wall'], converged=tot['fp64']['conv'],
                           systems=nsys_all),
                 speedup_mixed_over_fp64=sps['mixed'] / max(sps['fp64'], 1e-300),

@@ -221,21 +221,26 @@ int vh_solve_mixed(vh_geom* g, vh_precon* p, double jwmu_re, double jwmu_im, int
  * nearby one: any preconditioner gives the same solution).
  * method 0: FP64 left-preconditioned GMRES with the pgmres semantics
  *           (restart, maxit, tol) for every system;
- * method 1: mixed precision, FP64 flexible GMRES (outer_restart, maxit) with
- *           the FP32 inner GMRES (inner_restart, inner_tol).
+ * method 1: mixed precision, FP64 flexible GMRES (outer_restart, maxit) whose
+ *           preconditioner is an FP32 flexible GMRES (inner_restart,
+ *           inner_tol) on the FP32 matvec, preconditioned by the FP32
+ *           factors applied with FP32 work vectors.
  * All systems run in lock step and converged ones leave the batch; if the
  * Krylov workspaces exceed hbm_budget bytes (0: 80% of free memory) the batch
  * is processed in chunks.  tols (nullable, host) gives system s its own
  * tolerance tols[s] in place of tol: a sweep then follows the frequency rule
  * of VoxHenry_executer.m:219-223 (1e-12 below 1 MHz, 1e-8 above) in one
- * batch.  Outputs per system (each nullable): flag, relres (true residual
+ * batch.  x_init != 0 (method 1 only): x holds initial guesses and the
+ * outer FGMRES starts from b - A x (e.g. to tighten the tolerance of some
+ * systems after a first solve); otherwise x0 = 0 as in pgmres.m.
+ * Outputs per system (each nullable): flag, relres (true residual
  * ||b - A x|| / ||b|| of the returned x), iters (outer Arnoldi steps),
  * inner_iters (FP32 steps, method 1), resid_est (the solver's own residual
  * estimate at exit: method 0 the reference-style preconditioned
  * ||M r|| / ||M b|| that pgmres.m:126 stops on, method 1 the outer FGMRES
  * estimate of ||r|| / ||b||). */
 int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64_t nsys, const double* jwmu,
-                   const int* pidx, const double* b, double* x, int on_device, int method, double tol,
+                   const int* pidx, const double* b, double* x, int on_device, int x_init, int method, double tol,
                    const double* tols, int restart,
                    int maxit, int inner_restart, double inner_tol, double hbm_budget, int* flag, double* relres,
                    int* iters, int* inner_iters, double* resid_est);

@@ -58,7 +58,8 @@ SIGNATURES = {
     'vh_matvec_noalloc': [_P, _D, _D, _I, _I, _I64, _P, _I64, _P, _I64],
     'vh_matvec_timed': [_P, _D, _D, _I, _I, _I64, _P, _I64, _P, _I64, _I, _P, _P, _P],
     'vh_fft3d': [_P, _P, _I, _I, _I],
-    'vh_solve_batch': [_P, _I64, _P, _I64, _P, _P, _P, _P, _I, _I, _D, _P, _I, _I, _I, _D, _D, _P, _P, _P, _P, _P],
+    'vh_solve_batch': [_P, _I64, _P, _I64, _P, _P, _P, _P, _I, _I, _I, _D, _P, _I, _I, _I, _D, _D, _P, _P, _P, _P,
+                       _P],
     'vh_solve_batch_bytes': [_P, _I, _I, _I, _P],
     'vh_geom_currents_on_grid': [_P, _I64, _P, _I64, _P, _I64, _P, _I],
     'vh_precon_create': [_P, C.POINTER(_P)],

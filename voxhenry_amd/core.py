@@ -269,7 +269,7 @@ class Geometry:
         return x.reshape(np.shape(b)), dict(flag=flag, relres=relres, outer_iters=oit, inner_iters=iit)
 
     def solve_batch(self, B, jwmu, precons, pidx, method='mixed', tol=1e-8, restart=None, maxit=None,
-                    inner_restart=40, inner_tol=1e-5, hbm_budget=0.0):
+                    inner_restart=30, inner_tol=1e-5, hbm_budget=0.0, x0=None):
         """Many (port, frequency) systems in one batched solve (vh_solve_batch).
 
         B: (nsys, n) rhs, a numpy array (host) or a complex128 torch tensor on
@@ -277,7 +277,9 @@ class Geometry:
         precons: list of Precon; pidx: (nsys,) index into precons.  method
         'fp64' (pgmres semantics, default restart 100 / maxit 10) or 'mixed'
         (default outer restart 8 / maxit 4).  tol: one tolerance, or one per
-        system (e.g. reference_tol of each system's frequency)."""
+        system (e.g. reference_tol of each system's frequency).  x0 (mixed
+        only): initial guesses of the shape of B; the outer FGMRES starts
+        from B - A x0 (default: zero, as pgmres.m)."""
         m = 0 if method == 'fp64' else 1
         if restart is None:
             restart = 100 if m == 0 else 8
@@ -301,21 +303,26 @@ class Geometry:
             tols = np.ascontiguousarray(np.asarray(tol, dtype=np.float64).ravel())
             assert tols.size == nsys, 'one tolerance per system'
             tol = float(tols.max())
-        X = _torch().zeros_like(BB) if on_dev else np.zeros_like(BB)
+        if x0 is None:
+            X = _torch().zeros_like(BB) if on_dev else np.zeros_like(BB)
+        else:
+            X = x0.clone().contiguous() if on_dev else np.ascontiguousarray(x0, dtype=np.complex128).copy()
+            assert tuple(X.shape) == tuple(BB.shape), 'x0: one guess per system'
         flag = np.zeros(nsys, np.int32)
         relres = np.zeros(nsys)
         its = np.zeros(nsys, np.int32)
         iits = np.zeros(nsys, np.int32)
         rest = np.zeros(nsys)
         check(lib().vh_solve_batch(self.h, len(precons), C.cast(hs, C.c_void_p), nsys, ptr(jw), ptr(pi), ptr(BB),
-                                   ptr(X), int(on_dev), m, float(tol), ptr(tols) if tols is not None else None,
+                                   ptr(X), int(on_dev), int(x0 is not None), m, float(tol),
+                                   ptr(tols) if tols is not None else None,
                                    restart, maxit, inner_restart, inner_tol,
                                    float(hbm_budget),
                                    ptr(flag), ptr(relres), ptr(its), ptr(iits), ptr(rest)))
         # resid_est: fp64 the reference-style preconditioned residual (pgmres.m:126), mixed the outer estimate
         return X, dict(flag=flag, relres=relres, iters=its, inner_iters=iits, resid_est=rest)
 
-    def batch_bytes(self, method='mixed', restart=None, inner_restart=40):
+    def batch_bytes(self, method='mixed', restart=None, inner_restart=30):
         """Device bytes per system that solve_batch budgets on this geometry
         (Krylov workspace + the system's share of the matvec workspaces)."""
         m = 0 if method == 'fp64' else 1

@@ -31,7 +31,8 @@ vh::KrylovCache& krylov_cache(vh::Geometry& G) {
 // Px*Py*N per component, sized for complex128).
 double batch_bytes_per_system(const vh::Geometry& G, int method, int restart, int inner_restart) {
   const bool mixed = method == 1;
-  double b = (double)vh::KrylovCache::bytes(G.n(), 1, restart, mixed, mixed ? inner_restart : 0, false, mixed);
+  // mixed: the FP32 inner solve is flexible (its basis Z is stored too)
+  double b = (double)vh::KrylovCache::bytes(G.n(), 1, restart, mixed, mixed ? inner_restart : 0, mixed, mixed);
   const vh::GridDims& gd = G.g;
   const double a = 5.0 * gd.Px * (double)gd.M * gd.N, bgrid = 5.0 * gd.Px * (double)gd.Py * gd.N;
   return b + bgrid * 16 + a * (16 + (mixed ? 8 : 0));
@@ -694,7 +695,7 @@ int vh_ldl_solve_host(int64_t n, const int64_t* Ap, const int64_t* Ai, const dou
 }
 
 int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64_t nsys, const double* jwmu,
-                   const int* pidx, const double* b, double* x, int on_device, int method, double tol,
+                   const int* pidx, const double* b, double* x, int on_device, int x_init, int method, double tol,
                    const double* tols, int restart,
                    int maxit, int inner_restart, double inner_tol, double hbm_budget, int* flag, double* relres,
                    int* iters, int* inner_iters, double* resid_est) {
@@ -704,6 +705,7 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
     VH_REQUIRE(G.has_tensor && G.has_inc, "geometry not complete (tensor, incidence)");
     VH_REQUIRE(nprecon >= 1 && nsys >= 1 && nsys <= 13107, "nprecon / nsys out of range");
     VH_REQUIRE(method == 0 || method == 1, "method must be 0 (fp64) or 1 (mixed)");
+    VH_REQUIRE(!x_init || method == 1, "an initial guess (x_init) needs method 1 (the pgmres semantics start from 0)");
     VH_REQUIRE(restart >= 1 && restart <= 127 && inner_restart >= 1 && inner_restart <= 127 && maxit >= 1,
                "bad restart / maxit");
     for (int64_t k = 0; k < nprecon; ++k) VH_REQUIRE(precons[k] && precons[k]->P.ready, "preconditioner not prepared");
@@ -719,6 +721,10 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
     io.in(b, nsys, n, 16, on_device, st, n);
     double2* dx = (double2*)(on_device ? (void*)x : io.y);
     const double2* db = (const double2*)io.x;
+    if (x_init && !on_device) {
+      VH_HIP(hipStreamSynchronize(st));   // the clearing of io.y is queued on st
+      h2d(dx, x, (size_t)nsys * n * 16);
+    }
     // per-slot j*omega*mu on the device
     vh::DevBuf<double2> alphas(nsys);
     h2d(alphas.p, jwmu, nsys * sizeof(double2));
@@ -762,7 +768,7 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
     std::vector<vh::SysStats> stats(nsys);
     if (vh::trace_on()) fprintf(stderr, "[vh] solve_batch nsys %ld chunk %d budget %.1f GB per_sys %.3f GB\n",
                                 (long)nsys, chunk, budget / 1e9, per_sys / 1e9);
-    K.layout(n, chunk, restart, mixed, mixed ? inner_restart : 0, false, mixed);
+    K.layout(n, chunk, restart, mixed, mixed ? inner_restart : 0, mixed, mixed);
     vh::KrylovWork<double>& W64 = K.w64;
     // each chunk is solved with chunk-relative slots 0..cnt-1 (workspaces laid
     // out for the chunk); coff is the chunk's first system
@@ -778,8 +784,11 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
     vh::OpFn<double> M64 = [&](const std::vector<int>& act, const int* sl, const double2* xx, double2* yy) {
       vh::precon_apply_cols<double>(G, *sym, tabs64.p, pidx + coff, act, xx, n, yy, n);
     };
+    // inner preconditioner of the mixed solve: FP32 factors on FP32 work
+    // vectors, exact to FP32 rounding per level (not a fixed linear
+    // operator), inside a flexible inner GMRES
     vh::OpFn<float> M32 = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
-      vh::precon_apply_cols<float>(G, *sym, tabs32.p, pidx + coff, act, xx, n, yy, n);
+      vh::precon_apply_cols_wt<float, float>(G, *sym, tabs32.p, pidx + coff, act, xx, n, yy, n);
     };
     for (int64_t s0 = 0; s0 < nsys; s0 += chunk) {
       coff = s0;
@@ -800,8 +809,38 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
         vh::MixedOpts o;
         o.tol = tol; o.outer_restart = restart; o.outer_maxit = maxit;
         o.inner_restart = inner_restart; o.inner_tol = inner_tol;
+        o.x0 = x_init != 0;
+        o.inner_flexible = true;
         if (tols) o.tol_slot = &tolc;
         vh::fgmres_mixed(G.ctx, n, A64, A32, M32, bc, xc, sys, o, K, cs);
+        // Systems the outer iteration left above their tolerance continue
+        // from their solutions with the inner solve of a fixed linear FP32
+        // preconditioner (FP32 factors on FP64 work vectors, left
+        // preconditioning): rounding the work vectors to FP32 at every level
+        // can leave the inner solves of the hardest systems (a
+        // superconductor's low-frequency operator) too inaccurate for the
+        // outer iteration to reach the tolerance.
+        std::vector<int> again;
+        for (int i = 0; i < cnt; ++i)
+          if (cs[i].flag != 0) again.push_back(i);
+        if (!again.empty()) {
+          vh::OpFn<float> M32f = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
+            vh::precon_apply_cols<float>(G, *sym, tabs32.p, pidx + coff, act, xx, n, yy, n);
+          };
+          vh::MixedOpts o2 = o;
+          o2.x0 = true;
+          o2.inner_flexible = false;
+          std::vector<vh::SysStats> cs2;
+          vh::fgmres_mixed(G.ctx, n, A64, A32, M32f, bc, xc, again, o2, K, cs2);
+          for (int i : again) {
+            cs2[i].iters += cs[i].iters;
+            cs2[i].inner_iters += cs[i].inner_iters;
+            cs2[i].matvecs += cs[i].matvecs;
+            cs[i] = cs2[i];
+          }
+          if (vh::trace_on()) fprintf(stderr, "[vh] solve_batch: %zu systems continued with the FP64-work-vector inner solve\n",
+                                      again.size());
+        }
       }
       for (int i = 0; i < cnt; ++i) stats[s0 + i] = cs[i];
     }

@@ -38,12 +38,17 @@ __device__ __forceinline__ double wave_sum(double v) {
 // part[(a*kmax + j)*nblk + b] = sum over the chunks of block b of conj(V_j) w.
 // Blocks stride over chunks; each wave accumulates its own LDS row, so the
 // summation order is fixed (deterministic, no atomics).
+// need (optional): systems with need[a] == 0 are skipped (their partials are
+// not written; the reduction gives them zeros).  with_norm: entry k receives
+// ||w||^2 from the same pass over w.
 template <typename R>
 __global__ __launch_bounds__(KT) void k_mdot(const cplx<R>* __restrict__ V, int64_t vstride,
                                               const cplx<R>* __restrict__ w, int64_t ld, int64_t n, int k, int kmax,
-                                              const int* __restrict__ slots, double2* __restrict__ part, int nchunk) {
+                                              const int* __restrict__ slots, double2* __restrict__ part, int nchunk,
+                                              const int* __restrict__ need, int with_norm) {
   __shared__ double2 acc[4][KMAX];
   const int a = blockIdx.y;
+  if (need && !need[a]) return;
   const int s = slots[a];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int j = tid; j < 4 * KMAX; j += KT) { acc[j / KMAX][j % KMAX].x = 0; acc[j / KMAX][j % KMAX].y = 0; }
@@ -72,9 +77,16 @@ __global__ __launch_bounds__(KT) void k_mdot(const cplx<R>* __restrict__ V, int6
       sy = wave_sum(sy);
       if (lane == 0) { acc[wave][j].x += sx; acc[wave][j].y += sy; }
     }
+    if (with_norm) {
+      double sq = 0;
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) sq += (double)wv[e].x * wv[e].x + (double)wv[e].y * wv[e].y;
+      sq = wave_sum(sq);
+      if (lane == 0) acc[wave][k].x += sq;
+    }
   }
   __syncthreads();
-  for (int j = tid; j < k; j += KT) {
+  for (int j = tid; j < k + with_norm; j += KT) {
     double2 r;
     r.x = ((acc[0][j].x + acc[1][j].x) + acc[2][j].x) + acc[3][j].x;
     r.y = ((acc[0][j].y + acc[1][j].y) + acc[2][j].y) + acc[3][j].y;
@@ -83,11 +95,17 @@ __global__ __launch_bounds__(KT) void k_mdot(const cplx<R>* __restrict__ V, int6
 }
 
 // out[o] = sum_{q < np} part[o*np + q] with o = blockIdx.y*ostride + blockIdx.x
-// (fixed summation order: deterministic)
+// (fixed summation order: deterministic).  need (optional, per system a =
+// blockIdx.y, or blockIdx.x when ostride is 0): skipped systems get zero
+// (keep == 0) or keep their out value (keep != 0).
 __global__ __launch_bounds__(256) void k_reduce2(const double2* __restrict__ part, int np, double2* __restrict__ out,
-                                                 int ostride) {
+                                                 int ostride, const int* __restrict__ need, int keep) {
   __shared__ double sx[256], sy[256];
   const int o = blockIdx.y * ostride + blockIdx.x;
+  if (need && !need[ostride ? blockIdx.y : blockIdx.x]) {
+    if (!keep && threadIdx.x == 0) out[o] = make_double2(0.0, 0.0);
+    return;
+  }
   double ax = 0, ay = 0;
   for (int q = threadIdx.x; q < np; q += 256) { const double2 v = part[(int64_t)o * np + q]; ax += v.x; ay += v.y; }
   sx[threadIdx.x] = ax; sy[threadIdx.x] = ay;
@@ -105,10 +123,11 @@ template <typename R>
 __global__ __launch_bounds__(KT) void k_combine(const cplx<R>* __restrict__ V, int64_t vstride, cplx<R>* __restrict__ out,
                                                  int64_t ld, int64_t n, int k, int kmax, const int* __restrict__ slots,
                                                  const double2* __restrict__ coef, double beta, double sign,
-                                                 double2* __restrict__ npart, int nchunk) {
+                                                 double2* __restrict__ npart, int nchunk, const int* __restrict__ need) {
   __shared__ double2 cf[KMAX];
   __shared__ double wn[4];
   const int a = blockIdx.y;
+  if (need && !need[a]) return;
   const int s = slots[a];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int j = tid; j < k; j += KT) { double2 v = coef[(int64_t)a * kmax + j]; v.x *= sign; v.y *= sign; cf[j] = v; }
@@ -139,6 +158,17 @@ __global__ __launch_bounds__(KT) void k_combine(const cplx<R>* __restrict__ V, i
     __syncthreads();
     if (tid == 0) { double2 r; r.x = ((wn[0] + wn[1]) + wn[2]) + wn[3]; r.y = 0; npart[(int64_t)a * gridDim.x + blockIdx.x] = r; }
   }
+}
+
+// Selective second Gram-Schmidt pass: system a needs it when the first pass
+// removed more than half of ||w||^2 (||w'||^2 < eta2 ||w||^2, the
+// Kahan-Parlett criterion with eta = 1/sqrt(2)): then w' carries too little
+// of w for its rounding error to stay small against it.
+__global__ void k_reorth_flags(const double2* __restrict__ h, int kmax, int k, const double2* __restrict__ nrm2,
+                               int* __restrict__ need, int nact, double eta2) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nact) return;
+  need[a] = nrm2[a].x < eta2 * h[(int64_t)a * kmax + k].x ? 1 : 0;
 }
 
 // dst[s] = src[s] * (1/sqrt(nrm2[a]))  (scale by the inverse norm computed on the device)
@@ -220,31 +250,41 @@ VecOps<R>::VecOps(Context* c, int64_t n_, int64_t ld_, int kmax_, int nslot_, do
 // h[a][j] = V_j^H w for j < k (device, double2 [a][kmax])
 template <typename R>
 void VecOps<R>::mdot(const cplx<R>* V, int64_t vstride, const cplx<R>* w, int k, int nact, const int* slots,
-                     double2* h) {
+                     double2* h, const int* need, bool with_norm) {
   if (k <= 0 || nact <= 0) return;
+  VH_REQUIRE(k + (with_norm ? 1 : 0) <= kmax, "internal: Krylov basis larger than the reduction layout");
   hipStream_t st = ctx->stream;
   const int nb = blocks_for(nchunk, nact);
+  const int kk = k + (with_norm ? 1 : 0);
   hipLaunchKernelGGL((k_mdot<R>), dim3(nb, nact), dim3(KT), 0, st, V, vstride, w, ld, n, k, kmax, slots, part,
-                     nchunk);
+                     nchunk, need, with_norm ? 1 : 0);
   VH_LAUNCHED();
-  // reduce each (a, j < k) over nb partials into h[a][kmax]
-  hipLaunchKernelGGL(k_reduce2, dim3(k, nact), dim3(256), 0, st, part, nb, h, kmax);
+  // reduce each (a, j < kk) over nb partials into h[a][kmax]
+  hipLaunchKernelGGL(k_reduce2, dim3(kk, nact), dim3(256), 0, st, part, nb, h, kmax, need, 0);
+  VH_LAUNCHED();
+}
+
+template <typename R>
+void VecOps<R>::reorth_flags(const double2* h, int k, const double2* nrm2, int* need, int nact) {
+  if (nact <= 0) return;
+  hipLaunchKernelGGL(k_reorth_flags, dim3(ceil_div(nact, 256)), dim3(256), 0, ctx->stream, h, kmax, k, nrm2, need,
+                     nact, 0.5);
   VH_LAUNCHED();
 }
 
 // out = beta*out + sign*sum_j V_j coef_j ; if nrm2 != nullptr also ||out||^2 per system
 template <typename R>
 void VecOps<R>::combine(const cplx<R>* V, int64_t vstride, cplx<R>* out, int k, int nact, const int* slots,
-                        const double2* coef, double beta, double sign, double2* nrm2) {
+                        const double2* coef, double beta, double sign, double2* nrm2, const int* need) {
   if (nact <= 0) return;
   VH_REQUIRE(k <= KMAX, "Krylov basis larger than 128 vectors");
   hipStream_t st = ctx->stream;
   const int nb = blocks_for(nchunk, nact);
   hipLaunchKernelGGL((k_combine<R>), dim3(nb, nact), dim3(KT), 0, st, V, vstride, out, ld, n, k, kmax, slots, coef,
-                     beta, sign, nrm2 ? part : nullptr, nchunk);
+                     beta, sign, nrm2 ? part : nullptr, nchunk, need);
   VH_LAUNCHED();
   if (nrm2) {
-    hipLaunchKernelGGL(k_reduce2, dim3(nact), dim3(256), 0, st, part, nb, nrm2, 0);
+    hipLaunchKernelGGL(k_reduce2, dim3(nact), dim3(256), 0, st, part, nb, nrm2, 0, need, 1);
     VH_LAUNCHED();
   }
 }
@@ -254,9 +294,9 @@ void VecOps<R>::norm2(const cplx<R>* x, int nact, const int* slots, double2* nrm
   hipStream_t st = ctx->stream;
   const int nb = blocks_for(nchunk, nact);
   hipLaunchKernelGGL((k_mdot<R>), dim3(nb, nact), dim3(KT), 0, st, x, (int64_t)0, x, ld, n, 1, 1, slots, part,
-                     nchunk);
+                     nchunk, nullptr, 0);
   VH_LAUNCHED();
-  hipLaunchKernelGGL(k_reduce2, dim3(nact), dim3(256), 0, st, part, nb, nrm2, 0);
+  hipLaunchKernelGGL(k_reduce2, dim3(nact), dim3(256), 0, st, part, nb, nrm2, 0, nullptr, 0);
   VH_LAUNCHED();
 }
 

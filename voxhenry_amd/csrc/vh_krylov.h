@@ -15,9 +15,16 @@ template <typename R> struct VecOps {
   double2* part;   // reduction partials, part_size(kmax, nslot) entries (owned by the caller's workspace)
   VecOps(Context* c, int64_t n, int64_t ld, int kmax, int nslot, double2* part);
   static size_t part_size(int kmax, int nslot);
-  void mdot(const cplx<R>* V, int64_t vstride, const cplx<R>* w, int k, int nact, const int* slots, double2* h);
+  // h[a][j] = V_j^H w (j < k); with_norm: also h[a][k] = ||w||^2.  need
+  // (device, optional): systems with need[a] == 0 are skipped and get h = 0.
+  void mdot(const cplx<R>* V, int64_t vstride, const cplx<R>* w, int k, int nact, const int* slots, double2* h,
+            const int* need = nullptr, bool with_norm = false);
+  // need (optional): skipped systems keep out and nrm2 unchanged
   void combine(const cplx<R>* V, int64_t vstride, cplx<R>* out, int k, int nact, const int* slots,
-               const double2* coef, double beta, double sign, double2* nrm2);
+               const double2* coef, double beta, double sign, double2* nrm2, const int* need = nullptr);
+  // need[a] = 1 where a second Gram-Schmidt pass is due: ||w'||^2 (nrm2) <
+  // ||w||^2 / 2 (h[a][k], from mdot with_norm)
+  void reorth_flags(const double2* h, int k, const double2* nrm2, int* need, int nact);
   void norm2(const cplx<R>* x, int nact, const int* slots, double2* nrm2);
   void scale(const cplx<R>* src, cplx<R>* dst, int nact, const int* slots, const double2* nrm2);
   void sub_from(const cplx<R>* b, int64_t ldb, cplx<R>* y, int nact, const int* slots);
@@ -61,8 +68,10 @@ struct SolveOpts {
   double tol = 1e-8;
   bool final_residual = true;     // compute relres with one more matvec
   bool flexible = false;          // fpgmres semantics (right preconditioner)
-  bool reorth = false;            // classical Gram-Schmidt with a second pass (CGS2)
+  bool reorth = false;            // classical Gram-Schmidt with a second pass where the first one cancelled
+                                  // more than half of ||w||^2 (selective CGS2)
   const std::vector<double>* tol_slot = nullptr;   // per-slot tolerance (overrides tol)
+  bool x0 = false;                // x holds an initial guess (flexible or unpreconditioned solves only)
 };
 
 // Krylov workspace for nslot systems of length ld with restart m.  The large
@@ -80,6 +89,7 @@ template <typename R> struct KrylovWork {
   DevBuf<double2> h, h2, nrm, coef;
   DevBuf<double2> part;   // partials of the reductions (VecOps), sized for m + 1 vectors and nslot systems
   DevBuf<int> slots_dev, aux_slots;
+  DevBuf<int> need;       // per active system: second Gram-Schmidt pass due (selective CGS2)
   // pinned host staging: slot lists, LS coefficients, Hessenberg columns, norms
   PinnedBuf<int> h_slots, h_aux;
   PinnedBuf<double2> h_coef, h_rcoef, h_h, h_h2, h_nrm, h_bn;
@@ -106,6 +116,8 @@ struct MixedOpts {
   double inner_tol = 1e-5;          // tightest inner tolerance (the first inner solve)
   bool adaptive_inner = true;       // relax the inner tolerance as the outer residual falls
   const std::vector<double>* tol_slot = nullptr;   // per-slot outer tolerance (overrides tol)
+  bool x0 = false;                  // x holds an initial guess (the outer FGMRES starts from b - A x)
+  bool inner_flexible = false;      // inner solve right-preconditioned (fpgmres semantics): any M32 works
 };
 // Workspaces of the solvers, kept by a geometry across calls.  All large
 // vectors live in one device arena that only grows: releasing and allocating

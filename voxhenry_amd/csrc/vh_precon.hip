@@ -266,15 +266,16 @@ __device__ __forceinline__ double2 lane_sum(double2 v, int from, int to) {
   return v;
 }
 
-template <typename R, int NPG, bool FWD>
+template <typename R, typename WT, int NPG, bool FWD>
 __global__ __launch_bounds__(256) void k_blk_solve(const BlkInfo* __restrict__ info, const int* __restrict__ ep,
                                                    const int* __restrict__ esplit, const int* __restrict__ loc,
                                                    const int* __restrict__ ucol, const PcTab<R>* __restrict__ tab,
-                                                   PcGroupDev gr, int nn, double2* __restrict__ w,
-                                                   double2* __restrict__ w2, int stg_n, int llu, int llg) {
-  extern __shared__ double2 smem_pc[];
-  double2* stg = smem_pc;
-  double2* tt = smem_pc + stg_n;
+                                                   PcGroupDev gr, int nn, cplx<WT>* __restrict__ w,
+                                                   cplx<WT>* __restrict__ w2, int stg_n, int llu, int llg) {
+  using CW = cplx<WT>;
+  extern __shared__ __align__(16) unsigned char smem_pc_raw[];
+  CW* stg = (CW*)smem_pc_raw;
+  CW* tt = stg + stg_n;
   const int q = blockIdx.x / gr.ng, g = blockIdx.x - q * gr.ng;
   const BlkInfo bi = info[q];
   const int c0 = bi.c0, s = bi.s, u0 = bi.u0, nu = bi.nu;
@@ -282,21 +283,22 @@ __global__ __launch_bounds__(256) void k_blk_solve(const BlkInfo* __restrict__ i
   const R* __restrict__ val = FWD ? T.Rx : T.Cx;
   // forward: staged lines are y (w2), the block's own values b come from w;
   // backward: staged lines are x (w), the own values y from w2
-  const double2* __restrict__ src = FWD ? w2 : w;
-  const double2* __restrict__ own = FWD ? w : w2;
-  double2* __restrict__ dst = FWD ? w2 : w;
+  const CW* __restrict__ src = FWD ? w2 : w;
+  const CW* __restrict__ own = FWD ? w : w2;
+  CW* __restrict__ dst = FWD ? w2 : w;
   const int64_t gb = (int64_t)g * nn * NPG;
   const int tid = threadIdx.x;
   const int j = tid % NPG;
   // the block's own values (b, or y / D) go to t while the lines are staged
   for (int e = tid; e < s * NPG; e += blockDim.x) {
     const int i = e / NPG;
-    const double2 o = own[gb + (int64_t)(c0 + i) * NPG + (e - i * NPG)];
+    const CW o = own[gb + (int64_t)(c0 + i) * NPG + (e - i * NPG)];
     if (FWD) {
       tt[e] = o;
     } else {
       const double inv = 1.0 / (double)T.D[c0 + i];
-      tt[e] = make_double2(o.x * inv, o.y * inv);
+      CW q; q.x = (WT)((double)o.x * inv); q.y = (WT)((double)o.y * inv);
+      tt[e] = q;
     }
   }
   for (int e = tid; e < nu * NPG; e += blockDim.x) {
@@ -314,13 +316,14 @@ __global__ __launch_bounds__(256) void k_blk_solve(const BlkInfo* __restrict__ i
       double2 a = make_double2(0.0, 0.0);
       for (int p = lo + part; p < hi; p += lpr) {
         const double l = (double)val[p];
-        const double2 v = stg[loc[p] * NPG + j];
+        const CW v = stg[loc[p] * NPG + j];
         a.x += l * v.x; a.y += l * v.y;
       }
       a = lane_sum(a, NPG, lu);
       if (part == 0) {
-        tt[i * NPG + j].x -= a.x;
-        tt[i * NPG + j].y -= a.y;
+        CW& o = tt[i * NPG + j];
+        o.x = (WT)((double)o.x - a.x);
+        o.y = (WT)((double)o.y - a.y);
       }
     }
   }
@@ -334,15 +337,18 @@ __global__ __launch_bounds__(256) void k_blk_solve(const BlkInfo* __restrict__ i
       // row i of the transpose holds Linv(i..s-1, i) at i*s - i(i-1)/2
       const R* __restrict__ row = FWD ? Lb + (int64_t)i * (i + 1) / 2 : Lb + (int64_t)i * s - (int64_t)i * (i - 1) / 2;
       const int len = FWD ? i + 1 : s - i;
-      const double2* tv = FWD ? tt + j : tt + i * NPG + j;
+      const CW* tv = FWD ? tt + j : tt + i * NPG + j;
       double2 a = make_double2(0.0, 0.0);
       for (int jj = part; jj < len; jj += lpr) {
         const double l = (double)row[jj];
-        const double2 v = tv[jj * NPG];
+        const CW v = tv[jj * NPG];
         a.x += l * v.x; a.y += l * v.y;
       }
       a = lane_sum(a, NPG, lg);
-      if (part == 0) dst[gb + (int64_t)(c0 + i) * NPG + j] = a;
+      if (part == 0) {
+        CW o; o.x = (WT)a.x; o.y = (WT)a.y;
+        dst[gb + (int64_t)(c0 + i) * NPG + j] = o;
+      }
     }
   }
 }
@@ -1022,10 +1028,9 @@ static PcGroupDev group_table(PreconSymbolic& Y, const std::vector<int>& key, hi
 constexpr int kBlkCap = 2048;
 static int ilog2_floor(int v) { int l = 0; while ((2 << l) <= v) ++l; return l; }
 
-template <typename R>
-void precon_apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, const int* pidx,
-                       const std::vector<int>& slots, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy) {
-  using WT = double;
+template <typename R, typename WT>
+void precon_apply_cols_wt(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, const int* pidx,
+                          const std::vector<int>& slots, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy) {
   VH_REQUIRE(!slots.empty(), "preconditioner: no columns");
   hipStream_t st = G.ctx->stream;
   const int nn = (int)G.num_node;
@@ -1078,15 +1083,15 @@ void precon_apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, cons
         const int i1 = (int)(std::upper_bound(B.key.begin() + i0, B.key.begin() + b1, cap / npg) - B.key.begin());
         if (i1 > i0) {
           const int stg_n = B.umax[i1 - 1] * npg;
-          const size_t lds = (size_t)(stg_n + B.smax[i1 - 1] * npg) * sizeof(double2);
+          const size_t lds = (size_t)(stg_n + B.smax[i1 - 1] * npg) * sizeof(cplx<WT>);
           const dim3 grid((unsigned)((int64_t)(i1 - i0) * gr.ng));
           by_npg([&](auto N) {
             constexpr int n = decltype(N)::value;
             if (fwd)
-              hipLaunchKernelGGL((k_blk_solve<R, n, true>), grid, dim3(256), lds, st, info + i0, Y.Rp.p,
+              hipLaunchKernelGGL((k_blk_solve<R, WT, n, true>), grid, dim3(256), lds, st, info + i0, Y.Rp.p,
                                  Y.d_row_split.p, Y.d_f_loc.p, Y.d_f_ucol.p, tab, gr, nn, w, w2, stg_n, llu, llg);
             else
-              hipLaunchKernelGGL((k_blk_solve<R, n, false>), grid, dim3(256), lds, st, info + i0, Y.Cp.p,
+              hipLaunchKernelGGL((k_blk_solve<R, WT, n, false>), grid, dim3(256), lds, st, info + i0, Y.Cp.p,
                                  Y.d_col_split.p, Y.d_b_loc.p, Y.d_b_ucol.p, tab, gr, nn, w, w2, stg_n, llu, llg);
           });
           VH_LAUNCHED();
@@ -1166,6 +1171,14 @@ void precon_apply(Precon& P, const std::vector<int>& slots, const cplx<R>* x, in
   precon_apply_cols<R>(*P.G, *P.sym, tab, nullptr, slots, x, ldx, y, ldy);
 }
 
+template <typename R>
+void precon_apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, const int* pidx,
+                       const std::vector<int>& slots, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy) {
+  precon_apply_cols_wt<R, double>(G, Y, tab, pidx, slots, x, ldx, y, ldy);
+}
+
+template void precon_apply_cols_wt<float, float>(Geometry&, PreconSymbolic&, const PcTab<float>*, const int*,
+                                                 const std::vector<int>&, const float2*, int64_t, float2*, int64_t);
 template void precon_apply_cols<double>(Geometry&, PreconSymbolic&, const PcTab<double>*, const int*,
                                         const std::vector<int>&, const double2*, int64_t, double2*, int64_t);
 template void precon_apply_cols<float>(Geometry&, PreconSymbolic&, const PcTab<float>*, const int*,

@@ -42,6 +42,7 @@ void KrylovWork<R>::bind(char* base, int64_t n_, int64_t ld_, int m_, int nslot_
   nrm.alloc(nslot);
   slots_dev.alloc(nslot);
   aux_slots.alloc(nslot);
+  need.alloc(nslot);
   h_slots.alloc(nslot);
   h_aux.alloc(nslot);
   h_coef.alloc((size_t)nslot * (m + 1));
@@ -135,15 +136,24 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
 
   std::vector<int> act = sys;
   upload(act, W.slots_dev.p);
-  ops.zero(x, (int)act.size(), W.slots_dev.p);   // x0 = 0 for every system
+  VH_REQUIRE(!o.x0 || !left, "internal: an initial guess needs a flexible or unpreconditioned solve");
+  if (!o.x0) ops.zero(x, (int)act.size(), W.slots_dev.p);   // x0 = 0 for every system
   for (int s : sys) st[s] = SysStats();
-  // r = M b (pgmres.m:120-139) or b (fpgmres.m:91-95)
-  if (left) {
+  std::vector<double> bnorm(nslot, 0.0), bnorm_true(nslot, 0.0);
+  // r = M b (pgmres.m:120-139) or b (fpgmres.m:91-95); with an initial guess
+  // (flexible only) r = b - A x0, relative to ||b||
+  std::vector<double> r0(nslot, 1.0);
+  if (o.x0) {
+    ops.norm2(b, nsys, W.slots_dev.p, W.nrm.p);
+    VH_HIP(hipMemcpyAsync(W.h_bn.p, W.nrm.p, nsys * sizeof(double2), hipMemcpyDeviceToHost, strm));
+    A(act, W.slots_dev.p, x, W.r);
+    ops.sub_from(b, n, W.r, nsys, W.slots_dev.p);
+    for (int s : sys) st[s].matvecs++;
+  } else if (left) {
     (*M)(act, W.slots_dev.p, b, W.r);
   } else {
     convert_vectors<R, R>(ctx, b, n, W.r, n, n, nsys, W.slots_dev.p, W.slots_dev.p);
   }
-  std::vector<double> bnorm(nslot, 0.0), bnorm_true(nslot, 0.0);
   ops.norm2(W.r, nsys, W.slots_dev.p, W.nrm.p);
   fetch_norms(nsys);
   std::vector<int> outit(nslot, 0);
@@ -152,9 +162,15 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
     std::vector<int> keep;
     for (int a = 0; a < nsys; ++a) {
       const int s = sys[a];
-      bnorm[s] = std::sqrt(nbuf[a].x);
-      st[s].resvec.push_back(bnorm[s] > 0 ? 1.0 : 0.0);
-      if (bnorm[s] > 0 && 1.0 > tol_of(s)) keep.push_back(s);
+      if (o.x0) {
+        bnorm[s] = std::sqrt(W.h_bn[a].x);
+        r0[s] = bnorm[s] > 0 ? std::sqrt(nbuf[a].x) / bnorm[s] : 0.0;
+      } else {
+        bnorm[s] = std::sqrt(nbuf[a].x);
+        r0[s] = bnorm[s] > 0 ? 1.0 : 0.0;
+      }
+      st[s].resvec.push_back(r0[s]);
+      if (bnorm[s] > 0 && r0[s] > tol_of(s)) keep.push_back(s);
     }
     act.swap(keep);
   }
@@ -220,13 +236,16 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
       }
       for (int s : act) { st[s].iters++; st[s].matvecs++; }
       // classical Gram-Schmidt, one pass (pgmres_iter.m:107-111); with reorth a
-      // second pass re-projects w (the FP32 inner solve: one pass loses
-      // orthogonality once the residual nears sqrt(eps))
-      ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p);
-      ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, 1.0, -1.0, o.reorth ? nullptr : W.nrm.p);
+      // second pass re-projects w for the systems whose first pass cancelled
+      // more than half of ||w||^2 (the FP32 inner solve: one pass loses
+      // orthogonality once the residual nears sqrt(eps)); the others get a
+      // zero correction and keep their norm
+      ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, nullptr, o.reorth);
+      ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, 1.0, -1.0, W.nrm.p);
       if (o.reorth) {
-        ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h2.p);
-        ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h2.p, 1.0, -1.0, W.nrm.p);
+        ops.reorth_flags(W.h.p, k, W.nrm.p, W.need.p, nact);
+        ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h2.p, W.need.p);
+        ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h2.p, 1.0, -1.0, W.nrm.p, W.need.p);
         VH_HIP(hipMemcpyAsync(W.h_h2.p, W.h2.p, (size_t)nact * (m + 1) * sizeof(double2), hipMemcpyDeviceToHost,
                               strm));
       }
@@ -314,7 +333,7 @@ void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<f
   io.maxit = o.inner_maxit;
   io.tol = o.inner_tol;
   io.final_residual = false;
-  io.flexible = false;
+  io.flexible = o.inner_flexible;
   io.reorth = true;
   // Inner tolerance per system: the outer residual only has to fall from its
   // current value r (relative) to tol, so an inner solve that reduces its
@@ -345,6 +364,7 @@ void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<f
   oo.tol_slot = o.tol_slot;
   oo.final_residual = true;
   oo.flexible = true;
+  oo.x0 = o.x0;
   gmres_batch<double>(ctx, n, A64, &inner, b, x, sys, oo, W64, st);
   for (int s : sys) st[s].inner_iters = inner_iters[s];
 }

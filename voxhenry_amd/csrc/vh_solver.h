@@ -135,6 +135,13 @@ void schur_multiply(Precon& P, int nrhs, const double2* x, int64_t ldx, double2*
 // Several preconditioners of one geometry (sharing Y) applied in one pass:
 // slot s uses tab[pidx[s]] (tab[0] when pidx is null); tab is a device
 // array, pidx a host array indexed by slot.
+// The same with work vectors of type WT (precon_apply_cols: double).  With
+// FP32 factors and WT = float the apply is exact only to FP32 rounding at
+// every level, so it is not a fixed linear operator: callers use it inside a
+// flexible Krylov solve.
+template <typename R, typename WT>
+void precon_apply_cols_wt(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, const int* pidx,
+                          const std::vector<int>& slots, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy);
 template <typename R>
 void precon_apply_cols(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, const int* pidx,
                        const std::vector<int>& slots, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy);

@@ -544,8 +544,8 @@ static void cmd_solve_batch(int nlhs, mxArray* plhs[], int nrhs, const mxArray* 
   for (int64_t f = 0; f < nf && rc == 0; ++f) rc = vh_precon_create(S.geom, &ps[f]);
   if (rc == 0) rc = vh_precon_prepare_many(ps, nf, jw, NULL, 0);
   if (rc == 0)
-    rc = vh_solve_batch(S.geom, nf, ps, ns, jws, pidx, b, (double*)mxGetComplexDoubles(plhs[0]), 0, method, tmax,
-                        tols, 100, 10, 40, 1e-5, 0.0, flag, rr, its, iits, NULL);
+    rc = vh_solve_batch(S.geom, nf, ps, ns, jws, pidx, b, (double*)mxGetComplexDoubles(plhs[0]), 0, 0, method, tmax,
+                        tols, 100, 10, 30, 1e-5, 0.0, flag, rr, its, iits, NULL);
   if (rc != 0) snprintf(msgbuf, sizeof msgbuf, "%s", vh_last_error());
   for (int64_t f = 0; f < nf; ++f)
     if (ps[f]) vh_precon_destroy(ps[f]);

@@ -20,6 +20,17 @@ from .core import Geometry, Precon, MU0
 from ._lib import VH_C128, VH_C64
 
 
+# accuracy-driven tolerance of sweep_batched(rl_rtol=...): the R error of a
+# solve to a relative residual tol was measured at up to ~20 tol |Z| / R (the
+# 5-turn spiral at 100 GHz), so the tolerance is rl_rtol * (R / |Z|) / 20.
+# Below 1e-12 (the tightest residual the driver asks for) the target is not
+# pursued: such a system (R ~ 1e-9 |Z|, a superconductor at low frequency)
+# cannot have R to rl_rtol from any affordable solve; it is counted as
+# unresolved instead.
+RL_MARGIN = 20.0
+RL_TOL_FLOOR = 1e-12
+
+
 def reference_tol(f):
     """VoxHenry_executer.m:219-223."""
     return 1e-12 if f < 1e6 else 1e-8
@@ -123,7 +134,7 @@ class Solver:
         return out
 
     def sweep_batched(self, freqs, method='mixed', tol='reference', precons=None, pidx_of_freq=None, device=False,
-                      **kw):
+                      rl_rtol=None, **kw):
         """Every (port, frequency) system in one batched solve (vh_solve_batch).
 
         tol 'reference': each system gets the tolerance the reference's driver
@@ -133,11 +144,25 @@ class Solver:
         precons[pidx_of_freq[k]] preconditions frequency k (default: one
         preconditioner per frequency).  Every system uses the impedance its
         preconditioner was prepared with, so two-fluid conductors need one
-        preconditioner per frequency (their impedance depends on f)."""
+        preconditioner per frequency (their impedance depends on f).
+
+        rl_rtol (mixed only): relative accuracy wanted of R and L.  Where R
+        (or omega L) is a small part of |Z| a 1e-8 residual leaves it
+        inaccurate (its error grows like tol * |Z| / R), so after the first
+        solve the systems of every frequency whose worst port has
+        min(R, omega L) / |Z| = rho get their
+        tolerance tightened to rl_rtol * rho / RL_MARGIN when that is below
+        their tolerance and not below RL_TOL_FLOOR, and continue from their
+        solutions (warm-started FGMRES).  stats['refined'] counts the
+        systems that continued, stats['rl_unresolved'] those whose target
+        lies below the floor (their R is not fixed to rl_rtol); flag /
+        relres stay relative to the original tolerance."""
         freqs = np.asarray(freqs, float)
         if self.structure.get('lambdaL') is not None and precons is not None and \
                 len(set(np.asarray(pidx_of_freq).tolist())) < len(freqs):
             raise ValueError('two-fluid conductors need one preconditioner per frequency')
+        if rl_rtol is not None and method != 'mixed':
+            raise ValueError('rl_rtol needs the mixed method (warm-started FGMRES)')
         if self._z_f is None:
             self.G.set_impedance(impedance(self.structure, self.idx, freqs[0]))
             self._z_f = freqs[0]
@@ -145,34 +170,70 @@ class Solver:
             precons = self.precons_for(freqs)
             pidx_of_freq = np.arange(len(freqs))
         npo = self.nports
+        nf = len(freqs)
         jw = np.repeat(1j * 2 * np.pi * freqs * MU0, npo)
         pidx = np.repeat(np.asarray(pidx_of_freq), npo)
         if isinstance(tol, str):
             assert tol == 'reference', tol
             tol = np.repeat([reference_tol(f) for f in freqs], npo)
+        tols = np.broadcast_to(np.asarray(tol, float), (nf * npo,)).copy()
+        nz = np.flatnonzero(np.any(self.rhs != 0, axis=0))
+        Pnz = self.rhs[:, nz]
         if device:
             # right-hand sides and solutions stay in device memory; only the
             # port rows of the solutions (where the port vectors are nonzero)
             # come back, for Y = P^T X
             import torch
-            nz = np.flatnonzero(np.any(self.rhs != 0, axis=0))
             rhs = torch.tensor(self.rhs, dtype=torch.complex128, device='cuda')
-            B = rhs.repeat(len(freqs), 1).contiguous()
+            B = rhs.repeat(nf, 1).contiguous()
+            nzd = torch.tensor(nz, device='cuda')
             torch.cuda.current_stream().synchronize()   # this stream only: other geometries may run beside
-            t = time.perf_counter()
-            X, st = self.G.solve_batch(B, jw, precons, pidx, method=method, tol=tol, **kw)
-            Xp = X.index_select(1, torch.tensor(nz, device='cuda')).cpu().numpy()
-            st['solve_time'] = time.perf_counter() - t
-            Xp = Xp.reshape(len(freqs), npo, -1)
-            Pnz = self.rhs[:, nz]
-            Ys = [Pnz @ Xp[k].T for k in range(len(freqs))]
+            rows = lambda X: X.index_select(1, nzd).cpu().numpy()
         else:
-            B = np.repeat(self.rhs[None], len(freqs), axis=0).reshape(-1, self.G.n)
-            t = time.perf_counter()
-            X, st = self.G.solve_batch(B, jw, precons, pidx, method=method, tol=tol, **kw)
-            st['solve_time'] = time.perf_counter() - t
-            X = X.reshape(len(freqs), npo, -1)
-            Ys = [self.rhs @ X[k].T for k in range(len(freqs))]
+            B = np.repeat(self.rhs[None], nf, axis=0).reshape(-1, self.G.n)
+            rows = lambda X: X[:, nz]
+        t = time.perf_counter()
+        X, st = self.G.solve_batch(B, jw, precons, pidx, method=method, tol=tols, **kw)
+        Xp = rows(X).reshape(nf, npo, -1)
+        Ys = np.array([Pnz @ Xp[k].T for k in range(nf)])     # Y[kk, port] = sum(rhs_kk .* x_port)
+        st['refined'] = 0
+        st['rl_unresolved'] = 0
+        if rl_rtol is not None:
+            Zs = np.linalg.inv(Ys)
+            d = np.diagonal(Zs, axis1=1, axis2=2)
+            # the smaller of R / |Z| and omega L / |Z| over the ports: L is
+            # the small part of Z at low frequency as R is at high frequency
+            rho = np.min(np.minimum(np.abs(d.real), np.abs(d.imag)) / np.maximum(np.abs(d), 1e-300), axis=1)
+            target = np.repeat(rl_rtol * rho / RL_MARGIN, npo)
+            st['rl_unresolved'] = int(np.sum(target < RL_TOL_FLOOR))
+            want = np.minimum(tols, target)
+            sel = np.flatnonzero((want < tols * 0.999) & (target >= RL_TOL_FLOOR))
+            if len(sel):
+                if device:
+                    import torch
+                    idx = torch.tensor(sel, device='cuda')
+                    Bs, X0 = B.index_select(0, idx).contiguous(), X.index_select(0, idx).contiguous()
+                else:
+                    Bs, X0 = B[sel], X[sel]
+                X2, st2 = self.G.solve_batch(Bs, jw[sel], precons, pidx[sel], method=method, tol=want[sel], x0=X0,
+                                             **kw)
+                if device:
+                    X.index_copy_(0, idx, X2)
+                else:
+                    X[sel] = X2
+                for key in ('iters', 'inner_iters'):
+                    st[key][sel] += st2[key]
+                st['relres'][sel] = st2['relres']
+                st['resid_est'][sel] = st2['resid_est']
+                st['flag'][sel] = (st2['relres'] > tols[sel]).astype(st['flag'].dtype)
+                st['refined'] = len(sel)
+                ks = np.unique(sel // npo)
+                Xp2 = rows(X2).reshape(len(sel), -1)
+                for q, s_ in enumerate(sel):
+                    Xp[s_ // npo, s_ % npo] = Xp2[q]
+                for k in ks:
+                    Ys[k] = Pnz @ Xp[k].T
+        st['solve_time'] = time.perf_counter() - t
         out = dict(freq=freqs, Y=[], Z=[], R=[], L=[], stats=st)
         for k, f in enumerate(freqs):
             Y = Ys[k]
@@ -199,7 +260,7 @@ class Solver:
         return out
 
 
-def workspace_bytes(S, nsys, methods=('mixed', 'fp64'), inner_restart=40):
+def workspace_bytes(S, nsys, methods=('mixed', 'fp64'), inner_restart=30):
     """Device bytes the batched solves of `nsys` systems of solver S allocate
     when every listed method runs on it: the Krylov arena grows to the
     largest method's layout, the matvec keeps a pass-X grid per precision
