@@ -184,6 +184,7 @@ def matvec_bench(ctx, n, reps):
         del x, y
     out['c64_vs_c128_rel'] = float(np.linalg.norm(res['c64'] - res['c128']) / np.linalg.norm(res['c128']))
     out['zshape'] = tuned
+    out['shapes'] = G.shapes()
     out['grid'] = [L, M, N]
     out['pad'] = [Px, Py, Pz]
     G.close()

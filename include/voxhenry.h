@@ -139,12 +139,22 @@ int vh_matvec_timed(vh_geom* g, double jwmu_re, double jwmu_im, int prec, int mo
  * FFT is MATLAB's fftn): times every compiled pass-Z shape (threads per
  * workgroup x elements per thread, which fixes the radices and the kx tile
  * width) on nrhs device vectors of this geometry (convolution only, `reps`
- * calls each) and keeps the fastest for precision prec.  ms_shape (optional,
- * nshape entries) receives the average time of one matvec per shape;
- * *best receives the chosen index. */
+ * calls each) and keeps the fastest for precision prec; then, with that
+ * pass-Z shape, every compiled X/Y-pass shape (threads per workgroup x
+ * element bound of the x and y FFT passes, which fixes their radices and
+ * lines per workgroup).  Shapes that cannot hold the grid's pencils are not
+ * timed.  ms_shape (optional, room for *nshape entries on input) receives
+ * the average time of one matvec per pass-Z shape followed by one per X/Y
+ * shape (HUGE_VAL where not timed); *nshape receives the number of entries
+ * written; *best receives the chosen pass-Z index (vh_geom_get_shapes gives
+ * both choices). */
 int vh_geom_autotune(vh_geom* g, int prec, int64_t nrhs, int reps, int* best, double* ms_shape, int* nshape);
 /* Select a pass-Z shape by index (as reported by vh_geom_autotune). */
 int vh_geom_set_zshape(vh_geom* g, int prec, int shape);
+/* Select an X/Y-pass shape by index (refused when it cannot hold the grid's x or y pencils). */
+int vh_geom_set_xyshape(vh_geom* g, int prec, int shape);
+/* shapes[0..3] = pass-Z shape (FP32, FP64), X/Y-pass shape (FP32, FP64) in use. */
+int vh_geom_get_shapes(vh_geom* g, int* shapes);
 
 /* ---- preconditioner ---------------------------------------------------------
  * Replaces lse_sparse_precon_prepare.m (schur_invert + ldlt_decomp + CHOLMOD)

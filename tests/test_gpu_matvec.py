@@ -13,6 +13,7 @@ pytestmark = pytest.mark.gpu
 MU0 = 4 * np.pi * 1e-7
 TOL64 = 1e-12     # FP64 path vs oracle, relative 2-norm
 TOL32 = 2e-5      # FP32 path vs oracle, relative 2-norm
+NXY = 3           # compiled X/Y-pass shapes per precision
 
 
 def rel(a, b):
@@ -197,10 +198,12 @@ def test_matvec_radix_shapes(ctx, dims, hollow, pad_mode):
     G.close()
 
 
-@pytest.mark.parametrize('dims', [(16, 16, 16), (400, 4, 4), (9, 33, 17), (4, 4, 128), (5, 4, 100)])
+@pytest.mark.parametrize('dims', [(16, 16, 16), (400, 4, 4), (9, 33, 17), (4, 4, 128), (5, 4, 100), (600, 6, 3)])
 def test_matvec_every_zshape(ctx, dims):
-    """Every compiled pass-Z shape (the autotuner's choices) gives the same
-    operator as the oracle; the autotuner itself leaves a valid choice."""
+    """Every compiled pass-Z shape and every X/Y-pass shape (the
+    autotuner's choices) gives the same operator as the oracle; the
+    autotuner itself leaves a valid choice.  (600 x 6 x 3: Px = 1200 is too
+    long for the 128-thread X shape in FP64, which is then skipped.)"""
     import voxhenry_amd as vh
     say('zshapes', dims)
     L, M, N = dims
@@ -222,8 +225,22 @@ def test_matvec_every_zshape(ctx, dims):
             for k in range(2):
                 e = rel(Y[k].astype(complex), ref[k])
                 assert e < tol, (prec, sh, k, e)
+        # every X/Y-pass shape (threads x element bound of the x and y passes)
+        G.set_zshape(prec, 0)
+        for sh in range(NXY):
+            try:
+                G.set_xyshape(prec, sh)
+            except vh.VoxHenryError:
+                continue          # cannot hold this grid's pencils
+            Y = G.matvec(X if prec == vh.VH_C128 else X.astype(np.complex64), jwmu, prec, 0)
+            for k in range(2):
+                e = rel(Y[k].astype(complex), ref[k])
+                assert e < tol, (prec, 'xy', sh, k, e)
         best, ms = G.autotune(prec, nrhs=2, reps=1)
-        assert 0 <= best < nsh and len(ms) == nsh and np.all(ms > 0)
+        assert 0 <= best < nsh and len(ms) == nsh + NXY and np.all(ms > 0)
+        shp = G.shapes()
+        assert shp['z64' if prec == vh.VH_C128 else 'z32'] == best
+        assert np.isfinite(ms[nsh + shp['xy64' if prec == vh.VH_C128 else 'xy32']])
     G.close()
 
 

@@ -50,6 +50,8 @@ SIGNATURES = {
     'vh_geom_autotune': [_P, _I, _I64, _I, _P, _P, _P],
     'vh_precon_prepare_many': [_P, _I64, _P, _P, _I],
     'vh_geom_set_zshape': [_P, _I, _I],
+    'vh_geom_set_xyshape': [_P, _I, _I],
+    'vh_geom_get_shapes': [_P, _P],
     'vh_geom_set_incidence': [_P, _I64, _I64, _P, _P, _P],
     'vh_geom_set_impedance': [_P, _P],
     'vh_geom_build_topology': [_P, _I64, _P, _P, _P, _P, _I64, _P, _P],

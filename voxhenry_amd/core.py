@@ -334,8 +334,8 @@ class Geometry:
 
     def autotune(self, prec, nrhs=1, reps=3):
         """Time every compiled pass-Z shape on nrhs vectors of this geometry
-        and keep the fastest (vh_geom_autotune); returns (best index, ms per
-        shape)."""
+        and keep the fastest, then every X/Y-pass shape (vh_geom_autotune);
+        returns (best pass-Z index, ms per pass-Z shape then per X/Y shape)."""
         self.ctx.use_torch_stream()
         best = C.c_int(0)
         ms = np.zeros(16)
@@ -345,6 +345,15 @@ class Geometry:
 
     def set_zshape(self, prec, shape):
         check(lib().vh_geom_set_zshape(self.h, prec, int(shape)))
+
+    def set_xyshape(self, prec, shape):
+        check(lib().vh_geom_set_xyshape(self.h, prec, int(shape)))
+
+    def shapes(self):
+        """Pass shapes in use: dict(z32, z64, xy32, xy64)."""
+        v = np.zeros(4, np.int32)
+        check(lib().vh_geom_get_shapes(self.h, ptr(v)))
+        return dict(z32=int(v[0]), z64=int(v[1]), xy32=int(v[2]), xy64=int(v[3]))
 
     def fft3d(self, data, prec, direction):
         self.ctx.use_torch_stream()

@@ -265,7 +265,8 @@ int vh_geom_autotune(vh_geom* g, int prec, int64_t nrhs, int reps, int* best, do
     VH_REQUIRE(nrhs >= 1 && nrhs <= 13107 && reps >= 1, "nrhs / reps out of range");
     VH_REQUIRE(G.has_tensor, "autotune: Green's tensor not set");
     const int p = prec == VH_C128 ? 0 : 1;
-    const int ns = vh::matvec_num_zshapes(p);
+    const int q = prec == VH_C128 ? 1 : 0;    // index of the precision in G.zshape / G.xyshape
+    const int nz = vh::matvec_num_zshapes(p), nxy = vh::matvec_num_xyshapes(p);
     const int64_t n = G.num_curr;
     const size_t es = prec == VH_C128 ? 16 : 8;
     vh::DevBuf<char> x((size_t)nrhs * n * es), y((size_t)nrhs * n * es);
@@ -280,39 +281,57 @@ int vh_geom_autotune(vh_geom* g, int prec, int64_t nrhs, int reps, int* best, do
       else
         vh::matvec<float>(G, (int)nrhs, (const float2*)x.p, n, (float2*)y.p, n, 0.0, 1.0, 2);
     };
-    const int keep = G.zshape[prec == VH_C128 ? 1 : 0];
-    int arg = 0;
-    double tbest = 1e300;
+    auto timed = [&]() -> double {
+      run();   // warm-up
+      VH_HIP(hipEventRecord(e0, st));
+      for (int r = 0; r < reps; ++r) run();
+      VH_HIP(hipEventRecord(e1, st));
+      VH_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      VH_HIP(hipEventElapsedTime(&ms, e0, e1));
+      return ms / reps;
+    };
+    auto put = [&](int i, double t) { if (ms_shape && nshape && i < *nshape) ms_shape[i] = t; };
+    const int keep_z = G.zshape[q], keep_xy = G.xyshape[q];
+    int arg = 0, arg_xy = 0;
     try {
-      for (int i = 0; i < ns; ++i) {
-        if (ms_shape && nshape && i < *nshape) ms_shape[i] = HUGE_VAL;
-        // shapes that cannot hold a z pencil, or that the grid's kernel
-        // ignores, are not timed (a shape-less grid keeps shape 0)
+      // pass-Z shapes with the default X/Y shape; shapes that cannot hold a z
+      // pencil, or that the grid's kernel ignores, are not timed (a
+      // shape-less grid keeps shape 0)
+      G.xyshape[q] = 0;
+      double tbest = 1e300;
+      for (int i = 0; i < nz; ++i) {
+        put(i, HUGE_VAL);
         if (!vh::matvec_zshape_applies(G, p, i)) continue;
-        G.zshape[prec == VH_C128 ? 1 : 0] = i;
-        run();   // warm-up
-        VH_HIP(hipEventRecord(e0, st));
-        for (int r = 0; r < reps; ++r) run();
-        VH_HIP(hipEventRecord(e1, st));
-        VH_HIP(hipEventSynchronize(e1));
-        float ms = 0;
-        VH_HIP(hipEventElapsedTime(&ms, e0, e1));
-        const double t = ms / reps;
-        if (ms_shape && nshape && i < *nshape) ms_shape[i] = t;
+        G.zshape[q] = i;
+        const double t = timed();
+        put(i, t);
         if (t < tbest) { tbest = t; arg = i; }
       }
-      if (tbest == 1e300) arg = 0;
+      G.zshape[q] = arg;
+      // then the X/Y-pass shapes with that pass-Z shape
+      tbest = 1e300;
+      for (int i = 0; i < nxy; ++i) {
+        put(nz + i, HUGE_VAL);
+        if (!vh::matvec_xyshape_applies(G, p, i)) continue;
+        G.xyshape[q] = i;
+        const double t = timed();
+        put(nz + i, t);
+        if (t < tbest) { tbest = t; arg_xy = i; }
+      }
     } catch (...) {
-      G.zshape[prec == VH_C128 ? 1 : 0] = keep;
+      G.zshape[q] = keep_z;
+      G.xyshape[q] = keep_xy;
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
       throw;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    G.zshape[prec == VH_C128 ? 1 : 0] = arg;
+    G.zshape[q] = arg;
+    G.xyshape[q] = arg_xy;
     *best = arg;
-    if (nshape) *nshape = ns;
+    if (nshape) *nshape = nz + nxy;
   });
 }
 
@@ -323,6 +342,27 @@ int vh_geom_set_zshape(vh_geom* g, int prec, int shape) {
     const int p = prec == VH_C128 ? 0 : 1;
     VH_REQUIRE(shape >= 0 && shape < vh::matvec_num_zshapes(p), "shape index out of range");
     g->G.zshape[prec == VH_C128 ? 1 : 0] = shape;
+  });
+}
+
+int vh_geom_set_xyshape(vh_geom* g, int prec, int shape) {
+  return guard([&] {
+    need(g, "g");
+    VH_REQUIRE(prec == VH_C128 || prec == VH_C64, "prec must be VH_C128 or VH_C64");
+    const int p = prec == VH_C128 ? 0 : 1;
+    VH_REQUIRE(shape >= 0 && shape < vh::matvec_num_xyshapes(p), "shape index out of range");
+    VH_REQUIRE(vh::matvec_xyshape_applies(g->G, p, shape), "this X/Y-pass shape cannot hold the grid's pencils");
+    g->G.xyshape[prec == VH_C128 ? 1 : 0] = shape;
+  });
+}
+
+int vh_geom_get_shapes(vh_geom* g, int* shapes) {
+  return guard([&] {
+    need(g, "g"); need(shapes, "shapes");
+    shapes[0] = g->G.zshape[0];
+    shapes[1] = g->G.zshape[1];
+    shapes[2] = g->G.xyshape[0];
+    shapes[3] = g->G.xyshape[1];
   });
 }
 
@@ -840,6 +880,27 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
           }
           if (vh::trace_on()) fprintf(stderr, "[vh] solve_batch: %zu systems continued with the FP64-work-vector inner solve\n",
                                       again.size());
+          // last resort for what the FP32 inner solves cannot carry to the
+          // tolerance: FP64 flexible GMRES with the FP64 preconditioner from
+          // the current solutions (outer workspace, at most 20 cycles)
+          std::vector<int> last;
+          for (int i : again)
+            if (cs[i].flag != 0) last.push_back(i);
+          if (!last.empty()) {
+            vh::SolveOpts o3;
+            o3.restart = W64.m; o3.maxit = 20; o3.tol = tol; o3.final_residual = true; o3.flexible = true;
+            o3.x0 = true;
+            if (tols) o3.tol_slot = &tolc;
+            std::vector<vh::SysStats> cs3;
+            vh::gmres_batch<double>(G.ctx, n, A64, &M64, bc, xc, last, o3, W64, cs3);
+            for (int i : last) {
+              cs3[i].iters += cs[i].iters;
+              cs3[i].inner_iters = cs[i].inner_iters;
+              cs3[i].matvecs += cs[i].matvecs;
+              cs[i] = cs3[i];
+            }
+            if (vh::trace_on()) fprintf(stderr, "[vh] solve_batch: %zu systems finished in FP64\n", last.size());
+          }
         }
       }
       for (int i = 0; i < cnt; ++i) stats[s0 + i] = cs[i];

@@ -96,6 +96,7 @@ struct Geometry {
   int nports = 0;
   PassTimer* timer = nullptr;
   int zshape[2] = {0, 0};             // pass-Z shape index per precision [FP32, FP64] (autotuned)
+  int xyshape[2] = {0, 0};            // X/Y-pass shape index per precision [FP32, FP64] (autotuned)
   bool use_p2z = true;                // compile-time pass-Z kernels for power-of-two Pz
   std::shared_ptr<PreconSymbolic> precon_sym;   // shared by all preconditioners of this geometry
   std::mutex precon_sym_mu;                     // guards its construction
@@ -170,6 +171,10 @@ int matvec_num_zshapes(int prec);
 // threads x elements cannot hold one z pencil, or when the grid's z length is
 // served by the one-radix kernel, which has no shape.
 bool matvec_zshape_applies(const Geometry& G, int prec, int shape);
+// Compiled X/Y-pass shapes (threads x element bound of the x and y FFT
+// passes) and whether one can hold the grid's x and y pencils.
+int matvec_num_xyshapes(int prec);
+bool matvec_xyshape_applies(const Geometry& G, int prec, int shape);
 void geometry_set_spectrum_full(Geometry& G, const double* fN, int64_t P0, int64_t P1, int64_t P2,
                                 double scale_re, double scale_im);
 

@@ -32,8 +32,15 @@ template <typename R> __device__ __forceinline__ cplx<R> czero() { cplx<R> c; c.
 // and pencil.  X/Xi and Y/Yi hold one pencil element set per thread; Z holds
 // the five components' spectra of its pencils in registers between the
 // forward and the inverse transform, hence smaller bounds (ZShape below).
-constexpr int NT_XY = 256;
-template <typename R> constexpr int e_xy() { return sizeof(R) == 8 ? 8 : 16; }
+// X/Y-pass shapes the autotuner chooses from (threads per workgroup, element
+// bound); entry 0 is the default.
+struct XYShape { int nt, e; };
+constexpr XYShape kXYShapes32[] = {{256, 16}, {256, 8}, {128, 16}};
+constexpr XYShape kXYShapes64[] = {{256, 8}, {256, 4}, {128, 8}};
+template <typename R> constexpr int num_xyshapes() {
+  return sizeof(R) == 8 ? (int)(sizeof(kXYShapes64) / sizeof(XYShape)) : (int)(sizeof(kXYShapes32) / sizeof(XYShape));
+}
+template <typename R> static XYShape xyshape(int i) { return sizeof(R) == 8 ? kXYShapes64[i] : kXYShapes32[i]; }
 template <int N> constexpr int ilog2() { return N <= 1 ? 0 : 1 + ilog2<N / 2>(); }
 
 // ---------------------------------------------------------------------------
@@ -933,13 +940,12 @@ static void launch_z(int shape, int rl, dim3 grid, size_t lds, hipStream_t st, c
 
 // The passes on the pass-X output grid A (whose rows of empty x lines read
 // as zero) and the B grid of the caller.
-template <typename R>
-static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy,
-                       double are, double aim, int mode, const int* slots, const double2* alphas,
-                       const cplx<R>* const* zslot, cplx<R>* A, cplx<R>* B) {
+template <typename R, int EXY, int NT_XY>
+static void run_passes_t(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy,
+                         double are, double aim, int mode, const int* slots, const double2* alphas,
+                         const cplx<R>* const* zslot, cplx<R>* A, cplx<R>* B) {
   const GridDims& g = G.g;
   hipStream_t st = G.ctx->stream;
-  constexpr int EXY = e_xy<R>();
   const int zsh = std::min(std::max(G.zshape[sizeof(R) == 8 ? 1 : 0], 0), num_zshapes<R>() - 1);
   const ZShape Zs = zshape<R>(zsh);
   const int EZ = Zs.e, NTZ = Zs.nt;
@@ -1038,7 +1044,35 @@ static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cpl
   if (tm) tm->pending = true;
 }
 
+// The passes with the geometry's X/Y shape of this precision.
+template <typename R>
+static void run_passes(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy,
+                       double are, double aim, int mode, const int* slots, const double2* alphas,
+                       const cplx<R>* const* zslot, cplx<R>* A, cplx<R>* B) {
+  const int sh = std::min(std::max(G.xyshape[sizeof(R) == 8 ? 1 : 0], 0), num_xyshapes<R>() - 1);
+  if constexpr (sizeof(R) == 8) {
+    if (sh == 1) run_passes_t<R, 4, 256>(G, nrhs, x, ldx, y, ldy, are, aim, mode, slots, alphas, zslot, A, B);
+    else if (sh == 2) run_passes_t<R, 8, 128>(G, nrhs, x, ldx, y, ldy, are, aim, mode, slots, alphas, zslot, A, B);
+    else run_passes_t<R, 8, 256>(G, nrhs, x, ldx, y, ldy, are, aim, mode, slots, alphas, zslot, A, B);
+  } else {
+    if (sh == 1) run_passes_t<R, 8, 256>(G, nrhs, x, ldx, y, ldy, are, aim, mode, slots, alphas, zslot, A, B);
+    else if (sh == 2) run_passes_t<R, 16, 128>(G, nrhs, x, ldx, y, ldy, are, aim, mode, slots, alphas, zslot, A, B);
+    else run_passes_t<R, 16, 256>(G, nrhs, x, ldx, y, ldy, are, aim, mode, slots, alphas, zslot, A, B);
+  }
+}
+
 int matvec_num_zshapes(int prec) { return prec == 0 ? num_zshapes<double>() : num_zshapes<float>(); }
+int matvec_num_xyshapes(int prec) { return prec == 0 ? num_xyshapes<double>() : num_xyshapes<float>(); }
+
+template <typename R> static bool xyshape_applies(const Geometry& G, int shape) {
+  const XYShape z = xyshape<R>(shape);
+  return fft_tpp(G.fx.n, z.e, z.nt) >= 0 && fft_tpp(G.fy.n, z.e, z.nt) >= 0;
+}
+
+bool matvec_xyshape_applies(const Geometry& G, int prec, int shape) {
+  if (shape < 0 || shape >= matvec_num_xyshapes(prec)) return false;
+  return prec == 0 ? xyshape_applies<double>(G, shape) : xyshape_applies<float>(G, shape);
+}
 
 template <typename R> static bool zshape_applies(const Geometry& G, int shape) {
   const GridDims& g = G.g;

@@ -22,12 +22,13 @@ from ._lib import VH_C128, VH_C64
 
 # accuracy-driven tolerance of sweep_batched(rl_rtol=...): the R error of a
 # solve to a relative residual tol was measured at up to ~20 tol |Z| / R (the
-# 5-turn spiral at 100 GHz), so the tolerance is rl_rtol * (R / |Z|) / 20.
+# 5-turn spiral at 100 GHz; up to ~25 on the uneven batch's bars), so the
+# tolerance is rl_rtol * (R / |Z|) / 50.
 # Below 1e-12 (the tightest residual the driver asks for) the target is not
 # pursued: such a system (R ~ 1e-9 |Z|, a superconductor at low frequency)
 # cannot have R to rl_rtol from any affordable solve; it is counted as
 # unresolved instead.
-RL_MARGIN = 20.0
+RL_MARGIN = 50.0
 RL_TOL_FLOOR = 1e-12
 
 
