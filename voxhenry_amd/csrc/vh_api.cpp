@@ -33,6 +33,9 @@ double batch_bytes_per_system(const vh::Geometry& G, int method, int restart, in
   const bool mixed = method == 1;
   // mixed: the FP32 inner solve is flexible (its basis Z is stored too)
   double b = (double)vh::KrylovCache::bytes(G.n(), 1, restart, mixed, mixed ? inner_restart : 0, mixed, mixed);
+  // the per-system least-squares state and reduction columns of each Krylov workspace
+  b += (double)vh::KrylovWork<double>::small_bytes_per_slot(restart);
+  if (mixed) b += (double)vh::KrylovWork<float>::small_bytes_per_slot(inner_restart);
   const vh::GridDims& gd = G.g;
   const double a = 5.0 * gd.Px * (double)gd.M * gd.N, bgrid = 5.0 * gd.Px * (double)gd.Py * gd.N;
   return b + bgrid * 16 + a * (16 + (mixed ? 8 : 0));

@@ -4,15 +4,16 @@
 //   the same semantics: x0 = 0, left preconditioner, bnorm = ||M b||, one-pass
 //   classical Gram-Schmidt, stop when the least-squares residual drops below
 //   tol*bnorm, restart from r = V(:,1:k+1)*res, final true relres.  The small
-//   least-squares problem is solved by Givens rotations (same residuals as the
-//   reference's Hc\rhs up to rounding).  Runs a batch of independent systems in
-//   lock step; a system that converges is retired from the batch (slot map).
+//   least-squares problem is solved on the device by Givens rotations (same
+//   residuals as the reference's Hc\rhs up to rounding).  Runs a batch of
+//   independent systems in lock step; a system that converges is retired from
+//   the batch (slot map).
 // * fpgmres -- fpgmres.m:58-152 + fpgmres_iter.m:26-121 (flexible, right
 //   preconditioned, bnorm = ||b||).
 // * fgmres_mixed -- FP64 flexible GMRES whose preconditioner is an FP32
-//   left-preconditioned GMRES with the FP32 matvec and FP32 preconditioner:
-//   each outer step is one FP64 matvec, so the outer residual is the true
-//   FP64 residual and accuracy is recovered by the outer iteration.
+//   flexible GMRES with the FP32 matvec and FP32 preconditioner: each outer
+//   step is one FP64 matvec, so the outer residual is the true FP64 residual
+//   and accuracy is recovered by the outer iteration.
 //
 // Vectors of one batch live in slots of length ld: vector s of basis j is at
 // V + j*vstride + s*ld.
@@ -160,6 +161,164 @@ __global__ __launch_bounds__(KT) void k_combine(const cplx<R>* __restrict__ V, i
   }
 }
 
+// Rows a thread of the fused Gram-Schmidt kernels loads at once: UE rows of
+// KB basis elements (32 registers of complex64 data), all loads issued before
+// the arithmetic so every wave keeps UE*(KB+1) loads in flight.
+template <int KB> constexpr int fused_rows() { return KB >= 16 ? 1 : (KB >= 8 ? 2 : 4); }
+
+// First half of a fused two-pass classical Gram-Schmidt step (CGS2):
+// w <- w - sum_j V_j h1_j in place and, from the same pass over the basis,
+// the partials of h2_j = V_j^H w (the second pass's projections, j < k) and
+// of ||w||^2 (entry k), taken on the stored (rounded) w.  The KB + 1 running
+// sums of a thread stay in registers and meet in the wave and the block once,
+// at the end, in a fixed order (deterministic).
+template <typename R, int KB>
+__global__ __launch_bounds__(KT) void k_combine_dot(const cplx<R>* __restrict__ V, int64_t vstride,
+                                                     cplx<R>* __restrict__ w, int64_t ld, int64_t n, int k, int kmax,
+                                                     const int* __restrict__ slots, const double2* __restrict__ h1,
+                                                     double2* __restrict__ part, int nchunk) {
+  constexpr int UE = fused_rows<KB>();
+  static_assert(EPT % UE == 0, "rows per load group must divide the rows per thread");
+  __shared__ double2 cf[KB];
+  __shared__ double2 red[4][KB + 1];
+  const int a = blockIdx.y;
+  const int s = slots[a];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid < KB) cf[tid] = tid < k ? h1[(int64_t)a * kmax + tid] : make_double2(0.0, 0.0);
+  __syncthreads();
+  double ax_[KB + 1], ay_[KB + 1];
+#pragma unroll
+  for (int j = 0; j <= KB; ++j) { ax_[j] = 0; ay_[j] = 0; }
+  const cplx<R>* Vs = V + (int64_t)s * ld;
+  cplx<R>* ws = w + (int64_t)s * ld;
+  for (int c = blockIdx.x; c < nchunk; c += gridDim.x) {
+    const int64_t base = (int64_t)c * CHUNK;
+#pragma unroll 1
+    for (int e0 = 0; e0 < EPT; e0 += UE) {
+      cplx<R> vr[UE][KB], wv[UE];
+#pragma unroll
+      for (int u = 0; u < UE; ++u) {
+        const int64_t i = base + (e0 + u) * KT + tid;
+        if (i < n) {
+          // 32-bit element offset from a per-vector uniform base (n < 2^31):
+          // scalar base + vector offset addressing, no 64-bit address per lane
+          const uint32_t ii = (uint32_t)i;
+          wv[u] = ws[ii];
+#pragma unroll
+          for (int j = 0; j < KB; ++j)
+            if (j < k) vr[u][j] = (Vs + j * vstride)[ii];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UE; ++u) {
+        const int64_t i = base + (e0 + u) * KT + tid;
+        if (i < n) {
+          double x = wv[u].x, y = wv[u].y;
+#pragma unroll
+          for (int j = 0; j < KB; ++j)
+            if (j < k) {
+              x -= (double)vr[u][j].x * cf[j].x - (double)vr[u][j].y * cf[j].y;
+              y -= (double)vr[u][j].x * cf[j].y + (double)vr[u][j].y * cf[j].x;
+            }
+          cplx<R> r; r.x = (R)x; r.y = (R)y;
+          ws[i] = r;
+          const double rx = r.x, ry = r.y;
+#pragma unroll
+          for (int j = 0; j < KB; ++j)
+            if (j < k) {
+              ax_[j] += (double)vr[u][j].x * rx + (double)vr[u][j].y * ry;
+              ay_[j] += (double)vr[u][j].x * ry - (double)vr[u][j].y * rx;
+            }
+          ax_[KB] += rx * rx + ry * ry;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j <= KB; ++j) {
+    if (j < k || j == KB) {
+      const double sx = wave_sum(ax_[j]), sy = wave_sum(ay_[j]);
+      if (lane == 0) red[wave][j] = make_double2(sx, sy);
+    }
+  }
+  __syncthreads();
+  for (int j = tid; j <= k; j += KT) {
+    const int q = j < k ? j : KB;
+    double2 r;
+    r.x = ((red[0][q].x + red[1][q].x) + red[2][q].x) + red[3][q].x;
+    r.y = ((red[0][q].y + red[1][q].y) + red[2][q].y) + red[3][q].y;
+    part[((int64_t)a * kmax + j) * gridDim.x + blockIdx.x] = r;
+  }
+}
+
+// Second half of the fused CGS2 step: out = (w - sum_j V_j h2_j) / nu with
+// nu^2 = ||w||^2 - sum_j |h2_j|^2 (h2[a][k] = ||w||^2 from k_combine_dot; the
+// basis is orthonormal, so the norm of the corrected vector follows without
+// another pass over it); nrm2[a] = nu^2.  Every block derives nu from the
+// same values in the same order; nu = 0 (breakdown) gives a zero vector.
+template <typename R, int KB>
+__global__ __launch_bounds__(KT) void k_combine_scale(const cplx<R>* __restrict__ V, int64_t vstride,
+                                                       const cplx<R>* __restrict__ w, cplx<R>* __restrict__ out,
+                                                       int64_t ld, int64_t n, int k, int kmax,
+                                                       const int* __restrict__ slots, const double2* __restrict__ h2,
+                                                       double2* __restrict__ nrm2, int nchunk) {
+  constexpr int UE = fused_rows<KB>();
+  __shared__ double2 cf[KB];
+  __shared__ double inv_s;
+  const int a = blockIdx.y;
+  const int s = slots[a];
+  const int tid = threadIdx.x;
+  if (tid < KB) cf[tid] = tid < k ? h2[(int64_t)a * kmax + tid] : make_double2(0.0, 0.0);
+  __syncthreads();
+  if (tid == 0) {
+    double nu2 = h2[(int64_t)a * kmax + k].x;
+    for (int j = 0; j < k; ++j) nu2 -= cf[j].x * cf[j].x + cf[j].y * cf[j].y;
+    nu2 = nu2 > 0 ? nu2 : 0.0;
+    inv_s = nu2 > 0 ? 1.0 / sqrt(nu2) : 0.0;
+    if (blockIdx.x == 0) nrm2[a] = make_double2(nu2, 0.0);
+  }
+  __syncthreads();
+  const double inv = inv_s;
+  const cplx<R>* Vs = V + (int64_t)s * ld;
+  const cplx<R>* ws = w + (int64_t)s * ld;
+  cplx<R>* os = out + (int64_t)s * ld;
+  for (int c = blockIdx.x; c < nchunk; c += gridDim.x) {
+    const int64_t base = (int64_t)c * CHUNK;
+#pragma unroll 1
+    for (int e0 = 0; e0 < EPT; e0 += UE) {
+      cplx<R> vr[UE][KB], wv[UE];
+#pragma unroll
+      for (int u = 0; u < UE; ++u) {
+        const int64_t i = base + (e0 + u) * KT + tid;
+        if (i < n) {
+          // 32-bit element offset from a per-vector uniform base (n < 2^31):
+          // scalar base + vector offset addressing, no 64-bit address per lane
+          const uint32_t ii = (uint32_t)i;
+          wv[u] = ws[ii];
+#pragma unroll
+          for (int j = 0; j < KB; ++j)
+            if (j < k) vr[u][j] = (Vs + j * vstride)[ii];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < UE; ++u) {
+        const int64_t i = base + (e0 + u) * KT + tid;
+        if (i < n) {
+          double x = wv[u].x, y = wv[u].y;
+#pragma unroll
+          for (int j = 0; j < KB; ++j)
+            if (j < k) {
+              x -= (double)vr[u][j].x * cf[j].x - (double)vr[u][j].y * cf[j].y;
+              y -= (double)vr[u][j].x * cf[j].y + (double)vr[u][j].y * cf[j].x;
+            }
+          cplx<R> r; r.x = (R)(x * inv); r.y = (R)(y * inv);
+          os[i] = r;
+        }
+      }
+    }
+  }
+}
+
 // Selective second Gram-Schmidt pass: system a needs it when the first pass
 // removed more than half of ||w||^2 (||w'||^2 < eta2 ||w||^2, the
 // Kahan-Parlett criterion with eta = 1/sqrt(2)): then w' carries too little
@@ -290,6 +449,48 @@ void VecOps<R>::combine(const cplx<R>* V, int64_t vstride, cplx<R>* out, int k, 
 }
 
 template <typename R>
+void VecOps<R>::combine_dot(const cplx<R>* V, int64_t vstride, cplx<R>* w, int k, int nact, const int* slots,
+                            const double2* h1, double2* h2) {
+  if (nact <= 0) return;
+  VH_REQUIRE(k >= 1 && k <= kFusedMax && k + 1 <= kmax, "internal: fused Gram-Schmidt step outside its basis range");
+  hipStream_t st = ctx->stream;
+  const int nb = blocks_for(nchunk, nact);
+  const dim3 grid(nb, nact);
+  if (k <= 4)
+    hipLaunchKernelGGL((k_combine_dot<R, 4>), grid, dim3(KT), 0, st, V, vstride, w, ld, n, k, kmax, slots, h1, part,
+                       nchunk);
+  else if (k <= 8)
+    hipLaunchKernelGGL((k_combine_dot<R, 8>), grid, dim3(KT), 0, st, V, vstride, w, ld, n, k, kmax, slots, h1, part,
+                       nchunk);
+  else
+    hipLaunchKernelGGL((k_combine_dot<R, 16>), grid, dim3(KT), 0, st, V, vstride, w, ld, n, k, kmax, slots, h1, part,
+                       nchunk);
+  VH_LAUNCHED();
+  hipLaunchKernelGGL(k_reduce2, dim3(k + 1, nact), dim3(256), 0, st, part, nb, h2, kmax, nullptr, 0);
+  VH_LAUNCHED();
+}
+
+template <typename R>
+void VecOps<R>::combine_scale(const cplx<R>* V, int64_t vstride, const cplx<R>* w, cplx<R>* out, int k, int nact,
+                              const int* slots, const double2* h2, double2* nrm2) {
+  if (nact <= 0) return;
+  VH_REQUIRE(k >= 1 && k <= kFusedMax && k + 1 <= kmax, "internal: fused Gram-Schmidt step outside its basis range");
+  hipStream_t st = ctx->stream;
+  const int nb = blocks_for(nchunk, nact);
+  const dim3 grid(nb, nact);
+  if (k <= 4)
+    hipLaunchKernelGGL((k_combine_scale<R, 4>), grid, dim3(KT), 0, st, V, vstride, w, out, ld, n, k, kmax, slots, h2,
+                       nrm2, nchunk);
+  else if (k <= 8)
+    hipLaunchKernelGGL((k_combine_scale<R, 8>), grid, dim3(KT), 0, st, V, vstride, w, out, ld, n, k, kmax, slots, h2,
+                       nrm2, nchunk);
+  else
+    hipLaunchKernelGGL((k_combine_scale<R, 16>), grid, dim3(KT), 0, st, V, vstride, w, out, ld, n, k, kmax, slots,
+                       h2, nrm2, nchunk);
+  VH_LAUNCHED();
+}
+
+template <typename R>
 void VecOps<R>::norm2(const cplx<R>* x, int nact, const int* slots, double2* nrm2) {
   hipStream_t st = ctx->stream;
   const int nb = blocks_for(nchunk, nact);
@@ -343,71 +544,149 @@ template void convert_vectors<float, float>(Context*, const float2*, int64_t, fl
                                             const int*, const int*);
 
 // ---------------------------------------------------------------------------
-// Givens least squares for one system (host)
+// Least squares of the Arnoldi cycle (device, one thread per system)
 // ---------------------------------------------------------------------------
-using cd = std::complex<double>;
-
-void Givens::reset(int m, double beta) {
-  R.assign((size_t)m * m, 0.0);
-  H.assign((size_t)(m + 1) * m, 0.0);
-  g.assign(m + 1, 0.0);
-  c.assign(m, 0.0);
-  s.assign(m, 0.0);
-  g[0] = beta;
-  this->m = m;
-  k = 0;
-  beta0 = beta;
+__device__ __forceinline__ double2 zmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// conj(a) * b
+__device__ __forceinline__ double2 zcmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double2 zdiv(double2 a, double2 b) {
+  // Smith's algorithm (no overflow for |b| near the range limits)
+  if (fabs(b.x) >= fabs(b.y)) {
+    const double r = b.y / b.x, d = b.x + b.y * r;
+    return make_double2((a.x + a.y * r) / d, (a.y - a.x * r) / d);
+  }
+  const double r = b.x / b.y, d = b.x * r + b.y;
+  return make_double2((a.x * r + a.y) / d, (a.y * r - a.x) / d);
 }
 
-// append column k (h[0..k], h[k] = ||w||) -> returns |g_{k+1}|
-double Givens::push(const cd* h, double hnext) {
-  const int j = k;
-  for (int i = 0; i <= j; ++i) H[(size_t)i * m + j] = h[i];
-  H[(size_t)(j + 1) * m + j] = hnext;
-  std::vector<cd> col(j + 2);
-  for (int i = 0; i <= j; ++i) col[i] = h[i];
-  col[j + 1] = hnext;
+__global__ void k_ls_reset(LsDev L, const int* __restrict__ slots, int nact, const double2* __restrict__ nrm2) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nact) return;
+  const int s = slots[a];
+  const double beta = sqrt(nrm2[a].x);
+  L.beta[s] = beta;
+  double2* g = L.g + (size_t)s * (L.m + 1);
+  g[0] = make_double2(beta, 0.0);
+  for (int i = 1; i <= L.m; ++i) g[i] = make_double2(0.0, 0.0);
+}
+
+// Column j of H: h_ij (i <= j) from the Gram-Schmidt projections, h_{j+1,j} =
+// ||w||.  The earlier rotations are applied to it in order (each acts on
+// rows i, i+1), then the rotation that annihilates h_{j+1,j}:
+//   c = |a| / t, s = (a / |a|) conj(b) / t, r = (a / |a|) t,  t = hypot(|a|, |b|),
+// and g_{j+1} = -conj(s) g_j, g_j = c g_j (pgmres_iter.m:115-123 solves the
+// same least-squares problem with Hc\rhs).
+__global__ void k_ls_push(LsDev L, const int* __restrict__ slots, int nact, int j, const double2* __restrict__ h,
+                          const double2* __restrict__ h2, int kmax, const double2* __restrict__ nrm2,
+                          double* __restrict__ res) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nact) return;
+  const int s = slots[a], m = L.m;
+  double2* H = L.H + (size_t)s * (m + 1) * m;
+  double2* R = L.R + (size_t)s * m * m;
+  double2* g = L.g + (size_t)s * (m + 1);
+  double2* sn = L.s + (size_t)s * m;
+  double* cs = L.c + (size_t)s * m;
+  const double2* ha = h + (size_t)a * kmax;
+  const double2* h2a = h2 ? h2 + (size_t)a * kmax : nullptr;
+  auto col = [&](int i) {
+    double2 v = ha[i];
+    if (h2a) { v.x += h2a[i].x; v.y += h2a[i].y; }
+    return v;
+  };
+  const double hn = sqrt(nrm2[a].x);
+  double2 x = col(0);
+  H[j] = x;
   for (int i = 0; i < j; ++i) {
-    const cd t1 = col[i], t2 = col[i + 1];
-    col[i] = c[i] * t1 + s[i] * t2;
-    col[i + 1] = -std::conj(s[i]) * t1 + c[i] * t2;
+    const double2 y = col(i + 1);
+    H[(size_t)(i + 1) * m + j] = y;
+    const double c = cs[i];
+    const double2 sv = sn[i];
+    const double2 sy = zmul(sv, y), csx = zcmul(sv, x);
+    R[(size_t)i * m + j] = make_double2(c * x.x + sy.x, c * x.y + sy.y);
+    x = make_double2(c * y.x - csx.x, c * y.y - csx.y);
   }
-  const cd a = col[j], b = col[j + 1];
-  const double na = std::abs(a), nb = std::abs(b);
+  H[(size_t)(j + 1) * m + j] = make_double2(hn, 0.0);
+  const double na = hypot(x.x, x.y), nb = hn;
   double cc;
-  cd ss, rr;
-  if (nb == 0.0) { cc = 1.0; ss = 0.0; rr = a; }
-  else if (na == 0.0) { cc = 0.0; ss = std::conj(b) / nb; rr = nb; }
-  else {
-    const double t = std::hypot(na, nb);
+  double2 ss, rr;
+  if (nb == 0.0) {
+    cc = 1.0; ss = make_double2(0.0, 0.0); rr = x;
+  } else if (na == 0.0) {
+    cc = 0.0; ss = make_double2(1.0, 0.0); rr = make_double2(nb, 0.0);
+  } else {
+    const double t = hypot(na, nb);
+    const double2 u = make_double2(x.x / na, x.y / na);
     cc = na / t;
-    ss = (a / na) * std::conj(b) / t;
-    rr = (a / na) * t;
+    ss = make_double2(u.x * (nb / t), u.y * (nb / t));
+    rr = make_double2(u.x * t, u.y * t);
   }
-  c[j] = cc; s[j] = ss;
-  for (int i = 0; i < j; ++i) R[(size_t)i * m + j] = col[i];
+  cs[j] = cc;
+  sn[j] = ss;
   R[(size_t)j * m + j] = rr;
-  g[j + 1] = -std::conj(ss) * g[j];
-  g[j] = cc * g[j];
-  k = j + 1;
-  return std::abs(g[j + 1]);
+  const double2 gj = g[j];
+  const double2 cg = zcmul(ss, gj);
+  g[j + 1] = make_double2(-cg.x, -cg.y);
+  g[j] = make_double2(cc * gj.x, cc * gj.y);
+  res[a] = hypot(cg.x, cg.y);
 }
 
-void Givens::solve(std::vector<cd>& y) const {
-  y.assign(k, 0.0);
+// y = R^{-1} g (k unknowns) into coef[a]; rcoef[a][i] = beta e1 - H y (i <= k).
+__global__ void k_ls_solve(LsDev L, const int* __restrict__ slots, int nl, int k, int kmax, double2* __restrict__ coef,
+                           double2* __restrict__ rcoef) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nl) return;
+  const int s = slots[a], m = L.m;
+  const double2* H = L.H + (size_t)s * (m + 1) * m;
+  const double2* R = L.R + (size_t)s * m * m;
+  const double2* g = L.g + (size_t)s * (m + 1);
+  double2* y = coef + (size_t)a * kmax;
   for (int i = k - 1; i >= 0; --i) {
-    cd t = g[i];
-    for (int l = i + 1; l < k; ++l) t -= R[(size_t)i * m + l] * y[l];
-    y[i] = t / R[(size_t)i * m + i];
+    double2 t = g[i];
+    for (int l = i + 1; l < k; ++l) {
+      const double2 p = zmul(R[(size_t)i * m + l], y[l]);
+      t.x -= p.x; t.y -= p.y;
+    }
+    y[i] = zdiv(t, R[(size_t)i * m + i]);
+  }
+  if (!rcoef) return;
+  double2* r = rcoef + (size_t)a * kmax;
+  for (int i = 0; i <= k; ++i) {
+    double2 v = make_double2(i == 0 ? L.beta[s] : 0.0, 0.0);
+    for (int l = i > 0 ? i - 1 : 0; l < k; ++l) {   // H is upper Hessenberg: H(i, l) = 0 for l < i - 1
+      const double2 p = zmul(H[(size_t)i * m + l], y[l]);
+      v.x -= p.x; v.y -= p.y;
+    }
+    r[i] = v;
   }
 }
 
-// res = beta e1 - H y  (k+1 entries), the reference's rhs - Hc*y
-void Givens::residual(const std::vector<cd>& y, std::vector<cd>& res) const {
-  res.assign(k + 1, 0.0);
-  res[0] = beta0;
-  for (int i = 0; i <= k; ++i)
-    for (int l = 0; l < k; ++l) res[i] -= H[(size_t)i * m + l] * y[l];
+void ls_reset(Context* ctx, const LsDev& L, const int* slots, int nact, const double2* nrm2) {
+  if (nact <= 0) return;
+  hipLaunchKernelGGL(k_ls_reset, dim3(ceil_div(nact, 64)), dim3(64), 0, ctx->stream, L, slots, nact, nrm2);
+  VH_LAUNCHED();
+}
+
+void ls_push(Context* ctx, const LsDev& L, const int* slots, int nact, int j, const double2* h, const double2* h2,
+             int kmax, const double2* nrm2, double* res) {
+  if (nact <= 0) return;
+  VH_REQUIRE(j >= 0 && j < L.m && j < kmax, "internal: least-squares column outside the cycle");
+  hipLaunchKernelGGL(k_ls_push, dim3(ceil_div(nact, 64)), dim3(64), 0, ctx->stream, L, slots, nact, j, h, h2, kmax,
+                     nrm2, res);
+  VH_LAUNCHED();
+}
+
+void ls_solve(Context* ctx, const LsDev& L, const int* slots, int nl, int k, int kmax, double2* coef,
+              double2* rcoef) {
+  if (nl <= 0) return;
+  VH_REQUIRE(k >= 1 && k <= L.m && k + 1 <= kmax, "internal: least-squares solve outside the cycle");
+  hipLaunchKernelGGL(k_ls_solve, dim3(ceil_div(nl, 64)), dim3(64), 0, ctx->stream, L, slots, nl, k, kmax, coef,
+                     rcoef);
+  VH_LAUNCHED();
 }
 
 }  // namespace vh

@@ -25,6 +25,15 @@ template <typename R> struct VecOps {
   // need[a] = 1 where a second Gram-Schmidt pass is due: ||w'||^2 (nrm2) <
   // ||w||^2 / 2 (h[a][k], from mdot with_norm)
   void reorth_flags(const double2* h, int k, const double2* nrm2, int* need, int nact);
+  // Fused two-pass classical Gram-Schmidt (k <= kFusedMax): combine_dot does
+  // w <- w - V h1 in place and h2[a][j] = V_j^H w (j < k), h2[a][k] = ||w||^2
+  // in one pass over the basis; combine_scale writes out = (w - V h2) / nu
+  // with nu^2 = ||w||^2 - sum_j |h2_j|^2 and nrm2[a] = nu^2.
+  static constexpr int kFusedMax = 16;
+  void combine_dot(const cplx<R>* V, int64_t vstride, cplx<R>* w, int k, int nact, const int* slots,
+                   const double2* h1, double2* h2);
+  void combine_scale(const cplx<R>* V, int64_t vstride, const cplx<R>* w, cplx<R>* out, int k, int nact,
+                     const int* slots, const double2* h2, double2* nrm2);
   void norm2(const cplx<R>* x, int nact, const int* slots, double2* nrm2);
   void scale(const cplx<R>* src, cplx<R>* dst, int nact, const int* slots, const double2* nrm2);
   void sub_from(const cplx<R>* b, int64_t ldb, cplx<R>* y, int nact, const int* slots);
@@ -36,17 +45,28 @@ template <typename RD, typename RS>
 void convert_vectors(Context* ctx, const cplx<RS>* src, int64_t lds, cplx<RD>* dst, int64_t ldd, int64_t n, int nact,
                      const int* slots_src, const int* slots_dst);
 
-// Incremental Givens QR of the (k+1) x k Hessenberg matrix of one system.
-struct Givens {
-  int m = 0, k = 0;
-  double beta0 = 0;
-  std::vector<std::complex<double>> R, H, g, s;
-  std::vector<double> c;
-  void reset(int m, double beta);
-  double push(const std::complex<double>* h, double hnext);
-  void solve(std::vector<std::complex<double>>& y) const;
-  void residual(const std::vector<std::complex<double>>& y, std::vector<std::complex<double>>& res) const;
+// Least squares of an Arnoldi cycle on the device, one thread per system:
+// the (k+1) x k Hessenberg matrix H, its QR factor R by Givens rotations
+// (cosines c, sines s) and the rotated right-hand side g = Q^H beta e1 of
+// every slot stay in device memory across the steps of a cycle.
+struct LsDev {
+  double2* H;      // [slot][(m+1) * m], row i, column j at i*m + j
+  double2* R;      // [slot][m * m], upper triangle
+  double2* g;      // [slot][m + 1]
+  double2* s;      // [slot][m]
+  double* c;       // [slot][m]
+  double* beta;    // [slot]
+  int m;
 };
+// g = beta e1 with beta = sqrt(nrm2[a]) for the nact listed slots.
+void ls_reset(Context* ctx, const LsDev& L, const int* slots, int nact, const double2* nrm2);
+// Appends column j (h[a][i] (+ h2[a][i]) for i <= j, H(j+1, j) = sqrt(nrm2[a]))
+// and rotates it; res[a] = |g_{j+1}|, the least-squares residual norm.
+void ls_push(Context* ctx, const LsDev& L, const int* slots, int nact, int j, const double2* h, const double2* h2,
+             int kmax, const double2* nrm2, double* res);
+// coef[a][0..k) = y = R^{-1} g (back substitution); rcoef (optional)
+// [a][0..k] = beta e1 - H y, the reference's restart residual (rhs - Hc*y).
+void ls_solve(Context* ctx, const LsDev& L, const int* slots, int nl, int k, int kmax, double2* coef, double2* rcoef);
 
 // y_slot = Op(x_slot) for the systems listed in `act` (host) / `slots` (the
 // same list on the device).
@@ -86,13 +106,22 @@ template <typename R> struct KrylovWork {
   cplx<R>* w = nullptr;
   cplx<R>* t = nullptr;
   cplx<R>* r = nullptr;
-  DevBuf<double2> h, h2, nrm, coef;
+  DevBuf<double2> h, h2, nrm, coef, rcoef;
+  DevBuf<double2> lsH, lsR, lsg, lss;     // least-squares state per slot (LsDev)
+  DevBuf<double> lsc, lsbeta, lsres;
   DevBuf<double2> part;   // partials of the reductions (VecOps), sized for m + 1 vectors and nslot systems
   DevBuf<int> slots_dev, aux_slots;
   DevBuf<int> need;       // per active system: second Gram-Schmidt pass due (selective CGS2)
-  // pinned host staging: slot lists, LS coefficients, Hessenberg columns, norms
+  // pinned host staging: slot lists, least-squares residuals, norms
   PinnedBuf<int> h_slots, h_aux;
-  PinnedBuf<double2> h_coef, h_rcoef, h_h, h_h2, h_nrm, h_bn;
+  PinnedBuf<double> h_res;
+  PinnedBuf<double2> h_nrm, h_bn;
+  LsDev ls() { return LsDev{lsH.p, lsR.p, lsg.p, lss.p, lsc.p, lsbeta.p, m}; }
+  // device bytes of the per-slot small arrays (Hessenberg columns, least-squares state)
+  static size_t small_bytes_per_slot(int m) {
+    return sizeof(double2) * (size_t)(4 * (m + 1) + 1 + (size_t)(m + 1) * m + (size_t)m * m + (m + 1) + m) +
+           sizeof(double) * (size_t)(m + 2);
+  }
   static size_t bytes_needed(int64_t ld, int m, int nslot, bool flexible) {
     return sizeof(cplx<R>) * (size_t)ld * nslot * ((m + 1) + (flexible ? m : 0) + 3);
   }

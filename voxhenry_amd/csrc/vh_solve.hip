@@ -11,16 +11,16 @@
 //   relres = ||b - A x|| / ||b||;  flag = relres > tol
 // All systems advance together; one that meets the stopping rule gets its
 // update immediately and leaves the active set, so later steps only work on
-// the systems still running.  One host synchronisation per Arnoldi step brings
-// the new Hessenberg column of every active system to the host.
+// the systems still running.  The Hessenberg columns, the Givens QR and the
+// least-squares solutions stay on the device; one host synchronisation per
+// Arnoldi step brings back one residual norm per active system, from which
+// the host decides which systems retire.
 #include "vh_krylov.h"
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 
 namespace vh {
-
-using cd = std::complex<double>;
 
 template <typename R>
 void KrylovWork<R>::bind(char* base, int64_t n_, int64_t ld_, int m_, int nslot_, bool flexible_) {
@@ -38,6 +38,14 @@ void KrylovWork<R>::bind(char* base, int64_t n_, int64_t ld_, int m_, int nslot_
   h.alloc((size_t)nslot * (m + 1));
   h2.alloc((size_t)nslot * (m + 1));
   coef.alloc((size_t)nslot * (m + 1));
+  rcoef.alloc((size_t)nslot * (m + 1));
+  lsH.alloc((size_t)nslot * (m + 1) * m);
+  lsR.alloc((size_t)nslot * m * m);
+  lsg.alloc((size_t)nslot * (m + 1));
+  lss.alloc((size_t)nslot * m);
+  lsc.alloc((size_t)nslot * m);
+  lsbeta.alloc(nslot);
+  lsres.alloc(nslot);
   part.alloc(VecOps<R>::part_size(m + 1, nslot));
   nrm.alloc(nslot);
   slots_dev.alloc(nslot);
@@ -45,10 +53,7 @@ void KrylovWork<R>::bind(char* base, int64_t n_, int64_t ld_, int m_, int nslot_
   need.alloc(nslot);
   h_slots.alloc(nslot);
   h_aux.alloc(nslot);
-  h_coef.alloc((size_t)nslot * (m + 1));
-  h_rcoef.alloc((size_t)nslot * (m + 1));
-  h_h.alloc((size_t)nslot * (m + 1));
-  h_h2.alloc((size_t)nslot * (m + 1));
+  h_res.alloc(nslot);
   h_nrm.alloc(nslot);
   h_bn.alloc(nslot);
 }
@@ -118,21 +123,31 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
   auto tol_of = [&](int s) { return o.tol_slot ? (*o.tol_slot)[s] : o.tol; };
 
   // All host<->device transfers of the loop go through pinned staging buffers
-  // of the workspace; each staging buffer is rewritten only after the stream
-  // has been synchronised since its last copy.
+  // of the workspace; a staging buffer is rewritten only after the stream has
+  // been synchronised since its last copy (pending[] tracks the copies issued
+  // since the last synchronisation; a previous call on this workspace may
+  // have left one in flight).
   const double2* nbuf = W.h_nrm.p;
-  const double2* hbuf = W.h_h.p;
-  auto upload = [&](const std::vector<int>& v, int* dst) {
-    int* stage = dst == W.aux_slots.p ? W.h_aux.p : W.h_slots.p;
-    std::copy(v.begin(), v.end(), stage);
-    VH_HIP(hipMemcpyAsync(dst, stage, v.size() * sizeof(int), hipMemcpyHostToDevice, strm));
-  };
-  auto fetch_norms = [&](int cnt) {
-    VH_HIP(hipMemcpyAsync(W.h_nrm.p, W.nrm.p, cnt * sizeof(double2), hipMemcpyDeviceToHost, strm));
+  bool pending[2] = {true, true};
+  auto sync = [&]() {
     const auto t0 = clk::now();
     VH_HIP(hipStreamSynchronize(strm));
     t_wait += std::chrono::duration<double>(clk::now() - t0).count();
+    pending[0] = pending[1] = false;
   };
+  auto upload = [&](const std::vector<int>& v, int* dst) {
+    const int q = dst == W.aux_slots.p ? 1 : 0;
+    int* stage = q ? W.h_aux.p : W.h_slots.p;
+    if (pending[q]) sync();
+    std::copy(v.begin(), v.end(), stage);
+    VH_HIP(hipMemcpyAsync(dst, stage, v.size() * sizeof(int), hipMemcpyHostToDevice, strm));
+    pending[q] = true;
+  };
+  auto fetch_norms = [&](int cnt) {
+    VH_HIP(hipMemcpyAsync(W.h_nrm.p, W.nrm.p, cnt * sizeof(double2), hipMemcpyDeviceToHost, strm));
+    sync();
+  };
+  const LsDev ls = W.ls();
 
   std::vector<int> act = sys;
   upload(act, W.slots_dev.p);
@@ -157,7 +172,6 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
   ops.norm2(W.r, nsys, W.slots_dev.p, W.nrm.p);
   fetch_norms(nsys);
   std::vector<int> outit(nslot, 0);
-  std::vector<Givens> gv(nslot);
   {
     std::vector<int> keep;
     for (int a = 0; a < nsys; ++a) {
@@ -175,47 +189,30 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
     act.swap(keep);
   }
 
-  std::vector<cd> y, res;
-  // x += basis * y for the listed systems (k vectors); optionally r = V res (k+1 vectors)
+  // x += basis * y for the listed systems (k vectors); optionally r = V res
+  // (k+1 vectors), with y and res from the device least-squares state
   auto finish = [&](const std::vector<int>& L, int k, bool restart_r) {
     const int nl = (int)L.size();
     if (nl == 0) return;
-    double2* cbuf = W.h_coef.p;
-    double2* rbuf = W.h_rcoef.p;
-    std::fill(cbuf, cbuf + (size_t)nl * (m + 1), double2{0, 0});
-    std::fill(rbuf, rbuf + (size_t)nl * (m + 1), double2{0, 0});
-    for (int a = 0; a < nl; ++a) {
-      const int s = L[a];
-      gv[s].solve(y);
-      for (int j = 0; j < k; ++j) { cbuf[(size_t)a * (m + 1) + j].x = y[j].real(); cbuf[(size_t)a * (m + 1) + j].y = y[j].imag(); }
-      if (restart_r) {
-        gv[s].residual(y, res);
-        for (int j = 0; j <= k; ++j) { rbuf[(size_t)a * (m + 1) + j].x = res[j].real(); rbuf[(size_t)a * (m + 1) + j].y = res[j].imag(); }
-      }
+    for (int s : L) {
       st[s].inner = k;
       st[s].outer = outit[s];
     }
     upload(L, W.aux_slots.p);
-    VH_HIP(hipMemcpyAsync(W.coef.p, cbuf, (size_t)nl * (m + 1) * sizeof(double2), hipMemcpyHostToDevice, strm));
+    ls_solve(ctx, ls, W.aux_slots.p, nl, k, m + 1, W.coef.p, restart_r ? W.rcoef.p : nullptr);
     const C* basis = o.flexible ? W.Z : W.V;
     ops.combine(basis, vstr, x, k, nl, W.aux_slots.p, W.coef.p, 1.0, 1.0, nullptr);      // pgmres_iter.m:136-152
-    if (restart_r) {
-      // the device coefficient buffer is read by the kernel above on the same stream, so it can be reused
-      VH_HIP(hipMemcpyAsync(W.coef.p, rbuf, (size_t)nl * (m + 1) * sizeof(double2), hipMemcpyHostToDevice, strm));
-      ops.combine(W.V, vstr, W.r, k + 1, nl, W.aux_slots.p, W.coef.p, 0.0, 1.0, nullptr);  // :153
-    }
-    VH_HIP(hipStreamSynchronize(strm));   // host buffers above are reused
+    if (restart_r) ops.combine(W.V, vstr, W.r, k + 1, nl, W.aux_slots.p, W.rcoef.p, 0.0, 1.0, nullptr);  // :153
   };
 
   while (!act.empty()) {
     for (int s : act) outit[s]++;
     int nact = (int)act.size();
     upload(act, W.slots_dev.p);
-    // V(:,1) = r/||r||  (pgmres_iter.m:55)
+    // V(:,1) = r/||r||  (pgmres_iter.m:55); g = ||r|| e1
     ops.norm2(W.r, nact, W.slots_dev.p, W.nrm.p);
     ops.scale(W.r, W.V, nact, W.slots_dev.p, W.nrm.p);
-    fetch_norms(nact);
-    for (int a = 0; a < nact; ++a) gv[act[a]].reset(m, std::sqrt(nbuf[a].x));
+    ls_reset(ctx, ls, W.slots_dev.p, nact, W.nrm.p);
     int k = 0;
     while (k < m && !act.empty()) {
       ++k;
@@ -235,34 +232,39 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
         }
       }
       for (int s : act) { st[s].iters++; st[s].matvecs++; }
-      // classical Gram-Schmidt, one pass (pgmres_iter.m:107-111); with reorth a
-      // second pass re-projects w for the systems whose first pass cancelled
-      // more than half of ||w||^2 (the FP32 inner solve: one pass loses
-      // orthogonality once the residual nears sqrt(eps)); the others get a
-      // zero correction and keep their norm
-      ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, nullptr, o.reorth);
-      ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, 1.0, -1.0, W.nrm.p);
-      if (o.reorth) {
-        ops.reorth_flags(W.h.p, k, W.nrm.p, W.need.p, nact);
-        ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h2.p, W.need.p);
-        ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h2.p, 1.0, -1.0, W.nrm.p, W.need.p);
-        VH_HIP(hipMemcpyAsync(W.h_h2.p, W.h2.p, (size_t)nact * (m + 1) * sizeof(double2), hipMemcpyDeviceToHost,
-                              strm));
+      // classical Gram-Schmidt, one pass (pgmres_iter.m:107-111).  With reorth
+      // (the FP32 inner solve: one pass loses orthogonality once the residual
+      // nears sqrt(eps)) a second pass re-projects w.  Up to kFusedMax basis
+      // vectors the two passes take three sweeps over the basis (projection;
+      // update fused with the second projection; second update fused with
+      // the normalisation); beyond, the second pass is selective: it runs for
+      // the systems whose first pass cancelled more than half of ||w||^2, the
+      // others get a zero correction and keep their norm.
+      const bool fused = o.reorth && k <= VecOps<R>::kFusedMax;
+      if (fused) {
+        ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p);
+        ops.combine_dot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, W.h2.p);
+        ops.combine_scale(W.V, vstr, W.w, W.V + (int64_t)k * vstr, k, nact, W.slots_dev.p, W.h2.p, W.nrm.p);
+      } else {
+        ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, nullptr, o.reorth);
+        ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, 1.0, -1.0, W.nrm.p);
+        if (o.reorth) {
+          ops.reorth_flags(W.h.p, k, W.nrm.p, W.need.p, nact);
+          ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h2.p, W.need.p);
+          ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h2.p, 1.0, -1.0, W.nrm.p, W.need.p);
+        }
+        ops.scale(W.w, W.V + (int64_t)k * vstr, nact, W.slots_dev.p, W.nrm.p);
       }
-      ops.scale(W.w, W.V + (int64_t)k * vstr, nact, W.slots_dev.p, W.nrm.p);
-      VH_HIP(hipMemcpyAsync(W.h_h.p, W.h.p, (size_t)nact * (m + 1) * sizeof(double2), hipMemcpyDeviceToHost, strm));
-      fetch_norms(nact);
+      // column k of H and its Givens rotation (pgmres_iter.m:112-123)
+      ls_push(ctx, ls, W.slots_dev.p, nact, k - 1, W.h.p, o.reorth ? W.h2.p : nullptr, m + 1, W.nrm.p, W.lsres.p);
+      VH_HIP(hipMemcpyAsync(W.h_res.p, W.lsres.p, (size_t)nact * sizeof(double), hipMemcpyDeviceToHost, strm));
+      sync();
       ++nsteps;
       const auto th = clk::now();
       std::vector<int> conv, keep;
-      std::vector<cd> hc(k);
       for (int a = 0; a < nact; ++a) {
         const int s = act[a];
-        for (int j = 0; j < k; ++j) hc[j] = cd(hbuf[(size_t)a * (m + 1) + j].x, hbuf[(size_t)a * (m + 1) + j].y);
-        if (o.reorth)
-          for (int j = 0; j < k; ++j) hc[j] += cd(W.h_h2[(size_t)a * (m + 1) + j].x, W.h_h2[(size_t)a * (m + 1) + j].y);
-        const double hn = std::sqrt(nbuf[a].x);
-        const double rres = gv[s].push(hc.data(), hn);
+        const double rres = W.h_res[a];
         st[s].resvec.push_back(rres / bnorm[s]);
         if (!(rres >= tol_of(s) * bnorm[s])) conv.push_back(s);   // pgmres_iter.m:126 (NaN also stops)
         else keep.push_back(s);
@@ -283,7 +285,7 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
   }
 
   if (trace_on())
-    fprintf(stderr, "[vh] gmres<%s> nsys %d m %d: %d steps, alloc %.3f s, gpu wait %.3f s, host LS %.3f s, total %.3f s\n",
+    fprintf(stderr, "[vh] gmres<%s> nsys %d m %d: %d steps, alloc %.3f s, gpu wait %.3f s, host %.3f s, total %.3f s\n",
             sizeof(R) == 8 ? "f64" : "f32", nsys, m, nsteps, t_alloc, t_wait, t_host,
             std::chrono::duration<double>(clk::now() - t_start).count());
   // relres = ||b - A x|| / ||b||  (pgmres.m:181-186)
