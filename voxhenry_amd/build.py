@@ -30,7 +30,9 @@ def hipcc():
 
 
 def sources():
-    return sorted(f for f in os.listdir(CSRC) if f.endswith(('.hip', '.cpp')))
+    # the FFT-pass instantiation units are the slowest to compile: start them first
+    fs = [f for f in os.listdir(CSRC) if f.endswith(('.hip', '.cpp'))]
+    return sorted(fs, key=lambda f: (not f.startswith(('vh_matvec_xy', 'vh_matvec_z')), f))
 
 
 def compile_one(src):
