@@ -268,9 +268,17 @@ def method_summary(nsys, dt, st):
 
 
 RL_RTOL = 1e-6     # accuracy of R and L the mixed sweeps of the configuration lines refine to (sweep_batched)
+# The timed passes of a configuration line: label -> (solve_batch method, extra
+# sweep arguments).  'mixed' and 'fp64_true' both carry every system to its
+# true-residual tolerance (the solves/s ratio compares them); 'mixed_rl'
+# continues the systems whose R or omega L is a small part of |Z| until both
+# are resolved to RL_RTOL; 'fp64' is the reference's pgmres semantics (stops
+# on the preconditioned residual: its unconverged systems are not counted).
+CONFIG_PASSES = (('mixed', 'mixed', {}), ('mixed_rl', 'mixed', dict(rl_rtol=RL_RTOL)),
+                 ('fp64_true', 'fp64_true', {}), ('fp64', 'fp64', {}))
 
 
-def sweep_config(ctx, st, freqs, methods=('mixed', 'fp64'), reps=1, keep=None, tight=True, **solve_kw):
+def sweep_config(ctx, st, freqs, passes=CONFIG_PASSES, reps=1, keep=None, tight=True, **solve_kw):
     """Batched (port x frequency) solve of one geometry, device-resident:
     set-up (tensor, topology, preconditioners) untimed, then one warm-up and
     `reps` timed sweeps per method (the reference driver's tolerance per
@@ -289,26 +297,27 @@ def sweep_config(ctx, st, freqs, methods=('mixed', 'fp64'), reps=1, keep=None, t
                grid=list(st['sigma'].shape))
     nsys = len(freqs) * S.nports
     res = {}
-    for m in methods:
-        kw = dict(solve_kw, rl_rtol=RL_RTOL) if m == 'mixed' else solve_kw
+    for label, m, extra in passes:
+        kw = dict(solve_kw, **extra)
         S.sweep_batched(freqs, method=m, precons=precs, pidx_of_freq=pidx, device=True, **kw)
+        torch.cuda.synchronize()
         t = time.perf_counter()
         for _ in range(reps):
             r = S.sweep_batched(freqs, method=m, precons=precs, pidx_of_freq=pidx, device=True, **kw)
         torch.cuda.synchronize()
-        res[m] = r
-        out[m] = method_summary(nsys, (time.perf_counter() - t) / reps, r['stats'])
-        out[m]['refined'] = int(r['stats'].get('refined', 0))
+        res[label] = r
+        out[label] = method_summary(nsys, (time.perf_counter() - t) / reps, r['stats'])
+        out[label]['refined'] = int(r['stats'].get('refined', 0))
     if tight:
-        ref = S.sweep_batched(freqs, method='fp64', tol=TIGHT, precons=precs, pidx_of_freq=pidx, device=True,
+        ref = S.sweep_batched(freqs, method='fp64_true', tol=TIGHT, precons=precs, pidx_of_freq=pidx, device=True,
                               maxit=20, **solve_kw)
         out['ref_relres_max'] = float(ref['stats']['relres'].max())
-        for m in methods:
-            out[m]['err_vs_tight_fp64'] = rl_errors(res[m]['Z'], ref['Z'], freqs)
+        for label in res:
+            out[label]['err_vs_tight_fp64'] = rl_errors(res[label]['Z'], ref['Z'], freqs)
         out['R_ohm'] = [float(v) for v in ref['R'][:, 0, 0]]
         out['L_H'] = [float(v) for v in ref['L'][:, 0, 0]]
-    if 'mixed' in out and 'fp64' in out:
-        out['speedup_mixed_over_fp64'] = out['mixed']['solves_per_s'] / max(out['fp64']['solves_per_s'], 1e-300)
+    if 'mixed' in out and 'fp64_true' in out:
+        out['speedup_mixed_over_fp64'] = out['mixed']['solves_per_s'] / max(out['fp64_true']['solves_per_s'], 1e-300)
     if keep is not None:
         keep.append((S, precs, pidx))
         return out
@@ -320,14 +329,18 @@ def sweep_config(ctx, st, freqs, methods=('mixed', 'fp64'), reps=1, keep=None, t
 
 
 def compact_config(c):
-    """The few figures of a configuration line that go into the stdout line."""
-    o = dict(mixed=r3(c['mixed']['solves_per_s']), fp64=r3(c['fp64']['solves_per_s']),
-             ratio=r3(c.get('speedup_mixed_over_fp64')), mixed_refined=c['mixed'].get('refined', 0),
-             converged='%d/%d mixed, %d/%d fp64' % (c['mixed']['converged'], c['mixed']['systems'],
-                                                    c['fp64']['converged'], c['fp64']['systems']))
-    if 'err_vs_tight_fp64' in c['mixed']:
-        em, ef = c['mixed']['err_vs_tight_fp64'], c['fp64']['err_vs_tight_fp64']
-        o['RL_err_vs_fp64_1e-12'] = dict(mixed=[r3(em['R']), r3(em['L'])], fp64=[r3(ef['R']), r3(ef['L'])])
+    """The few figures of a configuration line that go into the stdout line:
+    solves/s of each pass, the mixed / fp64_true ratio, converged counts and
+    the [R, L] errors against the FP64 sweep to 1e-12."""
+    labels = [k for k, _, _ in CONFIG_PASSES if k in c]
+    o = {k: r3(c[k]['solves_per_s']) for k in labels}
+    o['ratio_mixed_over_fp64_true'] = r3(c.get('speedup_mixed_over_fp64'))
+    o['converged'] = ', '.join('%s %d/%d' % (k, c[k]['converged'], c[k]['systems']) for k in labels)
+    if 'mixed_rl' in c:
+        o['mixed_rl_refined'] = c['mixed_rl'].get('refined', 0)
+    if all('err_vs_tight_fp64' in c[k] for k in labels):
+        o['RL_err_vs_fp64_1e-12'] = {k: [r3(c[k]['err_vs_tight_fp64']['R']), r3(c[k]['err_vs_tight_fp64']['L'])]
+                                     for k in labels}
     return o
 
 
@@ -354,7 +367,8 @@ def uneven_config(count, nfreq, seed=0, workers=8, budget_frac=0.7):
     free0 = torch.cuda.mem_get_info()[0]
     budget = budget_frac * free0
     cap = 0.5 * budget                 # one geometry's workspaces: at most half the budget (it chunks beyond)
-    tot = dict(mixed=dict(wall=0.0, conv=0), fp64=dict(wall=0.0, conv=0))
+    labels = [k for k, _, _ in CONFIG_PASSES]
+    tot = {k: dict(wall=0.0, conv=0) for k in labels}
     nsys_all = 0
     worst = {}
     kinds = {}
@@ -394,37 +408,36 @@ def uneven_config(count, nfreq, seed=0, workers=8, budget_frac=0.7):
         groups.append(len(jobs))
 
         def pool(method, **kw):
-            if method == 'mixed':
-                kw.setdefault('rl_rtol', RL_RTOL)
             torch.cuda.synchronize()
             outs, wall = vsol.sweep_concurrent([(j['S'], freqs, j['precs'], pidx, j['stream'], j['ws']) for j in jobs],
                                                method=method, workers=workers, **kw)
             torch.cuda.synchronize()
             return outs, wall
 
-        pool('mixed')                                   # warm-up
-        refs, _ = pool('fp64', tol=TIGHT, maxit=20)     # the R/L reference (and the FP64 warm-up)
+        pool('mixed')                                        # warm-up
+        refs, _ = pool('fp64_true', tol=TIGHT, maxit=20)     # the R/L reference (and the FP64 warm-up)
         res = {}
-        for m in ('mixed', 'fp64'):
-            outs, wall = pool(m)
-            res[m] = outs
+        for label, m, extra in CONFIG_PASSES:
+            outs, wall = pool(m, **extra)
+            res[label] = outs
             conv = sum(int(np.sum(o['stats']['flag'] == 0)) for o in outs)
-            tot[m]['wall'] += wall
-            tot[m]['conv'] += conv
-            tot[m]['refined'] = tot[m].get('refined', 0) + sum(int(o['stats'].get('refined', 0)) for o in outs)
-            tot[m]['rl_unresolved'] = tot[m].get('rl_unresolved', 0) + sum(int(o['stats'].get('rl_unresolved', 0))
-                                                                            for o in outs)
+            tot[label]['wall'] += wall
+            tot[label]['conv'] += conv
+            tot[label]['refined'] = tot[label].get('refined', 0) + sum(int(o['stats'].get('refined', 0))
+                                                                        for o in outs)
+            tot[label]['rl_unresolved'] = tot[label].get('rl_unresolved', 0) + \
+                sum(int(o['stats'].get('rl_unresolved', 0)) for o in outs)
             log('[bench]   uneven group of %d, %s: %d/%d converged in %.3f s'
-                % (len(jobs), m, conv, sum(j['nsys'] for j in jobs), wall))
+                % (len(jobs), label, conv, sum(j['nsys'] for j in jobs), wall))
         for k, j in enumerate(jobs):
             ref = refs[k]
             nsys_all += j['nsys']
-            kd = kinds.setdefault(j['kind'], dict(geometries=0, systems=0, mixed_conv=0, fp64_conv=0))
+            kd = kinds.setdefault(j['kind'], dict(geometries=0, systems=0, **{k + '_conv': 0 for k in labels}))
             kd['geometries'] += 1
             kd['systems'] += j['nsys']
             rec = dict(kind=j['kind'], grid=list(geos[j['i']][1]['sigma'].shape), ports=j['S'].nports,
                        nv=int(len(j['S'].idx)), ref_relres_max=float(ref['stats']['relres'].max()))
-            for m in ('mixed', 'fp64'):
+            for m in labels:
                 o = res[m][k]
                 c = int(np.sum(o['stats']['flag'] == 0))
                 kd[m + '_conv'] += c
@@ -444,15 +457,14 @@ def uneven_config(count, nfreq, seed=0, workers=8, budget_frac=0.7):
         del jobs
         torch.cuda.empty_cache()
     sps = {m: tot[m]['conv'] / tot[m]['wall'] for m in tot}
-    return dict(seed=seed, geometries=count, nfreq=nfreq, systems=nsys_all, workers=workers, groups=groups,
-                budget_GB=budget / 1e9, setup_s=setup_s, wall_s=time.perf_counter() - t_all,
-                mixed=dict(solves_per_s=sps['mixed'], s=tot['mixed']['wall'], converged=tot['mixed']['conv'],
-                           systems=nsys_all, refined=tot['mixed'].get('refined', 0),
-                           rl_unresolved=tot['mixed'].get('rl_unresolved', 0)),
-                fp64=dict(solves_per_s=sps['fp64'], s=tot['fp64']['wall'], converged=tot['fp64']['conv'],
-                          systems=nsys_all),
-                speedup_mixed_over_fp64=sps['mixed'] / max(sps['fp64'], 1e-300),
-                by_kind=kinds, err_vs_tight_fp64_max=worst, per_geometry=per)
+    out = dict(seed=seed, geometries=count, nfreq=nfreq, systems=nsys_all, workers=workers, groups=groups,
+               budget_GB=budget / 1e9, setup_s=setup_s, wall_s=time.perf_counter() - t_all,
+               speedup_mixed_over_fp64=sps['mixed'] / max(sps['fp64_true'], 1e-300),
+               by_kind=kinds, err_vs_tight_fp64_max=worst, per_geometry=per)
+    for m in labels:
+        out[m] = dict(solves_per_s=sps[m], s=tot[m]['wall'], converged=tot[m]['conv'], systems=nsys_all,
+                      refined=tot[m].get('refined', 0), rl_unresolved=tot[m].get('rl_unresolved', 0))
+    return out
 
 
 def timed_steps(fn, steps, warmup):
@@ -626,14 +638,14 @@ def main():
     mixed = method_summary(nsol, s_mixed, stm)
     fp64 = None
     if not args.no_fp64:
-        s64, (Z64, st64) = timed_steps(lambda: bus_step(S, freqs, precs, 'fp64', data), max(1, args.steps // 2),
+        s64, (Z64, st64) = timed_steps(lambda: bus_step(S, freqs, precs, 'fp64_true', data), max(1, args.steps // 2),
                                        max(1, args.warmup))
         log('[bench] fp64 step %.3f s' % s64)
         fp64 = method_summary(nsol, s64, st64)
         # the reference-style stopping residual ||M r|| / ||M b|| (pgmres.m:126)
         fp64['precond_resid_max'] = float(st64['resid_est'].max())
     log('[bench] bus FP64 reference solve to %g' % TIGHT)
-    Zt, stt = bus_step(S, freqs, precs, 'fp64', bus_rhs(S, freqs, TIGHT), maxit=20)
+    Zt, stt = bus_step(S, freqs, precs, 'fp64_true', bus_rhs(S, freqs, TIGHT), maxit=20)
     ref_relres = float(stt['relres'].max())
     mixed['err_vs_tight_fp64'] = rl_errors(Zm, Zt, freqs)
     if fp64:

@@ -231,10 +231,22 @@ int vh_solve_mixed(vh_geom* g, vh_precon* p, double jwmu_re, double jwmu_im, int
  * nearby one: any preconditioner gives the same solution).
  * method 0: FP64 left-preconditioned GMRES with the pgmres semantics
  *           (restart, maxit, tol) for every system;
+ * method 2: method 0, then every system it leaves with a true residual above
+ *           its tolerance (pgmres stops on the preconditioned residual)
+ *           continues from its solution in FP64 flexible GMRES with the FP64
+ *           preconditioner (restart 50, up to 40 cycles), which
+ *           stops on the true residual: the plain FP64 path carried to the
+ *           same true-residual tolerance as method 1;
  * method 1: mixed precision, FP64 flexible GMRES (outer_restart, maxit) whose
  *           preconditioner is an FP32 flexible GMRES (inner_restart,
  *           inner_tol) on the FP32 matvec, preconditioned by the FP32
- *           factors applied with FP32 work vectors.
+ *           factors applied with FP32 work vectors.  A system whose inner
+ *           solves end a restart cycle above max(1e-2, 100x their tolerance)
+ *           (its FP32 Krylov process stalls, e.g. on a superconductor's
+ *           operator), and any system the outer iteration leaves above
+ *           tolerance, continues from its solution in FP64 flexible GMRES
+ *           with the FP64 preconditioner (restart 50 within the budget, up
+ *           to 40 cycles).
  * All systems run in lock step and converged ones leave the batch; if the
  * Krylov workspaces exceed hbm_budget bytes (0: 80% of free memory) the batch
  * is processed in chunks.  tols (nullable, host) gives system s its own

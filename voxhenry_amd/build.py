@@ -35,8 +35,21 @@ def sources():
     return sorted(fs, key=lambda f: (not f.startswith(('vh_matvec_xy', 'vh_matvec_z')), f))
 
 
+def _up_to_date(obj, src):
+    """VH_INCREMENTAL=1 (development only): skip a source whose object is newer
+    than it and than every header of csrc/ and include/."""
+    if os.environ.get('VH_INCREMENTAL') != '1' or not os.path.exists(obj):
+        return False
+    inc = os.path.join(HERE, '..', 'include')
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith('.h')] + \
+        [os.path.join(inc, f) for f in os.listdir(inc)]
+    return os.path.getmtime(obj) > max(os.path.getmtime(d) for d in deps)
+
+
 def compile_one(src):
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + '.o')
+    if _up_to_date(obj, src):
+        return obj
     cmd = [hipcc(), '-c'] + COMMON + [os.path.join(CSRC, src), '-o', obj]
     if src.endswith('.cpp'):
         cmd = [hipcc(), '-c', '-x', 'hip'] + COMMON + [os.path.join(CSRC, src), '-o', obj]

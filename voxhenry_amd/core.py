@@ -11,6 +11,8 @@ import numpy as np
 from ._lib import lib, check, ptr, VH_C128, VH_C64, VH_PC_SCHUR_INVERT, VH_PC_SCHUR_GMRES
 
 MU0 = 4 * np.pi * 1e-7        # EMconstants.m:13
+# solve_batch methods (the method argument of vh_solve_batch)
+METHODS = {'fp64': 0, 'mixed': 1, 'fp64_true': 2}
 
 
 def _torch():
@@ -275,16 +277,19 @@ class Geometry:
         B: (nsys, n) rhs, a numpy array (host) or a complex128 torch tensor on
         the device (then X is a device tensor too); jwmu: (nsys,) complex;
         precons: list of Precon; pidx: (nsys,) index into precons.  method
-        'fp64' (pgmres semantics, default restart 100 / maxit 10) or 'mixed'
-        (default outer restart 8 / maxit 4).  tol: one tolerance, or one per
+        'fp64' (pgmres semantics, default restart 100 / maxit 10), 'fp64_true'
+        (the same, then the systems it leaves above tol -- it stops on the
+        preconditioned residual -- continue in FP64 flexible GMRES until the
+        true residual meets tol) or 'mixed' (default outer restart 8 /
+        maxit 4).  tol: one tolerance, or one per
         system (e.g. reference_tol of each system's frequency).  x0 (mixed
         only): initial guesses of the shape of B; the outer FGMRES starts
         from B - A x0 (default: zero, as pgmres.m)."""
-        m = 0 if method == 'fp64' else 1
+        m = METHODS[method]
         if restart is None:
-            restart = 100 if m == 0 else 8
+            restart = 8 if m == 1 else 100
         if maxit is None:
-            maxit = 10 if m == 0 else 4
+            maxit = 4 if m == 1 else 10
         on_dev = not isinstance(B, np.ndarray)
         if on_dev:
             torch = _torch()
@@ -325,9 +330,9 @@ class Geometry:
     def batch_bytes(self, method='mixed', restart=None, inner_restart=30):
         """Device bytes per system that solve_batch budgets on this geometry
         (Krylov workspace + the system's share of the matvec workspaces)."""
-        m = 0 if method == 'fp64' else 1
+        m = METHODS[method]
         if restart is None:
-            restart = 100 if m == 0 else 8
+            restart = 8 if m == 1 else 100
         b = np.zeros(1)
         check(lib().vh_solve_batch_bytes(self.h, m, int(restart), int(inner_restart), ptr(b)))
         return float(b[0])

@@ -31,7 +31,8 @@ vh::KrylovCache& krylov_cache(vh::Geometry& G) {
 // Px*Py*N per component, sized for complex128).
 double batch_bytes_per_system(const vh::Geometry& G, int method, int restart, int inner_restart) {
   const bool mixed = method == 1;
-  // mixed: the FP32 inner solve is flexible (its basis Z is stored too)
+  // mixed: the FP32 inner solve is flexible (its basis Z is stored too);
+  // method 2 continues in a flexible FP64 solve laid out in the same bytes
   double b = (double)vh::KrylovCache::bytes(G.n(), 1, restart, mixed, mixed ? inner_restart : 0, mixed, mixed);
   // the per-system least-squares state and reduction columns of each Krylov workspace
   b += (double)vh::KrylovWork<double>::small_bytes_per_slot(restart);
@@ -747,7 +748,7 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
     auto& G = g->G;
     VH_REQUIRE(G.has_tensor && G.has_inc, "geometry not complete (tensor, incidence)");
     VH_REQUIRE(nprecon >= 1 && nsys >= 1 && nsys <= 13107, "nprecon / nsys out of range");
-    VH_REQUIRE(method == 0 || method == 1, "method must be 0 (fp64) or 1 (mixed)");
+    VH_REQUIRE(method >= 0 && method <= 2, "method must be 0 (fp64), 1 (mixed) or 2 (fp64 to the true residual)");
     VH_REQUIRE(!x_init || method == 1, "an initial guess (x_init) needs method 1 (the pgmres semantics start from 0)");
     VH_REQUIRE(restart >= 1 && restart <= 127 && inner_restart >= 1 && inner_restart <= 127 && maxit >= 1,
                "bad restart / maxit");
@@ -843,67 +844,74 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
       if (tols) tolc.assign(tols + s0, tols + s0 + cnt);
       const double2* bc = db + s0 * n;
       double2* xc = dx + s0 * n;
-      if (method == 0) {
+      if (method != 1) {
         vh::SolveOpts o;
         o.restart = restart; o.maxit = maxit; o.tol = tol; o.final_residual = true; o.flexible = false;
         if (tols) o.tol_slot = &tolc;
+        if (method == 2) K.layout(n, chunk, restart, false, 0, false, false);
         vh::gmres_batch<double>(G.ctx, n, A64, &M64, bc, xc, sys, o, W64, cs);
+        std::vector<int> again;
+        for (int i = 0; i < cnt && method == 2; ++i)
+          if (cs[i].flag != 0) again.push_back(i);
+        if (!again.empty()) {
+          // method 2: the systems the pgmres semantics left above their
+          // tolerance (it stops on the preconditioned residual) continue from
+          // their solutions in FP64 flexible GMRES with the FP64
+          // preconditioner, which stops on the true residual (restart 50, its
+          // workspace in the arena grown within the budget)
+          int rc = 50;
+          while (rc > 8 && (double)vh::KrylovCache::bytes(n, chunk, rc, true, 0, false, false) > budget) rc = rc * 3 / 4;
+          K.layout(n, chunk, rc, true, 0, false, false);
+          vh::SolveOpts o3;
+          o3.restart = rc; o3.maxit = 40; o3.tol = tol; o3.final_residual = true; o3.flexible = true;
+          o3.x0 = true;
+          if (tols) o3.tol_slot = &tolc;
+          std::vector<vh::SysStats> cs3;
+          vh::gmres_batch<double>(G.ctx, n, A64, &M64, bc, xc, again, o3, K.w64, cs3);
+          for (int i : again) {
+            cs3[i].iters += cs[i].iters;
+            cs3[i].matvecs += cs[i].matvecs;
+            cs[i] = cs3[i];
+          }
+          if (vh::trace_on()) fprintf(stderr, "[vh] solve_batch: %zu systems continued to the true residual\n",
+                                      again.size());
+        }
       } else {
+        if (s0 > 0) K.layout(n, chunk, restart, true, inner_restart, true, true);   // after a continuation
         vh::MixedOpts o;
         o.tol = tol; o.outer_restart = restart; o.outer_maxit = maxit;
         o.inner_restart = inner_restart; o.inner_tol = inner_tol;
         o.x0 = x_init != 0;
         o.inner_flexible = true;
+        o.escalate_inner = inner_restart;
         if (tols) o.tol_slot = &tolc;
         vh::fgmres_mixed(G.ctx, n, A64, A32, M32, bc, xc, sys, o, K, cs);
-        // Systems the outer iteration left above their tolerance continue
-        // from their solutions with the inner solve of a fixed linear FP32
-        // preconditioner (FP32 factors on FP64 work vectors, left
-        // preconditioning): rounding the work vectors to FP32 at every level
-        // can leave the inner solves of the hardest systems (a
-        // superconductor's low-frequency operator) too inaccurate for the
-        // outer iteration to reach the tolerance.
-        std::vector<int> again;
+        // Systems left above their tolerance -- those whose first inner
+        // restart cycle missed its tolerance (the FP32 Krylov process
+        // converges too slowly on them, e.g. a superconductor's operator)
+        // and any the outer iteration did not finish -- continue from their
+        // solutions in FP64 flexible GMRES with the FP64 preconditioner.  Its
+        // workspace takes the arena (grown within the budget to restart 50).
+        std::vector<int> last;
         for (int i = 0; i < cnt; ++i)
-          if (cs[i].flag != 0) again.push_back(i);
-        if (!again.empty()) {
-          vh::OpFn<float> M32f = [&](const std::vector<int>& act, const int* sl, const float2* xx, float2* yy) {
-            vh::precon_apply_cols<float>(G, *sym, tabs32.p, pidx + coff, act, xx, n, yy, n);
-          };
-          vh::MixedOpts o2 = o;
-          o2.x0 = true;
-          o2.inner_flexible = false;
-          std::vector<vh::SysStats> cs2;
-          vh::fgmres_mixed(G.ctx, n, A64, A32, M32f, bc, xc, again, o2, K, cs2);
-          for (int i : again) {
-            cs2[i].iters += cs[i].iters;
-            cs2[i].inner_iters += cs[i].inner_iters;
-            cs2[i].matvecs += cs[i].matvecs;
-            cs[i] = cs2[i];
+          if (cs[i].flag != 0) last.push_back(i);
+        if (!last.empty()) {
+          int rc = 50;
+          while (rc > 8 && (double)vh::KrylovCache::bytes(n, chunk, rc, true, 0, false, false) > budget) rc = rc * 3 / 4;
+          K.layout(n, chunk, rc, true, 0, false, false);
+          vh::SolveOpts o3;
+          o3.restart = rc; o3.maxit = 40; o3.tol = tol; o3.final_residual = true; o3.flexible = true;
+          o3.x0 = true;
+          if (tols) o3.tol_slot = &tolc;
+          std::vector<vh::SysStats> cs3;
+          vh::gmres_batch<double>(G.ctx, n, A64, &M64, bc, xc, last, o3, K.w64, cs3);
+          for (int i : last) {
+            cs3[i].iters += cs[i].iters;
+            cs3[i].inner_iters = cs[i].inner_iters;
+            cs3[i].matvecs += cs[i].matvecs;
+            cs[i] = cs3[i];
           }
-          if (vh::trace_on()) fprintf(stderr, "[vh] solve_batch: %zu systems continued with the FP64-work-vector inner solve\n",
-                                      again.size());
-          // last resort for what the FP32 inner solves cannot carry to the
-          // tolerance: FP64 flexible GMRES with the FP64 preconditioner from
-          // the current solutions (outer workspace, at most 20 cycles)
-          std::vector<int> last;
-          for (int i : again)
-            if (cs[i].flag != 0) last.push_back(i);
-          if (!last.empty()) {
-            vh::SolveOpts o3;
-            o3.restart = W64.m; o3.maxit = 20; o3.tol = tol; o3.final_residual = true; o3.flexible = true;
-            o3.x0 = true;
-            if (tols) o3.tol_slot = &tolc;
-            std::vector<vh::SysStats> cs3;
-            vh::gmres_batch<double>(G.ctx, n, A64, &M64, bc, xc, last, o3, W64, cs3);
-            for (int i : last) {
-              cs3[i].iters += cs[i].iters;
-              cs3[i].inner_iters = cs[i].inner_iters;
-              cs3[i].matvecs += cs[i].matvecs;
-              cs[i] = cs3[i];
-            }
-            if (vh::trace_on()) fprintf(stderr, "[vh] solve_batch: %zu systems finished in FP64\n", last.size());
-          }
+          if (vh::trace_on()) fprintf(stderr, "[vh] solve_batch: %zu systems finished in FP64 (restart %d)\n", last.size(), rc);
         }
       }
       for (int i = 0; i < cnt; ++i) stats[s0 + i] = cs[i];
@@ -923,7 +931,7 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
 int vh_solve_batch_bytes(vh_geom* g, int method, int restart, int inner_restart, double* bytes_per_system) {
   return guard([&] {
     need(g, "g"); need(bytes_per_system, "bytes_per_system");
-    VH_REQUIRE(method == 0 || method == 1, "method must be 0 (fp64) or 1 (mixed)");
+    VH_REQUIRE(method >= 0 && method <= 2, "method must be 0 (fp64), 1 (mixed) or 2 (fp64 to the true residual)");
     VH_REQUIRE(restart >= 1 && restart <= 127 && inner_restart >= 1 && inner_restart <= 127, "bad restart");
     *bytes_per_system = batch_bytes_per_system(g->G, method, restart, inner_restart);
   });

@@ -92,6 +92,10 @@ struct SolveOpts {
                                   // more than half of ||w||^2 (selective CGS2)
   const std::vector<double>* tol_slot = nullptr;   // per-slot tolerance (overrides tol)
   bool x0 = false;                // x holds an initial guess (flexible or unpreconditioned solves only)
+  // per-slot stop request (nullable): a system whose entry is set when a step
+  // ends takes the update of the steps done so far and leaves the iteration
+  // (its flag then follows from its residual)
+  const std::vector<char>* stop_slot = nullptr;
 };
 
 // Krylov workspace for nslot systems of length ld with restart m.  The large
@@ -147,6 +151,12 @@ struct MixedOpts {
   const std::vector<double>* tol_slot = nullptr;   // per-slot outer tolerance (overrides tol)
   bool x0 = false;                  // x holds an initial guess (the outer FGMRES starts from b - A x)
   bool inner_flexible = false;      // inner solve right-preconditioned (fpgmres semantics): any M32 works
+  // a system whose inner solves have taken escalate_inner FP32 steps in all
+  // and whose last inner solve ended above max(escalate_resid, 100x its
+  // tolerance) leaves the mixed iteration after that outer step (0: never),
+  // for the caller to finish in FP64: its FP32 Krylov process stalls
+  int escalate_inner = 0;
+  double escalate_resid = 1e-2;
 };
 // Workspaces of the solvers, kept by a geometry across calls.  All large
 // vectors live in one device arena that only grows: releasing and allocating

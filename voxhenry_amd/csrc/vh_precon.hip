@@ -42,69 +42,84 @@ struct PcGroupDev {
   const int* gslot;   // ng * NPG
 };
 
+// Work-vector lines of one node: the NPG ports of a node are contiguous, so
+// a line moves as 16-byte vector accesses.
+template <typename WT, int NPG> __device__ __forceinline__ void line_store(cplx<WT>* p, const cplx<WT> (&v)[NPG]) {
+#pragma unroll
+  for (int j = 0; j < NPG; ++j) p[j] = v[j];
+}
+
 // w[g][iperm[v]][j] = b[v] - sum_k Ar(v,k) * Ainv[k] * a[k] of column (g, j).
-// A workgroup covers NN = 256/NPG consecutive nodes v of group g.  The sums
-// are formed with the node fastest (each system's right-hand side and
-// currents are read in contiguous runs: consecutive nodes touch consecutive
-// currents) and transposed through shared memory, so that the ports of a
-// node write their work-vector line together.
+// One thread per (node v, group g = blockIdx.y) serves all NPG ports: the
+// row of Ar and the Ainv factors are loaded once, then every port's currents
+// (a wave reads 64 consecutive nodes' currents of one port, which lie in a
+// few contiguous runs) are in flight together, and the node's line of the
+// work vector is written whole.
 template <typename R, typename WT, int NPG>
 __global__ __launch_bounds__(256) void k_pc_gather(const cplx<R>* __restrict__ x, int64_t ldx, cplx<WT>* __restrict__ w,
                                                    int nn, const int* __restrict__ iperm, const int* __restrict__ arp,
                                                    const int* __restrict__ arc, const R* __restrict__ arv,
                                                    const PcTab<R>* __restrict__ tab, PcGroupDev gr, int64_t nc) {
-  constexpr int NN = 256 / NPG;
-  __shared__ cplx<WT> tile[NN * NPG];
-  const int tiles = (nn + NN - 1) / NN;
-  const int g = blockIdx.x / tiles, v0 = (blockIdx.x - g * tiles) * NN;
-  {
-    const int nl = threadIdx.x % NN, j = threadIdx.x / NN;
-    const int node = v0 + nl;
-    const int slot = gr.gslot[g * NPG + j];
-    cplx<WT> o; o.x = 0; o.y = 0;
-    if (node < nn && slot >= 0) {
-      const R* ainv = tab[gr.gtab[g]].ainv;
-      const cplx<R>* xr = x + (int64_t)slot * ldx;
-      const cplx<R> b = xr[nc + node];
-      double sx = b.x, sy = b.y;
-      // a node touches at most 6 currents: the entries of a short row are
-      // loaded all at once (independent loads in flight), a longer row loops
-      const int p0 = arp[node], p1 = arp[node + 1];
-      constexpr int D = 8;
-      if (p1 - p0 <= D) {
-        int kk[D];
-        R vv[D];
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.y;
+  if (v >= nn) return;
+  const R* __restrict__ ainv = tab[gr.gtab[g]].ainv;
+  const int p0 = arp[v], p1 = arp[v + 1];
+  // a node touches at most 6 currents: a short row is loaded all at once
+  constexpr int D = 6;
+  int kk[D];
+  double cf[D];
+  const bool short_row = p1 - p0 <= D;
+  if (short_row) {
 #pragma unroll
-        for (int u = 0; u < D; ++u) {
-          kk[u] = p0 + u < p1 ? arc[p0 + u] : 0;
-          vv[u] = p0 + u < p1 ? arv[p0 + u] : R(0);
-        }
+    for (int u = 0; u < D; ++u) kk[u] = p0 + u < p1 ? arc[p0 + u] : 0;
 #pragma unroll
-        for (int u = 0; u < D; ++u) {
-          if (p0 + u < p1) {
-            const double cf = (double)vv[u] * (double)ainv[kk[u]];
-            const cplx<R> a = xr[kk[u]];
-            sx -= cf * a.x; sy -= cf * a.y;
-          }
-        }
-      } else {
-        for (int p = p0; p < p1; ++p) {
-          const int k = arc[p];
-          const double cf = (double)arv[p] * (double)ainv[k];
-          const cplx<R> a = xr[k];
-          sx -= cf * a.x; sy -= cf * a.y;
+    for (int u = 0; u < D; ++u) cf[u] = p0 + u < p1 ? (double)arv[p0 + u] * (double)ainv[kk[u]] : 0.0;
+  }
+  cplx<WT> o[NPG];
+  // ports in chunks of four: each chunk has all its loads in flight at once
+  constexpr int PC = NPG < 4 ? NPG : 4;
+#pragma unroll
+  for (int j0 = 0; j0 < NPG; j0 += PC) {
+    cplx<R> b[PC], a[PC][D];
+    int sl[PC];
+#pragma unroll
+    for (int jj = 0; jj < PC; ++jj) {
+      sl[jj] = gr.gslot[g * NPG + j0 + jj];
+      const cplx<R>* xr = x + (int64_t)(sl[jj] >= 0 ? sl[jj] : 0) * ldx;
+      if (sl[jj] >= 0) {
+        b[jj] = xr[nc + v];
+        if (short_row) {
+#pragma unroll
+          for (int u = 0; u < D; ++u)
+            if (p0 + u < p1) a[jj][u] = xr[kk[u]];
         }
       }
-      o.x = (WT)sx; o.y = (WT)sy;
     }
-    tile[nl * NPG + j] = o;
+#pragma unroll
+    for (int jj = 0; jj < PC; ++jj) {
+      cplx<WT> r; r.x = 0; r.y = 0;
+      if (sl[jj] >= 0) {
+        double sx = b[jj].x, sy = b[jj].y;
+        if (short_row) {
+#pragma unroll
+          for (int u = 0; u < D; ++u)
+            if (p0 + u < p1) { sx -= cf[u] * a[jj][u].x; sy -= cf[u] * a[jj][u].y; }
+        } else {
+          const cplx<R>* xr = x + (int64_t)sl[jj] * ldx;
+          for (int p = p0; p < p1; ++p) {
+            const int k = arc[p];
+            const double c = (double)arv[p] * (double)ainv[k];
+            const cplx<R> av = xr[k];
+            sx -= c * av.x; sy -= c * av.y;
+          }
+        }
+        r.x = (WT)sx; r.y = (WT)sy;
+      }
+      o[j0 + jj] = r;
+    }
   }
-  __syncthreads();
-  {
-    const int j = threadIdx.x % NPG, nl = threadIdx.x / NPG;
-    const int node = v0 + nl;
-    if (node < nn) w[((int64_t)g * nn + iperm[node]) * NPG + j] = tile[nl * NPG + j];
-  }
+  line_store<WT, NPG>(w + ((int64_t)g * nn + iperm[v]) * NPG, o);
 }
 
 // Lanes of the level kernels: an item is (row q of the level, group g),
@@ -139,20 +154,20 @@ template <int NPG, int LPR> struct PLanes {
 // w holds b, t and finally x; w2 holds y then u.
 template <typename R, typename WT, int NPG, int LPR>
 __global__ __launch_bounds__(256) void k_sn_update_fwd(const int* __restrict__ rows, int nrows,
-                                                       const int* __restrict__ rp, const int* __restrict__ split,
-                                                       const int* __restrict__ ri, const PcTab<R>* __restrict__ tab,
+                                                       const int* __restrict__ fp, const int* __restrict__ fi,
+                                                       const PcTab<R>* __restrict__ tab,
                                                        PcGroupDev gr, int nn, cplx<WT>* __restrict__ w,
                                                        const cplx<WT>* __restrict__ w2) {
   const PLanes<NPG, LPR> ln;
   if (ln.item >= (int64_t)nrows * gr.ng) return;   // all lanes of the item
   const int q = (int)(ln.item / gr.ng), g = (int)(ln.item - (int64_t)q * gr.ng);
   const int r = rows[q];
-  const R* rx = tab[gr.gtab[g]].Rx;
+  const R* fx = tab[gr.gtab[g]].Fx;
   const cplx<WT>* wg = w2 + (int64_t)g * nn * NPG + ln.j;
   double2 s; s.x = 0; s.y = 0;
-  for (int p = rp[r] + ln.part; p < split[r]; p += LPR) {
-    const double l = rx[p];
-    const cplx<WT> v = wg[(int64_t)ri[p] * NPG];
+  for (int p = fp[r] + ln.part; p < fp[r + 1]; p += LPR) {
+    const double l = fx[p];
+    const cplx<WT> v = wg[(int64_t)fi[p] * NPG];
     s.x += l * v.x; s.y += l * v.y;
   }
   s = ln.sum(s);
@@ -192,8 +207,8 @@ __global__ __launch_bounds__(256) void k_sn_gemv_fwd(const int* __restrict__ row
 
 template <typename R, typename WT, int NPG, int LPR>
 __global__ __launch_bounds__(256) void k_sn_update_bwd(const int* __restrict__ rows, int nrows,
-                                                       const int* __restrict__ cp, const int* __restrict__ split,
-                                                       const int* __restrict__ ci, const PcTab<R>* __restrict__ tab,
+                                                       const int* __restrict__ bp, const int* __restrict__ bi,
+                                                       const PcTab<R>* __restrict__ tab,
                                                        PcGroupDev gr, int nn, const cplx<WT>* __restrict__ w,
                                                        cplx<WT>* __restrict__ w2) {
   const PLanes<NPG, LPR> ln;
@@ -201,12 +216,12 @@ __global__ __launch_bounds__(256) void k_sn_update_bwd(const int* __restrict__ r
   const int q = (int)(ln.item / gr.ng), g = (int)(ln.item - (int64_t)q * gr.ng);
   const int r = rows[q];
   const PcTab<R>& T = tab[gr.gtab[g]];
-  const R* cx = T.Cx;
+  const R* bx = T.Bx;
   const cplx<WT>* wg = w + (int64_t)g * nn * NPG + ln.j;
   double2 s; s.x = 0; s.y = 0;
-  for (int p = split[r] + ln.part; p < cp[r + 1]; p += LPR) {
-    const double l = cx[p];
-    const cplx<WT> v = wg[(int64_t)ci[p] * NPG];
+  for (int p = bp[r] + ln.part; p < bp[r + 1]; p += LPR) {
+    const double l = bx[p];
+    const cplx<WT> v = wg[(int64_t)bi[p] * NPG];
     s.x += l * v.x; s.y += l * v.y;
   }
   s = ln.sum(s);
@@ -251,36 +266,50 @@ __global__ __launch_bounds__(256) void k_sn_gemv_bwd(const int* __restrict__ row
 // off-block update of all the block's rows and its dense inverse product:
 //   forward   t_r = b_r - sum_j L(r,j) y_j   then  y_k = Linv_k t_k
 //   backward  u_r = y_r/D_r - sum_i L(i,r) x_i  then  x_k = Linv_k' u_k
-// The work-vector lines of the block's column union (ucol[up[k] ..
-// up[k+1]), sorted) are staged in shared memory once, so each line is read
-// from memory once per (block, group) instead of once per entry; an entry
-// addresses its line through loc[p], the position of its index in the union.
-// t (or u) stays in shared memory for the dense product.  Shared memory:
-// stg_n >= NPG * (largest union) elements for the lines, then NPG * s for t.
+// Everything the two products read is first staged in shared memory, in one
+// round of independent loads after the block's metadata: the work-vector
+// lines of the block's column union (ucol[up[k] .. up[k+1]), sorted; an entry
+// addresses its line through loc[p], the position of its index in the union),
+// the block's own values, the row ranges, and -- where they fit (BlkLds) --
+// the block's off-block entries (values and union positions, one contiguous
+// run in the packed layout) and the packed dense inverse.  The products then run out of shared memory, so a
+// workgroup waits on memory about three times instead of once per row pass.
 // Lanes: NPG ports x 2^llu parts per row in the update, NPG x 2^llg in the
-// product (each at most 64).  (Serving several groups per workgroup was
-// measured slower: the larger shared-memory footprint cuts the number of
-// workgroups in flight, which is what hides the dependent loads here.)
+// product (each at most 64 and at most NT).  The workgroup size NT follows
+// the tier's largest block (smaller workgroups for small blocks: more of
+// them in flight per CU hide the staging latency).
 __device__ __forceinline__ double2 lane_sum(double2 v, int from, int to) {
   for (int o = from; o < to; o <<= 1) { v.x += __shfl_xor(v.x, o, 64); v.y += __shfl_xor(v.y, o, 64); }
   return v;
 }
 
-template <typename R, typename WT, int NPG, bool FWD>
-__global__ __launch_bounds__(256) void k_blk_solve(const BlkInfo* __restrict__ info, const int* __restrict__ ep,
-                                                   const int* __restrict__ esplit, const int* __restrict__ loc,
+// Shared-memory regions of one block-kernel launch, in elements: staged
+// lines and own values (complex work type), packed inverse and entry span
+// (0: read from global memory instead), and the row ranges (2 per row).
+struct BlkLds {
+  int stg_n, tt_n, inv_n, ent_n;
+};
+
+template <typename R, typename WT, int NPG, bool FWD, int NT>
+__global__ __launch_bounds__(NT) void k_blk_solve(const BlkInfo* __restrict__ info, const int* __restrict__ pp,
+                                                   const int* __restrict__ loc,
                                                    const int* __restrict__ ucol, const PcTab<R>* __restrict__ tab,
                                                    PcGroupDev gr, int nn, cplx<WT>* __restrict__ w,
-                                                   cplx<WT>* __restrict__ w2, int stg_n, int llu, int llg) {
+                                                   cplx<WT>* __restrict__ w2, BlkLds lay, int llu, int llg) {
   using CW = cplx<WT>;
   extern __shared__ __align__(16) unsigned char smem_pc_raw[];
   CW* stg = (CW*)smem_pc_raw;
-  CW* tt = stg + stg_n;
+  CW* tt = stg + lay.stg_n;
+  R* sinv = (R*)(tt + lay.tt_n);
+  R* sval = sinv + lay.inv_n;
+  int* sloc = (int*)(sval + lay.ent_n);
+  int* rng = sloc + lay.ent_n;
   const int q = blockIdx.x / gr.ng, g = blockIdx.x - q * gr.ng;
   const BlkInfo bi = info[q];
-  const int c0 = bi.c0, s = bi.s, u0 = bi.u0, nu = bi.nu;
+  const int c0 = bi.c0, s = bi.s, u0 = bi.u0, nu = bi.nu, e0 = bi.e0, ne = bi.ne;
   const PcTab<R>& T = tab[gr.gtab[g]];
-  const R* __restrict__ val = FWD ? T.Rx : T.Cx;
+  const R* __restrict__ val = FWD ? T.Fx : T.Bx;
+  const R* __restrict__ Lb = (FWD ? T.Li : T.LiT) + bi.off;
   // forward: staged lines are y (w2), the block's own values b come from w;
   // backward: staged lines are x (w), the own values y from w2
   const CW* __restrict__ src = FWD ? w2 : w;
@@ -289,34 +318,82 @@ __global__ __launch_bounds__(256) void k_blk_solve(const BlkInfo* __restrict__ i
   const int64_t gb = (int64_t)g * nn * NPG;
   const int tid = threadIdx.x;
   const int j = tid % NPG;
-  // the block's own values (b, or y / D) go to t while the lines are staged
-  for (int e = tid; e < s * NPG; e += blockDim.x) {
-    const int i = e / NPG;
-    const CW o = own[gb + (int64_t)(c0 + i) * NPG + (e - i * NPG)];
-    if (FWD) {
-      tt[e] = o;
-    } else {
-      const double inv = 1.0 / (double)T.D[c0 + i];
-      CW q; q.x = (WT)((double)o.x * inv); q.y = (WT)((double)o.y * inv);
-      tt[e] = q;
+  const bool inv_lds = lay.inv_n > 0, ent_lds = lay.ent_n > 0;
+  // ---- staging: every load below depends on the block metadata only (the
+  // lines also on their union index); each thread issues up to four loads of
+  // a region before storing them
+  {
+    constexpr int U = 4;
+    for (int b = tid; b < nu * NPG; b += U * NT) {
+      int cq[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) cq[u] = b + u * NT < nu * NPG ? ucol[u0 + (b + u * NT) / NPG] : 0;
+      CW v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (b + u * NT < nu * NPG) v[u] = src[gb + (int64_t)cq[u] * NPG + (b + u * NT) % NPG];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (b + u * NT < nu * NPG) stg[b + u * NT] = v[u];
     }
-  }
-  for (int e = tid; e < nu * NPG; e += blockDim.x) {
-    const int cq = e / NPG, jj = e - cq * NPG;
-    stg[e] = src[gb + (int64_t)ucol[u0 + cq] * NPG + jj];
+    for (int b = tid; b < s * NPG; b += U * NT) {
+      CW v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (b + u * NT < s * NPG) v[u] = own[gb + (int64_t)c0 * NPG + b + u * NT];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int e = b + u * NT;
+        if (e < s * NPG) {
+          if (FWD) {
+            tt[e] = v[u];
+          } else {
+            const double inv = 1.0 / (double)T.D[c0 + e / NPG];
+            CW o; o.x = (WT)((double)v[u].x * inv); o.y = (WT)((double)v[u].y * inv);
+            tt[e] = o;
+          }
+        }
+      }
+    }
+    for (int i = tid; i < s; i += NT) {
+      rng[2 * i] = pp[c0 + i] - e0;
+      rng[2 * i + 1] = pp[c0 + i + 1] - e0;
+    }
+    if (inv_lds) {
+      const int ni = s * (s + 1) / 2;
+      for (int b = tid; b < ni; b += U * NT) {
+        R v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (b + u * NT < ni) v[u] = Lb[b + u * NT];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (b + u * NT < ni) sinv[b + u * NT] = v[u];
+      }
+    }
+    if (ent_lds) {
+      for (int b = tid; b < ne; b += U * NT) {
+        R v[U];
+        int l[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (b + u * NT < ne) { v[u] = val[e0 + b + u * NT]; l[u] = loc[e0 + b + u * NT]; }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (b + u * NT < ne) { sval[b + u * NT] = v[u]; sloc[b + u * NT] = l[u]; }
+      }
+    }
   }
   __syncthreads();
   {
     const int lu = NPG << llu, lpr = 1 << llu;
-    const int part = (tid / NPG) & (lpr - 1), rs = tid / lu, rpp = blockDim.x / lu;
+    const int part = (tid / NPG) & (lpr - 1), rs = tid / lu, rpp = NT / lu;
     for (int i = rs; i < s; i += rpp) {
-      const int r = c0 + i;
-      const int lo = FWD ? ep[r] : esplit[r];
-      const int hi = FWD ? esplit[r] : ep[r + 1];
+      const int lo = rng[2 * i], hi = rng[2 * i + 1];
       double2 a = make_double2(0.0, 0.0);
       for (int p = lo + part; p < hi; p += lpr) {
-        const double l = (double)val[p];
-        const CW v = stg[loc[p] * NPG + j];
+        const double l = ent_lds ? (double)sval[p] : (double)val[e0 + p];
+        const CW v = stg[(ent_lds ? sloc[p] : loc[e0 + p]) * NPG + j];
         a.x += l * v.x; a.y += l * v.y;
       }
       a = lane_sum(a, NPG, lu);
@@ -330,12 +407,12 @@ __global__ __launch_bounds__(256) void k_blk_solve(const BlkInfo* __restrict__ i
   __syncthreads();
   {
     const int lg = NPG << llg, lpr = 1 << llg;
-    const int part = (tid / NPG) & (lpr - 1), rs = tid / lg, rpp = blockDim.x / lg;
-    const R* __restrict__ Lb = (FWD ? T.Li : T.LiT) + bi.off;
+    const int part = (tid / NPG) & (lpr - 1), rs = tid / lg, rpp = NT / lg;
+    const R* __restrict__ Lrow = inv_lds ? sinv : Lb;
     for (int i = rs; i < s; i += rpp) {
       // forward: row i of Linv holds Linv(i, 0..i) at i(i+1)/2; backward:
       // row i of the transpose holds Linv(i..s-1, i) at i*s - i(i-1)/2
-      const R* __restrict__ row = FWD ? Lb + (int64_t)i * (i + 1) / 2 : Lb + (int64_t)i * s - (int64_t)i * (i - 1) / 2;
+      const R* __restrict__ row = FWD ? Lrow + (int64_t)i * (i + 1) / 2 : Lrow + (int64_t)i * s - (int64_t)i * (i - 1) / 2;
       const int len = FWD ? i + 1 : s - i;
       const CW* tv = FWD ? tt + j : tt + i * NPG + j;
       double2 a = make_double2(0.0, 0.0);
@@ -385,42 +462,70 @@ __global__ __launch_bounds__(256) void k_pc_scatter_nodes(const cplx<WT>* __rest
 }
 
 // y[k] = (a[k] + sum_i ArTp(k,i) w[g][i][j]) * Ainv[k]   (ArTp columns are permuted node ids).
-// Thread -> (group g = blockIdx.y, current k, port j) with the port fastest:
-// a wave covers 64/NPG consecutive currents of NPG systems (one segment per
-// system for a and y) and reads each work-vector line whole.
+// One thread per (current k, group g = blockIdx.y) serves all NPG ports: the
+// current's incidence row and Ainv[k] are loaded once, the lines of its nodes
+// (at most 6) are read whole, and a wave reads and writes 64 consecutive
+// currents of each port.
 template <typename R, typename WT, int NPG>
 __global__ __launch_bounds__(256) void k_pc_currents(const cplx<R>* __restrict__ x, int64_t ldx,
                                                      const cplx<WT>* __restrict__ w, cplx<R>* __restrict__ y,
                                                      int64_t ldy, int nn, int64_t nc, const int* __restrict__ tp,
                                                      const int* __restrict__ tc, const R* __restrict__ tv,
                                                      const PcTab<R>* __restrict__ tab, PcGroupDev gr) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = (int)(t % NPG);
-  const int64_t k = t / NPG;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int g = blockIdx.y;
-  const int slot = gr.gslot[g * NPG + j];
-  if (k >= nc || slot < 0) return;
-  const cplx<R> a = x[(int64_t)slot * ldx + k];
-  double sx = a.x, sy = a.y;
-  if (tc) {
-    // a current touches at most 2 nodes: both loaded at once
-    const cplx<WT>* wg = w + (int64_t)g * nn * NPG + j;
-    const int p0 = tp[k], p1 = tp[k + 1];
-    if (p1 - p0 <= 2) {
-      const int n0 = p0 < p1 ? tc[p0] : 0, n1 = p0 + 1 < p1 ? tc[p0 + 1] : 0;
-      const double v0 = p0 < p1 ? (double)tv[p0] : 0.0, v1 = p0 + 1 < p1 ? (double)tv[p0 + 1] : 0.0;
-      if (p0 < p1) { const cplx<WT> v = wg[(int64_t)n0 * NPG]; sx += v0 * v.x; sy += v0 * v.y; }
-      if (p0 + 1 < p1) { const cplx<WT> v = wg[(int64_t)n1 * NPG]; sx += v1 * v.x; sy += v1 * v.y; }
-    } else {
-      for (int p = p0; p < p1; ++p) {
-        const cplx<WT> v = wg[(int64_t)tc[p] * NPG];
-        sx += (double)tv[p] * v.x; sy += (double)tv[p] * v.y;
-      }
+  if (k >= nc) return;
+  const double ai = (double)tab[gr.gtab[g]].ainv[k];
+  const cplx<WT>* wg = w + (int64_t)g * nn * NPG;
+  constexpr int D = 6;
+  int p0 = 0, p1 = 0;
+  if (tc) { p0 = tp[k]; p1 = tp[k + 1]; }
+  const bool short_row = p1 - p0 <= D;
+  int nd[D];
+  double cv[D];
+  if (short_row) {
+#pragma unroll
+    for (int u = 0; u < D; ++u) {
+      nd[u] = p0 + u < p1 ? tc[p0 + u] : 0;
+      cv[u] = p0 + u < p1 ? (double)tv[p0 + u] : 0.0;
     }
   }
-  const double ai = tab[gr.gtab[g]].ainv[k];
-  cplx<R> o; o.x = (R)(sx * ai); o.y = (R)(sy * ai);
-  y[(int64_t)slot * ldy + k] = o;
+  constexpr int PC = NPG < 4 ? NPG : 4;
+#pragma unroll
+  for (int j0 = 0; j0 < NPG; j0 += PC) {
+    int sl[PC];
+    cplx<R> a[PC];
+#pragma unroll
+    for (int jj = 0; jj < PC; ++jj) {
+      sl[jj] = gr.gslot[g * NPG + j0 + jj];
+      if (sl[jj] >= 0) a[jj] = x[(int64_t)sl[jj] * ldx + k];
+    }
+    double sx[PC], sy[PC];
+#pragma unroll
+    for (int jj = 0; jj < PC; ++jj) { sx[jj] = sl[jj] >= 0 ? (double)a[jj].x : 0.0; sy[jj] = sl[jj] >= 0 ? (double)a[jj].y : 0.0; }
+    if (short_row) {
+#pragma unroll
+      for (int u = 0; u < D; ++u)
+        if (p0 + u < p1) {
+          const cplx<WT>* ln = wg + (int64_t)nd[u] * NPG + j0;
+#pragma unroll
+          for (int jj = 0; jj < PC; ++jj) { const cplx<WT> v = ln[jj]; sx[jj] += cv[u] * v.x; sy[jj] += cv[u] * v.y; }
+        }
+    } else {
+      for (int p = p0; p < p1; ++p) {
+        const cplx<WT>* ln = wg + (int64_t)tc[p] * NPG + j0;
+        const double c = (double)tv[p];
+#pragma unroll
+        for (int jj = 0; jj < PC; ++jj) { const cplx<WT> v = ln[jj]; sx[jj] += c * v.x; sy[jj] += c * v.y; }
+      }
+    }
+#pragma unroll
+    for (int jj = 0; jj < PC; ++jj)
+      if (sl[jj] >= 0) {
+        cplx<R> o; o.x = (R)(sx[jj] * ai); o.y = (R)(sy[jj] * ai);
+        y[(int64_t)sl[jj] * ldy + k] = o;
+      }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -549,31 +654,37 @@ static void block_unions(const std::vector<int>& sn_ptr, const int* idx, int64_t
 }
 
 // Per level, along the sorted blocks: the sort key max(size, union) and the
-// prefix maxima of sizes and unions (the block kernel sizes its shared
-// memory from them).
-static void level_extents(const std::vector<int>& sn_ptr, const std::vector<int>& up, LevelBlocks& B) {
+// prefix maxima of sizes, unions and entry spans (the block kernel sizes its
+// shared memory from them).  ptr: the row (forward, CSR) or column
+// (backward, CSC) pointers of L.
+static void level_extents(const std::vector<int>& sn_ptr, const std::vector<int>& up, const std::vector<int>& ptr,
+                          LevelBlocks& B) {
   const int nb = (int)B.blk.size();
   B.key.assign(nb, 0);
   B.smax.assign(nb, 0);
   B.umax.assign(nb, 0);
+  B.emax.assign(nb, 0);
   for (int l = 0; l + 1 < (int)B.bptr.size(); ++l) {
-    int ms = 0, mu = 0;
+    int ms = 0, mu = 0, me = 0;
     for (int i = B.bptr[l]; i < B.bptr[l + 1]; ++i) {
       const int k = B.blk[i];
       const int sz = sn_ptr[k + 1] - sn_ptr[k], un = up[k + 1] - up[k];
       B.key[i] = std::max(sz, un);
       ms = std::max(ms, sz);
       mu = std::max(mu, un);
+      me = std::max(me, ptr[sn_ptr[k + 1]] - ptr[sn_ptr[k]]);
       B.smax[i] = ms;
       B.umax[i] = mu;
+      B.emax[i] = me;
     }
   }
 }
 
-// Per sorted block position: rows, union range and inverse offset (what the
-// block kernel reads first, in one load).
+// Per sorted block position: rows, union range, inverse offset and entry span
+// (what the block kernel reads first, in one load).
 static std::vector<BlkInfo> block_info(const LevelBlocks& B, const std::vector<int>& sn_ptr,
-                                       const std::vector<int64_t>& sn_off, const std::vector<int>& up) {
+                                       const std::vector<int64_t>& sn_off, const std::vector<int>& up,
+                                       const std::vector<int>& ptr) {
   std::vector<BlkInfo> v(B.blk.size());
   for (size_t i = 0; i < B.blk.size(); ++i) {
     const int k = B.blk[i];
@@ -582,6 +693,8 @@ static std::vector<BlkInfo> block_info(const LevelBlocks& B, const std::vector<i
     v[i].u0 = up[k];
     v[i].nu = up[k + 1] - up[k];
     v[i].off = sn_off[k];
+    v[i].e0 = ptr[sn_ptr[k]];
+    v[i].ne = ptr[sn_ptr[k + 1]] - ptr[sn_ptr[k]];
   }
   return v;
 }
@@ -662,8 +775,32 @@ static void build_levels(Geometry& G, const LdlFactor& F, PreconSymbolic& Y) {
   level_blocks(lb, Y.sn_ptr, block_keys(Y.sn_ptr, Y.b_up), Y.bl);
   Y.f_lptr = Y.fl.lptr; Y.f_rows = Y.fl.rows;
   Y.b_lptr = Y.bl.lptr; Y.b_rows = Y.bl.rows;
-  level_extents(Y.sn_ptr, Y.f_up, Y.fl);
-  level_extents(Y.sn_ptr, Y.b_up, Y.bl);
+  // off-block entries packed contiguously: forward row by row (CSR order),
+  // backward column by column (CSC order); the values of every frequency
+  // are stored in these orders (fsrc / bsrc: their CSC positions)
+  {
+    std::vector<int> csr2csc(Rp[n]);
+    for (int p = 0; p < Rp[n]; ++p) csr2csc[Y.csc2csr[p]] = p;
+    Y.fp.assign(n + 1, 0);
+    Y.bp.assign(n + 1, 0);
+    Y.fi.clear(); Y.floc.clear(); Y.fsrc.clear();
+    Y.bi.clear(); Y.bloc.clear(); Y.bsrc.clear();
+    for (int r = 0; r < n; ++r) {
+      for (int p = Rp[r]; p < Y.row_split[r]; ++p) {
+        Y.fi.push_back(Ri[p]); Y.floc.push_back(Y.f_loc[p]); Y.fsrc.push_back(csr2csc[p]);
+      }
+      Y.fp[r + 1] = (int)Y.fi.size();
+      for (int q = Y.col_split[r]; q < F.Lp[r + 1]; ++q) {
+        Y.bi.push_back(F.Li[q]); Y.bloc.push_back(Y.b_loc[q]); Y.bsrc.push_back(q);
+      }
+      Y.bp[r + 1] = (int)Y.bi.size();
+    }
+  }
+  std::vector<int>().swap(Y.f_loc);     // (only the packed copies are used from here on)
+  std::vector<int>().swap(Y.b_loc);
+  std::vector<int>().swap(Y.csc2csr);
+  level_extents(Y.sn_ptr, Y.f_up, Y.fp, Y.fl);
+  level_extents(Y.sn_ptr, Y.b_up, Y.bp, Y.bl);
   if (trace_on()) {
     // block statistics: sizes, and the dense panels a block-wise update would use
     int64_t off_nnz = 0, fpanel = 0, bpanel = 0, fu = 0, bu = 0;
@@ -746,22 +883,20 @@ static void build_levels(Geometry& G, const LdlFactor& F, PreconSymbolic& Y) {
   }
   upload(Y.d_perm, F.perm);
   upload(Y.d_iperm, F.iperm);
-  upload(Y.Rp, Rp);
-  upload(Y.Ri, Ri);
-  upload(Y.Cp, F.Lp);
-  upload(Y.Ci, F.Li);
+  upload(Y.d_fp, Y.fp);
+  upload(Y.d_fi, Y.fi);
+  upload(Y.d_floc, Y.floc);
+  upload(Y.d_bp, Y.bp);
+  upload(Y.d_bi, Y.bi);
+  upload(Y.d_bloc, Y.bloc);
   upload(Y.d_sn_ptr, Y.sn_ptr);
   upload(Y.d_sn_of_row, Y.sn_of_row);
-  upload(Y.d_row_split, Y.row_split);
-  upload(Y.d_col_split, Y.col_split);
   upload(Y.d_f_rows, Y.f_rows);
   upload(Y.d_b_rows, Y.b_rows);
-  upload(Y.d_f_info, block_info(Y.fl, Y.sn_ptr, Y.sn_off, Y.f_up));
-  upload(Y.d_b_info, block_info(Y.bl, Y.sn_ptr, Y.sn_off, Y.b_up));
+  upload(Y.d_f_info, block_info(Y.fl, Y.sn_ptr, Y.sn_off, Y.f_up, Y.fp));
+  upload(Y.d_b_info, block_info(Y.bl, Y.sn_ptr, Y.sn_off, Y.b_up, Y.bp));
   upload(Y.d_f_ucol, Y.f_ucol);
-  upload(Y.d_f_loc, Y.f_loc);
   upload(Y.d_b_ucol, Y.b_ucol);
-  upload(Y.d_b_loc, Y.b_loc);
   upload(Y.d_sn_off, Y.sn_off);
   // Ar' with permuted node columns
   std::vector<int> tcp(G.atc_h.size());
@@ -868,12 +1003,13 @@ void precon_prepare(Precon& P, double jre, double jim, const std::complex<double
     build_levels(G, P.F, Y);
     lk.unlock();
   }
-  std::vector<double> Rx(Y.nnzL);
-  for (int64_t p = 0; p < Y.nnzL; ++p) Rx[Y.csc2csr[p]] = P.F.Lx[p];
+  std::vector<double> Fx(Y.fsrc.size()), Bx(Y.bsrc.size());
+  for (size_t q = 0; q < Fx.size(); ++q) Fx[q] = P.F.Lx[Y.fsrc[q]];
+  for (size_t q = 0; q < Bx.size(); ++q) Bx[q] = P.F.Lx[Y.bsrc[q]];
   std::vector<double> Li, LiT;
   block_inverses(Y, P.F, Li, LiT);
-  upload(P.Rx64, Rx); upload_f32(P.Rx32, Rx);
-  upload(P.Cx64, P.F.Lx); upload_f32(P.Cx32, P.F.Lx);
+  upload(P.Fx64, Fx); upload_f32(P.Fx32, Fx);
+  upload(P.Bx64, Bx); upload_f32(P.Bx32, Bx);
   upload(P.D64, P.F.D); upload_f32(P.D32, P.F.D);
   upload(P.ainv64, ainv); upload_f32(P.ainv32, ainv);
   upload(P.Li64, Li); upload_f32(P.Li32, Li);
@@ -886,8 +1022,8 @@ void precon_prepare(Precon& P, double jre, double jim, const std::complex<double
     VH_HIP(hipMemcpy(P.z64.p, P.zh.data(), nv * sizeof(double2), hipMemcpyHostToDevice));
     VH_HIP(hipMemcpy(P.z32.p, z32.data(), nv * sizeof(float2), hipMemcpyHostToDevice));
   }
-  P.tab64_h = PcTab<double>{P.Rx64.p, P.Cx64.p, P.D64.p, P.ainv64.p, P.Li64.p, P.LiT64.p, P.z64.p};
-  P.tab32_h = PcTab<float>{P.Rx32.p, P.Cx32.p, P.D32.p, P.ainv32.p, P.Li32.p, P.LiT32.p, P.z32.p};
+  P.tab64_h = PcTab<double>{P.Fx64.p, P.Bx64.p, P.D64.p, P.ainv64.p, P.Li64.p, P.LiT64.p, P.z64.p};
+  P.tab32_h = PcTab<float>{P.Fx32.p, P.Bx32.p, P.D32.p, P.ainv32.p, P.Li32.p, P.LiT32.p, P.z32.p};
   P.tab64.alloc(1);
   P.tab32.alloc(1);
   VH_HIP(hipMemcpy(P.tab64.p, &P.tab64_h, sizeof(PcTab<double>), hipMemcpyHostToDevice));
@@ -1028,6 +1164,26 @@ static PcGroupDev group_table(PreconSymbolic& Y, const std::vector<int>& key, hi
 constexpr int kBlkCap = 2048;
 static int ilog2_floor(int v) { int l = 0; while ((2 << l) <= v) ++l; return l; }
 
+// Shared memory of a block-kernel tier: the staged lines and own values
+// always; the packed inverses and the entry spans of its blocks when each
+// fits kBlkStageCap bytes and the whole stays within kBlkLdsMax.
+constexpr size_t kBlkStageCap = 16384, kBlkLdsMax = 65536;
+template <typename R, typename WT> static size_t blk_lds_bytes_t(const BlkLds& a, int smax) {
+  return (size_t)(a.stg_n + a.tt_n) * sizeof(cplx<WT>) + (size_t)a.inv_n * sizeof(R) +
+         (size_t)a.ent_n * (sizeof(R) + sizeof(int)) + 2 * (size_t)smax * sizeof(int);
+}
+template <typename R, typename WT> static BlkLds blk_lds_t(int umax, int smax, int emax, int npg) {
+  BlkLds a;
+  a.stg_n = umax * npg;
+  a.tt_n = smax * npg;
+  const size_t ni = (size_t)smax * (smax + 1) / 2;
+  a.inv_n = ni * sizeof(R) <= kBlkStageCap ? (int)ni : 0;
+  a.ent_n = (size_t)emax * (sizeof(R) + sizeof(int)) <= kBlkStageCap ? emax : 0;
+  if (blk_lds_bytes_t<R, WT>(a, smax) > kBlkLdsMax) a.ent_n = 0;
+  if (blk_lds_bytes_t<R, WT>(a, smax) > kBlkLdsMax) a.inv_n = 0;
+  return a;
+}
+
 template <typename R, typename WT>
 void precon_apply_cols_wt(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, const int* pidx,
                           const std::vector<int>& slots, const cplx<R>* x, int64_t ldx, cplx<R>* y, int64_t ldy) {
@@ -1040,6 +1196,7 @@ void precon_apply_cols_wt(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, c
   const int npg = key[0];
   const PcGroupDev gr = group_table(Y, key, st);
   const int64_t cols = (int64_t)gr.ng * npg;
+  VH_REQUIRE(cols < 65536, "preconditioner: too many columns");
   const size_t need = 2 * (size_t)nn * cols * sizeof(cplx<WT>);
   if (Y.ws.n < need) Y.ws.alloc(need);
   cplx<WT>* w = (cplx<WT>*)Y.ws.p;
@@ -1055,17 +1212,20 @@ void precon_apply_cols_wt(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, c
     }
   };
   if (nn > 0) {
-    const int64_t items = (int64_t)nn * cols;
     by_npg([&](auto N) {
       constexpr int n = decltype(N)::value;
-      hipLaunchKernelGGL((k_pc_gather<R, WT, n>), dim3((unsigned)(gr.ng * ceil_div(nn, 256 / n))), dim3(256), 0, st, x,
-                         ldx, w, nn, Y.d_iperm.p, G.Ar.ptr.p, G.Ar.col.p, arv, tab, gr, nc);
+      hipLaunchKernelGGL((k_pc_gather<R, WT, n>), dim3((unsigned)ceil_div(nn, 256), (unsigned)gr.ng), dim3(256), 0, st,
+                         x, ldx, w, nn, Y.d_iperm.p, G.Ar.ptr.p, G.Ar.col.p, arv, tab, gr, nc);
     });
     VH_LAUNCHED();
     // Blocks whose t fits the shared memory (s * npg <= kBlkCap) go through
     // the block kernel; the rows of larger blocks through the row kernels.
     // (VH_PC_BLOCK: bit 0 forward, bit 1 backward; default both)
     static const int blk_on = [] { const char* e = std::getenv("VH_PC_BLOCK"); return e ? std::atoi(e) : 3; }();
+    auto blk_lds = [](int umax, int smax, int emax, int np) { return blk_lds_t<R, WT>(umax, smax, emax, np); };
+    auto blk_lds_bytes = [](const BlkLds& a, int smax) { return blk_lds_bytes_t<R, WT>(a, smax); };
+    const bool trace_plan = trace_on() && !Y.plan_traced;
+    Y.plan_traced = true;
     auto blk_launch = [&](const LevelBlocks& B, int l, bool fwd, int lpr_u, int lpr_g) -> int {
       const int b0 = B.bptr[l], b1 = B.bptr[l + 1];
       if (!(blk_on & (fwd ? 1 : 2))) return B.lptr[l];
@@ -1080,19 +1240,40 @@ void precon_apply_cols_wt(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, c
       // so small blocks do not reserve the shared memory of large ones
       int i0 = b0;
       for (const int cap : {256, 1024, kBlkCap}) {
-        const int i1 = (int)(std::upper_bound(B.key.begin() + i0, B.key.begin() + b1, cap / npg) - B.key.begin());
+        int i1 = (int)(std::upper_bound(B.key.begin() + i0, B.key.begin() + b1, cap / npg) - B.key.begin());
+        // (the prefix maxima grow along the sorted blocks: drop the tail whose
+        // staging would not fit; those blocks take the row kernels)
+        while (i1 > i0 && blk_lds_bytes(blk_lds(B.umax[i1 - 1], B.smax[i1 - 1], B.emax[i1 - 1], npg), B.smax[i1 - 1]) >
+                              kBlkLdsMax)
+          --i1;
         if (i1 > i0) {
-          const int stg_n = B.umax[i1 - 1] * npg;
-          const size_t lds = (size_t)(stg_n + B.smax[i1 - 1] * npg) * sizeof(cplx<WT>);
+          const BlkLds lay = blk_lds(B.umax[i1 - 1], B.smax[i1 - 1], B.emax[i1 - 1], npg);
+          const size_t lds = blk_lds_bytes(lay, B.smax[i1 - 1]);
+          if (trace_plan)
+            fprintf(stderr, "[vh] pc %s level %d tier %d: %d blocks x %d groups, s<=%d union<=%d entries<=%d, lds %zu%s%s\n",
+                    fwd ? "fwd" : "bwd", l, cap, i1 - i0, gr.ng, B.smax[i1 - 1], B.umax[i1 - 1], B.emax[i1 - 1], lds,
+                    lay.inv_n ? " inv" : "", lay.ent_n ? " ent" : "");
           const dim3 grid((unsigned)((int64_t)(i1 - i0) * gr.ng));
+          // workgroup size: one thread per value of the tier's largest t (s *
+          // npg values), in [64, 256] (half or a quarter of that measured
+          // slower on the bus: the staging then takes more rounds per thread)
+          int nt = 64;
+          while (nt < 256 && nt < B.smax[i1 - 1] * npg) nt <<= 1;
+          const int lu = std::min(llu, ilog2_floor(nt / npg)), lg = std::min(llg, ilog2_floor(nt / npg));
           by_npg([&](auto N) {
             constexpr int n = decltype(N)::value;
-            if (fwd)
-              hipLaunchKernelGGL((k_blk_solve<R, WT, n, true>), grid, dim3(256), lds, st, info + i0, Y.Rp.p,
-                                 Y.d_row_split.p, Y.d_f_loc.p, Y.d_f_ucol.p, tab, gr, nn, w, w2, stg_n, llu, llg);
-            else
-              hipLaunchKernelGGL((k_blk_solve<R, WT, n, false>), grid, dim3(256), lds, st, info + i0, Y.Cp.p,
-                                 Y.d_col_split.p, Y.d_b_loc.p, Y.d_b_ucol.p, tab, gr, nn, w, w2, stg_n, llu, llg);
+            auto go = [&](auto ntc) {
+              constexpr int NTB = decltype(ntc)::value;
+              if (fwd)
+                hipLaunchKernelGGL((k_blk_solve<R, WT, n, true, NTB>), grid, dim3(NTB), lds, st, info + i0, Y.d_fp.p,
+                                   Y.d_floc.p, Y.d_f_ucol.p, tab, gr, nn, w, w2, lay, lu, lg);
+              else
+                hipLaunchKernelGGL((k_blk_solve<R, WT, n, false, NTB>), grid, dim3(NTB), lds, st, info + i0,
+                                   Y.d_bp.p, Y.d_bloc.p, Y.d_b_ucol.p, tab, gr, nn, w, w2, lay, lu, lg);
+            };
+            if (nt == 64) go(IC<64>{});
+            else if (nt == 128) go(IC<128>{});
+            else go(IC<256>{});
           });
           VH_LAUNCHED();
         }
@@ -1104,12 +1285,13 @@ void precon_apply_cols_wt(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, c
     for (int l = 0; l < nlf; ++l) {
       const int a = blk_launch(Y.fl, l, true, Y.f_lpr_u[l], Y.f_lpr_g[l]), nr = Y.f_lptr[l + 1] - a;
       if (nr == 0) continue;
+      if (trace_plan) fprintf(stderr, "[vh] pc fwd level %d: %d rows by the row kernels\n", l, nr);
       const int64_t it = (int64_t)nr * gr.ng;
       if (Y.f_upd_nnz[l] > 0) {
         with_lanes(npg, Y.f_lpr_u[l], [&](auto N, auto L) {
           constexpr int n = decltype(N)::value, lp = decltype(L)::value;
           hipLaunchKernelGGL((k_sn_update_fwd<R, WT, n, lp>), lane_grid(it, n * lp), dim3(256), 0, st,
-                             Y.d_f_rows.p + a, nr, Y.Rp.p, Y.d_row_split.p, Y.Ri.p, tab, gr, nn, w, w2);
+                             Y.d_f_rows.p + a, nr, Y.d_fp.p, Y.d_fi.p, tab, gr, nn, w, w2);
         });
         VH_LAUNCHED();
       }
@@ -1124,11 +1306,12 @@ void precon_apply_cols_wt(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, c
     for (int l = 0; l < nlb; ++l) {
       const int a = blk_launch(Y.bl, l, false, Y.b_lpr_u[l], Y.b_lpr_g[l]), nr = Y.b_lptr[l + 1] - a;
       if (nr == 0) continue;
+      if (trace_plan) fprintf(stderr, "[vh] pc bwd level %d: %d rows by the row kernels\n", l, nr);
       const int64_t it = (int64_t)nr * gr.ng;
       with_lanes(npg, Y.b_lpr_u[l], [&](auto N, auto L) {
         constexpr int n = decltype(N)::value, lp = decltype(L)::value;
         hipLaunchKernelGGL((k_sn_update_bwd<R, WT, n, lp>), lane_grid(it, n * lp), dim3(256), 0, st, Y.d_b_rows.p + a,
-                           nr, Y.Cp.p, Y.d_col_split.p, Y.Ci.p, tab, gr, nn, w, w2);
+                           nr, Y.d_bp.p, Y.d_bi.p, tab, gr, nn, w, w2);
       });
       VH_LAUNCHED();
       with_lanes(npg, Y.b_lpr_g[l], [&](auto N, auto L) {
@@ -1146,10 +1329,9 @@ void precon_apply_cols_wt(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, c
     VH_LAUNCHED();
   }
   const R* atv = d ? (const R*)G.ArT.v64.p : (const R*)G.ArT.v32.p;
-  VH_REQUIRE(cols < 65536, "preconditioner: too many columns");
   by_npg([&](auto N) {
     constexpr int n = decltype(N)::value;
-    hipLaunchKernelGGL((k_pc_currents<R, WT, n>), dim3(ceil_div(nc * n, 256), (unsigned)gr.ng), dim3(256), 0,
+    hipLaunchKernelGGL((k_pc_currents<R, WT, n>), dim3((unsigned)ceil_div(nc, 256), (unsigned)gr.ng), dim3(256), 0,
                        st, x, ldx, w, y, ldy, nn, nc, G.ArT.ptr.p, nn > 0 ? Y.ArTp_col.p : nullptr, atv, tab, gr);
   });
   VH_LAUNCHED();

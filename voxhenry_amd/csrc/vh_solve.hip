@@ -267,6 +267,7 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
         const double rres = W.h_res[a];
         st[s].resvec.push_back(rres / bnorm[s]);
         if (!(rres >= tol_of(s) * bnorm[s])) conv.push_back(s);   // pgmres_iter.m:126 (NaN also stops)
+        else if (o.stop_slot && (*o.stop_slot)[s]) conv.push_back(s);
         else keep.push_back(s);
       }
       t_host += std::chrono::duration<double>(clk::now() - th).count();
@@ -346,6 +347,7 @@ void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<f
   // [inner_tol, 0.1]; the first inner solve (r = 1) uses inner_tol.
   std::vector<double> itol(nslot, o.inner_tol);
   io.tol_slot = &itol;
+  std::vector<char> stop(nslot, 0);
   // preconditioner of the outer iteration: z = (FP32 GMRES solve of A z = v)
   OpFn<double> inner = [&](const std::vector<int>& act, const int* slots, const double2* v, double2* z) {
     const int na = (int)act.size();
@@ -357,7 +359,18 @@ void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<f
     convert_vectors<float, double>(ctx, v, n, v32, n, n, na, slots, slots);
     gmres_batch<float>(ctx, n, A32, &M32, v32, z32, act, io, W32, inner_st);
     convert_vectors<double, float>(ctx, z32, n, z, n, n, na, slots, slots);
-    for (int s : act) inner_iters[s] += inner_st[s].iters;
+    for (int s : act) {
+      inner_iters[s] += inner_st[s].iters;
+      // escalate a system whose inner solve ended above escalate_resid (and
+      // above 100x its own tolerance): its FP32 process is stalling, not just
+      // slow (measured on the uneven batch: 1e-3 also escalated the thick
+      // bars and wires at high frequency, whose FP32 solves finish faster
+      // than FP64; 1e-1 left the superconducting strips to crawl in FP32)
+      const double ir = inner_st[s].resvec.empty() ? 1.0 : inner_st[s].resvec.back();
+      if (o.escalate_inner > 0 && inner_iters[s] >= o.escalate_inner &&
+          !(ir <= std::max(100.0 * itol[s], o.escalate_resid)))
+        stop[s] = 1;
+    }
   };
   SolveOpts oo;
   oo.restart = o.outer_restart;
@@ -367,6 +380,7 @@ void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<f
   oo.final_residual = true;
   oo.flexible = true;
   oo.x0 = o.x0;
+  oo.stop_slot = &stop;
   gmres_batch<double>(ctx, n, A64, &inner, b, x, sys, oo, W64, st);
   for (int s : sys) st[s].inner_iters = inner_iters[s];
 }

@@ -8,8 +8,8 @@ namespace vh {
 
 // Device pointers to the factor values of one preconditioner, in one precision.
 template <typename R> struct PcTab {
-  const R* Rx;     // L in CSR (row copy)
-  const R* Cx;     // L in CSC
+  const R* Fx;     // off-block entries of L, packed row by row (forward solve)
+  const R* Bx;     // off-block entries of L, packed column by column (backward solve)
   const R* D;
   const R* ainv;
   const R* Li;     // inverses of the diagonal blocks, packed row-major lower
@@ -33,17 +33,20 @@ struct PcGroups {
 // Blocks of the levels of one solve direction: blk[bptr[l] .. bptr[l+1]) are
 // the blocks of level l in ascending key = max(size, column union),
 // rows[lptr[l] .. lptr[l+1]) their rows in the same order (rowoff[i]: first
-// row of blk[i] in rows); smax[i] / umax[i] are the largest size / union
-// among the blocks of its level up to and including blk[i].
+// row of blk[i] in rows); smax[i] / umax[i] / emax[i] are the largest size /
+// union / entry span among the blocks of its level up to and including blk[i].
 // What the block kernel needs of one block: rows [c0, c0+s), column union
-// ucol[u0 .. u0+nu), packed inverse at offset off.
+// ucol[u0 .. u0+nu), packed inverse at offset off, and the span [e0, e0+ne)
+// of the L entries of its rows (forward: CSR rows c0..c0+s-1; backward: CSC
+// columns), which holds every off-block entry the block's update reads.
 struct BlkInfo {
   int c0, s, u0, nu;
   int64_t off;
+  int e0, ne;
 };
 
 struct LevelBlocks {
-  std::vector<int> bptr, blk, lptr, rows, rowoff, key, smax, umax;
+  std::vector<int> bptr, blk, lptr, rows, rowoff, key, smax, umax, emax;
 };
 
 // Pattern data of the Schur factor, shared by every frequency of a geometry:
@@ -66,16 +69,22 @@ struct PreconSymbolic {
   // rows reach; backward: the rows below it its columns reach) and the
   // position of every entry's index in its block's union
   std::vector<int> f_up, f_ucol, f_loc, b_up, b_ucol, b_loc;
+  // off-block entries of L (outside the diagonal blocks, the only ones the
+  // solves read): forward packed row by row, fp[r] .. fp[r+1] with column fi
+  // and union position floc; backward packed column by column (bp, bi,
+  // bloc); fsrc / bsrc: CSC positions of the packed entries (value upload)
+  std::vector<int> fp, fi, floc, fsrc, bp, bi, bloc, bsrc;
   // lanes per (row, column) item of each level's kernels, from the mean row
   // length (update: entries outside the block; gemv: entries of the inverse)
   std::vector<int> f_lpr_u, f_lpr_g, b_lpr_u, b_lpr_g;
   std::vector<int64_t> f_upd_nnz;     // update entries of each forward level (0: no update launch)
   DevBuf<char> ws;                    // apply workspace (two [group][node][port] vectors), shared
   PcGroups groups;
-  DevBuf<int> d_perm, d_iperm, Rp, Ri, Cp, Ci, ArTp_col;
-  DevBuf<int> d_sn_ptr, d_sn_of_row, d_row_split, d_col_split, d_f_rows, d_b_rows;
+  bool plan_traced = false;           // VH_TRACE: the launch plan was printed
+  DevBuf<int> d_perm, d_iperm, d_fp, d_fi, d_floc, d_bp, d_bi, d_bloc, ArTp_col;
+  DevBuf<int> d_sn_ptr, d_sn_of_row, d_f_rows, d_b_rows;
   DevBuf<BlkInfo> d_f_info, d_b_info;  // per sorted block position (LevelBlocks order)
-  DevBuf<int> d_f_ucol, d_f_loc, d_b_ucol, d_b_loc;
+  DevBuf<int> d_f_ucol, d_b_ucol;
   DevBuf<int64_t> d_sn_off;
 };
 
@@ -98,10 +107,11 @@ struct Precon {
   std::shared_ptr<PreconSymbolic> sym;
   LdlFactor F;                        // numeric values (Lx, D) of this frequency
   std::vector<double> ainv_h;
-  // device values: L (CSR and CSC), D, A_inv, inverses of the diagonal blocks
+  // device values: L (off-block entries packed per solve direction), D,
+  // A_inv, inverses of the diagonal blocks
   // (row-major lower: Linv; the transpose row-major: LinvT)
-  DevBuf<double> Rx64, Cx64, D64, ainv64, Li64, LiT64;
-  DevBuf<float> Rx32, Cx32, D32, ainv32, Li32, LiT32;
+  DevBuf<double> Fx64, Bx64, D64, ainv64, Li64, LiT64;
+  DevBuf<float> Fx32, Bx32, D32, ainv32, Li32, LiT32;
   std::vector<std::complex<double>> zh;   // impedance per voxel it was prepared with
   DevBuf<double2> z64;                // device copies of zh (the batched matvec reads them per system)
   DevBuf<float2> z32;

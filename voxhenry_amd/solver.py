@@ -10,7 +10,9 @@ VoxHenry_executer.m:84-328:
   L = |Im Z|/(2 pi f) (:327-328).
 
 method 'fp64' runs pgmres with the reference semantics in complex128 on the
-GPU; 'mixed' runs the FP64 flexible GMRES with the FP32 inner solver.
+GPU ('fp64_true' carries the systems it leaves above tolerance on to the true
+residual in FP64 flexible GMRES); 'mixed' runs the FP64 flexible GMRES with
+the FP32 inner solver.
 """
 import time
 
@@ -267,7 +269,7 @@ def workspace_bytes(S, nsys, methods=('mixed', 'fp64'), inner_restart=30):
     largest method's layout, the matvec keeps a pass-X grid per precision
     and one shared B grid (vh_solve_batch_bytes per method)."""
     per = max(S.G.batch_bytes(m, inner_restart=inner_restart) for m in methods)
-    if 'mixed' in methods and 'fp64' in methods:
+    if 'mixed' in methods and ('fp64' in methods or 'fp64_true' in methods):
         Px = S.G.pad[0]
         L, M, N = S.G.dims
         per += 5.0 * Px * M * N * 8     # the FP32 pass-X grid beside the FP64 one
