@@ -287,14 +287,74 @@ def test_superconductor_batched_sweep(ctx):
     freqs = np.logspace(6, 10, 4)
     ref = S.sweep(freqs, method='fp64', tol=1e-12)
     b64 = S.sweep_batched(freqs, method='fp64', tol=1e-10)
+    b64t = S.sweep_batched(freqs, method='fp64_true', tol=1e-10)
     bmx = S.sweep_batched(freqs, method='mixed', tol=1e-10)
-    assert np.all(bmx['stats']['flag'] == 0)
-    for out, t in ((b64, 1e-5), (bmx, 1e-6)):
+    assert np.all(bmx['stats']['flag'] == 0) and np.all(b64t['stats']['flag'] == 0)
+    for out, t in ((b64, 1e-5), (b64t, 1e-6), (bmx, 1e-6)):
         eZ = max(np.linalg.norm(out['Z'][k] - ref['Z'][k]) / np.linalg.norm(ref['Z'][k]) for k in range(len(freqs)))
         dR = np.max(np.abs(out['R'][:, 0, 0] - ref['R'][:, 0, 0]) / np.abs(ref['Z'][:, 0, 0]))
         dL = np.max(np.abs(out['L'][:, 0, 0] - ref['L'][:, 0, 0]) / ref['L'][:, 0, 0])
         say('superconductor err Z %.2e R/|Z| %.2e L %.2e' % (eZ, dR, dL))
         assert eZ < t and dR < t and dL < t
+
+
+def test_fp64_true_continues_flagged_systems(ctx):
+    """Method fp64_true: the pgmres pass, then the systems it leaves above tol
+    (a superconducting strip: pgmres stops on the preconditioned residual,
+    its true residual stays above 1e-8 at about half the frequencies)
+    continue in FP64 FGMRES until the true residual meets tol.  Systems the
+    first pass converged are returned untouched (bit-identical to method
+    fp64); every system ends at or below tol; the continued ones agree with a
+    1e-12 reference to the tolerance's resolution."""
+    import torch
+    import voxhenry_amd as vh
+    from voxhenry_amd import structures, solver
+    say('fp64_true continuation')
+    geo = structures.strip(120, 10, thickness=2)
+    S = solver.Solver(ctx, geo)
+    freqs = np.logspace(6, 10, 8)
+    precs = S.precons_for(freqs)
+    npo = S.nports
+    B = torch.tensor(np.repeat(S.rhs, len(freqs), axis=0), dtype=torch.complex128, device='cuda')
+    jw = np.repeat(1j * 2 * np.pi * freqs * MU0, npo)
+    pidx = np.repeat(np.arange(len(freqs)), npo)
+    X0, s0 = S.G.solve_batch(B, jw, precs, pidx, method='fp64', tol=1e-8)
+    X2, s2 = S.G.solve_batch(B, jw, precs, pidx, method='fp64_true', tol=1e-8)
+    Xt, st = S.G.solve_batch(B, jw, precs, pidx, method='fp64_true', tol=1e-12, maxit=20)
+    say('pgmres pass converged %d/%d; fp64_true %d/%d, max relres %.2e' % (
+        np.sum(s0['flag'] == 0), len(freqs) * npo, np.sum(s2['flag'] == 0), len(freqs) * npo, s2['relres'].max()))
+    assert np.any(s0['flag'] != 0), 'the strip should leave pgmres-semantics systems above tol'
+    assert np.all(s2['flag'] == 0) and np.all(s2['relres'] <= 1e-8)
+    same = np.flatnonzero(s0['flag'] == 0)
+    assert torch.equal(X0[same], X2[same])
+    for k in np.flatnonzero(s0['flag'] != 0):
+        err = (torch.linalg.norm(X2[k] - Xt[k]) / torch.linalg.norm(Xt[k])).item()
+        assert err < 1e-5, (k, err)
+    for P in precs:
+        P.close()
+
+
+def test_mixed_escalation_matches_fp64(ctx):
+    """A strip whose FP32 inner process stalls: the mixed solve hands such
+    systems to FP64 FGMRES; every system converges to tol and Z equals the
+    FP64 solution carried to the same tolerance."""
+    from voxhenry_amd import structures, solver
+    say('mixed escalation')
+    geo = structures.strip(420, 22, thickness=2)
+    S = solver.Solver(ctx, geo)
+    freqs = np.logspace(6, 10, 6)
+    precs = S.precons_for(freqs)
+    pidx = np.arange(len(freqs))
+    ref = S.sweep_batched(freqs, method='fp64_true', tol=1e-12, precons=precs, pidx_of_freq=pidx, maxit=20)
+    mx = S.sweep_batched(freqs, method='mixed', tol=1e-8, precons=precs, pidx_of_freq=pidx)
+    st = mx['stats']
+    say('mixed: converged %d/%d, outer+FP64 steps %.1f, inner %.1f, max relres %.2e' % (
+        np.sum(st['flag'] == 0), len(st['flag']), st['iters'].mean(), st['inner_iters'].mean(), st['relres'].max()))
+    assert np.all(st['flag'] == 0) and np.all(st['relres'] <= 1e-8)
+    eZ = max(np.linalg.norm(mx['Z'][k] - ref['Z'][k]) / np.linalg.norm(ref['Z'][k]) for k in range(len(freqs)))
+    assert eZ < 1e-6, eZ
+    for P in precs:
+        P.close()
 
 
 def test_prepare_many_matches_serial(ctx):
