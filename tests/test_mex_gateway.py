@@ -326,5 +326,9 @@ def test_gateway_solve_batch():
     assert np.all(flagm == 0) and np.all(relm[0] <= 1e-8)
     for k in range(1, 3):
         assert abs(b @ Xm[:, k] - b @ X[:, k]) / abs(b @ X[:, k]) < 1e-6
+    # fp64_true: every column of this bar converges in the pgmres pass, so the
+    # solutions are those of 'fp64' bit for bit
+    Xt, flagt = m('solve_batch', B, jw, pidx, 'fp64_true', tols, nout=2)
+    assert np.all(flagt == 0) and np.array_equal(Xt, X)
     m('reset')
     assert m.locked() == 0 and m.leaked() == 0

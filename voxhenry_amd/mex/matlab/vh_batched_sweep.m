@@ -4,7 +4,8 @@ function [R_jL_mat, Y_mat, flag, relres] = vh_batched_sweep(dx, z_real, z_imag, 
 % impedance does not depend on f; two-fluid conductors go through the
 % per-frequency loop).  The arguments are the executer's own variables at
 % the start of its frequency loop (Ae with the port rows removed, rhs_vect
-% truncated, :181-192); method is 'fp64' (pgmres semantics) or 'mixed'.
+% truncated, :181-192); method is 'fp64' (pgmres semantics), 'fp64_true' (pgmres, then FP64 FGMRES to
+% the true residual for the columns it leaves above tol) or 'mixed'.
 % Tolerances follow the executer's rule (:219-223); Y, Z and R + jL are
 % assembled as at :311-328.
 mu = 4*pi*1e-7;

@@ -14,7 +14,7 @@
  *
  * the batched tier (every port x frequency system in one device solve):
  *
- *   [X,flag,relres,iters,inner] = vh_mex('solve_batch', B, jwmu, pidx, 'fp64' | 'mixed', tol)
+ *   [X,flag,relres,iters,inner] = vh_mex('solve_batch', B, jwmu, pidx, 'fp64' | 'fp64_true' | 'mixed', tol)
  *
  * and the lower-level commands the prepare step is made of:
  *
@@ -478,8 +478,10 @@ static void cmd_gmres(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]
  * is solved at frequency jwmu(pidx(s)) (jwmu: the j*omega*mu of nf
  * frequencies, pidx 1-based) with that frequency's preconditioner, prepared
  * here for the geometry's current impedance; method 'fp64' (pgmres
- * semantics, restart 100, 10 cycles) or 'mixed' (FP64 flexible outer, FP32
- * inner); tol a scalar or one per column.  Outputs are 1 x nsys rows. */
+ * semantics, restart 100, 10 cycles), 'fp64_true' (the same, then the
+ * columns it leaves above tol continue in FP64 FGMRES to the true residual)
+ * or 'mixed' (FP64 flexible outer, restart 8, 4 cycles; FP32 inner, restart
+ * 30); tol a scalar or one per column.  Outputs are 1 x nsys rows. */
 static void cmd_solve_batch(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   nargs(nrhs, 6, 6, nlhs, 5, "solve_batch");
   need_operator();
@@ -494,9 +496,10 @@ static void cmd_solve_batch(int nlhs, mxArray* plhs[], int nrhs, const mxArray* 
   const int64_t nf = (int64_t)mxGetNumberOfElements(prhs[2]);
   if (!is_real_double(prhs[3])) fail("voxhenry:usage", "pidx must be a real double vector");
   char mth[16];
-  if (!mxIsChar(prhs[4]) || mxGetString(prhs[4], mth, sizeof mth) != 0 || (strcmp(mth, "fp64") && strcmp(mth, "mixed")))
-    fail("voxhenry:usage", "method must be 'fp64' or 'mixed'");
-  const int method = strcmp(mth, "mixed") == 0;
+  if (!mxIsChar(prhs[4]) || mxGetString(prhs[4], mth, sizeof mth) != 0 ||
+      (strcmp(mth, "fp64") && strcmp(mth, "mixed") && strcmp(mth, "fp64_true")))
+    fail("voxhenry:usage", "method must be 'fp64', 'fp64_true' or 'mixed'");
+  const int method = strcmp(mth, "mixed") == 0 ? 1 : strcmp(mth, "fp64_true") == 0 ? 2 : 0;
   int64_t ns = 0;
   double* tmp = NULL;
   const double* b = columns(prhs[1], S.n, "B", &ns, &tmp);
@@ -545,7 +548,7 @@ static void cmd_solve_batch(int nlhs, mxArray* plhs[], int nrhs, const mxArray* 
   if (rc == 0) rc = vh_precon_prepare_many(ps, nf, jw, NULL, 0);
   if (rc == 0)
     rc = vh_solve_batch(S.geom, nf, ps, ns, jws, pidx, b, (double*)mxGetComplexDoubles(plhs[0]), 0, 0, method, tmax,
-                        tols, 100, 10, 30, 1e-5, 0.0, flag, rr, its, iits, NULL);
+                        tols, method == 1 ? 8 : 100, method == 1 ? 4 : 10, 30, 1e-5, 0.0, flag, rr, its, iits, NULL);
   if (rc != 0) snprintf(msgbuf, sizeof msgbuf, "%s", vh_last_error());
   for (int64_t f = 0; f < nf; ++f)
     if (ps[f]) vh_precon_destroy(ps[f]);
