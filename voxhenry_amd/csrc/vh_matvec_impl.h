@@ -732,10 +732,12 @@ __global__ __launch_bounds__(256) void k_node_rows(
     const int* __restrict__ arc, const R* __restrict__ arv, int num_node, int64_t num_curr,
     const int* __restrict__ slots, int nrhs) {
   // One thread per node serves up to RB right-hand sides (blockIdx.y selects
-  // the group): the row's indices and values are loaded once for all of them
-  // instead of being streamed again for every vector.  A node touches at
-  // most 6 currents; a longer row takes the looping path.
-  constexpr int RB = 8, D = 8;
+  // the group): the row's indices and values are loaded once for all of
+  // them, and the currents of RC right-hand sides at a time are loaded
+  // before any is used (a wave reads 64 consecutive nodes' currents, which lie
+  // in a few contiguous runs).  A node touches at most 6 currents; a longer
+  // row takes the looping path.
+  constexpr int RB = 8, RC = 4, D = 6;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= num_node) return;
   const int r0 = blockIdx.y * RB, r1 = min(nrhs, r0 + RB);
@@ -748,17 +750,30 @@ __global__ __launch_bounds__(256) void k_node_rows(
       kk[u] = p0 + u < p1 ? arc[p0 + u] : 0;
       vv[u] = p0 + u < p1 ? arv[p0 + u] : R(0);
     }
-    for (int q = r0; q < r1; ++q) {
-      const int rhs = slots ? slots[q] : q;
-      const cplx<R>* xs = x + (int64_t)rhs * ldx;
-      cplx<R> s = czero<R>();
 #pragma unroll
-      for (int u = 0; u < D; ++u)
-        if (p0 + u < p1) {
-          const cplx<R> v = xs[kk[u]];
-          s.x += vv[u] * v.x; s.y += vv[u] * v.y;
+    for (int q0 = 0; q0 < RB; q0 += RC) {
+      cplx<R> a[RC][D];
+      int rs[RC];
+#pragma unroll
+      for (int c = 0; c < RC; ++c) {
+        const int q = r0 + q0 + c;
+        rs[c] = q < r1 ? (slots ? slots[q] : q) : -1;
+        if (rs[c] >= 0) {
+          const cplx<R>* xs = x + (int64_t)rs[c] * ldx;
+#pragma unroll
+          for (int u = 0; u < D; ++u)
+            if (p0 + u < p1) a[c][u] = xs[kk[u]];
         }
-      y[(int64_t)rhs * ldy + num_curr + i] = s;
+      }
+#pragma unroll
+      for (int c = 0; c < RC; ++c) {
+        if (rs[c] < 0) continue;
+        cplx<R> s = czero<R>();
+#pragma unroll
+        for (int u = 0; u < D; ++u)
+          if (p0 + u < p1) { s.x += vv[u] * a[c][u].x; s.y += vv[u] * a[c][u].y; }
+        y[(int64_t)rs[c] * ldy + num_curr + i] = s;
+      }
     }
   } else {
     for (int q = r0; q < r1; ++q) {
