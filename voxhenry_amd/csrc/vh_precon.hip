@@ -653,7 +653,15 @@ static void block_unions(const std::vector<int>& sn_ptr, const int* idx, int64_t
                    "internal: block union index");
 }
 
-// Per level, along the sorted blocks: the sort key max(size, union) and the
+// Sort key of a block for the block kernel's tiers: the larger of its size,
+// its column union and an eighth of its off-block entries.  A block whose
+// entries are many for its workgroup (the top separators of a compact
+// conductor: 1e5 entries in one block) thereby falls to the row kernels,
+// which spread its rows over many workgroups (numex1: one such block took
+// 1.5 ms in one workgroup per group).
+static int block_key(int sz, int un, int ne) { return std::max(std::max(sz, un), (ne + 7) / 8); }
+
+// Per level, along the sorted blocks: the sort key (block_key) and the
 // prefix maxima of sizes, unions and entry spans (the block kernel sizes its
 // shared memory from them).  ptr: the row (forward, CSR) or column
 // (backward, CSC) pointers of L.
@@ -669,7 +677,7 @@ static void level_extents(const std::vector<int>& sn_ptr, const std::vector<int>
     for (int i = B.bptr[l]; i < B.bptr[l + 1]; ++i) {
       const int k = B.blk[i];
       const int sz = sn_ptr[k + 1] - sn_ptr[k], un = up[k + 1] - up[k];
-      B.key[i] = std::max(sz, un);
+      B.key[i] = block_key(sz, un, ptr[sn_ptr[k + 1]] - ptr[sn_ptr[k]]);
       ms = std::max(ms, sz);
       mu = std::max(mu, un);
       me = std::max(me, ptr[sn_ptr[k + 1]] - ptr[sn_ptr[k]]);
@@ -699,9 +707,10 @@ static std::vector<BlkInfo> block_info(const LevelBlocks& B, const std::vector<i
   return v;
 }
 
-static std::vector<int> block_keys(const std::vector<int>& sn_ptr, const std::vector<int>& up) {
+static std::vector<int> block_keys(const std::vector<int>& sn_ptr, const std::vector<int>& up, const std::vector<int>& ptr) {
   std::vector<int> key(sn_ptr.size() - 1);
-  for (size_t k = 0; k + 1 < sn_ptr.size(); ++k) key[k] = std::max(sn_ptr[k + 1] - sn_ptr[k], up[k + 1] - up[k]);
+  for (size_t k = 0; k + 1 < sn_ptr.size(); ++k)
+    key[k] = block_key(sn_ptr[k + 1] - sn_ptr[k], up[k + 1] - up[k], ptr[sn_ptr[k + 1]] - ptr[sn_ptr[k]]);
   return key;
 }
 
@@ -771,10 +780,6 @@ static void build_levels(Geometry& G, const LdlFactor& F, PreconSymbolic& Y) {
                Y.f_up, Y.f_ucol, Y.f_loc);
   block_unions(Y.sn_ptr, F.Li.data(), F.Lp[n], [&](int r) { return Y.col_split[r]; },
                [&](int r) { return F.Lp[r + 1]; }, Y.b_up, Y.b_ucol, Y.b_loc);
-  level_blocks(lf, Y.sn_ptr, block_keys(Y.sn_ptr, Y.f_up), Y.fl);
-  level_blocks(lb, Y.sn_ptr, block_keys(Y.sn_ptr, Y.b_up), Y.bl);
-  Y.f_lptr = Y.fl.lptr; Y.f_rows = Y.fl.rows;
-  Y.b_lptr = Y.bl.lptr; Y.b_rows = Y.bl.rows;
   // off-block entries packed contiguously: forward row by row (CSR order),
   // backward column by column (CSC order); the values of every frequency
   // are stored in these orders (fsrc / bsrc: their CSC positions)
@@ -799,6 +804,10 @@ static void build_levels(Geometry& G, const LdlFactor& F, PreconSymbolic& Y) {
   std::vector<int>().swap(Y.f_loc);     // (only the packed copies are used from here on)
   std::vector<int>().swap(Y.b_loc);
   std::vector<int>().swap(Y.csc2csr);
+  level_blocks(lf, Y.sn_ptr, block_keys(Y.sn_ptr, Y.f_up, Y.fp), Y.fl);
+  level_blocks(lb, Y.sn_ptr, block_keys(Y.sn_ptr, Y.b_up, Y.bp), Y.bl);
+  Y.f_lptr = Y.fl.lptr; Y.f_rows = Y.fl.rows;
+  Y.b_lptr = Y.bl.lptr; Y.b_rows = Y.bl.rows;
   level_extents(Y.sn_ptr, Y.f_up, Y.fp, Y.fl);
   level_extents(Y.sn_ptr, Y.b_up, Y.bp, Y.bl);
   if (trace_on()) {
@@ -1100,7 +1109,9 @@ static dim3 lane_grid(int64_t nitems, int lanes) { return dim3((unsigned)ceil_di
 // Groups of the columns: the slots of each preconditioner in chunks of npg.
 // npg (1, 2, 4 or 8) minimises an estimate of the level kernels' traffic:
 // per group and entry of L, its value and index (12 bytes) and the line of
-// the work vector (at least 64 bytes, 16 per port).
+// the work vector (at least 64 bytes, 16 per port); on a tie the smaller npg
+// (one system per preconditioner gets groups of one, not of four with three
+// empty slots whose lanes idle).
 static int build_groups(const int* pidx, const std::vector<int>& slots, std::vector<int>& key) {
   std::vector<int> order;            // preconditioners in order of first use
   std::vector<std::vector<int>> cols;
@@ -1119,7 +1130,7 @@ static int build_groups(const int* pidx, const std::vector<int>& slots, std::vec
     int64_t ng = 0;
     for (auto& c : cols) ng += ceil_div((int64_t)c.size(), (int64_t)npg);
     const double cost = (double)ng * (12.0 + std::max(64.0, 16.0 * npg));
-    if (npg == 1 || cost <= best_cost) { best = npg; best_cost = cost; }
+    if (npg == 1 || cost < best_cost) { best = npg; best_cost = cost; }
   }
   // key: npg, ng, gtab[ng], gslot[ng * npg]
   int ng = 0;
