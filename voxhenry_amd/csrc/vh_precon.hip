@@ -654,12 +654,15 @@ static void block_unions(const std::vector<int>& sn_ptr, const int* idx, int64_t
 }
 
 // Sort key of a block for the block kernel's tiers: the larger of its size,
-// its column union and an eighth of its off-block entries.  A block whose
+// its column union and a quarter of its off-block entries (a half, an
+// eighth, a sixteenth measured: numex1 sweep 0.074 / 0.081 / 0.089 / 0.089 s
+// mixed, bus step 0.616 / 0.599 / 0.600 / 0.607 s; FP64 0.632 / 0.620 /
+// 0.634 / 0.646 s).  A block whose
 // entries are many for its workgroup (the top separators of a compact
 // conductor: 1e5 entries in one block) thereby falls to the row kernels,
 // which spread its rows over many workgroups (numex1: one such block took
 // 1.5 ms in one workgroup per group).
-static int block_key(int sz, int un, int ne) { return std::max(std::max(sz, un), (ne + 7) / 8); }
+static int block_key(int sz, int un, int ne) { return std::max(std::max(sz, un), (ne + 3) / 4); }
 
 // Per level, along the sorted blocks: the sort key (block_key) and the
 // prefix maxima of sizes, unions and entry spans (the block kernel sizes its
