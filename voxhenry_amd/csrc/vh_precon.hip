@@ -144,6 +144,34 @@ template <int NPG, int LPR> struct PLanes {
   }
 };
 
+// sum over p = p0, p0 + LPR, ... < p1 of val[p] * wg[idx[p] * NPG]: the entries
+// of four strides are loaded before their lines, so a lane has four
+// independent line loads in flight (rows of the top separators hold
+// thousands of entries).
+template <typename WT, int NPG, int LPR, typename R>
+__device__ __forceinline__ double2 strided_dot(const R* __restrict__ val, const int* __restrict__ idx,
+                                               const cplx<WT>* __restrict__ wg, int p0, int p1) {
+  double2 s = make_double2(0.0, 0.0);
+  int p = p0;
+  for (; p + 3 * LPR < p1; p += 4 * LPR) {
+    int i[4];
+    double l[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { i[u] = idx[p + u * LPR]; l[u] = (double)val[p + u * LPR]; }
+    cplx<WT> v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = wg[(int64_t)i[u] * NPG];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { s.x += l[u] * v[u].x; s.y += l[u] * v[u].y; }
+  }
+  for (; p < p1; p += LPR) {
+    const double l = (double)val[p];
+    const cplx<WT> v = wg[(int64_t)idx[p] * NPG];
+    s.x += l * v.x; s.y += l * v.y;
+  }
+  return s;
+}
+
 // Supernodal solves with inverted diagonal blocks.  Block k owns rows and
 // columns [c0, c1).  Forward (L y = b), blocks in increasing level order:
 //   t_r = b_r - sum_{j < c0} L(r,j) y_j   (k_sn_update_fwd, rows of the level)
@@ -164,12 +192,7 @@ __global__ __launch_bounds__(256) void k_sn_update_fwd(const int* __restrict__ r
   const int r = rows[q];
   const R* fx = tab[gr.gtab[g]].Fx;
   const cplx<WT>* wg = w2 + (int64_t)g * nn * NPG + ln.j;
-  double2 s; s.x = 0; s.y = 0;
-  for (int p = fp[r] + ln.part; p < fp[r + 1]; p += LPR) {
-    const double l = fx[p];
-    const cplx<WT> v = wg[(int64_t)fi[p] * NPG];
-    s.x += l * v.x; s.y += l * v.y;
-  }
+  double2 s = strided_dot<WT, NPG, LPR>(fx, fi, wg, fp[r] + ln.part, fp[r + 1]);
   s = ln.sum(s);
   if (ln.part == 0) {
     cplx<WT>& o = w[((int64_t)g * nn + r) * NPG + ln.j];
@@ -218,12 +241,7 @@ __global__ __launch_bounds__(256) void k_sn_update_bwd(const int* __restrict__ r
   const PcTab<R>& T = tab[gr.gtab[g]];
   const R* bx = T.Bx;
   const cplx<WT>* wg = w + (int64_t)g * nn * NPG + ln.j;
-  double2 s; s.x = 0; s.y = 0;
-  for (int p = bp[r] + ln.part; p < bp[r + 1]; p += LPR) {
-    const double l = bx[p];
-    const cplx<WT> v = wg[(int64_t)bi[p] * NPG];
-    s.x += l * v.x; s.y += l * v.y;
-  }
+  double2 s = strided_dot<WT, NPG, LPR>(bx, bi, wg, bp[r] + ln.part, bp[r + 1]);
   s = ln.sum(s);
   if (ln.part == 0) {
     cplx<WT>& o = w2[((int64_t)g * nn + r) * NPG + ln.j];
