@@ -409,7 +409,25 @@ __global__ __launch_bounds__(NT) void k_blk_solve(const BlkInfo* __restrict__ in
     for (int i = rs; i < s; i += rpp) {
       const int lo = rng[2 * i], hi = rng[2 * i + 1];
       double2 a = make_double2(0.0, 0.0);
-      for (int p = lo + part; p < hi; p += lpr) {
+      int p = lo + part;
+      // four strides at a time: their entries (from LDS or memory) are all
+      // requested before any is used
+      for (; p + 3 * lpr < hi; p += 4 * lpr) {
+        double l[4];
+        int c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int q = p + u * lpr;
+          l[u] = ent_lds ? (double)sval[q] : (double)val[e0 + q];
+          c[u] = ent_lds ? sloc[q] : loc[e0 + q];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const CW v = stg[c[u] * NPG + j];
+          a.x += l[u] * v.x; a.y += l[u] * v.y;
+        }
+      }
+      for (; p < hi; p += lpr) {
         const double l = ent_lds ? (double)sval[p] : (double)val[e0 + p];
         const CW v = stg[(ent_lds ? sloc[p] : loc[e0 + p]) * NPG + j];
         a.x += l * v.x; a.y += l * v.y;
@@ -434,7 +452,18 @@ __global__ __launch_bounds__(NT) void k_blk_solve(const BlkInfo* __restrict__ in
       const int len = FWD ? i + 1 : s - i;
       const CW* tv = FWD ? tt + j : tt + i * NPG + j;
       double2 a = make_double2(0.0, 0.0);
-      for (int jj = part; jj < len; jj += lpr) {
+      int jj = part;
+      for (; jj + 3 * lpr < len; jj += 4 * lpr) {
+        double l[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) l[u] = (double)row[jj + u * lpr];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const CW v = tv[(jj + u * lpr) * NPG];
+          a.x += l[u] * v.x; a.y += l[u] * v.y;
+        }
+      }
+      for (; jj < len; jj += lpr) {
         const double l = (double)row[jj];
         const CW v = tv[jj * NPG];
         a.x += l * v.x; a.y += l * v.y;
