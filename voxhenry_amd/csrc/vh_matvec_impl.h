@@ -479,26 +479,84 @@ __global__ __launch_bounds__(NT, OCC) void k_pass_z1_pair(cplx<R>* __restrict__ 
 // compile-time zero / unused.  Same data flow and results as k_pass_z.
 // ---------------------------------------------------------------------------
 template <typename R, int DIR, class P, int PZ, int TPP, int T, int S, int LAST> struct MidStages {
-  __device__ __forceinline__ static void run(const cplx<R>* __restrict__ tw, int jl, cplx<R>* const (&bufs)[2], int t) {
+  __device__ __forceinline__ static void run(const cplx<R>* __restrict__ tw, int jl, cplx<R>* b0, cplx<R>* b1, int t) {
     if constexpr (S < LAST) {
-      const cplx<R>* in = bufs[(S - 1) & 1];
-      cplx<R>* out = bufs[S & 1];
+      const cplx<R>* in = ((S - 1) & 1) ? b1 : b0;
+      cplx<R>* out = (S & 1) ? b1 : b0;
       auto ld = [&](int i) { return in[i * T + t]; };
       auto st = [&](int o, cplx<R> v) { out[o * T + t] = v; };
       sstage<R, DIR, PZ, P::rad(S), P::ns(S), TPP>(tw + P::tw_off(S), jl, ld, st);
       __syncthreads();
-      MidStages<R, DIR, P, PZ, TPP, T, S + 1, LAST>::run(tw, jl, bufs, t);
+      MidStages<R, DIR, P, PZ, TPP, T, S + 1, LAST>::run(tw, jl, b0, b1, t);
     }
   }
 };
 
-template <typename R> constexpr int e_p2() { return sizeof(R) == 8 ? 4 : 8; }
+// The twiddles of a stage depend only on the thread, so the optimiser hoists
+// their loads out of the five-component loops and keeps every stage's table
+// in registers (54 VGPRs in complex64, 72 in complex128), which costs a wave
+// slot per SIMD or spills.  Offsetting the table pointer by a zero the
+// optimiser cannot see through, once per component, makes each iteration load
+// them where they are used (from L1).
+template <typename T> __device__ __forceinline__ const T* opaque_ptr(const T* p) {
+  int z = 0;
+  asm volatile("" : "+s"(z));
+  return p + z;
+}
 
-template <typename R, int PZ, int NT, bool HALF>
-__global__ __launch_bounds__(NT, (sizeof(R) == 4 && HALF) ? 3 : 1) void k_pass_z_p2(cplx<R>* __restrict__ B, GridDims g, const cplx<R>* __restrict__ twf,
+// Rows of the B grid of one right-hand side through a buffer resource: the
+// lane's byte offset (its (kx, ky) column and first z row) is one VGPR and
+// each row offset is a uniform SGPR, where 64-bit flat addresses of the five
+// components' rows would hold 40 VGPRs for the whole kernel.  The host checks
+// that the right-hand side's five component grids span less than 2^31 bytes.
+constexpr uint32_t kBufWord3 = 0x00020000u;   // raw buffer, 32-bit data format (CDNA)
+template <typename R> struct ZRows {
+  __amdgpu_buffer_rsrc_t rs;
+  uint32_t vo;   // lane byte offset
+  uint32_t pb;   // bytes per z row (plane)
+  __device__ __forceinline__ ZRows(cplx<R>* rhs_base, uint32_t bytes, uint32_t lane_elem, uint32_t plane_elems) {
+    rs = __builtin_amdgcn_make_buffer_rsrc(rhs_base, 0, (int)bytes, (int)kBufWord3);
+    vo = lane_elem * (uint32_t)sizeof(cplx<R>);
+    pb = plane_elems * (uint32_t)sizeof(cplx<R>);
+  }
+  __device__ __forceinline__ cplx<R> ld(int row) const {
+    if constexpr (sizeof(R) == 4)
+      return __builtin_bit_cast(cplx<R>, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, (uint32_t)row * pb, 0));
+    else
+      return __builtin_bit_cast(cplx<R>, __builtin_amdgcn_raw_buffer_load_b128(rs, vo, (uint32_t)row * pb, 0));
+  }
+  __device__ __forceinline__ void st(int row, cplx<R> v) const {
+    if constexpr (sizeof(R) == 4)
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(rs, 0, 0, 0)), v),
+                                            rs, vo, (uint32_t)row * pb, 0);
+    else
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b128(rs, 0, 0, 0)), v),
+                                             rs, vo, (uint32_t)row * pb, 0);
+  }
+};
+
+#ifndef VH_ZW32E8
+#define VH_ZW32E8 4
+#endif
+#ifndef VH_ZW32E4
+#define VH_ZW32E4 3
+#endif
+#ifndef VH_ZW64
+#define VH_ZW64 2
+#endif
+#ifndef VH_ZALL32E8
+#define VH_ZALL32E8 0
+#endif
+#ifndef VH_ZALL32E4
+#define VH_ZALL32E4 1
+#endif
+template <typename R, int E> constexpr int zp2_waves() { return sizeof(R) == 8 ? VH_ZW64 : (E == 8 ? VH_ZW32E8 : VH_ZW32E4); }
+template <typename R, int E, int NT> constexpr int zp2_minb() { return zp2_waves<R, E>() * 256 / NT > 0 ? zp2_waves<R, E>() * 256 / NT : 1; }
+
+template <typename R, int PZ, int NT, int E, bool HALF>
+__global__ __launch_bounds__(NT, (zp2_minb<R, E, NT>())) void k_pass_z_p2(cplx<R>* __restrict__ B, GridDims g, const cplx<R>* __restrict__ twf,
                                                   const cplx<R>* __restrict__ twi, TensorRef<R> tens) {
   using C = cplx<R>;
-  constexpr int E = e_p2<R>();
   constexpr int TPP = PZ / E, T = NT / TPP;
   static_assert(T >= 1 && (T & (T - 1)) == 0, "pencils per workgroup must be a power of two");
   using PF = P2Plan<PZ, E, false>;
@@ -511,7 +569,8 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && HALF) ? 3 : 1) void k_pass_z
   constexpr int BML = SStage<PZ, RL, TPP>::BM, NBL = SStage<PZ, RL, TPP>::NB;
   static_assert(PI::rad(0) == RL, "inverse plan starts with the last forward radix");
   extern __shared__ __align__(16) unsigned char smem[];
-  C* const bufs[2] = {(C*)smem, (C*)smem + T * PZ};
+  C* const buf0 = (C*)smem;
+  C* const buf1 = (C*)smem + T * PZ;
   const int tid = threadIdx.x;
   const int t = tid & (T - 1), jl = tid >> clog2(T);
   const int Px = g.Px, Py = g.Py, N = g.N;
@@ -529,16 +588,16 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && HALF) ? 3 : 1) void k_pass_z
   const int HxHy = g.Hx * g.Hy;
   const int rhs = blockIdx.y;
   const int64_t plane = (int64_t)Px * Py;
-  C* base = B + (int64_t)rhs * 5 * N * plane + (int64_t)ky * Px + kx;
+  const ZRows<R> zr(B + (int64_t)rhs * 5 * N * plane, (uint32_t)(5 * N * plane * (int64_t)sizeof(C)),
+                    (uint32_t)(ky * Px + kx + (int64_t)jl * plane), (uint32_t)plane);
   auto gload = [&](int c, C (&p)[BM0][R0]) {
-    const C* bc = base + (int64_t)c * N * plane;
 #pragma unroll
     for (int m = 0; m < BM0; ++m)
 #pragma unroll
       for (int r = 0; r < R0; ++r) {
-        const int i = jl + m * TPP + r * NB0;
-        if constexpr (HALF) p[m][r] = (r < R0 / 2 && valid) ? bc[(int64_t)i * plane] : czero<R>();   // i < Pz/2 iff r < R0/2
-        else p[m][r] = (valid && i < N) ? bc[(int64_t)i * plane] : czero<R>();
+        const int iu = m * TPP + r * NB0;   // row i = jl + iu (jl is in the lane offset)
+        if constexpr (HALF) p[m][r] = (r < R0 / 2 && valid) ? zr.ld(c * N + iu) : czero<R>();   // i < Pz/2 iff r < R0/2
+        else p[m][r] = (valid && jl + iu < N) ? zr.ld(c * N + iu) : czero<R>();
       }
   };
   C held[5][BML][RL];
@@ -548,7 +607,7 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && HALF) ? 3 : 1) void k_pass_z
   // input registers die as its spectrum forms (the upper half of each input
   // is the constant zero padding and takes no registers).  Otherwise (FP64,
   // twice the bytes, or a full-length input) one component is loaded ahead.
-  constexpr bool ALL = sizeof(R) == 4 && HALF;
+  constexpr bool ALL = sizeof(R) == 4 && HALF && (E == 8 ? VH_ZALL32E8 : VH_ZALL32E4);
   C in[ALL ? 5 : 2][BM0][R0];
   if constexpr (ALL) {
 #pragma unroll
@@ -559,17 +618,18 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && HALF) ? 3 : 1) void k_pass_z
 #pragma unroll
   for (int c = 0; c < 5; ++c) {
     C (&cur)[BM0][R0] = in[ALL ? c : (c & 1)];
+    const C* const twc = opaque_ptr(twf);
     if constexpr (!ALL) {
       if (c < 4) gload(c + 1, in[(c + 1) & 1]);
     }
     {
-      auto st = [&](int o, C val) { bufs[0][o * T + t] = val; };
-      sstage_regs<R, -1, PZ, R0, 1, TPP>(twf, jl, cur, st);
+      auto st = [&](int o, C val) { buf0[o * T + t] = val; };
+      sstage_regs<R, -1, PZ, R0, 1, TPP>(twc, jl, cur, st);
     }
     __syncthreads();
-    MidStages<R, -1, PF, PZ, TPP, T, 1, S - 1>::run(twf, jl, bufs, t);
+    MidStages<R, -1, PF, PZ, TPP, T, 1, S - 1>::run(twc, jl, buf0, buf1, t);
     {
-      const C* lin = bufs[(S - 2) & 1];
+      const C* lin = (((S - 2) & 1) ? buf1 : buf0);
 #pragma unroll
       for (int m = 0; m < BML; ++m)
 #pragma unroll
@@ -577,7 +637,7 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && HALF) ? 3 : 1) void k_pass_z
       // last stage: Ns * RL = Pz, so j < Ns and j mod Ns = j
 #pragma unroll
       for (int m = 0; m < BML; ++m)
-        butterfly<R, -1, RL>(held[c][m], twf + PF::tw_off(S - 1) + (jl + m * TPP) * (RL - 1), NSL > 1);
+        butterfly<R, -1, RL>(held[c][m], twc + PF::tw_off(S - 1) + (jl + m * TPP) * (RL - 1), NSL > 1);
     }
     __syncthreads();
   }
@@ -600,16 +660,16 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && HALF) ? 3 : 1) void k_pass_z
   constexpr int RI = PI::rad(S - 1), NSI = PI::ns(S - 1);
 #pragma unroll
   for (int c = 0; c < 5; ++c) {
+    const C* const tic = opaque_ptr(twi);
     {
-      auto st = [&](int o, C val) { bufs[0][o * T + t] = val; };
-      sstage_regs<R, +1, PZ, RL, 1, TPP>(twi, jl, held[c], st);
+      auto st = [&](int o, C val) { buf0[o * T + t] = val; };
+      sstage_regs<R, +1, PZ, RL, 1, TPP>(tic, jl, held[c], st);
     }
     __syncthreads();
-    MidStages<R, +1, PI, PZ, TPP, T, 1, S - 1>::run(twi, jl, bufs, t);
+    MidStages<R, +1, PI, PZ, TPP, T, 1, S - 1>::run(tic, jl, buf0, buf1, t);
     {
       // last inverse stage: Ns * RI = Pz, so output o = j + r*Ns with j < Ns
-      const C* in = bufs[(S - 2) & 1];
-      C* bc = base + (int64_t)c * N * plane;
+      const C* in = (((S - 2) & 1) ? buf1 : buf0);
       constexpr int BMI = SStage<PZ, RI, TPP>::BM, NBI = SStage<PZ, RI, TPP>::NB;
       static_assert(NBI == NSI, "last stage");
 #pragma unroll
@@ -618,12 +678,12 @@ __global__ __launch_bounds__(NT, (sizeof(R) == 4 && HALF) ? 3 : 1) void k_pass_z
         C w[RI];
 #pragma unroll
         for (int r = 0; r < RI; ++r) w[r] = in[(j + r * NBI) * T + t];
-        butterfly<R, +1, RI>(w, twi + PI::tw_off(S - 1) + j * (RI - 1), NSI > 1);
+        butterfly<R, +1, RI>(w, tic + PI::tw_off(S - 1) + j * (RI - 1), NSI > 1);
 #pragma unroll
         for (int r = 0; r < RI; ++r) {
-          const int o = j + r * NSI;
-          if constexpr (HALF) { if (r < RI / 2 && valid) bc[(int64_t)o * plane] = w[r]; }   // o < Pz/2 iff r < RI/2
-          else { if (valid && o < N) bc[(int64_t)o * plane] = w[r]; }
+          const int ou = m * TPP + r * NSI;   // row o = jl + ou
+          if constexpr (HALF) { if (r < RI / 2 && valid) zr.st(c * N + ou, w[r]); }   // o < Pz/2 iff r < RI/2
+          else { if (valid && jl + ou < N) zr.st(c * N + ou, w[r]); }
         }
       }
     }
@@ -925,21 +985,28 @@ static bool launch_z1(int nrhs, hipStream_t st, cplx<R>* B, const GridDims& g, T
 
 // Power-of-two Pz in [16, 512] with the precision's default element bound:
 // the compile-time kernel (returns false when it does not apply).
-template <typename R, int NT>
+template <typename R, int NT, int E>
 static bool launch_z_p2(int pz, int nrhs, hipStream_t st, cplx<R>* B, const GridDims& g, const Plan2& pf,
                         const Plan2& pi, TensorRef<R> tr) {
+  // the kernel addresses one right-hand side's five component grids through
+  // a buffer resource with 32-bit offsets
+  if ((int64_t)5 * g.N * g.Px * g.Py * (int64_t)sizeof(cplx<R>) >= (int64_t(1) << 31)) return false;
   auto go = [&](auto pzc) {
     constexpr int PZ = decltype(pzc)::value;
-    constexpr int TPP = PZ / e_p2<R>(), T = NT / TPP;
-    const dim3 grid((unsigned)(ceil_div(g.Px, T) * g.Py), nrhs);
-    const size_t lds = 2 * (size_t)T * PZ * sizeof(cplx<R>);
-    if (2 * g.N == PZ)
-      hipLaunchKernelGGL((k_pass_z_p2<R, PZ, NT, true>), grid, dim3(NT), lds, st, B, g, (const cplx<R>*)pf.tw,
-                         (const cplx<R>*)pi.tw, tr);
-    else
-      hipLaunchKernelGGL((k_pass_z_p2<R, PZ, NT, false>), grid, dim3(NT), lds, st, B, g, (const cplx<R>*)pf.tw,
-                         (const cplx<R>*)pi.tw, tr);
-    return true;
+    constexpr int TPP = PZ / E, T = NT / TPP;
+    if constexpr (T < 1) {
+      return false;
+    } else {
+      const dim3 grid((unsigned)(ceil_div(g.Px, T) * g.Py), nrhs);
+      const size_t lds = 2 * (size_t)T * PZ * sizeof(cplx<R>);
+      if (2 * g.N == PZ)
+        hipLaunchKernelGGL((k_pass_z_p2<R, PZ, NT, E, true>), grid, dim3(NT), lds, st, B, g, (const cplx<R>*)pf.tw,
+                           (const cplx<R>*)pi.tw, tr);
+      else
+        hipLaunchKernelGGL((k_pass_z_p2<R, PZ, NT, E, false>), grid, dim3(NT), lds, st, B, g, (const cplx<R>*)pf.tw,
+                           (const cplx<R>*)pi.tw, tr);
+      return true;
+    }
   };
   switch (pz) {
     case 16: return go(std::integral_constant<int, 16>());
@@ -990,9 +1057,14 @@ void pass_z(Geometry& G, int nrhs, cplx<R>* B) {
     const size_t lds = pzf.nstage > 1 ? (size_t)Tz * g.Pz * elem : 0;   // one stage: registers only
     const int ltz = log2i(tz);
     bool done = launch_z1<R>(nrhs, st, B, g, tr);
-    if (!done && EZ == e_p2<R>() && G.use_p2z) {
-      if (NTZ == 512) done = launch_z_p2<R, 512>(g.Pz, nrhs, st, B, g, pzf, pzi, tr);
-      else done = launch_z_p2<R, 256>(g.Pz, nrhs, st, B, g, pzf, pzi, tr);
+    if (!done && G.use_p2z) {
+      if (EZ == 4)
+        done = NTZ == 512 ? launch_z_p2<R, 512, 4>(g.Pz, nrhs, st, B, g, pzf, pzi, tr)
+                          : launch_z_p2<R, 256, 4>(g.Pz, nrhs, st, B, g, pzf, pzi, tr);
+      else if constexpr (sizeof(R) == 4)
+        if (EZ == 8)
+          done = NTZ == 512 ? launch_z_p2<R, 512, 8>(g.Pz, nrhs, st, B, g, pzf, pzi, tr)
+                            : launch_z_p2<R, 256, 8>(g.Pz, nrhs, st, B, g, pzf, pzi, tr);
     }
     if (!done)
       launch_z<R>(zsh, G.fz.last_radix(rmax_for(EZ)), grid, lds, st, B, ltz, g, pzf, pzi, tr);
