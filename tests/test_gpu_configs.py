@@ -346,6 +346,38 @@ def test_uneven_family_small_vs_oracle(ctx, kind):
     S.G.close()
 
 
+def test_uneven_strip_every_system_at_tolerance(ctx):
+    """Geometry 47 of the seeded batch (two-fluid strip 464 x 23 x 2, two
+    split terminals), 16 frequencies, mixed and fp64_true: every system ends
+    with its true residual at or below its tolerance.  (One of its systems
+    once ended at 1.0001e-8: the FGMRES estimate met 1e-8 where the
+    recomputed residual did not; such a system now restarts from its true
+    residual.)"""
+    import torch
+    from voxhenry_amd import structures, solver
+    freqs = np.logspace(6, 10, 16)
+    st = next(g for i, (k, g) in enumerate(structures.uneven_batch(0, 96)) if i == 47)
+    assert st['sigma'].shape == (464, 23, 2) and len(st['ports_P']) == 2
+    say('strip 47', st['sigma'].shape)
+    S = solver.Solver(ctx, st)
+    precs = S.precons_for(freqs)
+    npo = S.nports
+    B = torch.tensor(np.repeat(S.rhs[None], len(freqs), axis=0).reshape(-1, S.G.n), dtype=torch.complex128,
+                     device='cuda')
+    jw = np.repeat(1j * 2 * np.pi * freqs * MU0, npo)
+    pidx = np.repeat(np.arange(len(freqs)), npo)
+    tols = np.repeat([solver.reference_tol(f) for f in freqs], npo)
+    for method in ('mixed', 'fp64_true'):
+        X, stt = S.G.solve_batch(B, jw, precs, pidx, method=method, tol=tols)
+        say('  %s: max relres / tol %.4f' % (method, np.max(stt['relres'] / tols)))
+        assert np.all(stt['flag'] == 0) and np.all(stt['relres'] <= tols), (method, np.max(stt['relres'] / tols))
+        del X
+    del B
+    for P in precs:
+        P.close()
+    S.G.close()
+    torch.cuda.empty_cache()
+
 @pytest.mark.parametrize('kind', ['bar', 'wire', 'loop', 'strip'])
 def test_uneven_family_full_size(ctx, kind):
     """The first geometry of a family in the seeded 96-geometry batch at full

@@ -865,6 +865,7 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
           vh::SolveOpts o3;
           o3.restart = rc; o3.maxit = 40; o3.tol = tol; o3.final_residual = true; o3.flexible = true;
           o3.x0 = true;
+          o3.true_restart = true;
           if (tols) o3.tol_slot = &tolc;
           std::vector<vh::SysStats> cs3;
           vh::gmres_batch<double>(G.ctx, n, A64, &M64, bc, xc, again, o3, K.w64, cs3);
@@ -902,6 +903,7 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
           vh::SolveOpts o3;
           o3.restart = rc; o3.maxit = 40; o3.tol = tol; o3.final_residual = true; o3.flexible = true;
           o3.x0 = true;
+          o3.true_restart = true;
           if (tols) o3.tol_slot = &tolc;
           std::vector<vh::SysStats> cs3;
           vh::gmres_batch<double>(G.ctx, n, A64, &M64, bc, xc, last, o3, K.w64, cs3);

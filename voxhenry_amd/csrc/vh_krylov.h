@@ -96,6 +96,12 @@ struct SolveOpts {
   // ends takes the update of the steps done so far and leaves the iteration
   // (its flag then follows from its residual)
   const std::vector<char>* stop_slot = nullptr;
+  // flexible solves only: a system whose true residual (final_residual) is
+  // above its tolerance although the Arnoldi estimate met it restarts from
+  // that residual while outer iterations remain (the reference's fpgmres
+  // reports such a system with flag 1; the mixed solver and the FP64
+  // continuations, which count only systems at their tolerance, continue)
+  bool true_restart = false;
 };
 
 // Krylov workspace for nslot systems of length ld with restart m.  The large

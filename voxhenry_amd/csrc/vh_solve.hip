@@ -205,111 +205,129 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
     if (restart_r) ops.combine(W.V, vstr, W.r, k + 1, nl, W.aux_slots.p, W.rcoef.p, 0.0, 1.0, nullptr);  // :153
   };
 
-  while (!act.empty()) {
-    for (int s : act) outit[s]++;
-    int nact = (int)act.size();
-    upload(act, W.slots_dev.p);
-    // V(:,1) = r/||r||  (pgmres_iter.m:55); g = ||r|| e1
-    ops.norm2(W.r, nact, W.slots_dev.p, W.nrm.p);
-    ops.scale(W.r, W.V, nact, W.slots_dev.p, W.nrm.p);
-    ls_reset(ctx, ls, W.slots_dev.p, nact, W.nrm.p);
-    int k = 0;
-    while (k < m && !act.empty()) {
-      ++k;
-      nact = (int)act.size();
-      const C* vk = W.V + (int64_t)(k - 1) * vstr;
-      if (o.flexible) {
-        C* zk = W.Z + (int64_t)(k - 1) * vstr;
-        if (M) (*M)(act, W.slots_dev.p, vk, zk);
-        else convert_vectors<R, R>(ctx, vk, n, zk, n, n, nact, W.slots_dev.p, W.slots_dev.p);
-        A(act, W.slots_dev.p, zk, W.w);
-      } else {
-        if (M) {
-          A(act, W.slots_dev.p, vk, W.t);
-          (*M)(act, W.slots_dev.p, W.t, W.w);
+  VH_REQUIRE(!o.true_restart || (o.flexible && o.final_residual), "internal: true_restart needs a flexible solve");
+  std::vector<int> fin = sys;   // systems whose true residual the next check computes
+  for (;;) {
+    while (!act.empty()) {
+      for (int s : act) outit[s]++;
+      int nact = (int)act.size();
+      upload(act, W.slots_dev.p);
+      // V(:,1) = r/||r||  (pgmres_iter.m:55); g = ||r|| e1
+      ops.norm2(W.r, nact, W.slots_dev.p, W.nrm.p);
+      ops.scale(W.r, W.V, nact, W.slots_dev.p, W.nrm.p);
+      ls_reset(ctx, ls, W.slots_dev.p, nact, W.nrm.p);
+      int k = 0;
+      while (k < m && !act.empty()) {
+        ++k;
+        nact = (int)act.size();
+        const C* vk = W.V + (int64_t)(k - 1) * vstr;
+        if (o.flexible) {
+          C* zk = W.Z + (int64_t)(k - 1) * vstr;
+          if (M) (*M)(act, W.slots_dev.p, vk, zk);
+          else convert_vectors<R, R>(ctx, vk, n, zk, n, n, nact, W.slots_dev.p, W.slots_dev.p);
+          A(act, W.slots_dev.p, zk, W.w);
         } else {
-          A(act, W.slots_dev.p, vk, W.w);
+          if (M) {
+            A(act, W.slots_dev.p, vk, W.t);
+            (*M)(act, W.slots_dev.p, W.t, W.w);
+          } else {
+            A(act, W.slots_dev.p, vk, W.w);
+          }
+        }
+        for (int s : act) { st[s].iters++; st[s].matvecs++; }
+        // classical Gram-Schmidt, one pass (pgmres_iter.m:107-111).  With reorth
+        // (the FP32 inner solve: one pass loses orthogonality once the residual
+        // nears sqrt(eps)) a second pass re-projects w.  Up to kFusedMax basis
+        // vectors the two passes take three sweeps over the basis (projection;
+        // update fused with the second projection; second update fused with
+        // the normalisation); beyond, the second pass is selective: it runs for
+        // the systems whose first pass cancelled more than half of ||w||^2, the
+        // others get a zero correction and keep their norm.
+        const bool fused = o.reorth && k <= VecOps<R>::kFusedMax;
+        if (fused) {
+          ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p);
+          ops.combine_dot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, W.h2.p);
+          ops.combine_scale(W.V, vstr, W.w, W.V + (int64_t)k * vstr, k, nact, W.slots_dev.p, W.h2.p, W.nrm.p);
+        } else {
+          ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, nullptr, o.reorth);
+          ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, 1.0, -1.0, W.nrm.p);
+          if (o.reorth) {
+            ops.reorth_flags(W.h.p, k, W.nrm.p, W.need.p, nact);
+            ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h2.p, W.need.p);
+            ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h2.p, 1.0, -1.0, W.nrm.p, W.need.p);
+          }
+          ops.scale(W.w, W.V + (int64_t)k * vstr, nact, W.slots_dev.p, W.nrm.p);
+        }
+        // column k of H and its Givens rotation (pgmres_iter.m:112-123)
+        ls_push(ctx, ls, W.slots_dev.p, nact, k - 1, W.h.p, o.reorth ? W.h2.p : nullptr, m + 1, W.nrm.p, W.lsres.p);
+        VH_HIP(hipMemcpyAsync(W.h_res.p, W.lsres.p, (size_t)nact * sizeof(double), hipMemcpyDeviceToHost, strm));
+        sync();
+        ++nsteps;
+        const auto th = clk::now();
+        std::vector<int> conv, keep;
+        for (int a = 0; a < nact; ++a) {
+          const int s = act[a];
+          const double rres = W.h_res[a];
+          st[s].resvec.push_back(rres / bnorm[s]);
+          if (!(rres >= tol_of(s) * bnorm[s])) conv.push_back(s);   // pgmres_iter.m:126 (NaN also stops)
+          else if (o.stop_slot && (*o.stop_slot)[s]) conv.push_back(s);
+          else keep.push_back(s);
+        }
+        t_host += std::chrono::duration<double>(clk::now() - th).count();
+        if (!conv.empty()) {
+          finish(conv, k, false);
+          act.swap(keep);
+          if (!act.empty()) upload(act, W.slots_dev.p);
         }
       }
-      for (int s : act) { st[s].iters++; st[s].matvecs++; }
-      // classical Gram-Schmidt, one pass (pgmres_iter.m:107-111).  With reorth
-      // (the FP32 inner solve: one pass loses orthogonality once the residual
-      // nears sqrt(eps)) a second pass re-projects w.  Up to kFusedMax basis
-      // vectors the two passes take three sweeps over the basis (projection;
-      // update fused with the second projection; second update fused with
-      // the normalisation); beyond, the second pass is selective: it runs for
-      // the systems whose first pass cancelled more than half of ||w||^2, the
-      // others get a zero correction and keep their norm.
-      const bool fused = o.reorth && k <= VecOps<R>::kFusedMax;
-      if (fused) {
-        ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p);
-        ops.combine_dot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, W.h2.p);
-        ops.combine_scale(W.V, vstr, W.w, W.V + (int64_t)k * vstr, k, nact, W.slots_dev.p, W.h2.p, W.nrm.p);
-      } else {
-        ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, nullptr, o.reorth);
-        ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h.p, 1.0, -1.0, W.nrm.p);
-        if (o.reorth) {
-          ops.reorth_flags(W.h.p, k, W.nrm.p, W.need.p, nact);
-          ops.mdot(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h2.p, W.need.p);
-          ops.combine(W.V, vstr, W.w, k, nact, W.slots_dev.p, W.h2.p, 1.0, -1.0, W.nrm.p, W.need.p);
-        }
-        ops.scale(W.w, W.V + (int64_t)k * vstr, nact, W.slots_dev.p, W.nrm.p);
-      }
-      // column k of H and its Givens rotation (pgmres_iter.m:112-123)
-      ls_push(ctx, ls, W.slots_dev.p, nact, k - 1, W.h.p, o.reorth ? W.h2.p : nullptr, m + 1, W.nrm.p, W.lsres.p);
-      VH_HIP(hipMemcpyAsync(W.h_res.p, W.lsres.p, (size_t)nact * sizeof(double), hipMemcpyDeviceToHost, strm));
-      sync();
-      ++nsteps;
-      const auto th = clk::now();
-      std::vector<int> conv, keep;
-      for (int a = 0; a < nact; ++a) {
-        const int s = act[a];
-        const double rres = W.h_res[a];
-        st[s].resvec.push_back(rres / bnorm[s]);
-        if (!(rres >= tol_of(s) * bnorm[s])) conv.push_back(s);   // pgmres_iter.m:126 (NaN also stops)
-        else if (o.stop_slot && (*o.stop_slot)[s]) conv.push_back(s);
-        else keep.push_back(s);
-      }
-      t_host += std::chrono::duration<double>(clk::now() - th).count();
-      if (!conv.empty()) {
-        finish(conv, k, false);
+      if (!act.empty()) {
+        finish(act, k, true);
+        std::vector<int> keep;
+        for (int s : act) if (outit[s] < o.maxit) keep.push_back(s);
         act.swap(keep);
-        if (!act.empty()) upload(act, W.slots_dev.p);
       }
     }
-    if (!act.empty()) {
-      finish(act, k, true);
-      std::vector<int> keep;
-      for (int s : act) if (outit[s] < o.maxit) keep.push_back(s);
-      act.swap(keep);
-    }
-  }
 
-  if (trace_on())
-    fprintf(stderr, "[vh] gmres<%s> nsys %d m %d: %d steps, alloc %.3f s, gpu wait %.3f s, host %.3f s, total %.3f s\n",
-            sizeof(R) == 8 ? "f64" : "f32", nsys, m, nsteps, t_alloc, t_wait, t_host,
-            std::chrono::duration<double>(clk::now() - t_start).count());
-  // relres = ||b - A x|| / ||b||  (pgmres.m:181-186)
-  if (o.final_residual) {
-    upload(sys, W.slots_dev.p);
-    A(sys, W.slots_dev.p, x, W.t);
-    ops.sub_from(b, n, W.t, nsys, W.slots_dev.p);
-    ops.norm2(W.t, nsys, W.slots_dev.p, W.nrm.p);
-    fetch_norms(nsys);
+    // relres = ||b - A x|| / ||b||  (pgmres.m:181-186)
+    if (!o.final_residual) {
+      for (int s : sys) { st[s].relres = st[s].resvec.back(); st[s].flag = st[s].relres <= tol_of(s) ? 0 : 1; }
+      break;
+    }
+    const int nf = (int)fin.size();
+    upload(fin, W.slots_dev.p);
+    A(fin, W.slots_dev.p, x, W.t);
+    ops.sub_from(b, n, W.t, nf, W.slots_dev.p);
+    ops.norm2(W.t, nf, W.slots_dev.p, W.nrm.p);
+    fetch_norms(nf);
     const double2* bn = W.h_bn.p;
-    ops.norm2(b, nsys, W.slots_dev.p, W.nrm.p);
-    VH_HIP(hipMemcpyAsync(W.h_bn.p, W.nrm.p, nsys * sizeof(double2), hipMemcpyDeviceToHost, strm));
+    ops.norm2(b, nf, W.slots_dev.p, W.nrm.p);
+    VH_HIP(hipMemcpyAsync(W.h_bn.p, W.nrm.p, nf * sizeof(double2), hipMemcpyDeviceToHost, strm));
     VH_HIP(hipStreamSynchronize(strm));
-    for (int a = 0; a < nsys; ++a) {
-      const int s = sys[a];
+    std::vector<int> redo;
+    for (int a = 0; a < nf; ++a) {
+      const int s = fin[a];
       const double nb = std::sqrt(bn[a].x);
       st[s].relres = nb > 0 ? std::sqrt(nbuf[a].x) / nb : 0.0;
       st[s].flag = (st[s].relres <= tol_of(s)) ? 0 : 1;
       st[s].matvecs++;
+      if (o.true_restart && st[s].flag && outit[s] < o.maxit && !(o.stop_slot && (*o.stop_slot)[s]) &&
+          std::isfinite(st[s].relres)) {
+        redo.push_back(s);
+        st[s].resvec.push_back(st[s].relres);
+      }
     }
-  } else {
-    for (int s : sys) { st[s].relres = st[s].resvec.back(); st[s].flag = st[s].relres <= tol_of(s) ? 0 : 1; }
+    if (redo.empty()) break;
+    // restart those systems from their true residual r = b - A x (in W.t)
+    upload(redo, W.slots_dev.p);
+    convert_vectors<R, R>(ctx, W.t, n, W.r, n, n, (int)redo.size(), W.slots_dev.p, W.slots_dev.p);
+    if (trace_on()) fprintf(stderr, "[vh] gmres: %zu systems restart from their true residual\n", redo.size());
+    act = redo;
+    fin = redo;
   }
+  if (trace_on())
+    fprintf(stderr, "[vh] gmres<%s> nsys %d m %d: %d steps, alloc %.3f s, gpu wait %.3f s, host %.3f s, total %.3f s\n",
+            sizeof(R) == 8 ? "f64" : "f32", nsys, m, nsteps, t_alloc, t_wait, t_host,
+            std::chrono::duration<double>(clk::now() - t_start).count());
 }
 
 template void gmres_batch<double>(Context*, int64_t, const OpFn<double>&, const OpFn<double>*, const double2*, double2*,
@@ -383,6 +401,7 @@ void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<f
   oo.flexible = true;
   oo.x0 = o.x0;
   oo.stop_slot = &stop;
+  oo.true_restart = true;
   gmres_batch<double>(ctx, n, A64, &inner, b, x, sys, oo, W64, st);
   for (int s : sys) st[s].inner_iters = inner_iters[s];
 }
