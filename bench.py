@@ -657,7 +657,7 @@ def main():
 
     z = mv['c64']['passes']['Z']
     nt, e = ZSHAPES32[mv['zshape']['c64']]
-    kname = ('k_pass_z_p2<float, 256, %d, true>' % nt) if e == 8 else ('k_pass_z<float, %d, %d, 4>' % (e, nt))
+    kname = 'k_pass_z_p2<float, 256, %d, %d, true>' % (nt, e)   # compile-time kernel of the power-of-two Pz = 256
     roof = dict(kernel=kname + ' (z-FFT + Green product + inverse z-FFT), 128^3 grid, complex64, 1 vector per launch',
                 bound='hbm', achieved=z['GBs'], peak=HBM_PEAK_GBS, unit='GB/s',
                 frac=(z['GBs'] / HBM_PEAK_GBS) if z['GBs'] else None, traffic=recorded_traffic(),

@@ -735,52 +735,79 @@ __global__ __launch_bounds__(NT) void k_pass_x_inv(
   const int* vm = vmap + (int64_t)L * line;
   const int64_t lbase = (int64_t)L * line;
   const int4 run = valid && !dense ? runs[l] : make_int4(-1, 0, 0, 0);
+  // voxel id of output o of this line (-1: no voxel)
+  auto vid_of = [&](int o) -> int {
+    if (dense) return (int)(lbase + o);
+    if (run.x >= 0) return (o >= run.x && o < run.y) ? run.z + o : -1;
+    return vm[o];
+  };
+  // Modes 0 and 1: the diagonal term w z J and the incidence term Ar' phi of
+  // every output of the line do not depend on the transform, so they are
+  // gathered before it (their loads overlap each other instead of forming a
+  // chain per output element in the last stage) into the line's row `pre`
+  // of shared memory behind the transform's buffer.
+  C* const pre = (C*)smem + T * (lpad(Px - 1) + 1) + t * L;
+  if constexpr (mode <= 1) {
+    constexpr int U = 4;   // outputs per round: all their loads in flight together
+    auto coef = [](int c) { return c == 1 ? R(1) : c == 2 ? R(-1) : c == 3 ? R(0.5) : R(-0.5); };
+    const int K = atpk ? pinc.k[comp] : 0;
+    const int* const pk = atpk ? atpk + pinc.off[comp] : nullptr;
+    for (int o0 = jl; o0 < L; o0 += U * tpp) {
+      int vid[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int o = o0 + u * tpp;
+        vid[u] = (valid && o < L) ? vid_of(o) : -1;
+      }
+      C zx[U], jv[U];
+      int2 e[U][3];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int vi = vid[u] < 0 ? 0 : vid[u];
+        zx[u] = vid[u] >= 0 ? zv[vi] : czero<R>();
+        jv[u] = vid[u] >= 0 ? xs[(int64_t)comp * g.nv + vi] : czero<R>();
+        if constexpr (mode == 0) {
+          // packed row: up to three pairs of (node, coefficient code)
+          // without the row-pointer chain; K is uniform across the workgroup
+#pragma unroll
+          for (int q = 0; q < 3; ++q)
+            e[u][q] = (pk && vid[u] >= 0 && 2 * q < K) ? ((const int2*)(pk + (int64_t)vi * K))[q] : make_int2(0, 0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (vid[u] < 0) continue;
+        C acc = cscale(cmul(zx[u], jv[u]), w);
+        if constexpr (mode == 0) {
+          C sum = czero<R>();
+          if (pk) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+              const int ca = e[u][q].x & 7, cb = e[u][q].y & 7;
+              if (ca) { const C ph = xs[num_curr + (e[u][q].x >> 3)]; const R v = coef(ca); sum.x += v * ph.x; sum.y += v * ph.y; }
+              if (cb) { const C ph = xs[num_curr + (e[u][q].y >> 3)]; const R v = coef(cb); sum.x += v * ph.x; sum.y += v * ph.y; }
+            }
+          } else {
+            const int64_t row = (int64_t)comp * g.nv + vid[u];
+            for (int k = atp[row]; k < atp[row + 1]; ++k) {
+              const C ph = xs[num_curr + atc[k]];
+              sum.x += atv[k] * ph.x; sum.y += atv[k] * ph.y;
+            }
+          }
+          acc = csub(acc, sum);
+        }
+        pre[o0 + u * tpp] = acc;
+      }
+    }
+    __syncthreads();
+  }
   auto gst = [&](int o, C acc) {
     if (!valid || o >= L) return;
-    int vid;
-    if (dense) vid = (int)(lbase + o);
-    else if (run.x >= 0) vid = (o >= run.x && o < run.y) ? run.z + o : -1;
-    else vid = vm[o];
+    const int vid = vid_of(o);
     if (vid < 0) return;
-    const int64_t row = (int64_t)comp * g.nv + vid;
     acc = cmul(alpha, acc);
-    if constexpr (mode <= 1) acc = cadd(acc, cscale(cmul(zv[vid], xs[row]), w));
-    if constexpr (mode == 0) {
-      // a current touches at most 2 nodes: both potentials loaded at once
-      C s = czero<R>();
-      if (atpk) {
-        // packed row: nodes and coefficients without the row-pointer chain;
-        // the component (so the entry count) is uniform across the workgroup
-        const int K = pinc.k[comp];
-        const int2* e2 = (const int2*)(atpk + pinc.off[comp] + (int64_t)vid * K);
-        auto coef = [](int c) { return c == 1 ? R(1) : c == 2 ? R(-1) : c == 3 ? R(0.5) : R(-0.5); };
-        int2 e[3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) e[q] = 2 * q < K ? e2[q] : make_int2(0, 0);
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          const int ca = e[q].x & 7, cb = e[q].y & 7;
-          if (ca) { const C ph = xs[num_curr + (e[q].x >> 3)]; const R v = coef(ca); s.x += v * ph.x; s.y += v * ph.y; }
-          if (cb) { const C ph = xs[num_curr + (e[q].y >> 3)]; const R v = coef(cb); s.x += v * ph.x; s.y += v * ph.y; }
-        }
-        ys[row] = csub(acc, s);
-        return;
-      }
-      const int k0 = atp[row], k1 = atp[row + 1];
-      if (k1 - k0 <= 2) {
-        const int n0 = k0 < k1 ? atc[k0] : 0, n1 = k0 + 1 < k1 ? atc[k0 + 1] : 0;
-        const R v0 = k0 < k1 ? atv[k0] : R(0), v1 = k0 + 1 < k1 ? atv[k0 + 1] : R(0);
-        if (k0 < k1) { const C ph = xs[num_curr + n0]; s.x += v0 * ph.x; s.y += v0 * ph.y; }
-        if (k0 + 1 < k1) { const C ph = xs[num_curr + n1]; s.x += v1 * ph.x; s.y += v1 * ph.y; }
-      } else {
-        for (int k = k0; k < k1; ++k) {
-          const C ph = xs[num_curr + atc[k]];
-          s.x += atv[k] * ph.x; s.y += atv[k] * ph.y;
-        }
-      }
-      acc = csub(acc, s);
-    }
-    ys[row] = acc;
+    if constexpr (mode <= 1) acc = cadd(acc, pre[o]);
+    ys[(int64_t)comp * g.nv + vid] = acc;
   };
   fft_pencil<R, +1, E>(pl, lds, lx, jl, tpp, gld, gst);
 }
@@ -797,7 +824,7 @@ __global__ __launch_bounds__(256) void k_node_rows(
   // before any is used (a wave reads 64 consecutive nodes' currents, which lie
   // in a few contiguous runs).  A node touches at most 6 currents; a longer
   // row takes the looping path.
-  constexpr int RB = 8, RC = 4, D = 6;
+  constexpr int RB = 8, RC = sizeof(R) == 4 ? 8 : 4, D = 6;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= num_node) return;
   const int r0 = blockIdx.y * RB, r1 = min(nrhs, r0 + RB);
@@ -1140,9 +1167,12 @@ void run_passes_t(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>*
     alpha.x = (R)are;
     alpha.y = (R)aim;
     // the output mode is a template parameter: each variant is compiled for its own work
+    // modes 0 and 1 keep each line's diagonal and incidence terms (L
+    // elements) behind the transform's buffer
+    const size_t lds_xi = mode <= 1 ? (size_t)Tx * (lpad_host(g.Px - 1) + 1 + g.L) * elem : lds_x;
     auto xi = [&](auto mc) {
       constexpr int MODE = decltype(mc)::value;
-      hipLaunchKernelGGL((k_pass_x_inv<R, EXY, NT_XY, MODE>), grid, dim3(NT_XY), lds_x, st, B, x, ldx, y, ldy,
+      hipLaunchKernelGGL((k_pass_x_inv<R, EXY, NT_XY, MODE>), grid, dim3(NT_XY), lds_xi, st, B, x, ldx, y, ldy,
                          G.vmap.p, G.lines.p, G.runs.p, G.nlines, log2i(tx), g, px, zz, (R)(1.0 / G.dx), G.ArT.ptr.p,
                          G.ArT.col.p, atv, G.atpk.p, G.pinc, G.num_curr, slots, dense, zslot, alpha, alphas);
     };
