@@ -879,6 +879,17 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
         }
       } else {
         if (s0 > 0) K.layout(n, chunk, restart, true, inner_restart, true, true);   // after a continuation
+        // Systems of a two-fluid conductor (kinetic impedance) skip the FP32
+        // inner solves: their FP32 Krylov process stalls (the preconditioner's
+        // |A_inv| diagonal models the mostly inductive operator poorly and FP32
+        // rounding slows it further; on the uneven batch's strips 30 FP32
+        // steps left the residual near 1e-2), so they run the FP64 FGMRES
+        // below from the start.
+        std::vector<int> msys, dsys;
+        for (int i = 0; i < cnt; ++i) (precons[pidx[s0 + i]]->P.kinetic ? dsys : msys).push_back(i);
+        if (!x_init)
+          for (int i : dsys) VH_HIP(hipMemsetAsync(xc + (int64_t)i * n, 0, (size_t)n * sizeof(double2), st));
+        cs.resize(cnt);
         vh::MixedOpts o;
         o.tol = tol; o.outer_restart = restart; o.outer_maxit = maxit;
         o.inner_restart = inner_restart; o.inner_tol = inner_tol;
@@ -886,15 +897,15 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
         o.inner_flexible = true;
         o.escalate_inner = inner_restart;
         if (tols) o.tol_slot = &tolc;
-        vh::fgmres_mixed(G.ctx, n, A64, A32, M32, bc, xc, sys, o, K, cs);
+        if (!msys.empty()) vh::fgmres_mixed(G.ctx, n, A64, A32, M32, bc, xc, msys, o, K, cs);
         // Systems left above their tolerance -- those whose first inner
         // restart cycle missed its tolerance (the FP32 Krylov process
         // converges too slowly on them, e.g. a superconductor's operator)
         // and any the outer iteration did not finish -- continue from their
         // solutions in FP64 flexible GMRES with the FP64 preconditioner.  Its
         // workspace takes the arena (grown within the budget to restart 50).
-        std::vector<int> last;
-        for (int i = 0; i < cnt; ++i)
+        std::vector<int> last = dsys;
+        for (int i : msys)
           if (cs[i].flag != 0) last.push_back(i);
         if (!last.empty()) {
           int rc = 50;

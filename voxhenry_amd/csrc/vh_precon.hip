@@ -1010,6 +1010,7 @@ void precon_prepare(Precon& P, double jre, double jim, const std::complex<double
   std::complex<double> st[3];
   for (int k = 0; k < 3; ++k) st[k] = jw * std::complex<double>(G.st2[2 * k], G.st2[2 * k + 1]);
   P.zh.assign(zin ? zin : G.zh.data(), (zin ? zin : G.zh.data()) + nv);
+  P.kinetic = std::any_of(P.zh.begin(), P.zh.end(), [](const std::complex<double>& z) { return z.imag() != 0.0; });
   std::vector<double> ainv(nc);
   for (int v = 0; v < nv; ++v) {
     const std::complex<double> z = P.zh[v];

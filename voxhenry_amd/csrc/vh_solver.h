@@ -113,6 +113,7 @@ struct Precon {
   DevBuf<double> Fx64, Bx64, D64, ainv64, Li64, LiT64;
   DevBuf<float> Fx32, Bx32, D32, ainv32, Li32, LiT32;
   std::vector<std::complex<double>> zh;   // impedance per voxel it was prepared with
+  bool kinetic = false;               // some voxel's impedance has a kinetic (imaginary) part: a two-fluid conductor
   DevBuf<double2> z64;                // device copies of zh (the batched matvec reads them per system)
   DevBuf<float2> z32;
   DevBuf<PcTab<double>> tab64;        // one-entry tables for the single apply
