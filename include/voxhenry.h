@@ -242,11 +242,15 @@ int vh_solve_mixed(vh_geom* g, vh_precon* p, double jwmu_re, double jwmu_im, int
  *           inner_tol) on the FP32 matvec, preconditioned by the FP32
  *           factors applied with FP32 work vectors.  A system whose inner
  *           solves end a restart cycle above max(1e-2, 100x their tolerance)
- *           (its FP32 Krylov process stalls, e.g. on a superconductor's
- *           operator), and any system the outer iteration leaves above
- *           tolerance, continues from its solution in FP64 flexible GMRES
- *           with the FP64 preconditioner (restart 50 within the budget, up
- *           to 40 cycles).
+ *           (its FP32 Krylov process stalls), and any system the outer
+ *           iteration leaves above tolerance, continues from its solution in
+ *           FP64 flexible GMRES with the FP64 preconditioner (restart 50
+ *           within the budget, up to 40 cycles); a system whose
+ *           preconditioner was prepared with a kinetic (two-fluid)
+ *           impedance runs that FP64 solve from the start (its FP32 process
+ *           stalls).  In these FGMRES solves a system whose recomputed
+ *           residual misses the tolerance its estimate met restarts from
+ *           that residual.
  * All systems run in lock step and converged ones leave the batch; if the
  * Krylov workspaces exceed hbm_budget bytes (0: 80% of free memory) the batch
  * is processed in chunks.  tols (nullable, host) gives system s its own
