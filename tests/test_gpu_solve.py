@@ -335,9 +335,10 @@ def test_fp64_true_continues_flagged_systems(ctx):
 
 
 def test_mixed_escalation_matches_fp64(ctx):
-    """A strip whose FP32 inner process stalls: the mixed solve hands such
-    systems to FP64 FGMRES; every system converges to tol and Z equals the
-    FP64 solution carried to the same tolerance."""
+    """A two-fluid strip, whose FP32 inner process stalls: the mixed solve
+    runs its systems (kinetic impedance) in FP64 FGMRES from the start; every
+    system converges to tol and Z equals the FP64 solution carried to the
+    same tolerance."""
     from voxhenry_amd import structures, solver
     say('mixed escalation')
     geo = structures.strip(420, 22, thickness=2)
@@ -350,6 +351,31 @@ def test_mixed_escalation_matches_fp64(ctx):
     st = mx['stats']
     say('mixed: converged %d/%d, outer+FP64 steps %.1f, inner %.1f, max relres %.2e' % (
         np.sum(st['flag'] == 0), len(st['flag']), st['iters'].mean(), st['inner_iters'].mean(), st['relres'].max()))
+    assert np.all(st['flag'] == 0) and np.all(st['relres'] <= 1e-8)
+    assert np.all(st['inner_iters'] == 0)
+    eZ = max(np.linalg.norm(mx['Z'][k] - ref['Z'][k]) / np.linalg.norm(ref['Z'][k]) for k in range(len(freqs)))
+    assert eZ < 1e-6, eZ
+    for P in precs:
+        P.close()
+
+
+def test_mixed_inner_stall_escalates(ctx):
+    """A normal conductor with one-step inner restart cycles: a system whose
+    first inner cycle ends above 1e-2 is handed to FP64 FGMRES from its
+    current solution, the others finish in the mixed iteration; every system
+    converges to tol and Z equals the FP64 solution."""
+    from voxhenry_amd import structures, solver
+    geo = structures.bar(40, 6, 6)
+    S = solver.Solver(ctx, geo)
+    freqs = np.logspace(8, 10, 6)
+    precs = S.precons_for(freqs)
+    pidx = np.arange(len(freqs))
+    ref = S.sweep_batched(freqs, method='fp64_true', tol=1e-12, precons=precs, pidx_of_freq=pidx, maxit=20)
+    mx = S.sweep_batched(freqs, method='mixed', tol=1e-8, precons=precs, pidx_of_freq=pidx, inner_restart=1)
+    st = mx['stats']
+    esc = int(np.sum(st['inner_iters'] == 1))
+    say('inner restart 1: converged %d/%d, %d systems with one inner step, outer+FP64 steps %.1f' % (
+        np.sum(st['flag'] == 0), len(st['flag']), esc, st['iters'].mean()))
     assert np.all(st['flag'] == 0) and np.all(st['relres'] <= 1e-8)
     eZ = max(np.linalg.norm(mx['Z'][k] - ref['Z'][k]) / np.linalg.norm(ref['Z'][k]) for k in range(len(freqs)))
     assert eZ < 1e-6, eZ
