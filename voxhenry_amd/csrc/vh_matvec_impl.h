@@ -472,9 +472,8 @@ __global__ __launch_bounds__(NT, OCC) void k_pass_z1_pair(cplx<R>* __restrict__ 
 // pass Z for power-of-two Pz (16..512), compile-time plan: radices, thread
 // mapping and twiddle offsets are constants, the middle stages ping-pong
 // between two LDS buffers (no barrier between a stage's reads and writes),
-// and global loads run ahead of the transforms (FP32 with N = Pz/2: all five
-// components up front, at a register bound of three waves per SIMD;
-// otherwise one component ahead).  HALF (N = Pz/2, the usual padding) makes the
+// and global loads run ahead of the transforms (see zp2_all_ahead).  HALF
+// (N = Pz/2, the usual padding) makes the
 // upper half of the first stage's inputs and of the last stage's outputs
 // compile-time zero / unused.  Same data flow and results as k_pass_z.
 // ---------------------------------------------------------------------------
@@ -535,22 +534,16 @@ template <typename R> struct ZRows {
   }
 };
 
-#ifndef VH_ZW32E8
-#define VH_ZW32E8 4
-#endif
-#ifndef VH_ZW32E4
-#define VH_ZW32E4 3
-#endif
-#ifndef VH_ZW64
-#define VH_ZW64 2
-#endif
-#ifndef VH_ZALL32E8
-#define VH_ZALL32E8 0
-#endif
-#ifndef VH_ZALL32E4
-#define VH_ZALL32E4 1
-#endif
-template <typename R, int E> constexpr int zp2_waves() { return sizeof(R) == 8 ? VH_ZW64 : (E == 8 ? VH_ZW32E8 : VH_ZW32E4); }
+// Register budget of the compile-time pass-Z kernel, as waves per SIMD (the
+// launch bound asks the compiler to fit it): complex64 with 8 elements per
+// thread 4 (122 VGPRs; 5 spills), complex64 with 4 elements 3, complex128 2
+// (3 spills 40+ VGPRs and runs slower).
+template <typename R, int E> constexpr int zp2_waves() { return sizeof(R) == 8 ? 2 : (E == 8 ? 4 : 3); }
+// FP32 with 4 elements per thread loads all five components before the first
+// transform (each wave has its whole input in flight); with 8 elements one
+// component ahead measured faster at 4 waves per SIMD (0.248 vs 0.262 ms on
+// the 128^3 cube).
+template <typename R, int E> constexpr bool zp2_all_ahead() { return sizeof(R) == 4 && E == 4; }
 template <typename R, int E, int NT> constexpr int zp2_minb() { return zp2_waves<R, E>() * 256 / NT > 0 ? zp2_waves<R, E>() * 256 / NT : 1; }
 
 template <typename R, int PZ, int NT, int E, bool HALF>
@@ -601,13 +594,12 @@ __global__ __launch_bounds__(NT, (zp2_minb<R, E, NT>())) void k_pass_z_p2(cplx<R
       }
   };
   C held[5][BML][RL];
-  // Forward transforms.  FP32 with N = Pz/2: all five components are loaded
-  // before the first transform, so a wave has its whole input in flight at
-  // once (memory-level parallelism hides the HBM latency) and component c's
-  // input registers die as its spectrum forms (the upper half of each input
-  // is the constant zero padding and takes no registers).  Otherwise (FP64,
-  // twice the bytes, or a full-length input) one component is loaded ahead.
-  constexpr bool ALL = sizeof(R) == 4 && HALF && (E == 8 ? VH_ZALL32E8 : VH_ZALL32E4);
+  // Forward transforms.  With zp2_all_ahead (N = Pz/2) all five components
+  // are loaded before the first transform and component c's input registers
+  // die as its spectrum forms (the upper half of each input is the constant
+  // zero padding and takes no registers); otherwise one component is loaded
+  // ahead of the transform.
+  constexpr bool ALL = HALF && zp2_all_ahead<R, E>();
   C in[ALL ? 5 : 2][BM0][R0];
   if constexpr (ALL) {
 #pragma unroll
