@@ -898,12 +898,12 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
         o.escalate_inner = inner_restart;
         if (tols) o.tol_slot = &tolc;
         if (!msys.empty()) vh::fgmres_mixed(G.ctx, n, A64, A32, M32, bc, xc, msys, o, K, cs);
-        // Systems left above their tolerance -- those whose first inner
-        // restart cycle missed its tolerance (the FP32 Krylov process
-        // converges too slowly on them, e.g. a superconductor's operator)
-        // and any the outer iteration did not finish -- continue from their
-        // solutions in FP64 flexible GMRES with the FP64 preconditioner.  Its
-        // workspace takes the arena (grown within the budget to restart 50).
+        // The two-fluid systems, and the systems left above their tolerance
+        // -- those whose first inner restart cycle missed its tolerance (the
+        // FP32 Krylov process converges too slowly on them) and any the outer
+        // iteration did not finish -- continue from their solutions in FP64
+        // flexible GMRES with the FP64 preconditioner.  Its workspace takes
+        // the arena (grown within the budget to restart 50).
         std::vector<int> last = dsys;
         for (int i : msys)
           if (cs[i].flag != 0) last.push_back(i);
