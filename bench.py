@@ -362,8 +362,18 @@ def uneven_config(count, nfreq, seed=0, workers=8, budget_frac=0.7):
     freqs = np.logspace(6, 10, nfreq)
     pidx = np.arange(nfreq)
     geos = list(structures.uneven_batch(seed, count))
-    # largest first: the big geometries set the group sizes
-    order = sorted(range(len(geos)), key=lambda i: -np.count_nonzero(geos[i][1]['sigma']) * len(geos[i][1]['ports_P']))
+    # admission order: the largest remaining geometry, then the two smallest,
+    # and so on, so that every group mixes a few large geometries (which set
+    # its memory) with small ones that keep the other workers busy
+    desc = sorted(range(len(geos)), key=lambda i: -np.count_nonzero(geos[i][1]['sigma']) * len(geos[i][1]['ports_P']))
+    order, lo, hi = [], 0, len(desc) - 1
+    while lo <= hi:
+        order.append(desc[lo])
+        lo += 1
+        for _ in range(2):
+            if lo <= hi:
+                order.append(desc[hi])
+                hi -= 1
     free0 = torch.cuda.mem_get_info()[0]
     budget = budget_frac * free0
     cap = 0.5 * budget                 # one geometry's workspaces: at most half the budget (it chunks beyond)
