@@ -896,6 +896,15 @@ int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64
         o.x0 = x_init != 0;
         o.inner_flexible = true;
         o.escalate_inner = inner_restart;
+        // outer iteration: FP32-GMRES iterative refinement (every step tests
+        // the true FP64 residual with its one FP64 matvec), or FP64 FGMRES
+        // with VH_MIXED_OUTER=fgmres (one more FP64 matvec per system for the
+        // final true residual, and its basis and Gram-Schmidt)
+        static const bool outer_fgmres = [] {
+          const char* e = std::getenv("VH_MIXED_OUTER");
+          return e && std::string(e) == "fgmres";
+        }();
+        o.refine = !outer_fgmres;
         if (tols) o.tol_slot = &tolc;
         if (!msys.empty()) vh::fgmres_mixed(G.ctx, n, A64, A32, M32, bc, xc, msys, o, K, cs);
         // The two-fluid systems, and the systems left above their tolerance

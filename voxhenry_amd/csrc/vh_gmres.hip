@@ -357,6 +357,18 @@ __global__ void k_convert(const cplx<RS>* __restrict__ src, int64_t lds, cplx<RD
   dst[(int64_t)sd * ldd + i] = o;
 }
 
+template <typename RD, typename RS>
+__global__ void k_accumulate(const cplx<RS>* __restrict__ src, int64_t lds, cplx<RD>* __restrict__ dst, int64_t ldd,
+                             int64_t n, const int* __restrict__ slots) {
+  const int s = slots[blockIdx.y];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const cplx<RS> v = src[(int64_t)s * lds + i];
+  cplx<RD> o = dst[(int64_t)s * ldd + i];
+  o.x += (RD)v.x; o.y += (RD)v.y;
+  dst[(int64_t)s * ldd + i] = o;
+}
+
 template <typename R>
 __global__ void k_zero_slots(cplx<R>* __restrict__ x, int64_t ld, int64_t n, const int* __restrict__ slots) {
   const int s = slots[blockIdx.y];
@@ -542,6 +554,17 @@ template void convert_vectors<double, double>(Context*, const double2*, int64_t,
                                               const int*, const int*);
 template void convert_vectors<float, float>(Context*, const float2*, int64_t, float2*, int64_t, int64_t, int,
                                             const int*, const int*);
+
+template <typename RD, typename RS>
+void accumulate_vectors(Context* ctx, const cplx<RS>* src, int64_t lds, cplx<RD>* dst, int64_t ldd, int64_t n,
+                        int nact, const int* slots) {
+  if (nact <= 0) return;
+  hipLaunchKernelGGL((k_accumulate<RD, RS>), dim3(ceil_div(n, 256), nact), dim3(256), 0, ctx->stream, src, lds, dst,
+                     ldd, n, slots);
+  VH_LAUNCHED();
+}
+template void accumulate_vectors<double, float>(Context*, const float2*, int64_t, double2*, int64_t, int64_t, int,
+                                                const int*);
 
 // ---------------------------------------------------------------------------
 // Least squares of the Arnoldi cycle (device, one thread per system)

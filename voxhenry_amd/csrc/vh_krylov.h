@@ -44,6 +44,10 @@ template <typename R> struct VecOps {
 template <typename RD, typename RS>
 void convert_vectors(Context* ctx, const cplx<RS>* src, int64_t lds, cplx<RD>* dst, int64_t ldd, int64_t n, int nact,
                      const int* slots_src, const int* slots_dst);
+// dst[slot] += src[slot] (src converted to dst's precision) for the listed slots
+template <typename RD, typename RS>
+void accumulate_vectors(Context* ctx, const cplx<RS>* src, int64_t lds, cplx<RD>* dst, int64_t ldd, int64_t n,
+                        int nact, const int* slots);
 
 // Least squares of an Arnoldi cycle on the device, one thread per system:
 // the (k+1) x k Hessenberg matrix H, its QR factor R by Givens rotations
@@ -163,6 +167,14 @@ struct MixedOpts {
   // for the caller to finish in FP64: its FP32 Krylov process stalls
   int escalate_inner = 0;
   double escalate_resid = 1e-2;
+  // outer iteration: false -- FP64 FGMRES (basis of the FP32 corrections,
+  // one more FP64 matvec for the final true residual); true -- iterative
+  // refinement x += z, z the FP32 inner solution for r = b - A x, with r
+  // recomputed in FP64 after every correction (GMRES-IR: the residual each
+  // step tests is the true one; at most outer_restart * outer_maxit steps, a
+  // system whose residual falls by less than half in a step stops there
+  // unconverged, for the caller to finish in FP64)
+  bool refine = false;
 };
 // Workspaces of the solvers, kept by a geometry across calls.  All large
 // vectors live in one device arena that only grows: releasing and allocating

@@ -336,6 +336,100 @@ template void gmres_batch<double>(Context*, int64_t, const OpFn<double>&, const 
 template void gmres_batch<float>(Context*, int64_t, const OpFn<float>&, const OpFn<float>*, const float2*, float2*,
                                  const std::vector<int>&, const SolveOpts&, KrylovWork<float>&, std::vector<SysStats>&);
 
+// Iterative refinement with FP32 GMRES corrections (GMRES-IR): r = b - A x in
+// FP64, z = FP32 solution of A z = r (inner_solve writes z32), x += z, until
+// ||r|| <= tol ||b||.  One FP64 matvec per step, and the residual it tests is
+// the true FP64 residual of the current x.  A system leaves unconverged when
+// a step reduces its residual by less than half (its FP32 model of A or its
+// inner solve no longer contracts), when its inner solve escalated (stop), or
+// after outer_restart * outer_maxit steps.  Per system: iters = refinement
+// steps, resvec = relative true residuals (the first one of x0).
+template <class Inner>
+static void refine_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const double2* b, double2* x,
+                         const std::vector<int>& sys, const MixedOpts& o, KrylovWork<double>& W, const float2* z32,
+                         Inner& inner_solve, const std::vector<char>& stop, std::vector<SysStats>& st) {
+  hipStream_t strm = ctx->stream;
+  int nslot = 0;
+  for (int s : sys) nslot = std::max(nslot, s + 1);
+  VH_REQUIRE(W.r && W.ld == n && W.nslot >= nslot, "internal: refinement workspace not laid out for this solve");
+  VecOps<double> ops(ctx, n, n, W.m + 1, W.nslot, W.part.p);
+  if ((int)st.size() < nslot) st.resize(nslot);
+  auto tol_of = [&](int s) { return o.tol_slot ? (*o.tol_slot)[s] : o.tol; };
+  // slot lists reach the device through the pinned staging of W, rewritten
+  // only after the stream has drained (the previous copy may be in flight)
+  auto upload = [&](const std::vector<int>& v) {
+    VH_HIP(hipStreamSynchronize(strm));
+    std::copy(v.begin(), v.end(), W.h_slots.p);
+    VH_HIP(hipMemcpyAsync(W.slots_dev.p, W.h_slots.p, v.size() * sizeof(int), hipMemcpyHostToDevice, strm));
+  };
+  const int ns = (int)sys.size();
+  if (ns == 0) return;
+  for (int s : sys) st[s] = SysStats();
+  upload(sys);
+  ops.norm2(b, ns, W.slots_dev.p, W.nrm.p);
+  VH_HIP(hipMemcpyAsync(W.h_bn.p, W.nrm.p, ns * sizeof(double2), hipMemcpyDeviceToHost, strm));
+  const double2* src = b;   // right-hand side of the next correction: b while x = 0, then r
+  if (o.x0) {
+    A64(sys, W.slots_dev.p, x, W.r);
+    ops.sub_from(b, n, W.r, ns, W.slots_dev.p);
+    ops.norm2(W.r, ns, W.slots_dev.p, W.nrm.p);
+    VH_HIP(hipMemcpyAsync(W.h_nrm.p, W.nrm.p, ns * sizeof(double2), hipMemcpyDeviceToHost, strm));
+    src = W.r;
+  } else {
+    ops.zero(x, ns, W.slots_dev.p);
+  }
+  VH_HIP(hipStreamSynchronize(strm));
+  std::vector<double> bnorm(nslot, 0.0), rel(nslot, 0.0);
+  std::vector<int> act;
+  for (int a = 0; a < ns; ++a) {
+    const int s = sys[a];
+    bnorm[s] = std::sqrt(W.h_bn[a].x);
+    if (o.x0) {
+      rel[s] = bnorm[s] > 0 ? std::sqrt(W.h_nrm[a].x) / bnorm[s] : 0.0;
+      st[s].matvecs++;
+    } else {
+      rel[s] = bnorm[s] > 0 ? 1.0 : 0.0;
+    }
+    st[s].resvec.push_back(rel[s]);
+    if (rel[s] > tol_of(s)) act.push_back(s);
+  }
+  const int max_steps = std::max(1, o.outer_restart * o.outer_maxit);
+  for (int step = 0; step < max_steps && !act.empty(); ++step) {
+    const int na = (int)act.size();
+    upload(act);
+    inner_solve(act, W.slots_dev.p, src);
+    accumulate_vectors<double, float>(ctx, z32, n, x, n, n, na, W.slots_dev.p);
+    A64(act, W.slots_dev.p, x, W.r);
+    ops.sub_from(b, n, W.r, na, W.slots_dev.p);
+    ops.norm2(W.r, na, W.slots_dev.p, W.nrm.p);
+    VH_HIP(hipMemcpyAsync(W.h_nrm.p, W.nrm.p, na * sizeof(double2), hipMemcpyDeviceToHost, strm));
+    VH_HIP(hipStreamSynchronize(strm));
+    src = W.r;
+    std::vector<int> keep;
+    for (int a = 0; a < na; ++a) {
+      const int s = act[a];
+      const double r = bnorm[s] > 0 ? std::sqrt(W.h_nrm[a].x) / bnorm[s] : 0.0;
+      st[s].iters++;
+      st[s].matvecs++;
+      st[s].resvec.push_back(r);
+      const bool contracts = r <= 0.5 * rel[s];   // false for NaN
+      rel[s] = r;
+      if (r > tol_of(s) && contracts && !stop[s]) keep.push_back(s);
+    }
+    act.swap(keep);
+  }
+  for (int s : sys) {
+    st[s].relres = rel[s];
+    st[s].flag = rel[s] <= tol_of(s) ? 0 : 1;
+    st[s].outer = st[s].iters;
+  }
+  if (trace_on()) {
+    int nc = 0;
+    for (int s : sys) nc += st[s].flag == 0;
+    fprintf(stderr, "[vh] refine: %d of %d systems at tolerance\n", nc, ns);
+  }
+}
+
 void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<float>& A32, const OpFn<float>& M32,
                   const double2* b, double2* x, const std::vector<int>& sys, const MixedOpts& o, KrylovCache& K,
                   std::vector<SysStats>& st) {
@@ -368,8 +462,10 @@ void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<f
   std::vector<double> itol(nslot, o.inner_tol);
   io.tol_slot = &itol;
   std::vector<char> stop(nslot, 0);
-  // preconditioner of the outer iteration: z = (FP32 GMRES solve of A z = v)
-  OpFn<double> inner = [&](const std::vector<int>& act, const int* slots, const double2* v, double2* z) {
+  // z32 = FP32 GMRES solution of A z = v for the listed systems, each to its
+  // inner tolerance (from the outer relative residual, the last entry of
+  // st[s].resvec)
+  auto inner_solve = [&](const std::vector<int>& act, const int* slots, const double2* v) {
     const int na = (int)act.size();
     for (int s : act) {
       const double r = st[s].resvec.empty() ? 1.0 : st[s].resvec.back();
@@ -378,7 +474,6 @@ void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<f
     }
     convert_vectors<float, double>(ctx, v, n, v32, n, n, na, slots, slots);
     gmres_batch<float>(ctx, n, A32, &M32, v32, z32, act, io, W32, inner_st);
-    convert_vectors<double, float>(ctx, z32, n, z, n, n, na, slots, slots);
     for (int s : act) {
       inner_iters[s] += inner_st[s].iters;
       // escalate a system whose inner solve ended above escalate_resid (and
@@ -391,6 +486,16 @@ void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<f
           !(ir <= std::max(100.0 * itol[s], o.escalate_resid)))
         stop[s] = 1;
     }
+  };
+  if (o.refine) {
+    refine_mixed(ctx, n, A64, b, x, sys, o, W64, z32, inner_solve, stop, st);
+    for (int s : sys) st[s].inner_iters = inner_iters[s];
+    return;
+  }
+  // preconditioner of the outer iteration: z = (FP32 GMRES solve of A z = v)
+  OpFn<double> inner = [&](const std::vector<int>& act, const int* slots, const double2* v, double2* z) {
+    inner_solve(act, slots, v);
+    convert_vectors<double, float>(ctx, z32, n, z, n, n, (int)act.size(), slots, slots);
   };
   SolveOpts oo;
   oo.restart = o.outer_restart;
