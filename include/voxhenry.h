@@ -237,12 +237,19 @@ int vh_solve_mixed(vh_geom* g, vh_precon* p, double jwmu_re, double jwmu_im, int
  *           preconditioner (restart 50, up to 40 cycles), which
  *           stops on the true residual: the plain FP64 path carried to the
  *           same true-residual tolerance as method 1;
- * method 1: mixed precision, FP64 flexible GMRES (outer_restart, maxit) whose
- *           preconditioner is an FP32 flexible GMRES (inner_restart,
- *           inner_tol) on the FP32 matvec, preconditioned by the FP32
- *           factors applied with FP32 work vectors.  A system whose inner
+ * method 1: mixed precision, iterative refinement x += z (up to
+ *           outer_restart * maxit steps) where z is an FP32 flexible GMRES
+ *           solution (inner_restart, inner_tol, relaxed as the residual
+ *           falls) of A z = r on the FP32 matvec, preconditioned by the FP32
+ *           factors applied with FP32 work vectors, and r = b - A x is
+ *           recomputed with the FP64 matvec after every step, so each step
+ *           tests the true residual (with the environment variable
+ *           VH_MIXED_OUTER=fgmres the outer iteration is instead FP64
+ *           flexible GMRES with that inner solve as its preconditioner, as
+ *           in vh_solve_mixed).  A system whose inner
  *           solves end a restart cycle above max(1e-2, 100x their tolerance)
- *           (its FP32 Krylov process stalls), and any system the outer
+ *           (its FP32 Krylov process stalls), a system whose residual falls
+ *           by less than half in a step, and any system the outer
  *           iteration leaves above tolerance, continues from its solution in
  *           FP64 flexible GMRES with the FP64 preconditioner (restart 50
  *           within the budget, up to 40 cycles); a system whose
@@ -257,14 +264,16 @@ int vh_solve_mixed(vh_geom* g, vh_precon* p, double jwmu_re, double jwmu_im, int
  * tolerance tols[s] in place of tol: a sweep then follows the frequency rule
  * of VoxHenry_executer.m:219-223 (1e-12 below 1 MHz, 1e-8 above) in one
  * batch.  x_init != 0 (method 1 only): x holds initial guesses and the
- * outer FGMRES starts from b - A x (e.g. to tighten the tolerance of some
+ * outer iteration starts from b - A x (e.g. to tighten the tolerance of some
  * systems after a first solve); otherwise x0 = 0 as in pgmres.m.
  * Outputs per system (each nullable): flag, relres (true residual
- * ||b - A x|| / ||b|| of the returned x), iters (outer Arnoldi steps),
+ * ||b - A x|| / ||b|| of the returned x), iters (outer steps: Arnoldi
+ * steps, or refinement steps of method 1, plus any FP64 continuation),
  * inner_iters (FP32 steps, method 1), resid_est (the solver's own residual
  * estimate at exit: method 0 the reference-style preconditioned
- * ||M r|| / ||M b|| that pgmres.m:126 stops on, method 1 the outer FGMRES
- * estimate of ||r|| / ||b||). */
+ * ||M r|| / ||M b|| that pgmres.m:126 stops on, method 1 the last
+ * refinement residual ||r|| / ||b|| (the outer FGMRES estimate with
+ * VH_MIXED_OUTER=fgmres)). */
 int vh_solve_batch(vh_geom* g, int64_t nprecon, vh_precon* const* precons, int64_t nsys, const double* jwmu,
                    const int* pidx, const double* b, double* x, int on_device, int x_init, int method, double tol,
                    const double* tols, int restart,

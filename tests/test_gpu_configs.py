@@ -338,7 +338,7 @@ def test_uneven_family_small_vs_oracle(ctx, kind):
         if m == 'mixed' or not kind.startswith('strip'):
             # the FP64 pgmres semantics may stop a two-fluid system on its
             # preconditioned residual above tol (flag 1); the mixed path's
-            # outer FGMRES tests the true residual
+            # refinement and its FP64 continuation test the true residual
             assert np.all(out['stats']['flag'] == 0)
             assert eL < 1e-6 and eRZ < 1e-6 and eZ < 1e-6
         if m == 'mixed' and not kind.startswith('strip'):

@@ -280,10 +280,11 @@ class Geometry:
         'fp64' (pgmres semantics, default restart 100 / maxit 10), 'fp64_true'
         (the same, then the systems it leaves above tol -- it stops on the
         preconditioned residual -- continue in FP64 flexible GMRES until the
-        true residual meets tol) or 'mixed' (default outer restart 8 /
-        maxit 4).  tol: one tolerance, or one per
+        true residual meets tol) or 'mixed' (FP32-GMRES iterative refinement
+        on the FP64 residual, up to restart * maxit steps, default 8 x 4).
+        tol: one tolerance, or one per
         system (e.g. reference_tol of each system's frequency).  x0 (mixed
-        only): initial guesses of the shape of B; the outer FGMRES starts
+        only): initial guesses of the shape of B; the refinement starts
         from B - A x0 (default: zero, as pgmres.m)."""
         m = METHODS[method]
         if restart is None:

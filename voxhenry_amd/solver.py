@@ -156,7 +156,7 @@ class Solver:
         min(R, omega L) / |Z| = rho get their
         tolerance tightened to rl_rtol * rho / RL_MARGIN when that is below
         their tolerance and not below RL_TOL_FLOOR, and continue from their
-        solutions (warm-started FGMRES).  stats['refined'] counts the
+        solutions (warm-started refinement).  stats['refined'] counts the
         systems that continued, stats['rl_unresolved'] those whose target
         lies below the floor (their R is not fixed to rl_rtol); flag /
         relres stay relative to the original tolerance."""
@@ -165,7 +165,7 @@ class Solver:
                 len(set(np.asarray(pidx_of_freq).tolist())) < len(freqs):
             raise ValueError('two-fluid conductors need one preconditioner per frequency')
         if rl_rtol is not None and method != 'mixed':
-            raise ValueError('rl_rtol needs the mixed method (warm-started FGMRES)')
+            raise ValueError('rl_rtol needs the mixed method (warm-started refinement)')
         if self._z_f is None:
             self.G.set_impedance(impedance(self.structure, self.idx, freqs[0]))
             self._z_f = freqs[0]
