@@ -383,6 +383,34 @@ def test_mixed_inner_stall_escalates(ctx):
         P.close()
 
 
+def test_mixed_refinement_reports_true_residual(ctx):
+    """The mixed batched method refines on the FP64 residual: the relres it
+    reports equals ||b - A x|| / ||b|| recomputed here with the FP64 matvec
+    for every system, at a tolerance (1e-11) six decades below what an FP32
+    process reaches alone, and every system took FP32 inner steps and at
+    least two refinement steps (one FP32 solve cannot reach it)."""
+    import voxhenry_amd as vh
+    from voxhenry_amd import structures, solver
+    say('mixed refinement residuals')
+    S = solver.Solver(ctx, structures.bus(nbars=2, length=24, section=3, pitch=5, width_grid=10))
+    freqs = np.array([1e6, 1e8, 1e10])
+    precs = S.precons_for(freqs)
+    B = np.repeat(S.rhs[None], len(freqs), axis=0).reshape(-1, S.G.n)
+    jw = np.repeat(1j * 2 * np.pi * freqs * MU0, S.nports)
+    pidx = np.repeat(np.arange(len(freqs)), S.nports)
+    X, st = S.G.solve_batch(B, jw, precs, pidx, method='mixed', tol=1e-11)
+    rel = np.array([np.linalg.norm(B[s] - S.G.matvec(X[s], jw[s], vh.VH_C128, 0)) / np.linalg.norm(B[s])
+                    for s in range(len(B))])
+    say('refinement steps', st['iters'], 'inner', st['inner_iters'], 'relres', st['relres'], 'recomputed', rel)
+    assert np.all(st['flag'] == 0)
+    assert np.all(rel <= 1e-11)
+    assert np.all(np.abs(rel - st['relres']) <= 1e-2 * rel)
+    assert np.all(st['iters'] >= 2) and np.all(st['inner_iters'] >= 2)
+    for P in precs:
+        P.close()
+    S.G.close()
+
+
 def test_prepare_many_matches_serial(ctx):
     """Preconditioners prepared on host threads equal serially prepared ones."""
     import voxhenry_amd as vh
