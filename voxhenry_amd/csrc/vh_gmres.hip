@@ -10,10 +10,11 @@
 //   the batch (slot map).
 // * fpgmres -- fpgmres.m:58-152 + fpgmres_iter.m:26-121 (flexible, right
 //   preconditioned, bnorm = ||b||).
-// * fgmres_mixed -- FP64 flexible GMRES whose preconditioner is an FP32
-//   flexible GMRES with the FP32 matvec and FP32 preconditioner: each outer
-//   step is one FP64 matvec, so the outer residual is the true FP64 residual
-//   and accuracy is recovered by the outer iteration.
+// * fgmres_mixed -- FP32 flexible GMRES solves (FP32 matvec and
+//   preconditioner) inside an FP64 outer iteration that recovers the
+//   accuracy: iterative refinement on the FP64 residual b - A x, recomputed
+//   after every correction, or FP64 flexible GMRES with the FP32 solve as
+//   its preconditioner (vh_solve.hip).
 //
 // Vectors of one batch live in slots of length ld: vector s of basis j is at
 // V + j*vstride + s*ld.
@@ -357,6 +358,48 @@ __global__ void k_convert(const cplx<RS>* __restrict__ src, int64_t lds, cplx<RD
   dst[(int64_t)sd * ldd + i] = o;
 }
 
+// v[s] = (RD)(b[s] - y[s]) with per-block partials of ||b - y||^2 taken in
+// the source precision before the rounding: npart[a*nblk + b] (the
+// residual of an iterative-refinement step handed to a lower-precision
+// solve in one pass over b and y)
+template <typename RD, typename RS>
+__global__ __launch_bounds__(KT) void k_residual_to(const cplx<RS>* __restrict__ b, const cplx<RS>* __restrict__ y,
+                                                     cplx<RD>* __restrict__ v, int64_t ld, int64_t n,
+                                                     const int* __restrict__ slots, double2* __restrict__ npart,
+                                                     int nchunk) {
+  __shared__ double wn[4];
+  const int a = blockIdx.y;
+  const int s = slots[a];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const cplx<RS>* bs = b + (int64_t)s * ld;
+  const cplx<RS>* ys = y + (int64_t)s * ld;
+  cplx<RD>* vs = v + (int64_t)s * ld;
+  double nrm = 0;
+  for (int c = blockIdx.x; c < nchunk; c += gridDim.x) {
+    const int64_t base = (int64_t)c * CHUNK;
+    cplx<RS> bb[EPT], yy[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int64_t i = base + e * KT + tid;
+      if (i < n) { bb[e] = bs[i]; yy[e] = ys[i]; }
+    }
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int64_t i = base + e * KT + tid;
+      if (i < n) {
+        const double rx = (double)bb[e].x - (double)yy[e].x, ry = (double)bb[e].y - (double)yy[e].y;
+        nrm += rx * rx + ry * ry;
+        cplx<RD> o; o.x = (RD)rx; o.y = (RD)ry;
+        vs[i] = o;
+      }
+    }
+  }
+  nrm = wave_sum(nrm);
+  if (lane == 0) wn[wave] = nrm;
+  __syncthreads();
+  if (tid == 0) { double2 r; r.x = ((wn[0] + wn[1]) + wn[2]) + wn[3]; r.y = 0; npart[(int64_t)a * gridDim.x + blockIdx.x] = r; }
+}
+
 template <typename RD, typename RS>
 __global__ void k_accumulate(const cplx<RS>* __restrict__ src, int64_t lds, cplx<RD>* __restrict__ dst, int64_t ldd,
                              int64_t n, const int* __restrict__ slots) {
@@ -522,6 +565,19 @@ void VecOps<R>::scale(const cplx<R>* src, cplx<R>* dst, int nact, const int* slo
 }
 
 template <typename R>
+template <typename RD>
+void VecOps<R>::residual_to(const cplx<R>* b, const cplx<R>* y, cplx<RD>* v, int nact, const int* slots,
+                            double2* nrm2) {
+  if (nact <= 0) return;
+  hipStream_t st = ctx->stream;
+  const int nb = blocks_for(nchunk, nact);
+  hipLaunchKernelGGL((k_residual_to<RD, R>), dim3(nb, nact), dim3(KT), 0, st, b, y, v, ld, n, slots, part, nchunk);
+  VH_LAUNCHED();
+  hipLaunchKernelGGL(k_reduce2, dim3(nact), dim3(256), 0, st, part, nb, nrm2, 0, nullptr, 0);
+  VH_LAUNCHED();
+}
+
+template <typename R>
 void VecOps<R>::zero(cplx<R>* x, int nact, const int* slots) {
   if (nact <= 0) return;
   hipLaunchKernelGGL((k_zero_slots<R>), dim3(ceil_div(n, 256), nact), dim3(256), 0, ctx->stream, x, ld, n, slots);
@@ -546,6 +602,7 @@ void convert_vectors(Context* ctx, const cplx<RS>* src, int64_t lds, cplx<RD>* d
 
 template struct VecOps<double>;
 template struct VecOps<float>;
+template void VecOps<double>::residual_to<float>(const double2*, const double2*, float2*, int, const int*, double2*);
 template void convert_vectors<float, double>(Context*, const double2*, int64_t, float2*, int64_t, int64_t, int,
                                              const int*, const int*);
 template void convert_vectors<double, float>(Context*, const float2*, int64_t, double2*, int64_t, int64_t, int,

@@ -37,6 +37,10 @@ template <typename R> struct VecOps {
   void norm2(const cplx<R>* x, int nact, const int* slots, double2* nrm2);
   void scale(const cplx<R>* src, cplx<R>* dst, int nact, const int* slots, const double2* nrm2);
   void sub_from(const cplx<R>* b, int64_t ldb, cplx<R>* y, int nact, const int* slots);
+  // v[slot] = (RD)(b[slot] - y[slot]) (b, y, v of stride ld) and nrm2[a] =
+  // ||b - y||^2 before the rounding, in one pass
+  template <typename RD>
+  void residual_to(const cplx<R>* b, const cplx<R>* y, cplx<RD>* v, int nact, const int* slots, double2* nrm2);
   // x[slot] = 0 for the nact slots (one launch; no memset node in a captured graph)
   void zero(cplx<R>* x, int nact, const int* slots);
 };

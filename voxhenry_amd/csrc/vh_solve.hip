@@ -152,12 +152,14 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
   std::vector<int> act = sys;
   upload(act, W.slots_dev.p);
   VH_REQUIRE(!o.x0 || !left, "internal: an initial guess needs a flexible or unpreconditioned solve");
-  if (!o.x0) ops.zero(x, (int)act.size(), W.slots_dev.p);   // x0 = 0 for every system
   for (int s : sys) st[s] = SysStats();
   std::vector<double> bnorm(nslot, 0.0), bnorm_true(nslot, 0.0);
   // r = M b (pgmres.m:120-139) or b (fpgmres.m:91-95); with an initial guess
   // (flexible only) r = b - A x0, relative to ||b||
   std::vector<double> r0(nslot, 1.0);
+  // the first cycle's residual: b itself for a flexible or unpreconditioned
+  // solve from x0 = 0 (no copy), else W.r; later cycles always use W.r
+  const C* rsrc = W.r;
   if (o.x0) {
     ops.norm2(b, nsys, W.slots_dev.p, W.nrm.p);
     VH_HIP(hipMemcpyAsync(W.h_bn.p, W.nrm.p, nsys * sizeof(double2), hipMemcpyDeviceToHost, strm));
@@ -167,9 +169,9 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
   } else if (left) {
     (*M)(act, W.slots_dev.p, b, W.r);
   } else {
-    convert_vectors<R, R>(ctx, b, n, W.r, n, n, nsys, W.slots_dev.p, W.slots_dev.p);
+    rsrc = b;
   }
-  ops.norm2(W.r, nsys, W.slots_dev.p, W.nrm.p);
+  ops.norm2(rsrc, nsys, W.slots_dev.p, W.nrm.p);
   fetch_norms(nsys);
   std::vector<int> outit(nslot, 0);
   {
@@ -188,6 +190,21 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
     }
     act.swap(keep);
   }
+  // x0 = 0: the systems that start no cycle are zeroed here; the others get
+  // x = basis * y at their first update (no zero fill to read back)
+  std::vector<char> xzero(nslot, 0);
+  if (!o.x0) {
+    for (int s : act) xzero[s] = 1;
+    std::vector<int> idle;
+    for (int s : sys) if (!xzero[s]) idle.push_back(s);
+    if (!idle.empty()) {
+      upload(idle, W.aux_slots.p);
+      ops.zero(x, (int)idle.size(), W.aux_slots.p);
+    }
+  }
+  // W.nrm holds ||r||^2 of the systems of sys in order: the first cycle
+  // reuses it when every system starts one
+  bool norms_fresh = act == sys;
 
   // x += basis * y for the listed systems (k vectors); optionally r = V res
   // (k+1 vectors), with y and res from the device least-squares state
@@ -198,10 +215,17 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
       st[s].inner = k;
       st[s].outer = outit[s];
     }
+    // a list is either all first updates (the first cycle, which every
+    // system of it starts together) or all later ones
+    const bool first = xzero[L[0]] != 0;
+    for (int s : L) {
+      VH_REQUIRE((xzero[s] != 0) == first, "internal: mixed first and later solution updates");
+      xzero[s] = 0;
+    }
     upload(L, W.aux_slots.p);
     ls_solve(ctx, ls, W.aux_slots.p, nl, k, m + 1, W.coef.p, restart_r ? W.rcoef.p : nullptr);
     const C* basis = o.flexible ? W.Z : W.V;
-    ops.combine(basis, vstr, x, k, nl, W.aux_slots.p, W.coef.p, 1.0, 1.0, nullptr);      // pgmres_iter.m:136-152
+    ops.combine(basis, vstr, x, k, nl, W.aux_slots.p, W.coef.p, first ? 0.0 : 1.0, 1.0, nullptr);   // pgmres_iter.m:136-152
     if (restart_r) ops.combine(W.V, vstr, W.r, k + 1, nl, W.aux_slots.p, W.rcoef.p, 0.0, 1.0, nullptr);  // :153
   };
 
@@ -213,9 +237,11 @@ void gmres_batch(Context* ctx, int64_t n, const OpFn<R>& A, const OpFn<R>* M, co
       int nact = (int)act.size();
       upload(act, W.slots_dev.p);
       // V(:,1) = r/||r||  (pgmres_iter.m:55); g = ||r|| e1
-      ops.norm2(W.r, nact, W.slots_dev.p, W.nrm.p);
-      ops.scale(W.r, W.V, nact, W.slots_dev.p, W.nrm.p);
+      if (!norms_fresh) ops.norm2(rsrc, nact, W.slots_dev.p, W.nrm.p);
+      norms_fresh = false;
+      ops.scale(rsrc, W.V, nact, W.slots_dev.p, W.nrm.p);
       ls_reset(ctx, ls, W.slots_dev.p, nact, W.nrm.p);
+      rsrc = W.r;
       int k = 0;
       while (k < m && !act.empty()) {
         ++k;
@@ -346,8 +372,9 @@ template void gmres_batch<float>(Context*, int64_t, const OpFn<float>&, const Op
 // steps, resvec = relative true residuals (the first one of x0).
 template <class Inner>
 static void refine_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const double2* b, double2* x,
-                         const std::vector<int>& sys, const MixedOpts& o, KrylovWork<double>& W, const float2* z32,
-                         Inner& inner_solve, const std::vector<char>& stop, std::vector<SysStats>& st) {
+                         const std::vector<int>& sys, const MixedOpts& o, KrylovWork<double>& W, float2* v32,
+                         const float2* z32, Inner& inner_solve, const std::vector<char>& stop,
+                         std::vector<SysStats>& st) {
   hipStream_t strm = ctx->stream;
   int nslot = 0;
   for (int s : sys) nslot = std::max(nslot, s + 1);
@@ -368,15 +395,14 @@ static void refine_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const
   upload(sys);
   ops.norm2(b, ns, W.slots_dev.p, W.nrm.p);
   VH_HIP(hipMemcpyAsync(W.h_bn.p, W.nrm.p, ns * sizeof(double2), hipMemcpyDeviceToHost, strm));
-  const double2* src = b;   // right-hand side of the next correction: b while x = 0, then r
+  // right-hand side of the next correction: b while x = 0 (converted by the
+  // inner solve), then r = b - A x, which residual_to leaves in v32
+  const double2* src = b;
   if (o.x0) {
     A64(sys, W.slots_dev.p, x, W.r);
-    ops.sub_from(b, n, W.r, ns, W.slots_dev.p);
-    ops.norm2(W.r, ns, W.slots_dev.p, W.nrm.p);
+    ops.residual_to<float>(b, W.r, v32, ns, W.slots_dev.p, W.nrm.p);
     VH_HIP(hipMemcpyAsync(W.h_nrm.p, W.nrm.p, ns * sizeof(double2), hipMemcpyDeviceToHost, strm));
-    src = W.r;
-  } else {
-    ops.zero(x, ns, W.slots_dev.p);
+    src = nullptr;
   }
   VH_HIP(hipStreamSynchronize(strm));
   std::vector<double> bnorm(nslot, 0.0), rel(nslot, 0.0);
@@ -393,18 +419,30 @@ static void refine_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const
     st[s].resvec.push_back(rel[s]);
     if (rel[s] > tol_of(s)) act.push_back(s);
   }
+  // x0 = 0: the first correction is written over x (no zero fill to read
+  // back); the systems that take no step are zeroed here
+  bool x_zero = !o.x0;
+  if (x_zero && (int)act.size() < ns) {
+    std::vector<char> on(nslot, 0);
+    for (int s : act) on[s] = 1;
+    std::vector<int> idle;
+    for (int s : sys) if (!on[s]) idle.push_back(s);
+    upload(idle);
+    ops.zero(x, (int)idle.size(), W.slots_dev.p);
+  }
   const int max_steps = std::max(1, o.outer_restart * o.outer_maxit);
   for (int step = 0; step < max_steps && !act.empty(); ++step) {
     const int na = (int)act.size();
     upload(act);
     inner_solve(act, W.slots_dev.p, src);
-    accumulate_vectors<double, float>(ctx, z32, n, x, n, n, na, W.slots_dev.p);
+    if (x_zero) convert_vectors<double, float>(ctx, z32, n, x, n, n, na, W.slots_dev.p, W.slots_dev.p);
+    else accumulate_vectors<double, float>(ctx, z32, n, x, n, n, na, W.slots_dev.p);
+    x_zero = false;
     A64(act, W.slots_dev.p, x, W.r);
-    ops.sub_from(b, n, W.r, na, W.slots_dev.p);
-    ops.norm2(W.r, na, W.slots_dev.p, W.nrm.p);
+    ops.residual_to<float>(b, W.r, v32, na, W.slots_dev.p, W.nrm.p);
     VH_HIP(hipMemcpyAsync(W.h_nrm.p, W.nrm.p, na * sizeof(double2), hipMemcpyDeviceToHost, strm));
     VH_HIP(hipStreamSynchronize(strm));
-    src = W.r;
+    src = nullptr;
     std::vector<int> keep;
     for (int a = 0; a < na; ++a) {
       const int s = act[a];
@@ -464,7 +502,7 @@ void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<f
   std::vector<char> stop(nslot, 0);
   // z32 = FP32 GMRES solution of A z = v for the listed systems, each to its
   // inner tolerance (from the outer relative residual, the last entry of
-  // st[s].resvec)
+  // st[s].resvec); v == nullptr: the right-hand side is already in v32
   auto inner_solve = [&](const std::vector<int>& act, const int* slots, const double2* v) {
     const int na = (int)act.size();
     for (int s : act) {
@@ -472,7 +510,7 @@ void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<f
       const double t = o.tol_slot ? (*o.tol_slot)[s] : o.tol;
       itol[s] = o.adaptive_inner && r > 0 ? std::min(0.1, std::max(o.inner_tol, 0.2 * t / r)) : o.inner_tol;
     }
-    convert_vectors<float, double>(ctx, v, n, v32, n, n, na, slots, slots);
+    if (v) convert_vectors<float, double>(ctx, v, n, v32, n, n, na, slots, slots);
     gmres_batch<float>(ctx, n, A32, &M32, v32, z32, act, io, W32, inner_st);
     for (int s : act) {
       inner_iters[s] += inner_st[s].iters;
@@ -488,7 +526,7 @@ void fgmres_mixed(Context* ctx, int64_t n, const OpFn<double>& A64, const OpFn<f
     }
   };
   if (o.refine) {
-    refine_mixed(ctx, n, A64, b, x, sys, o, W64, z32, inner_solve, stop, st);
+    refine_mixed(ctx, n, A64, b, x, sys, o, W64, v32, z32, inner_solve, stop, st);
     for (int s : sys) st[s].inner_iters = inner_iters[s];
     return;
   }
