@@ -77,6 +77,8 @@ def parse():
     ap.add_argument('--probe-pass-z', action='store_true',
                     help='only the 128^3 matvec (3 calls in complex64, then 3 in complex128) and a reference '
                          'elementwise kernel of known traffic (1 GiB read + 1 GiB written), for hardware-counter runs')
+    ap.add_argument('--probe-bus-step', action='store_true',
+                    help='only the bus set-up and one mixed step (512 systems), for hardware-counter runs')
     return ap.parse_args()
 
 
@@ -629,6 +631,14 @@ def main():
     ctx.use_torch_stream()
     if args.probe_pass_z:
         probe_pass_z(ctx)
+        return
+    if args.probe_bus_step:
+        S, freqs, precs, _ = bus_setup(ctx, args.nfreq)
+        data = bus_rhs(S, freqs)
+        log('[probe] bus set-up done')
+        bus_step(S, freqs, precs, 'mixed', data)
+        torch.cuda.synchronize()
+        log('[probe] done')
         return
     detail = {}
 

@@ -50,6 +50,17 @@ template <> struct cplx_t<double> { using type = double2; };
 template <> struct cplx_t<float> { using type = float2; };
 template <typename R> using cplx = typename cplx_t<R>::type;
 
+// XCD-aware workgroup order: the hardware deals consecutive workgroups of a
+// launch round-robin over the 8 XCDs; xcd_tile maps the linear workgroup id
+// bid of nblk to a logical tile so that each XCD takes a contiguous range of
+// tiles, and neighbouring tiles (which share 128-byte lines) meet in the same
+// L2.  A bijection of [0, nblk).
+__device__ __forceinline__ int xcd_tile(int bid, int nblk) {
+  const int per = nblk >> 3, rem = nblk & 7;
+  const int x = bid & 7, local = bid >> 3;
+  return x * per + (x < rem ? x : rem) + local;
+}
+
 template <typename R> __host__ __device__ inline cplx<R> mk(R x, R y) { cplx<R> c; c.x = x; c.y = y; return c; }
 template <typename C> __host__ __device__ inline C cadd(C a, C b) { a.x += b.x; a.y += b.y; return a; }
 template <typename C> __host__ __device__ inline C csub(C a, C b) { a.x -= b.x; a.y -= b.y; return a; }

@@ -9,15 +9,6 @@
 
 namespace vh {
 
-// XCD-aware tile order: the hardware deals consecutive workgroups round-robin
-// over the 8 XCDs; give each XCD a contiguous range of tiles so neighbouring
-// tiles (which share 128-byte lines at tile edges) meet in the same L2.
-__device__ __forceinline__ int xcd_tile(int bid, int nblk) {
-  const int per = nblk >> 3, rem = nblk & 7;
-  const int x = bid & 7, local = bid >> 3;
-  return x * per + (x < rem ? x : rem) + local;
-}
-
 // Pass-Z tile order.  One row of the tensor octant, (|kx| tile, |ky|), serves
 // the four tiles (±kx, ±ky).  Rows ky and Py-ky are interleaved tile by tile
 // (row 0, and row Py/2 for even Py, are their own mirrors), and within a row
@@ -810,16 +801,21 @@ __global__ __launch_bounds__(256) void k_node_rows(
     const cplx<R>* __restrict__ x, int64_t ldx, cplx<R>* __restrict__ y, int64_t ldy, const int* __restrict__ arp,
     const int* __restrict__ arc, const R* __restrict__ arv, int num_node, int64_t num_curr,
     const int* __restrict__ slots, int nrhs) {
-  // One thread per node serves up to RB right-hand sides (blockIdx.y selects
-  // the group): the row's indices and values are loaded once for all of
-  // them, and the currents of RC right-hand sides at a time are loaded
-  // before any is used (a wave reads 64 consecutive nodes' currents, which lie
-  // in a few contiguous runs).  A node touches at most 6 currents; a longer
-  // row takes the looping path.
+  // One thread per node serves a group of up to RB right-hand sides: the
+  // row's indices and values are loaded once for all of them, and the
+  // currents of RC right-hand sides at a time are loaded before any is used.
+  // A node touches at most 6 currents; a longer row takes the looping path.
+  // One-dimensional grid of node blocks x right-hand-side groups in XCD
+  // order: the currents of consecutive y- and z-type nodes lie a grid row or
+  // plane apart, so a wave touches 64 lines that the next node blocks use
+  // again; those blocks run in the same L2.
   constexpr int RB = 8, RC = sizeof(R) == 4 ? 8 : 4, D = 6;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nbx = (num_node + 255) >> 8;
+  const int lt = xcd_tile(blockIdx.x, gridDim.x);
+  const int by = lt / nbx, bx = lt - by * nbx;
+  const int i = bx * 256 + threadIdx.x;
   if (i >= num_node) return;
-  const int r0 = blockIdx.y * RB, r1 = min(nrhs, r0 + RB);
+  const int r0 = by * RB, r1 = min(nrhs, r0 + RB);
   const int p0 = arp[i], p1 = arp[i + 1];
   if (p1 - p0 <= D) {
     int kk[D];
@@ -1177,7 +1173,8 @@ void run_passes_t(Geometry& G, int nrhs, const cplx<R>* x, int64_t ldx, cplx<R>*
   if (mode == 0 && G.num_node > 0) {
     const R* arv;
     if constexpr (sizeof(R) == 8) arv = G.Ar.v64.p; else arv = G.Ar.v32.p;
-    dim3 grid(ceil_div(G.num_node, 256), ceil_div(nrhs, 8));
+    VH_REQUIRE((int64_t)ceil_div(G.num_node, 256) * ceil_div(nrhs, 8) < (1LL << 31), "node rows: grid too large");
+    dim3 grid((unsigned)(ceil_div(G.num_node, 256) * ceil_div(nrhs, 8)));
     hipLaunchKernelGGL((k_node_rows<R>), grid, dim3(256), 0, st, x, ldx, y, ldy, G.Ar.ptr.p, G.Ar.col.p, arv,
                        (int)G.num_node, G.num_curr, slots, nrhs);
     VH_LAUNCHED();

@@ -50,7 +50,7 @@ template <typename WT, int NPG> __device__ __forceinline__ void line_store(cplx<
 }
 
 // w[g][iperm[v]][j] = b[v] - sum_k Ar(v,k) * Ainv[k] * a[k] of column (g, j).
-// One thread per (node v, group g = blockIdx.y) serves all NPG ports: the
+// One thread per (node v, group g) serves all NPG ports: the
 // row of Ar and the Ainv factors are loaded once, then every port's currents
 // (a wave reads 64 consecutive nodes' currents of one port, which lie in a
 // few contiguous runs) are in flight together, and the node's line of the
@@ -60,8 +60,14 @@ __global__ __launch_bounds__(256) void k_pc_gather(const cplx<R>* __restrict__ x
                                                    int nn, const int* __restrict__ iperm, const int* __restrict__ arp,
                                                    const int* __restrict__ arc, const R* __restrict__ arv,
                                                    const PcTab<R>* __restrict__ tab, PcGroupDev gr, int64_t nc) {
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = blockIdx.y;
+  // one-dimensional grid of node blocks x groups in XCD order (the currents
+  // of consecutive y- and z-type nodes lie a grid row or plane apart: the
+  // lines a wave touches are used again by the next node blocks, which run
+  // in the same L2)
+  const int nbx = (nn + 255) >> 8;
+  const int lt = xcd_tile(blockIdx.x, gridDim.x);
+  const int g = lt / nbx;
+  const int v = (lt - g * nbx) * 256 + threadIdx.x;
   if (v >= nn) return;
   const R* __restrict__ ainv = tab[gr.gtab[g]].ainv;
   const int p0 = arp[v], p1 = arp[v + 1];
@@ -509,7 +515,7 @@ __global__ __launch_bounds__(256) void k_pc_scatter_nodes(const cplx<WT>* __rest
 }
 
 // y[k] = (a[k] + sum_i ArTp(k,i) w[g][i][j]) * Ainv[k]   (ArTp columns are permuted node ids).
-// One thread per (current k, group g = blockIdx.y) serves all NPG ports: the
+// One thread per (current k, group g) serves all NPG ports: the
 // current's incidence row and Ainv[k] are loaded once, the lines of its nodes
 // (at most 6) are read whole, and a wave reads and writes 64 consecutive
 // currents of each port.
@@ -519,8 +525,11 @@ __global__ __launch_bounds__(256) void k_pc_currents(const cplx<R>* __restrict__
                                                      int64_t ldy, int nn, int64_t nc, const int* __restrict__ tp,
                                                      const int* __restrict__ tc, const R* __restrict__ tv,
                                                      const PcTab<R>* __restrict__ tab, PcGroupDev gr) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int g = blockIdx.y;
+  // one-dimensional grid of current blocks x groups in XCD order
+  const int nbx = (int)((nc + 255) >> 8);
+  const int lt = xcd_tile(blockIdx.x, gridDim.x);
+  const int g = lt / nbx;
+  const int64_t k = (int64_t)(lt - g * nbx) * 256 + threadIdx.x;
   if (k >= nc) return;
   const double ai = (double)tab[gr.gtab[g]].ainv[k];
   const cplx<WT>* wg = w + (int64_t)g * nn * NPG;
@@ -1262,6 +1271,8 @@ void precon_apply_cols_wt(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, c
   const PcGroupDev gr = group_table(Y, key, st);
   const int64_t cols = (int64_t)gr.ng * npg;
   VH_REQUIRE(cols < 65536, "preconditioner: too many columns");
+  VH_REQUIRE((int64_t)ceil_div(std::max<int64_t>(nc, nn), 256) * gr.ng < (1LL << 31),
+             "preconditioner: grid of the gather / current kernels too large");
   const size_t need = 2 * (size_t)nn * cols * sizeof(cplx<WT>);
   if (Y.ws.n < need) Y.ws.alloc(need);
   cplx<WT>* w = (cplx<WT>*)Y.ws.p;
@@ -1279,7 +1290,7 @@ void precon_apply_cols_wt(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, c
   if (nn > 0) {
     by_npg([&](auto N) {
       constexpr int n = decltype(N)::value;
-      hipLaunchKernelGGL((k_pc_gather<R, WT, n>), dim3((unsigned)ceil_div(nn, 256), (unsigned)gr.ng), dim3(256), 0, st,
+      hipLaunchKernelGGL((k_pc_gather<R, WT, n>), dim3((unsigned)(ceil_div(nn, 256) * gr.ng)), dim3(256), 0, st,
                          x, ldx, w, nn, Y.d_iperm.p, G.Ar.ptr.p, G.Ar.col.p, arv, tab, gr, nc);
     });
     VH_LAUNCHED();
@@ -1396,7 +1407,7 @@ void precon_apply_cols_wt(Geometry& G, PreconSymbolic& Y, const PcTab<R>* tab, c
   const R* atv = d ? (const R*)G.ArT.v64.p : (const R*)G.ArT.v32.p;
   by_npg([&](auto N) {
     constexpr int n = decltype(N)::value;
-    hipLaunchKernelGGL((k_pc_currents<R, WT, n>), dim3((unsigned)ceil_div(nc, 256), (unsigned)gr.ng), dim3(256), 0,
+    hipLaunchKernelGGL((k_pc_currents<R, WT, n>), dim3((unsigned)(ceil_div(nc, 256) * gr.ng)), dim3(256), 0,
                        st, x, ldx, w, y, ldy, nn, nc, G.ArT.ptr.p, nn > 0 ? Y.ArTp_col.p : nullptr, atv, tab, gr);
   });
   VH_LAUNCHED();
